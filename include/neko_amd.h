@@ -1,0 +1,207 @@
+/*
+ * neko_amd.h -- C ABI of libneko_amd.so, the MI355X (gfx950) implementation
+ * of Neko's pressure/velocity linear-solve hot path.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in entry points with the exact signatures Neko's Fortran binds
+ *     for its HIP backend (device pointers by value, integers by reference,
+ *     FP64 "real", 1-based gs maps, stream = Neko's glb_cmd_queue where the
+ *     reference takes none).  Each names the reference function it replaces.
+ *     Like the reference they return nothing and abort on a HIP error.
+ *
+ *  2. A handle API (nm_*) that owns device copies of one problem's geometry,
+ *     gs maps and mask, and runs whole solves with device-resident scalars.
+ *     Every nm_* function returns 0 on success or a status code; the message
+ *     of the last failure on the calling thread is nm_last_error().
+ *
+ * All fields use Neko's layout: column-major (lx,lx,lx,nel), i.e. point
+ * (i,j,k) of element e at i + lx*(j + lx*(k + lx*e)).  D(i,j) = dx[i+lx*j].
+ * Reference paths below are relative to the Neko source tree.
+ */
+#ifndef NEKO_AMD_H
+#define NEKO_AMD_H
+
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================
+ * 1. Drop-in entry points
+ * ====================================================================== */
+
+/* Neko's global command queue (src/device/device.F90:52, bind(c)); weak
+ * definition in the library, Neko's own wins when linked into Neko. */
+extern void *glb_cmd_queue;
+
+/* replaces hip_ax_helm, src/math/bcknd/device/hip/ax_helm.hip:64-67
+ * (Fortran interface src/math/bcknd/device/ax_helm_device.F90:53-64).
+ * w = sum_rst D^T (h1 G) D u per element; stream glb_cmd_queue. */
+void hip_ax_helm(void *w, void *u, void *dx, void *dy, void *dz, void *dxt,
+                 void *dyt, void *dzt, void *h1, void *g11, void *g22,
+                 void *g33, void *g12, void *g13, void *g23, int *nelv,
+                 int *lx);
+
+/* FP32 twin for Neko's single-precision build (--enable-real=sp); no
+ * reference counterpart with this name (the reference templates on real). */
+void hip_ax_helm_sp(void *w, void *u, void *dx, void *h1, void *g11,
+                    void *g22, void *g33, void *g12, void *g13, void *g23,
+                    int *nelv, int *lx, hipStream_t strm);
+
+/* replaces hip_gather_kernel / hip_scatter_kernel,
+ * src/gs/bcknd/device/hip/gs.hip:52-93 and :98-118 (bound in
+ * src/gs/bcknd/device/gs_device.F90).  op: 1 ADD, 2 MUL, 3 MIN, 4 MAX
+ * (src/gs/gs_ops.f90:35-36); o < 0 selects the plain shared-copy tail. */
+void hip_gather_kernel(void *v, int *m, int *o, void *dg, void *u, int *n,
+                       void *gd, int *nb, void *b, void *bo, int *op,
+                       hipStream_t stream);
+void hip_scatter_kernel(void *v, int *m, void *dg, void *u, int *n, void *gd,
+                        int *nb, void *b, void *bo, hipStream_t stream);
+
+/* replaces hip_glsc3, src/math/bcknd/device/hip/math.hip:891 */
+double hip_glsc3(void *a, void *b, void *c, int *n, hipStream_t stream);
+/* replaces hip_glsc3_many, math.hip:922 (v: device array of j pointers) */
+void hip_glsc3_many(double *h, void *w, void *v, void *mult, int *j, int *n,
+                    hipStream_t stream);
+
+/* replace the BLAS-1 wrappers of math.hip:55-720 used by the Krylov
+ * solvers (add2s1 :440, add2s2 :456, cmult2 :265, copy :55, rzero :231,
+ * cmult :251, cfill :377, add2 :394, col2 :601, col3 :616, sub2 :646,
+ * invcol1 :558, addcol3 :676, addcol4 :691) */
+void hip_add2s1(void *a, void *b, double *c1, int *n, hipStream_t strm);
+void hip_add2s2(void *a, void *b, double *c1, int *n, hipStream_t strm);
+void hip_cmult2(void *a, void *b, double *c1, int *n, hipStream_t strm);
+void hip_copy(void *a, void *b, int *n, hipStream_t strm);
+void hip_rzero(void *a, int *n, hipStream_t strm);
+void hip_cmult(void *a, double *c, int *n, hipStream_t strm);
+void hip_cfill(void *a, double *c, int *n, hipStream_t strm);
+void hip_add2(void *a, void *b, int *n, hipStream_t strm);
+void hip_col2(void *a, void *b, int *n, hipStream_t strm);
+void hip_col3(void *a, void *b, void *c, int *n, hipStream_t strm);
+void hip_sub2(void *a, void *b, int *n, hipStream_t strm);
+void hip_invcol1(void *a, int *n, hipStream_t strm);
+void hip_addcol3(void *a, void *b, void *c, int *n, hipStream_t strm);
+void hip_addcol4(void *a, void *b, void *c, void *d, int *n,
+                 hipStream_t strm);
+
+/* replaces hip_jacobi_update, src/krylov/bcknd/device/hip/pc_jacobi.hip:108.
+ * Computes the CPU formula (src/krylov/bcknd/cpu/pc_jacobi.f90:168-264)
+ * with every element treated as deformed; glb_cmd_queue. */
+void hip_jacobi_update(void *d, void *dxt, void *dyt, void *dzt, void *G11,
+                       void *G22, void *G33, void *G12, void *G13, void *G23,
+                       int *nel, int *lxp);
+/* Same with Neko's per-element flags msh%dfrmd_el (device int32 array):
+ * exact CPU semantics (pc_jacobi.f90:922-955 cross terms only on deformed
+ * elements).  No reference entry point carries the flags. */
+void nm_jacobi_update(void *d, void *dxt, void *G11, void *G22, void *G33,
+                      void *G12, void *G13, void *G23, void *dfrmd, int *nel,
+                      int *lxp, hipStream_t strm);
+
+/* ======================================================================
+ * 2. Handle API
+ * ====================================================================== */
+typedef struct nm_op_s nm_op_t;
+typedef struct nm_cg_s nm_cg_t;
+
+typedef struct {
+  int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
+  int lx;
+  double volume;
+  double h1c;
+  int ifh2;
+} nm_op_info_t;
+
+typedef struct {
+  int iter;          /* CG iterations (mixed: total inner FP32 iterations) */
+  int outer;         /* refinement sweeps (mixed), 0 for FP64 */
+  double res_start;  /* Neko-normalised residuals sqrt(r.mult.r/volume) */
+  double res_final;
+  int converged;
+} nm_monitor_t;
+
+const char *nm_last_error(void);
+
+/* --- setup stand-ins for Neko's space_t / mesh_t / dofmap_t / gs_init --- */
+/* GLL nodes, weights, D and D^T (space.f90:149-320; speclib.f90 ZWGLL/DGLL) */
+int nm_gll(int lx, double *z, double *w, double *d, double *dt);
+/* seeded synthetic inputs (this project's input specification) */
+double nm_rng_uniform(uint64_t seed, uint64_t idx);
+int nm_rng_normal_fill(uint64_t seed, int64_t n, double *out);
+/* unit-cube box mesh, element vertices (nel*8*3) in Neko's hex order
+ * (contrib/genmeshbox/genmeshbox.f90:200-216), interior vertices perturbed
+ * by amp*h*(2U-1) */
+int nm_box_mesh(int nelx, int nely, int nelz, double amp, uint64_t seed,
+                double *vxyz);
+/* msh%dfrmd_el (src/mesh/mesh.f90:437-467) */
+int nm_dfrmd(int64_t nel, const double *vxyz, int *dfrmd);
+/* trilinear GLL coordinates (src/sem/dofmap.f90:866-939 dofmap_xyzlin) */
+int nm_xyz(int64_t nel, int lx, const double *zg, const double *vxyz,
+           double *x, double *y, double *z);
+/* Neko's local gs maps of a box, bit-exact layout of gs_init_mapping
+ * (src/gs/gather_scatter.f90:513-1375): 1-based dg/gd, blk_len/blk_off */
+int nm_box_gs_sizes(int nelx, int nely, int nelz, int lx, int perx, int pery,
+                    int perz, int64_t *nlocal, int64_t *facet_offset);
+int nm_box_gs_map(int nelx, int nely, int nelz, int lx, int perx, int pery,
+                  int perz, int32_t *dg, int32_t *gd, int32_t *blk_len,
+                  int32_t *blk_off, int64_t *nlocal, int64_t *facet_offset,
+                  int64_t *nblks);
+/* zero-Dirichlet mask of a box (1-based, zero_dirichlet.f90 msk(1:m)) */
+int64_t nm_box_mask_size(int nelx, int nely, int nelz, int lx, int perx,
+                         int pery, int perz);
+int nm_box_mask(int nelx, int nely, int nelz, int lx, int perx, int pery,
+                int perz, int32_t *msk, int64_t *nmsk);
+
+/* --- problem (operator) handles ----------------------------------------- */
+/* Whole box problem built on the device (coef on the GPU). */
+int nm_op_create_box(nm_op_t **op, int nelx, int nely, int nelz, int lx,
+                     int perx, int pery, int perz, double amp, uint64_t seed,
+                     double h1, double h2, hipStream_t s);
+/* Problem from Neko's own host arrays (coef_t, gs_t, bc mask) -- what the
+ * Fortran ksp_t/pc_t plug-ins pass at init (src/krylov/krylov.f90:73-201,
+ * src/krylov/precon.f90:40-65). */
+int nm_op_create(nm_op_t **op, int nel, int lx, const double *dx,
+                 const double *g11, const double *g22, const double *g33,
+                 const double *g12, const double *g13, const double *g23,
+                 const double *B, const double *h1, double h1c,
+                 const double *h2, int ifh2, const int *dfrmd, int nlocal,
+                 int facet_offset, int nblks, const int *dg, const int *gd,
+                 const int *blk_len, const int *blk_off, int nmsk,
+                 const int *msk, hipStream_t s);
+void nm_op_free(nm_op_t *op);
+int nm_op_info(const nm_op_t *op, nm_op_info_t *info);
+/* device pointer of a field: 0..5 G11..G23, 6 B, 7 mult, 8 Jacobi 1/diag,
+ * 9..11 x,y,z, 12 h2*B, 13 jac */
+const void *nm_op_field(const nm_op_t *op, int which);
+int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
+                  int32_t *blk_len, int32_t *blk_off);
+int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
+/* w = A_local u ; w = gs(A_local u) ; w = mask(gs(A_local u)) (FP64) */
+int nm_op_ax_local(const nm_op_t *op, double *w, const double *u,
+                   hipStream_t s);
+int nm_op_ax_gs(const nm_op_t *op, double *w, const double *u, hipStream_t s);
+int nm_op_ax(const nm_op_t *op, double *w, const double *u, hipStream_t s);
+int nm_op_ax_local32(nm_op_t *op, float *w, const float *u, hipStream_t s);
+int nm_op_ax_gs32(nm_op_t *op, float *w, const float *u, hipStream_t s);
+int nm_op_gs(const nm_op_t *op, double *u, hipStream_t s);
+int nm_op_mask(const nm_op_t *op, double *u, hipStream_t s);
+
+/* --- Jacobi-PCG (ksp_t 'cg' + pc_t 'jacobi') ----------------------------- */
+/* precision 0: FP64; 1: FP32 inner CG inside FP64 iterative refinement */
+int nm_cg_create(nm_cg_t **cg, nm_op_t *op, int precision);
+void nm_cg_free(nm_cg_t *cg);
+int nm_cg_solve(nm_cg_t *cg, double *x, const double *f, double abs_tol,
+                int max_iter, double inner_rel, nm_monitor_t *mon,
+                hipStream_t stream);
+
+/* ordered copy helper for host harnesses (kind: hipMemcpyKind) */
+int nm_memcpy(void *dst, const void *src, size_t bytes, int kind,
+              hipStream_t strm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEKO_AMD_H */

@@ -1,0 +1,487 @@
+// Jacobi-preconditioned CG on the device (FP64, and FP32-inner / FP64
+// iterative refinement).  Semantics of Neko's ksp_t CG (krylov.f90:56-66,
+// 411-421; cg.f90:140-279): residual norm sqrt(r.mult.r)/sqrt(volume), stop
+// on an absolute tolerance, iteration count reported as Neko does.
+//
+// Unlike Neko's device CG (one host synchronisation per dot product), every
+// scalar stays on the device: alpha, beta and the stopping test are
+// computed by one-block "finish" kernels, every kernel of an iteration
+// exits early once the device flag `done` is set, and the host only polls
+// the flag between batches of graph-captured iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nm_common.h"
+#include "nm_kernels.h"
+#include "nm_op.h"
+
+namespace nm {
+
+struct CgScal {
+  double rtz, rtz_old, pap, rtr, alpha, beta, rnorm, tol, norm_fac, res_start;
+  double scale;  // refinement: inner rhs = scale * r
+  int iter, done, max_iter, converged;
+};
+
+// ---- kernels ---------------------------------------------------------------
+__device__ inline double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline void bsum2(double &a, double &b, double *sh) {
+  a = wsum(a);
+  b = wsum(b);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[wid] = a;
+    sh[8 + wid] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ra = 0.0, rb = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+      ra += sh[q];
+      rb += sh[8 + q];
+    }
+    a = ra;
+    b = rb;
+  }
+  __syncthreads();
+}
+
+// init: x = 0, p = 0, r = scale_in * f; partial rtr = sum r r mult,
+// rtz = sum r r mult dinv.  scale_in comes from the device (refinement) or 1.
+template <typename T, typename TX, typename TF>
+__global__ void __launch_bounds__(RED_THREADS)
+    cg_init_kernel(TX *__restrict__ x, T *__restrict__ r, T *__restrict__ p,
+                   const TF *__restrict__ f, const T *__restrict__ mult,
+                   const T *__restrict__ dinv, int64_t n,
+                   const CgScal *__restrict__ sc_in, double *__restrict__ part) {
+  __shared__ double sh[16];
+  const double sf = sc_in ? sc_in->scale : 1.0;
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T ri = (T)((double)f[i] * sf);
+    x[i] = TX(0);
+    p[i] = T(0);
+    r[i] = ri;
+    const double rm = (double)ri * (double)ri * (double)mult[i];
+    a += rm;
+    b += rm * (double)dinv[i];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = b;
+  }
+}
+
+// finish of init: rnorm, res_start, tolerance (absolute, or relative to the
+// start residual when rel > 0), done flag.
+__global__ void cg_init_fin_kernel(const double *__restrict__ part,
+                                   CgScal *__restrict__ sc, double abs_tol,
+                                   double rel, int max_iter) {
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) {
+    a += part[q];
+    b += part[RED_BLOCKS + q];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    sc->rtr = a;
+    sc->rtz = b;
+    sc->rtz_old = 1.0;
+    sc->beta = 0.0;
+    sc->rnorm = sqrt(a) * sc->norm_fac;
+    sc->res_start = sc->rnorm;
+    sc->tol = rel > 0.0 ? rel * sc->rnorm : abs_tol;
+    sc->iter = 0;
+    sc->max_iter = max_iter;
+    sc->converged = 0;
+    sc->done = (fabs(sc->rnorm) < 2.220446049250313e-16) ? 1 : 0;
+    if (sc->done) sc->converged = 1;
+  }
+}
+
+// p = dinv * r + beta * p
+template <typename T>
+__global__ void cg_p_kernel(T *__restrict__ p, const T *__restrict__ r,
+                            const T *__restrict__ dinv, int64_t n,
+                            const CgScal *__restrict__ sc) {
+  if (sc->done) return;
+  const T beta = (T)sc->beta;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+}
+
+// partial pap = sum w mult p
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+    cg_pap_kernel(const T *__restrict__ w, const T *__restrict__ mult,
+                  const T *__restrict__ p, int64_t n, const CgScal *__restrict__ sc,
+                  double *__restrict__ part) {
+  if (sc->done) return;
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    a += (double)w[i] * (double)mult[i] * (double)p[i];
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+__global__ void cg_alpha_kernel(const double *__restrict__ part, CgScal *__restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) a += part[q];
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    sc->pap = a;
+    sc->alpha = sc->rtz / a;
+  }
+}
+
+// x += alpha p; r -= alpha w; partial rtr and rtz of the new r
+template <typename T, typename TX>
+__global__ void __launch_bounds__(RED_THREADS)
+    cg_update_kernel(TX *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+                     const T *__restrict__ w, const T *__restrict__ mult,
+                     const T *__restrict__ dinv, int64_t n,
+                     const CgScal *__restrict__ sc, double *__restrict__ part) {
+  if (sc->done) return;
+  __shared__ double sh[16];
+  const double alpha = sc->alpha;
+  const T alpha_t = (T)alpha;
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    x[i] = x[i] + (TX)(alpha_t * p[i]);
+    const T ri = r[i] - alpha_t * w[i];
+    r[i] = ri;
+    const double rm = (double)ri * (double)ri * (double)mult[i];
+    a += rm;
+    b += rm * (double)dinv[i];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = b;
+  }
+}
+
+__global__ void cg_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) {
+    a += part[q];
+    b += part[RED_BLOCKS + q];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    sc->rtr = a;
+    sc->rtz_old = sc->rtz;
+    sc->rtz = b;
+    sc->beta = b / sc->rtz_old;
+    sc->rnorm = sqrt(a) * sc->norm_fac;
+    sc->iter = sc->iter + 1;
+    if (sc->rnorm < sc->tol) {
+      sc->done = 1;
+      sc->converged = 1;
+    } else if (sc->iter >= sc->max_iter) {
+      sc->done = 1;
+    }
+  }
+}
+
+// refinement helpers ---------------------------------------------------------
+// r64 = f - w (w = A x), partial rtr
+__global__ void __launch_bounds__(RED_THREADS)
+    resid_kernel(double *__restrict__ r, const double *__restrict__ f,
+                 const double *__restrict__ w, const double *__restrict__ mult,
+                 int64_t n, double *__restrict__ part) {
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ri = f[i] - w[i];
+    r[i] = ri;
+    a += ri * ri * mult[i];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+
+// outer scalars: rnorm of r64, scale = 1/||r|| for the FP32 inner problem
+__global__ void resid_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ outer,
+                                 CgScal *__restrict__ inner) {
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) a += part[q];
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    outer->rtr = a;
+    outer->rnorm = sqrt(a) * outer->norm_fac;
+    const double nr = sqrt(a);
+    inner->scale = nr > 0.0 ? 1.0 / nr : 1.0;
+  }
+}
+
+// x64 += e32 / scale
+__global__ void refine_add_kernel(double *__restrict__ x, const float *__restrict__ e,
+                                  int64_t n, const CgScal *__restrict__ inner) {
+  const double inv = 1.0 / inner->scale;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] += (double)e[i] * inv;
+}
+
+static inline unsigned vec_blocks(int64_t n) {
+  int64_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  if (nb < 1) nb = 1;
+  return (unsigned)nb;
+}
+
+// ---- solver object -----------------------------------------------------------
+// One CG iteration (T = vector type, TX = solution type) on stream s.
+template <typename T, typename TX>
+static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
+                        double *part, hipStream_t s) {
+  const OpFields<T> &F = op.fields<T>();
+  const int64_t n = op.n;
+  hipLaunchKernelGGL(cg_p_kernel<T>, dim3(vec_blocks(n)), dim3(256), 0, s, p, r, F.dinv, n, sc);
+  int st = op.apply<T>(w, p, &sc->done, s, true, true);
+  if (st) return st;
+  hipLaunchKernelGGL(cg_pap_kernel<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, w, F.mult, p,
+                     n, sc, part);
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+  hipLaunchKernelGGL((cg_update_kernel<T, TX>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x,
+                     r, p, w, F.mult, F.dinv, n, sc, part);
+  hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+struct CgSolver {
+  Op *op = nullptr;
+  int mixed = 0;
+  hipStream_t s = nullptr;
+  // FP64 state
+  double *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr, *f = nullptr;
+  // FP32 inner state
+  float *e32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
+  double *part = nullptr;
+  CgScal *sc = nullptr;      // device: [0] main/inner, [1] outer
+  CgScal *h_sc = nullptr;    // pinned host mirror
+  hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  int graph_iters = 4;
+  std::vector<void *> owned;
+  ~CgSolver() {
+    if (g64) (void)hipGraphExecDestroy(g64);
+    if (g32) (void)hipGraphExecDestroy(g32);
+    for (void *q : owned) (void)hipFree(q);
+    if (h_sc) (void)hipHostFree(h_sc);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  template <typename T>
+  T *alloc(int64_t cnt) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(cnt, 1)) != hipSuccess) return nullptr;
+    owned.push_back(q);
+    return (T *)q;
+  }
+};
+
+template <typename T, typename TX>
+static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
+                         hipGraphExec_t *out) {
+  hipGraph_t graph;
+  NM_HIP_TRY(hipStreamBeginCapture(cg->s, hipStreamCaptureModeThreadLocal));
+  int st = NM_OK;
+  for (int k = 0; k < cg->graph_iters && st == NM_OK; k++)
+    st = cg_iteration<T, TX>(*cg->op, x, r, p, w, sc, cg->part, cg->s);
+  hipError_t e = hipStreamEndCapture(cg->s, &graph);
+  if (st) return st;
+  NM_HIP_TRY(e);
+  NM_HIP_TRY(hipGraphInstantiate(out, graph, nullptr, nullptr, 0));
+  NM_HIP_TRY(hipGraphDestroy(graph));
+  return NM_OK;
+}
+
+// Run iterations (graph batches) until the device flag is set.
+static int run_until_done(CgSolver *cg, hipGraphExec_t g, CgScal *sc_dev,
+                          int max_iter) {
+  int launched = 0, batch = 1;
+  for (;;) {
+    for (int b = 0; b < batch; b++) {
+      NM_HIP_TRY(hipGraphLaunch(g, cg->s));
+      launched += cg->graph_iters;
+    }
+    NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, sc_dev, sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
+    NM_HIP_TRY(hipStreamSynchronize(cg->s));
+    if (cg->h_sc->done) return NM_OK;
+    if (launched > max_iter + 8 * cg->graph_iters) {
+      set_error("cg: device did not stop after %d iterations", launched);
+      return NM_ERR_STATE;
+    }
+    if (batch < 16) batch *= 2;
+  }
+}
+
+}  // namespace nm
+
+using namespace nm;
+
+struct nm_cg_s {
+  CgSolver cg;
+};
+
+extern "C" {
+
+typedef struct nm_op_s nm_op_t;
+typedef struct nm_cg_s nm_cg_t;
+
+typedef struct {
+  int iter;          // CG iterations (mixed: total inner FP32 iterations)
+  int outer;         // refinement sweeps (mixed), 0 for FP64
+  double res_start;  // Neko-normalised residual norms
+  double res_final;
+  int converged;
+} nm_monitor_t;
+
+// precision: 0 = FP64 throughout; 1 = FP32 inner CG inside FP64 refinement.
+int nm_cg_create(nm_cg_t **out, nm_op_t *oph, int precision) {
+  *out = nullptr;
+  nm_cg_t *h = new nm_cg_t;
+  CgSolver *cg = &h->cg;
+  cg->op = &oph->op;
+  cg->mixed = precision;
+  const int64_t n = cg->op->n;
+  auto fail = [&](int code) {
+    delete h;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&cg->s, hipStreamNonBlocking) != hipSuccess) return fail(NM_ERR_HIP);
+  if (!(cg->x = cg->alloc<double>(n)) || !(cg->r = cg->alloc<double>(n)) ||
+      !(cg->w = cg->alloc<double>(n)) || !(cg->f = cg->alloc<double>(n)) ||
+      !(cg->part = cg->alloc<double>(2 * RED_BLOCKS)) || !(cg->sc = cg->alloc<CgScal>(2))) {
+    set_error("nm_cg_create: out of device memory");
+    return fail(NM_ERR_NOMEM);
+  }
+  if (!precision) {
+    if (!(cg->p = cg->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+  } else {
+    if (!(cg->e32 = cg->alloc<float>(n)) || !(cg->r32 = cg->alloc<float>(n)) ||
+        !(cg->p32 = cg->alloc<float>(n)) || !(cg->w32 = cg->alloc<float>(n)))
+      return fail(NM_ERR_NOMEM);
+    int st = cg->op->ensure32(cg->s);
+    if (st) return fail(st);
+  }
+  if (hipHostMalloc((void **)&cg->h_sc, sizeof(CgScal) * 2, hipHostMallocDefault) != hipSuccess)
+    return fail(NM_ERR_NOMEM);
+  int st;
+  if (!precision)
+    st = capture_iters<double, double>(cg, cg->x, cg->r, cg->p, cg->w, cg->sc, &cg->g64);
+  else
+    st = capture_iters<float, float>(cg, cg->e32, cg->r32, cg->p32, cg->w32, cg->sc, &cg->g32);
+  if (st) return fail(st);
+  if (hipStreamSynchronize(cg->s) != hipSuccess) return fail(NM_ERR_HIP);
+  *out = h;
+  return NM_OK;
+}
+
+void nm_cg_free(nm_cg_t *h) { delete h; }
+
+// Solve A x = f.  x, f: device FP64 arrays of op->n entries (caller owned).
+// abs_tol: Neko absolute tolerance on sqrt(r.mult.r)/sqrt(volume).
+// Mixed mode: inner FP32 CG to `inner_rel` of its start residual (<= 0:
+// default 1e-4), outer FP64 residual r = f - A x until the absolute
+// tolerance is met.  `stream` is ordered before and after the solve.
+int nm_cg_solve(nm_cg_t *h, double *x, const double *f, double abs_tol,
+                int max_iter, double inner_rel, nm_monitor_t *mon,
+                hipStream_t stream) {
+  CgSolver *cg = &h->cg;
+  const Op &op = *cg->op;
+  const int64_t n = op.n;
+  hipEvent_t ev;
+  NM_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  NM_HIP_TRY(hipEventRecord(ev, stream));
+  NM_HIP_TRY(hipStreamWaitEvent(cg->s, ev, 0));
+  CgScal init{};
+  init.norm_fac = 1.0 / std::sqrt(op.volume);
+  init.scale = 1.0;
+  std::memcpy(&cg->h_sc[0], &init, sizeof(CgScal));
+  std::memcpy(&cg->h_sc[1], &init, sizeof(CgScal));
+  NM_HIP_TRY(hipMemcpyAsync(cg->sc, cg->h_sc, 2 * sizeof(CgScal), hipMemcpyHostToDevice, cg->s));
+  NM_HIP_TRY(hipMemcpyAsync(cg->f, f, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
+  int st = NM_OK;
+  std::memset(mon, 0, sizeof(*mon));
+  if (!cg->mixed) {
+    hipLaunchKernelGGL((cg_init_kernel<double, double, double>), dim3(RED_BLOCKS),
+                       dim3(RED_THREADS), 0, cg->s, cg->x, cg->r, cg->p, cg->f,
+                       op.f64.mult, op.f64.dinv, n, (const CgScal *)nullptr, cg->part);
+    hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                       cg->sc, abs_tol, 0.0, max_iter);
+    st = run_until_done(cg, cg->g64, cg->sc, max_iter);
+    if (st) return st;
+    mon->iter = cg->h_sc->iter;
+    mon->res_start = cg->h_sc->res_start;
+    mon->res_final = cg->h_sc->rnorm;
+    mon->converged = cg->h_sc->converged;
+    NM_HIP_TRY(hipMemcpyAsync(x, cg->x, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
+  } else {
+    // outer FP64 refinement: x = 0, r = f
+    CgScal *inner = cg->sc, *outer = cg->sc + 1;
+    const double rel = inner_rel > 0.0 ? inner_rel : 1e-4;
+    NM_HIP_TRY(hipMemsetAsync(cg->x, 0, sizeof(double) * n, cg->s));
+    int total = 0, sweeps = 0;
+    double res_start = -1.0, rnorm = 0.0;
+    for (;;) {
+      // r = f - A x (FP64 operator, gs, mask) and its norm
+      if ((st = op.apply<double>(cg->w, cg->x, nullptr, cg->s, true, true))) return st;
+      hipLaunchKernelGGL(resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->r,
+                         cg->f, cg->w, op.f64.mult, n, cg->part);
+      hipLaunchKernelGGL(resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                         outer, inner);
+      NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
+      NM_HIP_TRY(hipStreamSynchronize(cg->s));
+      rnorm = cg->h_sc[1].rnorm;
+      if (res_start < 0) res_start = rnorm;
+      if (rnorm < abs_tol || total >= max_iter || std::fabs(rnorm) < 2.220446049250313e-16) break;
+      // inner FP32 solve of A e = r / ||r||, relative tolerance
+      hipLaunchKernelGGL((cg_init_kernel<float, float, double>), dim3(RED_BLOCKS),
+                         dim3(RED_THREADS), 0, cg->s, cg->e32, cg->r32, cg->p32, cg->r,
+                         op.f32.mult, op.f32.dinv, n, (const CgScal *)inner, cg->part);
+      hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                         inner, 0.0, rel, max_iter - total);
+      if ((st = run_until_done(cg, cg->g32, inner, max_iter))) return st;
+      total += cg->h_sc->iter;
+      sweeps++;
+      hipLaunchKernelGGL(refine_add_kernel, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->x,
+                         cg->e32, n, inner);
+      if (sweeps > 64) break;
+    }
+    mon->iter = total;
+    mon->outer = sweeps;
+    mon->res_start = res_start;
+    mon->res_final = rnorm;
+    mon->converged = (rnorm < abs_tol) && total < max_iter;
+    NM_HIP_TRY(hipMemcpyAsync(x, cg->x, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
+  }
+  NM_HIP_TRY(hipEventRecord(ev, cg->s));
+  NM_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
+  NM_HIP_TRY(hipEventDestroy(ev));
+  return NM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Launchers of the gfx950 kernels (implemented in nm_kernels.hip).
+#pragma once
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace nm {
+
+// Reduction geometry: every reduction uses exactly RED_BLOCKS x RED_THREADS
+// threads and a fixed combine order, so a result depends only on the data
+// (deterministic across runs and across concurrent problems).
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+// Arguments of the tensor-product Helmholtz operator
+//   w = sum_{rst} D^T (h1 G) D u  (+ h2B * u)
+template <typename T>
+struct AxArgs {
+  T *w;
+  const T *u;
+  const T *dx;         // D(i,l) column-major, lx*lx
+  const T *h1;         // pointwise h1 field, or nullptr
+  T h1c;               // constant h1 when h1 == nullptr
+  const T *g[6];       // G11 G22 G33 G12 G13 G23
+  const T *h2b;        // premultiplied h2*B field, or nullptr (ifh2 false)
+  int64_t nel;
+  int lx;
+  const int *skip;     // device flag: launch is a no-op when *skip != 0
+};
+
+template <typename T>
+int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s);
+
+// Fused local gather-scatter ADD on the device layout:
+//   blocks  : entries gidx[blk_ptr[b] .. blk_ptr[b+1]) summed in order
+//   pairs   : entries gidx[pair0 + 2p], gidx[pair0 + 2p + 1]
+template <typename T>
+struct GsArgs {
+  T *u;
+  const int32_t *blk_ptr;  // nblks + 1 offsets (0-based)
+  const int32_t *gidx;     // nlocal 0-based local indices
+  int64_t nblks;
+  int64_t pair0;           // offset of the first facet pair in gidx
+  int64_t npairs;
+  const int *skip;
+};
+template <typename T>
+int launch_gs(const GsArgs<T> &a, hipStream_t s);
+
+// Neko-layout gather / scatter (1-based maps, gs_kernels semantics).
+template <typename T>
+int launch_neko_gather(T *v, int m, int o, const int *dg, const T *u,
+                       const int *gd, int nb, const int *b, const int *bo,
+                       int op, hipStream_t s);
+template <typename T>
+int launch_neko_scatter(const T *v, int m, const int *dg, T *u, const int *gd,
+                        int nb, const int *b, const int *bo, hipStream_t s);
+
+template <typename T>
+int launch_mask(T *x, const int32_t *msk0, int64_t m, const int *skip,
+                hipStream_t s);
+
+// Deterministic reductions: partial sums into part[RED_BLOCKS * nv] then a
+// one-block combine into out[nv] (double).
+int launch_glsc3_partial(const double *a, const double *b, const double *c,
+                         int64_t n, double *part, hipStream_t s);
+int launch_glsc3_many_partial(const double *w, const double *const *v,
+                              const double *mult, int j, int64_t n,
+                              double *part, hipStream_t s);
+int launch_combine(const double *part, int nv, double *out, hipStream_t s);
+
+// Elementwise (Neko device_math semantics, 0-based, any n)
+enum VecOp : int {
+  V_COPY, V_RZERO, V_ADD2S1, V_ADD2S2, V_COL2, V_COL3, V_SUB2, V_CMULT2,
+  V_INVCOL1, V_ADDCOL3, V_ADDCOL4, V_CMULT, V_ADD2, V_CFILL
+};
+template <typename T>
+int launch_vec(int op, T *a, const T *b, const T *c, const T *d, T s1,
+               int64_t n, hipStream_t s);
+
+// Geometric factors from GLL point coordinates (coef.f90 formulas).
+int launch_coef(int64_t nel, int lx, const double *x, const double *y,
+                const double *z, const double *dx, const double *w3,
+                double *g[6], double *jac, double *B, hipStream_t s);
+
+// Jacobi diagonal (CPU formula of pc_jacobi.f90), before h1/h2/gs/invert.
+int launch_jacobi_diag(int64_t nel, int lx, const double *dxt,
+                       const double *const g[6], const int *dfrmd,
+                       double *d, hipStream_t s);
+
+// Convert / cast
+int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s);
+
+}  // namespace nm

@@ -1,0 +1,638 @@
+// gfx950 kernels of the solve path: tensor-product Helmholtz operator,
+// gather-scatter, Dirichlet mask, deterministic reductions, BLAS-1 updates,
+// geometric factors and the Jacobi diagonal.
+//
+// Field layout everywhere: Neko's column-major (lx,lx,lx,nel), i.e. point
+// (i,j,k) of element e at i + lx*(j + lx*(k + lx*e)).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "nm_common.h"
+#include "nm_kernels.h"
+
+namespace nm {
+
+// ---------------------------------------------------------------------------
+// Ax: "k-step" tensor-product kernel.
+//
+// One lx*lx thread plane per element (EPB elements per workgroup).  Thread
+// (i,j) keeps the u column u(i,j,:) in registers, so the t-derivative and
+// the final D^T contraction in t never touch LDS; the r- and s-derivatives
+// and their transposes go through one lx*lx LDS plane per element.  The six
+// geometric factors are streamed once, coalesced (consecutive threads read
+// consecutive addresses of each plane).  Reference behaviour:
+// ax_helm_cpu.f90:180-290 (generic) / :1211-1341 (lx=8); device twin
+// ax_helm_kernel.h:149-355.
+// ---------------------------------------------------------------------------
+template <typename T, int LX, int EPB>
+__global__ void __launch_bounds__(LX *LX *EPB)
+    ax_kstep_kernel(T *__restrict__ w, const T *__restrict__ u,
+                    const T *__restrict__ dx, const T *__restrict__ h1, T h1c,
+                    const T *__restrict__ g11, const T *__restrict__ g22,
+                    const T *__restrict__ g33, const T *__restrict__ g12,
+                    const T *__restrict__ g13, const T *__restrict__ g23,
+                    const T *__restrict__ h2b, int64_t nel,
+                    const int *__restrict__ skip) {
+  constexpr int P = LX * LX;
+  __shared__ T sD[LX * LX];
+  __shared__ T su[EPB][P];
+  __shared__ T sr[EPB][P];
+  __shared__ T ss[EPB][P];
+
+  if (skip != nullptr && *skip) return;
+
+  const int t = threadIdx.x;
+  const int q = threadIdx.y;
+  const int i = t % LX;
+  const int j = t / LX;
+  const int64_t e = (int64_t)blockIdx.x * EPB + q;
+  const bool active = e < nel;
+
+  for (int s = t + q * P; s < LX * LX; s += P * EPB) sD[s] = dx[s];
+
+  const int64_t base = e * (int64_t)(LX * P) + t;
+  T uc[LX], ut[LX], wc[LX];
+#pragma unroll
+  for (int k = 0; k < LX; k++) uc[k] = active ? u[base + k * P] : T(0);
+
+  __syncthreads();
+
+  // t-derivative: ut(k) = sum_l D(k,l) u(i,j,l), all in registers.
+#pragma unroll
+  for (int k = 0; k < LX; k++) {
+    T acc = T(0);
+#pragma unroll
+    for (int l = 0; l < LX; l++) acc += sD[k + LX * l] * uc[l];
+    ut[k] = acc;
+  }
+
+#pragma unroll
+  for (int k = 0; k < LX; k++) {
+    su[q][t] = uc[k];
+    __syncthreads();
+    T ur = T(0), us = T(0);
+#pragma unroll
+    for (int l = 0; l < LX; l++) {
+      ur += sD[i + LX * l] * su[q][l + LX * j];
+      us += sD[j + LX * l] * su[q][i + LX * l];
+    }
+    const int64_t p = base + k * P;
+    T G11 = T(0), G22 = T(0), G33 = T(0), G12 = T(0), G13 = T(0), G23 = T(0);
+    T hh = h1c;
+    if (active) {
+      G11 = g11[p]; G22 = g22[p]; G33 = g33[p];
+      G12 = g12[p]; G13 = g13[p]; G23 = g23[p];
+      if (h1 != nullptr) hh = h1[p];
+    }
+    const T vt = ut[k];
+    const T rr = hh * (G11 * ur + G12 * us + G13 * vt);
+    const T rs = hh * (G12 * ur + G22 * us + G23 * vt);
+    const T rt = hh * (G13 * ur + G23 * us + G33 * vt);
+    ut[k] = rt;
+    sr[q][t] = rr;
+    ss[q][t] = rs;
+    __syncthreads();
+    T acc = T(0);
+#pragma unroll
+    for (int l = 0; l < LX; l++) acc += sD[l + LX * i] * sr[q][l + LX * j];
+#pragma unroll
+    for (int l = 0; l < LX; l++) acc += sD[l + LX * j] * ss[q][i + LX * l];
+    wc[k] = acc;
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int k = 0; k < LX; k++) {
+    T acc = wc[k];
+#pragma unroll
+    for (int l = 0; l < LX; l++) acc += sD[l + LX * k] * ut[l];
+    wc[k] = acc;
+  }
+  if (active) {
+    if (h2b != nullptr) {
+#pragma unroll
+      for (int k = 0; k < LX; k++) wc[k] += h2b[base + k * P] * uc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < LX; k++) w[base + k * P] = wc[k];
+  }
+}
+
+template <typename T, int LX, int EPB>
+static int ax_go(const AxArgs<T> &a, hipStream_t s) {
+  dim3 block(LX * LX, EPB);
+  dim3 grid((unsigned)((a.nel + EPB - 1) / EPB));
+  if (a.nel <= 0) return NM_OK;
+  hipLaunchKernelGGL((ax_kstep_kernel<T, LX, EPB>), grid, block, 0, s, a.w,
+                     a.u, a.dx, a.h1, a.h1c, a.g[0], a.g[1], a.g[2], a.g[3],
+                     a.g[4], a.g[5], a.h2b, a.nel, a.skip);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+template <typename T, int LX>
+static int ax_epb(const AxArgs<T> &a, int epb, hipStream_t s) {
+  constexpr int P = LX * LX;
+  if (epb <= 0) {
+    epb = 1;
+    while (P * epb * 2 <= 64 && epb < 16) epb *= 2;
+  }
+  switch (epb) {
+    case 1: return ax_go<T, LX, 1>(a, s);
+    case 2: if constexpr (P * 2 <= 256) return ax_go<T, LX, 2>(a, s); break;
+    case 4: if constexpr (P * 4 <= 256) return ax_go<T, LX, 4>(a, s); break;
+    case 8: if constexpr (P * 8 <= 256) return ax_go<T, LX, 8>(a, s); break;
+    case 16: if constexpr (P * 16 <= 256) return ax_go<T, LX, 16>(a, s); break;
+    default: break;
+  }
+  set_error("ax: elements-per-workgroup %d unsupported for lx=%d", epb, LX);
+  return NM_ERR_UNSUPPORTED;
+}
+
+template <typename T>
+int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s) {
+  switch (a.lx) {
+    case 2: return ax_epb<T, 2>(a, epb, s);
+    case 3: return ax_epb<T, 3>(a, epb, s);
+    case 4: return ax_epb<T, 4>(a, epb, s);
+    case 5: return ax_epb<T, 5>(a, epb, s);
+    case 6: return ax_epb<T, 6>(a, epb, s);
+    case 7: return ax_epb<T, 7>(a, epb, s);
+    case 8: return ax_epb<T, 8>(a, epb, s);
+    case 9: return ax_epb<T, 9>(a, epb, s);
+    case 10: return ax_epb<T, 10>(a, epb, s);
+    case 11: return ax_epb<T, 11>(a, epb, s);
+    case 12: return ax_epb<T, 12>(a, epb, s);
+    case 13: return ax_epb<T, 13>(a, epb, s);
+    case 14: return ax_epb<T, 14>(a, epb, s);
+    default:
+      set_error("ax: lx=%d unsupported (2..14)", a.lx);
+      return NM_ERR_UNSUPPORTED;
+  }
+}
+template int launch_ax<double>(const AxArgs<double> &, int, hipStream_t);
+template int launch_ax<float>(const AxArgs<float> &, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Gather-scatter ADD, fused: one thread per gs block (a run of local dofs
+// with the same gs id) or per facet pair.  Block sums are taken in the
+// order of Neko's sorted map (tmp = u1; tmp += u2; ...), which makes the
+// result bit-identical to gs_gather_kernel_add + gs_scatter_kernel
+// (gs_cpu.f90:98-138, :292-322).  No atomics, no intermediate v buffer.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gs_fused_kernel(T *__restrict__ u, const int32_t *__restrict__ blk_ptr,
+                    const int32_t *__restrict__ gidx, int64_t nblks,
+                    int64_t pair0, int64_t npairs,
+                    const int *__restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nblks) {
+    const int32_t s0 = blk_ptr[t], s1 = blk_ptr[t + 1];
+    T acc = u[gidx[s0]];
+    for (int32_t q = s0 + 1; q < s1; q++) acc += u[gidx[q]];
+    for (int32_t q = s0; q < s1; q++) u[gidx[q]] = acc;
+  } else if (t < nblks + npairs) {
+    const int64_t p = pair0 + 2 * (t - nblks);
+    const int32_t a = gidx[p], b = gidx[p + 1];
+    const T acc = u[a] + u[b];
+    u[a] = acc;
+    u[b] = acc;
+  }
+}
+
+template <typename T>
+int launch_gs(const GsArgs<T> &a, hipStream_t s) {
+  const int64_t items = a.nblks + a.npairs;
+  if (items <= 0) return NM_OK;
+  const int bs = 256;
+  hipLaunchKernelGGL(gs_fused_kernel<T>, dim3((unsigned)((items + bs - 1) / bs)),
+                     dim3(bs), 0, s, a.u, a.blk_ptr, a.gidx, a.nblks, a.pair0,
+                     a.npairs, a.skip);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+template int launch_gs<double>(const GsArgs<double> &, hipStream_t);
+template int launch_gs<float>(const GsArgs<float> &, hipStream_t);
+
+// Neko-layout gather: v(dg(k+1)) = op over u(gd(k+j)) for each block, then
+// the facet pairs (o > 0) or the plain shared copy (o < 0); gs_cpu.f90
+// semantics for ADD/MUL/MIN/MAX (gs_ops.f90:35-36).  One thread per block
+// and one per facet pair / copied entry.
+template <typename T>
+__device__ inline T gs_apply(int op, T a, T b) {
+  switch (op) {
+    case 2: return a * b;
+    case 3: return a < b ? a : b;
+    case 4: return a > b ? a : b;
+    default: return a + b;
+  }
+}
+
+template <typename T>
+__global__ void neko_gather_kernel(T *__restrict__ v, int m, int o,
+                                   const int *__restrict__ dg,
+                                   const T *__restrict__ u,
+                                   const int *__restrict__ gd, int nb,
+                                   const int *__restrict__ b,
+                                   const int *__restrict__ bo, int op) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nb) {
+    const int k = bo[t], len = b[t];
+    T acc = u[gd[k] - 1];
+    for (int q = 1; q < len; q++) acc = gs_apply(op, acc, u[gd[k + q] - 1]);
+    v[dg[k] - 1] = acc;
+    return;
+  }
+  const int64_t r = t - nb;
+  if (o < 0) {
+    const int64_t i = (int64_t)(-o) + r;  // 1-based
+    if (i <= m) v[dg[i - 1] - 1] = u[gd[i - 1] - 1];
+  } else {
+    const int64_t i = (int64_t)o + 2 * r;  // 1-based, pairs
+    if (i + 1 <= m) v[dg[i - 1] - 1] = gs_apply(op, u[gd[i - 1] - 1], u[gd[i] - 1]);
+  }
+}
+
+template <typename T>
+int launch_neko_gather(T *v, int m, int o, const int *dg, const T *u,
+                       const int *gd, int nb, const int *b, const int *bo,
+                       int op, hipStream_t s) {
+  if (m <= 0) return NM_OK;
+  int64_t tail = (o < 0) ? (int64_t)m - (-o) + 1 : ((int64_t)m - o + 1 + 1) / 2;
+  if (tail < 0) tail = 0;
+  const int64_t items = nb + tail;
+  if (items <= 0) return NM_OK;
+  hipLaunchKernelGGL(neko_gather_kernel<T>, dim3((unsigned)((items + 255) / 256)),
+                     dim3(256), 0, s, v, m, o, dg, u, gd, nb, b, bo, op);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+template int launch_neko_gather<double>(double *, int, int, const int *,
+                                        const double *, const int *, int,
+                                        const int *, const int *, int,
+                                        hipStream_t);
+template int launch_neko_gather<float>(float *, int, int, const int *,
+                                       const float *, const int *, int,
+                                       const int *, const int *, int,
+                                       hipStream_t);
+
+// Neko-layout scatter: u(gd(k+j)) = v(dg(k+1)) for blocks, then
+// u(gd(i)) = v(dg(i)) for the remaining entries (gs_cpu.f90:292-322).
+template <typename T>
+__global__ void neko_scatter_kernel(const T *__restrict__ v, int m,
+                                    const int *__restrict__ dg,
+                                    T *__restrict__ u,
+                                    const int *__restrict__ gd, int nb,
+                                    const int *__restrict__ b,
+                                    const int *__restrict__ bo) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int start = nb > 0 ? bo[nb - 1] + b[nb - 1] + 1 : 1;
+  if (t < nb) {
+    const int k = bo[t], len = b[t];
+    const T val = v[dg[k] - 1];
+    for (int q = 0; q < len; q++) u[gd[k + q] - 1] = val;
+    return;
+  }
+  const int64_t i = (int64_t)start + (t - nb);  // 1-based
+  if (i <= m) u[gd[i - 1] - 1] = v[dg[i - 1] - 1];
+}
+
+template <typename T>
+int launch_neko_scatter(const T *v, int m, const int *dg, T *u, const int *gd,
+                        int nb, const int *b, const int *bo, hipStream_t s) {
+  if (m <= 0) return NM_OK;
+  // The first entry after the blocks, bo(nb) + b(nb) + 1, is read by the
+  // kernel itself; launch an upper bound of nb + m items.
+  const int64_t items = (int64_t)nb + m;
+  hipLaunchKernelGGL(neko_scatter_kernel<T>, dim3((unsigned)((items + 255) / 256)),
+                     dim3(256), 0, s, v, m, dg, u, gd, nb, b, bo);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+template int launch_neko_scatter<double>(const double *, int, const int *,
+                                         double *, const int *, int,
+                                         const int *, const int *, hipStream_t);
+template int launch_neko_scatter<float>(const float *, int, const int *,
+                                        float *, const int *, int,
+                                        const int *, const int *, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Zero-Dirichlet mask: x(msk(i)) = 0 (zero_dirichlet.f90:92-118).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void mask_kernel(T *__restrict__ x, const int32_t *__restrict__ msk,
+                            int64_t m, const int *__restrict__ skip) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) x[msk[t]] = T(0);
+}
+
+template <typename T>
+int launch_mask(T *x, const int32_t *msk0, int64_t m, const int *skip,
+                hipStream_t s) {
+  if (m <= 0) return NM_OK;
+  hipLaunchKernelGGL(mask_kernel<T>, dim3((unsigned)((m + 255) / 256)), dim3(256),
+                     0, s, x, msk0, m, skip);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+template int launch_mask<double>(double *, const int32_t *, int64_t, const int *, hipStream_t);
+template int launch_mask<float>(float *, const int32_t *, int64_t, const int *, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions.  Wave64 butterfly (DPP/shuffle) then LDS across
+// the 4 waves of a 256-thread block; fixed grid, fixed combine order.
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int q = 0; q < nw; q++) r += sh[q];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+    glsc3_partial_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                         const double *__restrict__ c, int64_t n,
+                         double *__restrict__ part) {
+  __shared__ double sh[RED_THREADS / 64];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc += a[i] * b[i] * c[i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+int launch_glsc3_partial(const double *a, const double *b, const double *c,
+                         int64_t n, double *part, hipStream_t s) {
+  hipLaunchKernelGGL(glsc3_partial_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS),
+                     0, s, a, b, c, n, part);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// j weighted dot products w.(v_l * mult), l < j  (glsc3_many semantics)
+__global__ void __launch_bounds__(RED_THREADS)
+    glsc3_many_partial_kernel(const double *__restrict__ w,
+                              const double *const *__restrict__ v,
+                              const double *__restrict__ mult, int j, int64_t n,
+                              double *__restrict__ part) {
+  __shared__ double sh[RED_THREADS / 64];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int l = 0; l < j; l++) {
+    const double *vl = v[l];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+      acc += w[i] * vl[i] * mult[i];
+    const double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[(int64_t)l * RED_BLOCKS + blockIdx.x] = r;
+  }
+}
+
+int launch_glsc3_many_partial(const double *w, const double *const *v,
+                              const double *mult, int j, int64_t n,
+                              double *part, hipStream_t s) {
+  hipLaunchKernelGGL(glsc3_many_partial_kernel, dim3(RED_BLOCKS),
+                     dim3(RED_THREADS), 0, s, w, v, mult, j, n, part);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+__global__ void __launch_bounds__(RED_THREADS)
+    combine_kernel(const double *__restrict__ part, int nv, double *__restrict__ out) {
+  __shared__ double sh[RED_THREADS / 64];
+  for (int l = 0; l < nv; l++) {
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x)
+      acc += part[(int64_t)l * RED_BLOCKS + q];
+    const double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[l] = r;
+  }
+}
+
+int launch_combine(const double *part, int nv, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(combine_kernel, dim3(1), dim3(RED_THREADS), 0, s, part,
+                     nv, out);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise updates (device_math.F90 semantics).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void vec_kernel(int op, T *__restrict__ a, const T *__restrict__ b,
+                           const T *__restrict__ c, const T *__restrict__ d,
+                           T s1, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    switch (op) {
+      case V_COPY: a[i] = b[i]; break;
+      case V_RZERO: a[i] = T(0); break;
+      case V_ADD2S1: a[i] = s1 * a[i] + b[i]; break;       // a = c1 a + b
+      case V_ADD2S2: a[i] = a[i] + s1 * b[i]; break;       // a = a + c1 b
+      case V_COL2: a[i] = a[i] * b[i]; break;
+      case V_COL3: a[i] = b[i] * c[i]; break;
+      case V_SUB2: a[i] = a[i] - b[i]; break;
+      case V_CMULT2: a[i] = b[i] * s1; break;
+      case V_INVCOL1: a[i] = T(1) / a[i]; break;
+      case V_ADDCOL3: a[i] = a[i] + b[i] * c[i]; break;
+      case V_ADDCOL4: a[i] = a[i] + b[i] * c[i] * d[i]; break;
+      case V_CMULT: a[i] = a[i] * s1; break;
+      case V_ADD2: a[i] = a[i] + b[i]; break;
+      case V_CFILL: a[i] = s1; break;
+      default: break;
+    }
+  }
+}
+
+template <typename T>
+int launch_vec(int op, T *a, const T *b, const T *c, const T *d, T s1,
+               int64_t n, hipStream_t s) {
+  if (n <= 0) return NM_OK;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(vec_kernel<T>, dim3((unsigned)nb), dim3(256), 0, s, op, a,
+                     b, c, d, s1, n);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+template int launch_vec<double>(int, double *, const double *, const double *,
+                                const double *, double, int64_t, hipStream_t);
+template int launch_vec<float>(int, float *, const float *, const float *,
+                               const float *, float, int64_t, hipStream_t);
+
+__global__ void d2f_kernel(const double *__restrict__ a, float *__restrict__ b,
+                           int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    b[i] = (float)a[i];
+}
+int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s) {
+  if (n <= 0) return NM_OK;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(d2f_kernel, dim3((unsigned)nb), dim3(256), 0, s, a, b, n);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Geometric factors (coef.f90:714-1030 and :1112-1156): dx/dr via D in the
+// three directions, Jacobian, cofactor metrics, G = (grad r_i . grad r_j) *
+// jacinv * w3, B = jac * w3.  One thread per point, element staged in LDS.
+// ---------------------------------------------------------------------------
+template <int LX>
+__global__ void __launch_bounds__(LX *LX *LX <= 1024 ? LX *LX *LX : 1024)
+    coef_kernel(const double *__restrict__ X, const double *__restrict__ Y,
+                const double *__restrict__ Z, const double *__restrict__ dx,
+                const double *__restrict__ w3, double *__restrict__ g11,
+                double *__restrict__ g22, double *__restrict__ g33,
+                double *__restrict__ g12, double *__restrict__ g13,
+                double *__restrict__ g23, double *__restrict__ jac,
+                double *__restrict__ B) {
+  constexpr int N3 = LX * LX * LX;
+  __shared__ double sx[3][N3];
+  __shared__ double sD[LX * LX];
+  const int64_t e = blockIdx.x;
+  const int64_t off = e * N3;
+  for (int p = threadIdx.x; p < N3; p += blockDim.x) {
+    sx[0][p] = X[off + p];
+    sx[1][p] = Y[off + p];
+    sx[2][p] = Z[off + p];
+  }
+  for (int p = threadIdx.x; p < LX * LX; p += blockDim.x) sD[p] = dx[p];
+  __syncthreads();
+  for (int p = threadIdx.x; p < N3; p += blockDim.x) {
+    const int i = p % LX, j = (p / LX) % LX, k = p / (LX * LX);
+    double dr[3], ds[3], dt[3];
+    for (int c = 0; c < 3; c++) {
+      double a = 0.0, b = 0.0, cc = 0.0;
+      for (int l = 0; l < LX; l++) {
+        a += sD[i + LX * l] * sx[c][l + LX * (j + LX * k)];
+        b += sx[c][i + LX * (l + LX * k)] * sD[j + LX * l];
+        cc += sx[c][i + LX * (j + LX * l)] * sD[k + LX * l];
+      }
+      dr[c] = a; ds[c] = b; dt[c] = cc;
+    }
+    const double xr = dr[0], yr = dr[1], zr = dr[2];
+    const double xs = ds[0], ys = ds[1], zs = ds[2];
+    const double xt = dt[0], yt = dt[1], zt = dt[2];
+    double jc = 0.0;
+    jc = jc + (xr * ys * zt);
+    jc = jc + (xt * yr * zs);
+    jc = jc + (xs * yt * zr);
+    jc = jc - (xr * yt * zs);
+    jc = jc - (xs * yr * zt);
+    jc = jc - (xt * ys * zr);
+    const double rx = ys * zt - yt * zs, ry = xt * zs - xs * zt, rz = xs * yt - xt * ys;
+    const double sxx = yt * zr - yr * zt, sy = xr * zt - xt * zr, sz = xt * yr - xr * yt;
+    const double tx = yr * zs - ys * zr, ty = xs * zr - xr * zs, tz = xr * ys - xs * yr;
+    const double jinv = 1.0 / jc;
+    const double wq = w3[p];
+    const int64_t q = off + p;
+    g11[q] = (rx * rx + ry * ry + rz * rz) * jinv * wq;
+    g22[q] = (sxx * sxx + sy * sy + sz * sz) * jinv * wq;
+    g33[q] = (tx * tx + ty * ty + tz * tz) * jinv * wq;
+    g12[q] = (rx * sxx + ry * sy + rz * sz) * jinv * wq;
+    g13[q] = (rx * tx + ry * ty + rz * tz) * jinv * wq;
+    g23[q] = (sxx * tx + sy * ty + sz * tz) * jinv * wq;
+    jac[q] = jc;
+    B[q] = jc * wq;
+  }
+}
+
+int launch_coef(int64_t nel, int lx, const double *x, const double *y,
+                const double *z, const double *dx, const double *w3,
+                double *g[6], double *jac, double *B, hipStream_t s) {
+  if (nel <= 0) return NM_OK;
+  const int threads = lx * lx * lx <= 1024 ? lx * lx * lx : 1024;
+#define NM_COEF_CASE(L)                                                          \
+  case L:                                                                        \
+    hipLaunchKernelGGL(coef_kernel<L>, dim3((unsigned)nel), dim3(threads), 0, s, \
+                       x, y, z, dx, w3, g[0], g[1], g[2], g[3], g[4], g[5], jac, B); \
+    break;
+  switch (lx) {
+    NM_COEF_CASE(2) NM_COEF_CASE(3) NM_COEF_CASE(4) NM_COEF_CASE(5)
+    NM_COEF_CASE(6) NM_COEF_CASE(7) NM_COEF_CASE(8) NM_COEF_CASE(9)
+    NM_COEF_CASE(10) NM_COEF_CASE(11) NM_COEF_CASE(12)
+    default:
+      set_error("coef: lx=%d unsupported (2..12)", lx);
+      return NM_ERR_UNSUPPORTED;
+  }
+#undef NM_COEF_CASE
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi diagonal, CPU formula (pc_jacobi.f90:168-264): sum_l of G11(l,j,k)
+// dxt(i,l)^2 + G22(i,l,k) dyt(j,l)^2 + G33(i,j,l) dzt(k,l)^2, plus the
+// G12/G13/G23 corner corrections on deformed elements only.
+// ---------------------------------------------------------------------------
+__global__ void jacobi_diag_kernel(int64_t nel, int lx,
+                                   const double *__restrict__ dxt,
+                                   const double *__restrict__ G11,
+                                   const double *__restrict__ G22,
+                                   const double *__restrict__ G33,
+                                   const double *__restrict__ G12,
+                                   const double *__restrict__ G13,
+                                   const double *__restrict__ G23,
+                                   const int *__restrict__ dfrmd,
+                                   double *__restrict__ d) {
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * n3) return;
+  const int64_t e = idx / n3;
+  const int p = (int)(idx % n3);
+  const int i = p % lx, j = (p / lx) % lx, k = p / (lx * lx);
+  const int64_t o = e * n3;
+#define DT(a, b) dxt[(a) + lx * (b)]
+#define A(arr, a, b, c) arr[o + (a) + lx * ((b) + lx * (c))]
+  double acc = 0.0;
+  for (int l = 0; l < lx; l++) acc = acc + A(G11, l, j, k) * (DT(i, l) * DT(i, l));
+  for (int l = 0; l < lx; l++) acc = acc + A(G22, i, l, k) * (DT(j, l) * DT(j, l));
+  for (int l = 0; l < lx; l++) acc = acc + A(G33, i, j, l) * (DT(k, l) * DT(k, l));
+  const int L = lx - 1;
+  const bool ci = (i == 0 || i == L), cj = (j == 0 || j == L), ck = (k == 0 || k == L);
+  if (dfrmd[e] && ci && cj && ck) {
+    // the three correction loops of the reference visit each corner once
+    acc = acc + A(G12, i, j, k) * DT(i, i) * DT(j, j) + A(G13, i, j, k) * DT(i, i) * DT(k, k);
+    acc = acc + A(G12, i, j, k) * DT(j, j) * DT(i, i) + A(G23, i, j, k) * DT(j, j) * DT(k, k);
+    acc = acc + A(G13, i, j, k) * DT(k, k) * DT(i, i) + A(G23, i, j, k) * DT(k, k) * DT(j, j);
+  }
+  d[o + p] = acc;
+#undef DT
+#undef A
+}
+
+int launch_jacobi_diag(int64_t nel, int lx, const double *dxt,
+                       const double *const g[6], const int *dfrmd, double *d,
+                       hipStream_t s) {
+  const int64_t n = nel * lx * lx * lx;
+  if (n <= 0) return NM_OK;
+  hipLaunchKernelGGL(jacobi_diag_kernel, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, s, nel, lx, dxt, g[0], g[1], g[2], g[3],
+                     g[4], g[5], dfrmd, d);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+}  // namespace nm

@@ -1,0 +1,84 @@
+// Device-resident discretisation ("operator") shared by the solvers:
+// geometry, gs maps, mask, mass, multiplicity and the Jacobi diagonal of one
+// problem, in FP64 and (on demand) FP32.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "nm_common.h"
+#include "nm_kernels.h"
+
+namespace nm {
+
+template <typename T>
+struct OpFields {
+  T *dx = nullptr;        // D(i,l), lx*lx
+  T *g[6] = {nullptr};    // G11 G22 G33 G12 G13 G23
+  T *h1 = nullptr;        // pointwise h1 (nullptr: constant h1c)
+  T *h2b = nullptr;       // h2*B (nullptr: no h2 term)
+  T *mult = nullptr;
+  T *dinv = nullptr;      // Jacobi: 1/gs(h1 diag(K) + h2 B)
+};
+
+struct Op {
+  int64_t nel = 0, n = 0;
+  int lx = 0;
+  double h1c = 1.0;
+  bool ifh2 = false;
+  double volume = 0.0;
+  // gs (device layout)
+  int64_t nlocal = 0, nblks = 0, npairs = 0, pair0 = 0;
+  int32_t *blk_ptr = nullptr, *gidx = nullptr;
+  // Neko-layout copies (1-based) kept for the drop-in gather/scatter tests
+  std::vector<int32_t> h_dg, h_gd, h_blk_len, h_blk_off;
+  int64_t facet_offset = 0;
+  // mask
+  int64_t nmsk = 0;
+  int32_t *msk = nullptr;  // 0-based
+  int *dfrmd = nullptr;
+  double *dxt = nullptr;
+  double *B = nullptr, *jac = nullptr;
+  double *xyz[3] = {nullptr, nullptr, nullptr};
+  OpFields<double> f64;
+  OpFields<float> f32;
+  bool has32 = false;
+  int epb64 = 0, epb32 = 0;  // Ax elements per workgroup (0 = default)
+  double *red_part = nullptr;   // RED_BLOCKS * 4 doubles
+  double *red_out = nullptr;    // 4 doubles
+  std::vector<void *> owned;    // every device allocation of this op
+  int box[3] = {0, 0, 0}, per[3] = {0, 0, 0};
+
+  ~Op();
+  template <typename T>
+  T *alloc(int64_t count);
+  int ensure32(hipStream_t s);
+  template <typename T>
+  const OpFields<T> &fields() const;
+  // w = mask(gs(A_local u))
+  template <typename T>
+  int apply(T *w, const T *u, const int *skip, hipStream_t s, bool gs = true,
+            bool mask = true) const;
+  template <typename T>
+  int gs(T *u, const int *skip, hipStream_t s) const;
+};
+
+template <>
+inline const OpFields<double> &Op::fields<double>() const { return f64; }
+template <>
+inline const OpFields<float> &Op::fields<float>() const { return f32; }
+
+// Build the device layout of Neko's (1-based) local gs maps and finish the
+// derived fields (mult, volume, Jacobi) on the device.
+int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
+              const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
+              const int32_t *blk_off, hipStream_t s);
+int op_finish(Op *op, hipStream_t s);
+
+}  // namespace nm
+
+// Opaque handle of the C-ABI (include/neko_amd.h: nm_op_t).
+struct nm_op_s {
+  nm::Op op;
+};

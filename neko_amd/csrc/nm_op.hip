@@ -1,0 +1,499 @@
+// Operator construction (box meshes or Neko-supplied arrays) and the
+// C-ABI handle functions around it.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nm_common.h"
+#include "nm_kernels.h"
+#include "nm_op.h"
+
+extern "C" {
+int nm_gll(int lx, double *z, double *w, double *d, double *dt);
+int nm_box_mesh(int nelx, int nely, int nelz, double amp, uint64_t seed,
+                double *vxyz);
+int nm_dfrmd(int64_t nel, const double *vxyz, int *dfrmd);
+int nm_box_gs_sizes(int nelx, int nely, int nelz, int lx, int perx, int pery,
+                    int perz, int64_t *nlocal, int64_t *facet_offset);
+int nm_box_gs_map(int nelx, int nely, int nelz, int lx, int perx, int pery,
+                  int perz, int32_t *dg, int32_t *gd, int32_t *blk_len,
+                  int32_t *blk_off, int64_t *nlocal_out,
+                  int64_t *facet_offset_out, int64_t *nblks_out);
+int64_t nm_box_mask_size(int nelx, int nely, int nelz, int lx, int perx,
+                         int pery, int perz);
+int nm_box_mask(int nelx, int nely, int nelz, int lx, int perx, int pery,
+                int perz, int32_t *msk, int64_t *nmsk);
+}
+
+namespace nm {
+
+Op::~Op() {
+  for (void *p : owned) (void)hipFree(p);
+}
+
+template <typename T>
+T *Op::alloc(int64_t count) {
+  void *p = nullptr;
+  if (count <= 0) count = 1;
+  if (hipMalloc(&p, sizeof(T) * (size_t)count) != hipSuccess) return nullptr;
+  owned.push_back(p);
+  return (T *)p;
+}
+template double *Op::alloc<double>(int64_t);
+template float *Op::alloc<float>(int64_t);
+template int32_t *Op::alloc<int32_t>(int64_t);
+
+#define NM_ALLOC(var, T, cnt)                                        \
+  do {                                                               \
+    (var) = op->alloc<T>(cnt);                                       \
+    if (!(var)) {                                                    \
+      set_error("out of device memory (%lld x %zu B)",               \
+                (long long)(cnt), sizeof(T));                        \
+      return NM_ERR_NOMEM;                                           \
+    }                                                                \
+  } while (0)
+
+template <typename T>
+int Op::gs(T *u, const int *skip, hipStream_t s) const {
+  GsArgs<T> a{u, blk_ptr, gidx, nblks, pair0, npairs, skip};
+  return launch_gs<T>(a, s);
+}
+template int Op::gs<double>(double *, const int *, hipStream_t) const;
+template int Op::gs<float>(float *, const int *, hipStream_t) const;
+
+template <typename T>
+int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
+              bool do_mask) const {
+  const OpFields<T> &F = fields<T>();
+  AxArgs<T> a;
+  a.w = w;
+  a.u = u;
+  a.dx = F.dx;
+  a.h1 = F.h1;
+  a.h1c = (T)h1c;
+  for (int c = 0; c < 6; c++) a.g[c] = F.g[c];
+  a.h2b = ifh2 ? F.h2b : nullptr;
+  a.nel = nel;
+  a.lx = lx;
+  a.skip = skip;
+  int st = launch_ax<T>(a, std::is_same<T, double>::value ? epb64 : epb32, s);
+  if (st) return st;
+  if (do_gs) {
+    st = gs<T>(w, skip, s);
+    if (st) return st;
+  }
+  if (do_mask) st = launch_mask<T>(w, msk, nmsk, skip, s);
+  return st;
+}
+template int Op::apply<double>(double *, const double *, const int *,
+                               hipStream_t, bool, bool) const;
+template int Op::apply<float>(float *, const float *, const int *, hipStream_t,
+                              bool, bool) const;
+
+int Op::ensure32(hipStream_t s) {
+  Op *op = this;
+  if (has32) return NM_OK;
+  NM_ALLOC(f32.dx, float, lx * lx);
+  NM_HIP_TRY(launch_d2f(f64.dx, f32.dx, lx * lx, s) ? hipErrorUnknown : hipSuccess);
+  for (int c = 0; c < 6; c++) {
+    NM_ALLOC(f32.g[c], float, n);
+    if (launch_d2f(f64.g[c], f32.g[c], n, s)) return NM_ERR_HIP;
+  }
+  if (f64.h1) {
+    NM_ALLOC(f32.h1, float, n);
+    if (launch_d2f(f64.h1, f32.h1, n, s)) return NM_ERR_HIP;
+  }
+  if (f64.h2b) {
+    NM_ALLOC(f32.h2b, float, n);
+    if (launch_d2f(f64.h2b, f32.h2b, n, s)) return NM_ERR_HIP;
+  }
+  NM_ALLOC(f32.mult, float, n);
+  if (launch_d2f(f64.mult, f32.mult, n, s)) return NM_ERR_HIP;
+  NM_ALLOC(f32.dinv, float, n);
+  if (launch_d2f(f64.dinv, f32.dinv, n, s)) return NM_ERR_HIP;
+  has32 = true;
+  return NM_OK;
+}
+
+int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
+              const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
+              const int32_t *blk_off, hipStream_t s) {
+  // Device layout: blk_ptr (nblks+1), gidx = gd - 1 (0-based).  Entries
+  // from facet_offset on are (nlocal - facet_offset + 1) / 2 pairs.
+  op->nlocal = nlocal;
+  op->nblks = nblks;
+  op->facet_offset = facet_offset;
+  op->pair0 = facet_offset - 1;
+  op->npairs = (nlocal - (facet_offset - 1)) / 2;
+  if ((nlocal - (facet_offset - 1)) % 2 != 0) {
+    set_error("gs: odd number of facet entries");
+    return NM_ERR_ARG;
+  }
+  std::vector<int32_t> ptr(nblks + 1), gi(nlocal > 0 ? nlocal : 1);
+  for (int64_t b = 0; b < nblks; b++) ptr[b] = blk_off[b];
+  ptr[nblks] = nblks > 0 ? blk_off[nblks - 1] + blk_len[nblks - 1] : 0;
+  if (ptr[nblks] != facet_offset - 1) {
+    set_error("gs: blocks end at %d, facet offset %lld", ptr[nblks],
+              (long long)facet_offset);
+    return NM_ERR_ARG;
+  }
+  for (int64_t i = 0; i < nlocal; i++) gi[i] = gd[i] - 1;
+  NM_ALLOC(op->blk_ptr, int32_t, nblks + 1);
+  NM_ALLOC(op->gidx, int32_t, nlocal);
+  NM_HIP_TRY(hipMemcpyAsync(op->blk_ptr, ptr.data(), sizeof(int32_t) * (nblks + 1),
+                            hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipMemcpyAsync(op->gidx, gi.data(), sizeof(int32_t) * nlocal,
+                            hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+// mult = 1/gs(1), volume = sum B, dinv = 1/gs(h1 diag(K) + h2 B)
+// (coef.f90:403-419, :1142-1155; pc_jacobi.f90:93-165).
+int op_finish(Op *op, hipStream_t s) {
+  const int64_t n = op->n;
+  if (!op->f64.mult) NM_ALLOC(op->f64.mult, double, n);
+  int st = launch_vec<double>(V_CFILL, op->f64.mult, nullptr, nullptr, nullptr, 1.0, n, s);
+  if (st) return st;
+  if ((st = op->gs<double>(op->f64.mult, nullptr, s))) return st;
+  if ((st = launch_vec<double>(V_INVCOL1, op->f64.mult, nullptr, nullptr, nullptr, 0.0, n, s))) return st;
+  if (!op->red_part) NM_ALLOC(op->red_part, double, RED_BLOCKS * 4);
+  if (!op->red_out) NM_ALLOC(op->red_out, double, 4);
+  // volume = sum(B) via glsc3(B, 1, 1); dinv serves as the ones vector
+  // before it receives the Jacobi diagonal.
+  if (!op->f64.dinv) NM_ALLOC(op->f64.dinv, double, n);
+  double *dd = op->f64.dinv;
+  if ((st = launch_vec<double>(V_CFILL, dd, nullptr, nullptr, nullptr, 1.0, n, s))) return st;
+  if ((st = launch_glsc3_partial(op->B, dd, dd, n, op->red_part, s))) return st;
+  if ((st = launch_combine(op->red_part, 1, op->red_out, s))) return st;
+  NM_HIP_TRY(hipMemcpyAsync(&op->volume, op->red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  const double *g[6];
+  for (int c = 0; c < 6; c++) g[c] = op->f64.g[c];
+  if ((st = launch_jacobi_diag(op->nel, op->lx, op->dxt, g, op->dfrmd, dd, s))) return st;
+  if (op->f64.h1) {
+    if ((st = launch_vec<double>(V_COL2, dd, op->f64.h1, nullptr, nullptr, 0.0, n, s))) return st;
+  } else if (op->h1c != 1.0) {
+    if ((st = launch_vec<double>(V_CMULT, dd, nullptr, nullptr, nullptr, op->h1c, n, s))) return st;
+  }
+  if (op->ifh2 && op->f64.h2b) {
+    if ((st = launch_vec<double>(V_ADD2, dd, op->f64.h2b, nullptr, nullptr, 0.0, n, s))) return st;
+  }
+  if ((st = op->gs<double>(dd, nullptr, s))) return st;
+  if ((st = launch_vec<double>(V_INVCOL1, dd, nullptr, nullptr, nullptr, 0.0, n, s))) return st;
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+// Trilinear GLL coordinates from the 8 element vertices (dofmap_xyzlin).
+__global__ void xyz_kernel(int64_t nel, int lx, const double *__restrict__ zg,
+                           const double *__restrict__ vxyz,
+                           double *__restrict__ X, double *__restrict__ Y,
+                           double *__restrict__ Z) {
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * n3) return;
+  const int64_t e = idx / n3;
+  const int p = (int)(idx % n3);
+  const int i = p % lx, j = (p / lx) % lx, k = p / (lx * lx);
+  const double ri[2] = {(1.0 - zg[i]) / 2.0, (1.0 + zg[i]) / 2.0};
+  const double sj[2] = {(1.0 - zg[j]) / 2.0, (1.0 + zg[j]) / 2.0};
+  const double tk[2] = {(1.0 - zg[k]) / 2.0, (1.0 + zg[k]) / 2.0};
+  for (int c = 0; c < 3; c++) {
+    double acc = 0.0;
+    for (int cz = 0; cz < 2; cz++) {
+      double ay = 0.0;
+      for (int cy = 0; cy < 2; cy++) {
+        const double *v = vxyz + (e * 8 + 4 * cz + 2 * cy) * 3 + c;
+        const double ax = ri[0] * v[0] + ri[1] * v[3];
+        ay += sj[cy] * ax;
+      }
+      acc += tk[cz] * ay;
+    }
+    (c == 0 ? X : (c == 1 ? Y : Z))[idx] = acc;
+  }
+}
+
+}  // namespace nm
+
+using namespace nm;
+
+
+extern "C" {
+
+typedef struct nm_op_s nm_op_t;
+
+// Build a whole box problem: host mesh/gs maps/mask, device coordinates,
+// geometric factors, mass, multiplicity and Jacobi diagonal.
+// h1 constant; h2 constant (h2 != 0 turns the mass term on, ifh2).
+int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
+                     int perx, int pery, int perz, double amp, uint64_t seed,
+                     double h1, double h2, hipStream_t s) {
+  *out = nullptr;
+  if (lx < 2 || lx > 12) {
+    set_error("nm_op_create_box: lx=%d outside 2..12", lx);
+    return NM_ERR_ARG;
+  }
+  if ((perx && nelx < 3) || (pery && nely < 3) || (perz && nelz < 3)) {
+    set_error("nm_op_create_box: periodic direction needs >= 3 elements");
+    return NM_ERR_ARG;
+  }
+  nm_op_t *h = new nm_op_t;
+  Op *op = &h->op;
+  op->nel = (int64_t)nelx * nely * nelz;
+  op->lx = lx;
+  op->n = op->nel * lx * lx * lx;
+  op->h1c = h1;
+  op->ifh2 = h2 != 0.0;
+  op->box[0] = nelx; op->box[1] = nely; op->box[2] = nelz;
+  op->per[0] = perx; op->per[1] = pery; op->per[2] = perz;
+  const int64_t n = op->n, nel = op->nel;
+  int st = NM_OK;
+  auto fail = [&](int code) {
+    delete h;
+    return code;
+  };
+  // space
+  std::vector<double> z(lx), w(lx), d(lx * lx), dt(lx * lx), w3(lx * lx * lx);
+  nm_gll(lx, z.data(), w.data(), d.data(), dt.data());
+  for (int k = 0; k < lx; k++)
+    for (int j = 0; j < lx; j++)
+      for (int i = 0; i < lx; i++) w3[i + lx * (j + lx * k)] = w[i] * w[j] * w[k];
+  // mesh
+  std::vector<double> vxyz(nel * 24);
+  std::vector<int> dfrmd(nel);
+  nm_box_mesh(nelx, nely, nelz, amp, seed, vxyz.data());
+  nm_dfrmd(nel, vxyz.data(), dfrmd.data());
+  // device arrays
+  double *d_z, *d_v, *d_w3;
+  if (!(op->f64.dx = op->alloc<double>(lx * lx)) || !(op->dxt = op->alloc<double>(lx * lx)) ||
+      !(d_z = op->alloc<double>(lx)) || !(d_w3 = op->alloc<double>(lx * lx * lx)) ||
+      !(d_v = op->alloc<double>(nel * 24)) || !(op->dfrmd = op->alloc<int32_t>(nel)))
+    return fail(NM_ERR_NOMEM);
+  for (int c = 0; c < 3; c++)
+    if (!(op->xyz[c] = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+  for (int c = 0; c < 6; c++)
+    if (!(op->f64.g[c] = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+  if (!(op->B = op->alloc<double>(n)) || !(op->jac = op->alloc<double>(n)))
+    return fail(NM_ERR_NOMEM);
+  hipMemcpyAsync(op->f64.dx, d.data(), sizeof(double) * lx * lx, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(op->dxt, dt.data(), sizeof(double) * lx * lx, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(d_z, z.data(), sizeof(double) * lx, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(d_w3, w3.data(), sizeof(double) * lx * lx * lx, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(d_v, vxyz.data(), sizeof(double) * nel * 24, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(op->dfrmd, dfrmd.data(), sizeof(int) * nel, hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(xyz_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     nel, lx, d_z, d_v, op->xyz[0], op->xyz[1], op->xyz[2]);
+  if (hipGetLastError() != hipSuccess) return fail(NM_ERR_HIP);
+  if ((st = launch_coef(nel, lx, op->xyz[0], op->xyz[1], op->xyz[2], op->f64.dx, d_w3,
+                        op->f64.g, op->jac, op->B, s)))
+    return fail(st);
+  if (op->ifh2) {
+    if (!(op->f64.h2b = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+    launch_vec<double>(V_CMULT2, op->f64.h2b, op->B, nullptr, nullptr, h2, n, s);
+  }
+  // gs maps + mask on the host (Neko's layout), then device layout
+  int64_t nlocal, foff, nb;
+  if ((st = nm_box_gs_sizes(nelx, nely, nelz, lx, perx, pery, perz, &nlocal, &foff)))
+    return fail(st);
+  op->h_dg.resize(nlocal);
+  op->h_gd.resize(nlocal);
+  op->h_blk_len.resize(foff);
+  op->h_blk_off.resize(foff);
+  if ((st = nm_box_gs_map(nelx, nely, nelz, lx, perx, pery, perz, op->h_dg.data(),
+                          op->h_gd.data(), op->h_blk_len.data(), op->h_blk_off.data(),
+                          &nlocal, &foff, &nb)))
+    return fail(st);
+  op->h_blk_len.resize(nb);
+  op->h_blk_off.resize(nb);
+  if ((st = op_set_gs(op, nlocal, foff, nb, op->h_dg.data(), op->h_gd.data(),
+                      op->h_blk_len.data(), op->h_blk_off.data(), s)))
+    return fail(st);
+  const int64_t mcap = nm_box_mask_size(nelx, nely, nelz, lx, perx, pery, perz);
+  std::vector<int32_t> msk(mcap > 0 ? mcap : 1);
+  int64_t m = 0;
+  nm_box_mask(nelx, nely, nelz, lx, perx, pery, perz, msk.data(), &m);
+  for (int64_t i = 0; i < m; i++) msk[i] -= 1;
+  op->nmsk = m;
+  if (!(op->msk = op->alloc<int32_t>(m))) return fail(NM_ERR_NOMEM);
+  hipMemcpyAsync(op->msk, msk.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("nm_op_create_box: device setup failed");
+    return fail(NM_ERR_HIP);
+  }
+  if ((st = op_finish(op, s))) return fail(st);
+  *out = h;
+  return NM_OK;
+}
+
+// Build a problem from Neko's own host arrays (what a Fortran caller holds
+// after space_t / coef_t / gs_t / bc setup).  All pointers are host memory;
+// gs maps and mask are 1-based as in Neko.  h1 may be NULL (constant h1c);
+// h2 may be NULL when ifh2 == 0.  dfrmd may be NULL (all deformed).
+int nm_op_create(nm_op_t **out, int nel, int lx, const double *dx,
+                 const double *g11, const double *g22, const double *g33,
+                 const double *g12, const double *g13, const double *g23,
+                 const double *B, const double *h1, double h1c,
+                 const double *h2, int ifh2, const int *dfrmd, int nlocal,
+                 int facet_offset, int nblks, const int *dg, const int *gd,
+                 const int *blk_len, const int *blk_off, int nmsk,
+                 const int *msk, hipStream_t s) {
+  *out = nullptr;
+  if (nel <= 0 || lx < 2 || lx > 14) {
+    set_error("nm_op_create: bad nel/lx");
+    return NM_ERR_ARG;
+  }
+  nm_op_t *hnd = new nm_op_t;
+  Op *op = &hnd->op;
+  op->nel = nel;
+  op->lx = lx;
+  op->n = (int64_t)nel * lx * lx * lx;
+  op->h1c = h1c;
+  op->ifh2 = ifh2 != 0;
+  const int64_t n = op->n;
+  auto fail = [&](int code) {
+    delete hnd;
+    return code;
+  };
+  std::vector<double> dt(lx * lx);
+  for (int i = 0; i < lx; i++)
+    for (int j = 0; j < lx; j++) dt[j + lx * i] = dx[i + lx * j];
+  if (!(op->f64.dx = op->alloc<double>(lx * lx)) || !(op->dxt = op->alloc<double>(lx * lx)) ||
+      !(op->B = op->alloc<double>(n)) || !(op->dfrmd = op->alloc<int32_t>(nel)))
+    return fail(NM_ERR_NOMEM);
+  hipMemcpyAsync(op->f64.dx, dx, sizeof(double) * lx * lx, hipMemcpyHostToDevice, s);
+  hipMemcpyAsync(op->dxt, dt.data(), sizeof(double) * lx * lx, hipMemcpyHostToDevice, s);
+  const double *gs_[6] = {g11, g22, g33, g12, g13, g23};
+  for (int c = 0; c < 6; c++) {
+    if (!(op->f64.g[c] = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+    hipMemcpyAsync(op->f64.g[c], gs_[c], sizeof(double) * n, hipMemcpyHostToDevice, s);
+  }
+  hipMemcpyAsync(op->B, B, sizeof(double) * n, hipMemcpyHostToDevice, s);
+  if (h1) {
+    if (!(op->f64.h1 = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+    hipMemcpyAsync(op->f64.h1, h1, sizeof(double) * n, hipMemcpyHostToDevice, s);
+  }
+  if (op->ifh2) {
+    if (!h2) {
+      set_error("nm_op_create: ifh2 set without h2");
+      return fail(NM_ERR_ARG);
+    }
+    std::vector<double> hb(n);
+    for (int64_t i = 0; i < n; i++) hb[i] = h2[i] * B[i];
+    if (!(op->f64.h2b = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
+    hipMemcpyAsync(op->f64.h2b, hb.data(), sizeof(double) * n, hipMemcpyHostToDevice, s);
+    hipStreamSynchronize(s);
+  }
+  std::vector<int> df(nel, 1);
+  if (dfrmd)
+    for (int e = 0; e < nel; e++) df[e] = dfrmd[e];
+  hipMemcpyAsync(op->dfrmd, df.data(), sizeof(int) * nel, hipMemcpyHostToDevice, s);
+  op->h_dg.assign(dg, dg + nlocal);
+  op->h_gd.assign(gd, gd + nlocal);
+  op->h_blk_len.assign(blk_len, blk_len + nblks);
+  op->h_blk_off.assign(blk_off, blk_off + nblks);
+  int st = op_set_gs(op, nlocal, facet_offset, nblks, dg, gd, blk_len, blk_off, s);
+  if (st) return fail(st);
+  std::vector<int32_t> m0(nmsk > 0 ? nmsk : 1);
+  for (int i = 0; i < nmsk; i++) m0[i] = msk[i] - 1;
+  op->nmsk = nmsk;
+  if (!(op->msk = op->alloc<int32_t>(nmsk))) return fail(NM_ERR_NOMEM);
+  hipMemcpyAsync(op->msk, m0.data(), sizeof(int32_t) * (nmsk > 0 ? nmsk : 1), hipMemcpyHostToDevice, s);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(NM_ERR_HIP);
+  if ((st = op_finish(op, s))) return fail(st);
+  *out = hnd;
+  return NM_OK;
+}
+
+void nm_op_free(nm_op_t *h) { delete h; }
+
+typedef struct {
+  int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
+  int lx;
+  double volume;
+  double h1c;
+  int ifh2;
+} nm_op_info_t;
+
+int nm_op_info(const nm_op_t *h, nm_op_info_t *info) {
+  const Op &op = h->op;
+  info->nel = op.nel;
+  info->n = op.n;
+  info->nlocal = op.nlocal;
+  info->nblks = op.nblks;
+  info->npairs = op.npairs;
+  info->nmsk = op.nmsk;
+  info->facet_offset = op.facet_offset;
+  info->lx = op.lx;
+  info->volume = op.volume;
+  info->h1c = op.h1c;
+  info->ifh2 = op.ifh2;
+  return NM_OK;
+}
+
+// Device pointer of a named field (for zero-copy access by a caller that
+// owns no copy): 0..5 G11..G23, 6 B, 7 mult, 8 Jacobi dinv, 9..11 x/y/z,
+// 12 h2*B, 13 jac.
+const void *nm_op_field(const nm_op_t *h, int which) {
+  const Op &op = h->op;
+  switch (which) {
+    case 0: case 1: case 2: case 3: case 4: case 5: return op.f64.g[which];
+    case 6: return op.B;
+    case 7: return op.f64.mult;
+    case 8: return op.f64.dinv;
+    case 9: case 10: case 11: return op.xyz[which - 9];
+    case 12: return op.f64.h2b;
+    case 13: return op.jac;
+    default: return nullptr;
+  }
+}
+
+// Neko-layout gs maps / mask held by the operator (host copies).
+int nm_op_gs_maps(const nm_op_t *h, int32_t *dg, int32_t *gd, int32_t *blk_len,
+                  int32_t *blk_off) {
+  const Op &op = h->op;
+  std::memcpy(dg, op.h_dg.data(), sizeof(int32_t) * op.h_dg.size());
+  std::memcpy(gd, op.h_gd.data(), sizeof(int32_t) * op.h_gd.size());
+  std::memcpy(blk_len, op.h_blk_len.data(), sizeof(int32_t) * op.h_blk_len.size());
+  std::memcpy(blk_off, op.h_blk_off.data(), sizeof(int32_t) * op.h_blk_off.size());
+  return NM_OK;
+}
+
+int nm_op_set_epb(nm_op_t *h, int epb64, int epb32) {
+  h->op.epb64 = epb64;
+  h->op.epb32 = epb32;
+  return NM_OK;
+}
+
+// w = A_local u (no gs, no mask), FP64, device pointers.
+int nm_op_ax_local(const nm_op_t *h, double *w, const double *u, hipStream_t s) {
+  return h->op.apply<double>(w, u, nullptr, s, false, false);
+}
+// w = mask(gs(A_local u)) (the operator every Krylov iteration applies).
+int nm_op_ax(const nm_op_t *h, double *w, const double *u, hipStream_t s) {
+  return h->op.apply<double>(w, u, nullptr, s, true, true);
+}
+// w = gs(A_local u) (Ax+gs, no mask) FP64
+int nm_op_ax_gs(const nm_op_t *h, double *w, const double *u, hipStream_t s) {
+  return h->op.apply<double>(w, u, nullptr, s, true, false);
+}
+int nm_op_ax_gs32(nm_op_t *h, float *w, const float *u, hipStream_t s) {
+  int st = h->op.ensure32(s);
+  if (st) return st;
+  return h->op.apply<float>(w, u, nullptr, s, true, false);
+}
+int nm_op_ax_local32(nm_op_t *h, float *w, const float *u, hipStream_t s) {
+  int st = h->op.ensure32(s);
+  if (st) return st;
+  return h->op.apply<float>(w, u, nullptr, s, false, false);
+}
+int nm_op_gs(const nm_op_t *h, double *u, hipStream_t s) {
+  return h->op.gs<double>(u, nullptr, s);
+}
+int nm_op_mask(const nm_op_t *h, double *u, hipStream_t s) {
+  return launch_mask<double>(u, h->op.msk, h->op.nmsk, nullptr, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+"""Host-side mirror of Neko's solve-path interfaces on top of the C ABI.
+
+Names follow Neko: a `Problem` bundles what Neko keeps in space_t, mesh_t,
+dofmap_t, coef_t, gs_t and the zero-Dirichlet bc list; `AxHelm.compute`
+mirrors ax_t%compute (src/math/ax.f90:70-84); `Problem.gs_op` mirrors
+gs_t%op(u, n, GS_OP_ADD) (src/gs/gather_scatter.f90:1613); `CG.solve`
+mirrors ksp_t%solve returning a ksp_monitor_t (src/krylov/krylov.f90:56-66).
+
+Device memory is torch tensors on the current HIP device; every compute
+call goes to libneko_amd.so.  There is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from ._lib import Monitor, OpInfo, check, lib
+
+GS_OP_ADD, GS_OP_MUL, GS_OP_MIN, GS_OP_MAX = 1, 2, 3, 4  # gs_ops.f90:35-36
+
+FIELD_G11, FIELD_G22, FIELD_G33, FIELD_G12, FIELD_G13, FIELD_G23 = range(6)
+FIELD_B, FIELD_MULT, FIELD_DINV, FIELD_X, FIELD_Y, FIELD_Z = range(6, 12)
+FIELD_H2B, FIELD_JAC = 12, 13
+
+
+def _ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+@dataclass
+class KspMonitor:
+    """ksp_monitor_t (krylov.f90:56-66) plus the refinement sweep count."""
+    iter: int
+    res_start: float
+    res_final: float
+    converged: bool
+    outer: int = 0
+
+
+class Space:
+    """GLL space (space_t): nodes, weights, D and D^T (column-major)."""
+
+    def __init__(self, lx: int):
+        self.lx = lx
+        self.zg = np.zeros(lx)
+        self.wx = np.zeros(lx)
+        self.dx = np.zeros(lx * lx)
+        self.dxt = np.zeros(lx * lx)
+        check(lib().nm_gll(lx, self.zg.ctypes.data, self.wx.ctypes.data,
+                           self.dx.ctypes.data, self.dxt.ctypes.data), "nm_gll")
+        w = self.wx
+        self.w3 = np.einsum("k,j,i->kji", w, w, w).reshape(-1).copy()
+
+
+def box_mesh(nelx, nely, nelz, amp=0.0, seed=0):
+    v = np.zeros(nelx * nely * nelz * 24)
+    check(lib().nm_box_mesh(nelx, nely, nelz, float(amp), int(seed),
+                            v.ctypes.data), "nm_box_mesh")
+    return v
+
+
+def box_gs_map(nelx, nely, nelz, lx, periodic=(0, 0, 0)):
+    """Neko's local gs maps of a box (1-based numpy arrays)."""
+    L = lib()
+    nl, fo, nb = C.c_int64(), C.c_int64(), C.c_int64()
+    p = [int(bool(q)) for q in periodic]
+    check(L.nm_box_gs_sizes(nelx, nely, nelz, lx, *p, C.byref(nl), C.byref(fo)),
+          "nm_box_gs_sizes")
+    dg = np.zeros(nl.value, dtype=np.int32)
+    gd = np.zeros(nl.value, dtype=np.int32)
+    bl = np.zeros(max(fo.value, 1), dtype=np.int32)
+    bo = np.zeros(max(fo.value, 1), dtype=np.int32)
+    check(L.nm_box_gs_map(nelx, nely, nelz, lx, *p, dg.ctypes.data,
+                          gd.ctypes.data, bl.ctypes.data, bo.ctypes.data,
+                          C.byref(nl), C.byref(fo), C.byref(nb)), "nm_box_gs_map")
+    return dict(dg=dg, gd=gd, blk_len=bl[: nb.value].copy(),
+                blk_off=bo[: nb.value].copy(), nlocal=nl.value,
+                facet_offset=fo.value, nblks=nb.value)
+
+
+def box_mask(nelx, nely, nelz, lx, periodic=(0, 0, 0)):
+    L = lib()
+    p = [int(bool(q)) for q in periodic]
+    m = L.nm_box_mask_size(nelx, nely, nelz, lx, *p)
+    msk = np.zeros(max(m, 1), dtype=np.int32)
+    mm = C.c_int64()
+    check(L.nm_box_mask(nelx, nely, nelz, lx, *p, msk.ctypes.data, C.byref(mm)),
+          "nm_box_mask")
+    return msk[: mm.value].copy()
+
+
+class Problem:
+    """One discretised problem resident on the device (geometry, gs maps,
+    mask, mass, multiplicity, Jacobi diagonal)."""
+
+    def __init__(self, handle: int, keep=None):
+        self.h = C.c_void_p(handle)
+        self._keep = keep
+        info = OpInfo()
+        check(lib().nm_op_info(self.h, C.byref(info)), "nm_op_info")
+        self.info = info
+        self.n = info.n
+        self.lx = info.lx
+        self.nel = info.nel
+        self.volume = info.volume
+
+    # ---- construction ---------------------------------------------------
+    @classmethod
+    def box(cls, nelx, nely, nelz, lx, periodic=(0, 0, 0), amp=0.0, seed=0,
+            h1=1.0, h2=0.0):
+        h = C.c_void_p()
+        p = [int(bool(q)) for q in periodic]
+        check(lib().nm_op_create_box(C.byref(h), nelx, nely, nelz, lx, *p,
+                                     float(amp), int(seed), float(h1),
+                                     float(h2), _stream()), "nm_op_create_box")
+        pr = cls(h.value)
+        pr.box_dims = (nelx, nely, nelz)
+        pr.periodic = tuple(p)
+        return pr
+
+    @classmethod
+    def from_arrays(cls, nel, lx, dx, g, B, mult_unused, gs, msk, h1=None,
+                    h1c=1.0, h2=None, dfrmd=None):
+        """From Neko-style host arrays (numpy): g = [G11,G22,G33,G12,G13,G23],
+        gs = dict(dg, gd, blk_len, blk_off, nlocal, facet_offset, nblks)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in [dx, *g, B]]
+        h1a = None if h1 is None else np.ascontiguousarray(h1, dtype=np.float64)
+        h2a = None if h2 is None else np.ascontiguousarray(h2, dtype=np.float64)
+        dfa = None if dfrmd is None else np.ascontiguousarray(dfrmd, dtype=np.int32)
+        ints = [np.ascontiguousarray(gs[k], dtype=np.int32)
+                for k in ("dg", "gd", "blk_len", "blk_off")]
+        m = np.ascontiguousarray(msk, dtype=np.int32)
+        h = C.c_void_p()
+        vp = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        check(lib().nm_op_create(
+            C.byref(h), int(nel), int(lx), *[a.ctypes.data for a in arrs],
+            vp(h1a), float(h1c), vp(h2a), int(h2a is not None), vp(dfa),
+            int(gs["nlocal"]), int(gs["facet_offset"]), int(gs["nblks"]),
+            *[a.ctypes.data for a in ints], int(len(m)), m.ctypes.data,
+            _stream()), "nm_op_create")
+        return cls(h.value, keep=(arrs, h1a, h2a, dfa, ints, m))
+
+    def free(self):
+        if self.h:
+            lib().nm_op_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ---- data -----------------------------------------------------------
+    def field(self, which: int) -> torch.Tensor:
+        """Copy of a device field as a new torch tensor."""
+        ptr = lib().nm_op_field(self.h, which)
+        if not ptr:
+            raise ValueError(f"field {which} not present")
+        out = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        _memcpy_d2d(out.data_ptr(), ptr, self.n * 8)
+        return out
+
+    def gs_maps(self):
+        info = self.info
+        dg = np.zeros(info.nlocal, dtype=np.int32)
+        gd = np.zeros(info.nlocal, dtype=np.int32)
+        bl = np.zeros(max(info.nblks, 1), dtype=np.int32)
+        bo = np.zeros(max(info.nblks, 1), dtype=np.int32)
+        check(lib().nm_op_gs_maps(self.h, dg.ctypes.data, gd.ctypes.data,
+                                  bl.ctypes.data, bo.ctypes.data), "gs_maps")
+        return dict(dg=dg, gd=gd, blk_len=bl[: info.nblks],
+                    blk_off=bo[: info.nblks], nlocal=info.nlocal,
+                    facet_offset=info.facet_offset, nblks=info.nblks)
+
+    def set_epb(self, epb64=0, epb32=0):
+        check(lib().nm_op_set_epb(self.h, int(epb64), int(epb32)), "set_epb")
+
+    # ---- operations (Neko names) ----------------------------------------
+    def ax_local(self, w: torch.Tensor, u: torch.Tensor):
+        """ax_t%compute: w = A_local u (no assembly)."""
+        if u.dtype == torch.float32:
+            check(lib().nm_op_ax_local32(self.h, _ptr(w), _ptr(u), _stream()),
+                  "ax_local32")
+        else:
+            check(lib().nm_op_ax_local(self.h, _ptr(w), _ptr(u), _stream()),
+                  "ax_local")
+        return w
+
+    def ax_gs(self, w, u):
+        """w = gs(A_local u): the Ax+gs unit of work."""
+        if u.dtype == torch.float32:
+            check(lib().nm_op_ax_gs32(self.h, _ptr(w), _ptr(u), _stream()), "ax_gs32")
+        else:
+            check(lib().nm_op_ax_gs(self.h, _ptr(w), _ptr(u), _stream()), "ax_gs")
+        return w
+
+    def ax_full(self, w, u):
+        """Ax%compute + gs_h%op(ADD) + blst%apply (cg.f90:207-209)."""
+        check(lib().nm_op_ax(self.h, _ptr(w), _ptr(u), _stream()), "ax")
+        return w
+
+    def gs_op(self, u, op=GS_OP_ADD):
+        if op != GS_OP_ADD:
+            raise NotImplementedError("handle gs is ADD only; use the drop-in "
+                                      "hip_gather_kernel for MUL/MIN/MAX")
+        check(lib().nm_op_gs(self.h, _ptr(u), _stream()), "gs")
+        return u
+
+    def bc_apply(self, u):
+        check(lib().nm_op_mask(self.h, _ptr(u), _stream()), "mask")
+        return u
+
+    def rhs_normal(self, seed: int) -> torch.Tensor:
+        """RHS i.i.d. N(0,1) per local dof (seeded), made consistent by one
+        gs sum, then masked."""
+        f = np.zeros(self.n)
+        check(lib().nm_rng_normal_fill(int(seed), self.n, f.ctypes.data), "rng")
+        t = torch.from_numpy(f).to("cuda")
+        self.gs_op(t)
+        self.bc_apply(t)
+        return t
+
+    def rhs_sine(self) -> torch.Tensor:
+        """b = mask(gs(B * 3 pi^2 sin(pi x) sin(pi y) sin(pi z)))."""
+        x, y, z = (self.field(c) for c in (FIELD_X, FIELD_Y, FIELD_Z))
+        pi = np.pi
+        f = self.field(FIELD_B) * (3 * pi * pi) * torch.sin(pi * x) * \
+            torch.sin(pi * y) * torch.sin(pi * z)
+        f = f.contiguous()
+        self.gs_op(f)
+        self.bc_apply(f)
+        return f
+
+
+class CG:
+    """Jacobi-preconditioned CG (ksp_t 'cg' with pc_t 'jacobi').
+    precision='fp64' or 'mixed' (FP32 inner CG in FP64 refinement)."""
+
+    def __init__(self, problem: Problem, precision: str = "fp64"):
+        self.problem = problem
+        h = C.c_void_p()
+        check(lib().nm_cg_create(C.byref(h), problem.h,
+                                 0 if precision == "fp64" else 1), "nm_cg_create")
+        self.h = h
+        self.precision = precision
+
+    def solve(self, x: torch.Tensor, f: torch.Tensor, abs_tol: float,
+              max_iter: int = 100000, inner_rel: float = 0.0) -> KspMonitor:
+        mon = Monitor()
+        check(lib().nm_cg_solve(self.h, _ptr(x), _ptr(f), float(abs_tol),
+                                int(max_iter), float(inner_rel), C.byref(mon),
+                                _stream()), "nm_cg_solve")
+        return KspMonitor(mon.iter, mon.res_start, mon.res_final,
+                          bool(mon.converged), mon.outer)
+
+    def free(self):
+        if self.h:
+            lib().nm_cg_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _memcpy_d2d(dst: int, src: int, nbytes: int):
+    check(lib().nm_memcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3,
+                          _stream()), "nm_memcpy")

@@ -1,0 +1,1313 @@
+/*
+ * neko_oracle.c -- CPU FP64 restatement of Neko's linear-solve hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the Neko source tree).  Compile with -ffp-contract=off so no FMA
+ * contraction changes the rounding of the Fortran expressions restated here.
+ *
+ * Pinning status of each function (how it is checked independently of this
+ * file) is listed in the header of tests/test_oracle.py.
+ */
+#include "neko_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX4(i, j, k, e, lx) \
+  ((int64_t)(i) + (int64_t)(lx) * ((j) + (int64_t)(lx) * ((k) + (int64_t)(lx) * (e))))
+
+/* ===================================================================== */
+/* Synthetic input generator.  Not part of Neko: this is the seeded input */
+/* specification shared with the product (neko_amd/csrc/nm_setup.cpp).   */
+/* ===================================================================== */
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t or_rng_u64(uint64_t seed, uint64_t idx) {
+  uint64_t k = mix64(seed + 0x9e3779b97f4a7c15ULL);
+  return mix64(k ^ (idx * 0xd1b54a32d192ed03ULL + 0x9e3779b97f4a7c15ULL));
+}
+
+double or_rng_uniform(uint64_t seed, uint64_t idx) {
+  return (double)(or_rng_u64(seed, idx) >> 11) * 0x1.0p-53;
+}
+
+double or_rng_normal(uint64_t seed, uint64_t idx) {
+  double u1 = 1.0 - or_rng_uniform(seed, 2 * idx);   /* (0,1] */
+  double u2 = or_rng_uniform(seed, 2 * idx + 1);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+/* ===================================================================== */
+/* a1: GLL points, weights and derivative matrix                         */
+/* ===================================================================== */
+static double gammaf(double x) {
+  /* speclib.f90 GAMMAF (tabulated special values only) */
+  const double pi = 4.0 * atan(1.0);
+  double g = 1.0;
+  const double eps = 2.220446049250313e-16;
+  if (fabs(x - (-0.5)) < eps) g = -2.0 * sqrt(pi);
+  if (fabs(x - 0.5) < eps) g = sqrt(pi);
+  if (fabs(x - 1.0) < eps) g = 1.0;
+  if (fabs(x - 2.0) < eps) g = 1.0;
+  if (fabs(x - 1.5) < eps) g = sqrt(pi) / 2.0;
+  if (fabs(x - 2.5) < eps) g = 1.5 * sqrt(pi) / 2.0;
+  if (fabs(x - 3.5) < eps) g = 0.5 * (2.5 * (1.5 * sqrt(pi)));
+  if (fabs(x - 3.0) < eps) g = 2.0;
+  if (fabs(x - 4.0) < eps) g = 6.0;
+  if (fabs(x - 5.0) < eps) g = 24.0;
+  if (fabs(x - 6.0) < eps) g = 120.0;
+  return g;
+}
+
+static double pnormj(int n, double alpha, double beta) {
+  /* speclib.f90 PNORMJ */
+  double dn = (double)n, cnst = alpha + beta + 1.0, prod;
+  if (n <= 1) {
+    prod = gammaf(dn + alpha) * gammaf(dn + beta);
+    prod = prod / (gammaf(dn) * gammaf(dn + alpha + beta));
+    return prod * pow(2.0, cnst) / (2.0 * dn + cnst);
+  }
+  prod = gammaf(alpha + 1.0) * gammaf(beta + 1.0);
+  prod = prod / (2.0 * (1.0 + cnst) * gammaf(cnst + 1.0));
+  prod = prod * (1.0 + alpha) * (2.0 + alpha);
+  prod = prod * (1.0 + beta) * (2.0 + beta);
+  for (int i = 3; i <= n; i++) {
+    double di = (double)i;
+    double frac = (di + alpha) * (di + beta) / (di * (di + alpha + beta));
+    prod = prod * frac;
+  }
+  return prod * pow(2.0, cnst) / (2.0 * dn + cnst);
+}
+
+static void jacobf(double *poly, double *pder, double *polym1, double *pderm1,
+                   double *polym2, double *pderm2, int n, double alp,
+                   double bet, double x) {
+  /* speclib.f90 JACOBF (three-term recurrence) */
+  double apb = alp + bet, polyl, pderl, polyn, pdern, psave = 0, pdsave = 0;
+  *poly = 1.0;
+  *pder = 0.0;
+  if (n == 0) return;
+  polyl = *poly;
+  pderl = *pder;
+  *poly = (alp - bet + (apb + 2.0) * x) / 2.0;
+  *pder = (apb + 2.0) / 2.0;
+  if (n == 1) return;
+  for (int k = 2; k <= n; k++) {
+    double dk = (double)k;
+    double a1 = 2.0 * dk * (dk + apb) * (2.0 * dk + apb - 2.0);
+    double a2 = (2.0 * dk + apb - 1.0) * (alp * alp - bet * bet);
+    double b3 = (2.0 * dk + apb - 2.0);
+    double a3 = b3 * (b3 + 1.0) * (b3 + 2.0);
+    double a4 = 2.0 * (dk + alp - 1.0) * (dk + bet - 1.0) * (2.0 * dk + apb);
+    polyn = ((a2 + a3 * x) * (*poly) - a4 * polyl) / a1;
+    pdern = ((a2 + a3 * x) * (*pder) - a4 * pderl + a3 * (*poly)) / a1;
+    psave = polyl;
+    pdsave = pderl;
+    polyl = *poly;
+    *poly = polyn;
+    pderl = *pder;
+    *pder = pdern;
+  }
+  *polym1 = polyl;
+  *pderm1 = pderl;
+  *polym2 = psave;
+  *pderm2 = pdsave;
+}
+
+static void jacg(double *xjac, int np, double alpha, double beta) {
+  /* speclib.f90 JACG: Newton with deflation, KSTOP=10, EPS=1e-12 */
+  const int kstop = 10;
+  const double eps = 1.0e-12;
+  const double pi = 4.0 * atan(1.0);
+  int n = np - 1;
+  double dth = pi / (2.0 * (double)n + 2.0);
+  double x, xlast = 0.0, p, pd, pm1, pdm1, pm2, pdm2;
+  for (int j = 1; j <= np; j++) {
+    if (j == 1) {
+      x = cos((2.0 * ((double)j - 1.0) + 1.0) * dth);
+    } else {
+      double x1 = cos((2.0 * ((double)j - 1.0) + 1.0) * dth);
+      x = (x1 + xlast) / 2.0;
+    }
+    for (int k = 1; k <= kstop; k++) {
+      jacobf(&p, &pd, &pm1, &pdm1, &pm2, &pdm2, np, alpha, beta, x);
+      double recsum = 0.0;
+      for (int i = 1; i <= j - 1; i++) recsum = recsum + 1.0 / (x - xjac[np - i]);
+      double delx = -p / (pd - recsum * p);
+      x = x + delx;
+      if (fabs(delx) < eps) break;
+    }
+    xjac[np - j] = x;
+    xlast = x;
+  }
+  for (int i = 1; i <= np; i++) {
+    double xmin = 2.0;
+    int jmin = i;
+    for (int j = i; j <= np; j++) {
+      if (xjac[j - 1] < xmin) {
+        xmin = xjac[j - 1];
+        jmin = j;
+      }
+    }
+    if (jmin != i) {
+      double sw = xjac[i - 1];
+      xjac[i - 1] = xjac[jmin - 1];
+      xjac[jmin - 1] = sw;
+    }
+  }
+}
+
+static void zwgjd(double *z, double *w, int np, double alpha, double beta) {
+  /* speclib.f90 ZWGJD */
+  double p, pd, pm1 = 0, pdm1 = 0, pm2, pdm2;
+  if (np == 1) {
+    double apb = alpha + beta;
+    z[0] = (beta - alpha) / (apb + 2.0);
+    w[0] = gammaf(alpha + 1.0) * gammaf(beta + 1.0) / gammaf(apb + 2.0) *
+           pow(2.0, apb + 1.0);
+    return;
+  }
+  jacg(z, np, alpha, beta);
+  int n = np - 1, np1 = n + 1, np2 = n + 2;
+  double dnp1 = (double)np1, dnp2 = (double)np2;
+  double fac1 = dnp1 + alpha + beta + 1.0;
+  double fac2 = fac1 + dnp1;
+  double fac3 = fac2 + 1.0;
+  double fnorm = pnormj(np1, alpha, beta);
+  double rcoef = (fnorm * fac2 * fac3) / (2.0 * fac1 * dnp2);
+  for (int i = 0; i < np; i++) {
+    jacobf(&p, &pd, &pm1, &pdm1, &pm2, &pdm2, np2, alpha, beta, z[i]);
+    w[i] = -rcoef / (p * pdm1);
+  }
+}
+
+static double endw(int n, double alpha, double beta, int second) {
+  /* speclib.f90 ENDW1 (second=0) / ENDW2 (second=1) */
+  if (n == 0) return 0.0;
+  double apb = alpha + beta, f1, f2, f3 = 0, fint1, fint2;
+  if (!second) {
+    f1 = gammaf(alpha + 2.0) * gammaf(beta + 1.0) / gammaf(apb + 3.0);
+  } else {
+    f1 = gammaf(alpha + 1.0) * gammaf(beta + 2.0) / gammaf(apb + 3.0);
+  }
+  f1 = f1 * (apb + 2.0) * pow(2.0, apb + 2.0) / 2.0;
+  if (n == 1) return f1;
+  if (!second) {
+    fint1 = gammaf(alpha + 2.0) * gammaf(beta + 1.0) / gammaf(apb + 3.0);
+  } else {
+    fint1 = gammaf(alpha + 1.0) * gammaf(beta + 2.0) / gammaf(apb + 3.0);
+  }
+  fint1 = fint1 * pow(2.0, apb + 2.0);
+  fint2 = gammaf(alpha + 2.0) * gammaf(beta + 2.0) / gammaf(apb + 4.0);
+  fint2 = fint2 * pow(2.0, apb + 3.0);
+  if (!second) {
+    f2 = (-2.0 * (beta + 2.0) * fint1 + (apb + 4.0) * fint2) * (apb + 3.0) / 4.0;
+  } else {
+    f2 = (2.0 * (alpha + 2.0) * fint1 - (apb + 4.0) * fint2) * (apb + 3.0) / 4.0;
+  }
+  if (n == 2) return f2;
+  for (int i = 3; i <= n; i++) {
+    double di = (double)(i - 1);
+    double abn = alpha + beta + di;
+    double abnn = abn + di;
+    double a1 = -(2.0 * (di + alpha) * (di + beta)) / (abn * abnn * (abnn + 1.0));
+    double a2 = (2.0 * (alpha - beta)) / (abnn * (abnn + 2.0));
+    double a3 = (2.0 * (abn + 1.0)) / ((abnn + 2.0) * (abnn + 1.0));
+    f3 = -(a2 * f2 + a1 * f1) / a3;
+    f1 = f2;
+    f2 = f3;
+  }
+  return f3;
+}
+
+void or_zwgll(double *z, double *w, int np) {
+  /* speclib.f90:179-186 ZWGLL -> ZWGLJ -> ZWGLJD (alpha = beta = 0) */
+  const double alpha = 0.0, beta = 0.0;
+  double p, pd, pm1, pdm1, pm2, pdm2;
+  int n = np - 1, nm1 = n - 1;
+  if (nm1 > 0) zwgjd(z + 1, w + 1, nm1, alpha + 1.0, beta + 1.0);
+  z[0] = -1.0;
+  z[np - 1] = 1.0;
+  for (int i = 1; i < np - 1; i++) w[i] = w[i] / (1.0 - z[i] * z[i]);
+  jacobf(&p, &pd, &pm1, &pdm1, &pm2, &pdm2, n, alpha, beta, z[0]);
+  w[0] = endw(n, alpha, beta, 0) / (2.0 * pd);
+  jacobf(&p, &pd, &pm1, &pdm1, &pm2, &pdm2, n, alpha, beta, z[np - 1]);
+  w[np - 1] = endw(n, alpha, beta, 1) / (2.0 * pd);
+}
+
+static double pnleg(double z, int n) {
+  /* speclib.f90 PNLEG */
+  double p1 = 1.0, p2, p3;
+  if (n == 0) return p1;
+  p2 = z;
+  p3 = p2;
+  for (int k = 1; k <= n - 1; k++) {
+    double fk = (double)k;
+    p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0);
+    p1 = p2;
+    p2 = p3;
+  }
+  return p3;
+}
+
+void or_dgll(double *d, double *dt, const double *z, int nz) {
+  /* speclib.f90:967-999 DGLL; D(i,j) stored at d[i + nz*j] */
+  int n = nz - 1;
+  if (nz == 1) {
+    d[0] = 0.0;
+    dt[0] = 0.0;
+    return;
+  }
+  double fn = (double)n;
+  double d0 = fn * (fn + 1.0) / 4.0;
+  for (int i = 0; i < nz; i++) {
+    for (int j = 0; j < nz; j++) {
+      double v;
+      if (i != j) {
+        v = pnleg(z[i], n) / (pnleg(z[j], n) * (z[i] - z[j]));
+      } else if (i == 0) {
+        v = -d0;
+      } else if (i == nz - 1) {
+        v = d0;
+      } else {
+        v = 0.0;
+      }
+      d[i + nz * j] = v;
+      dt[j + nz * i] = v;
+    }
+  }
+}
+
+/* ===================================================================== */
+/* a2: box mesh (contrib/genmeshbox/genmeshbox.f90:200-345 conventions)  */
+/* ===================================================================== */
+void or_box_mesh(int nelx, int nely, int nelz, int perx, int pery, int perz,
+                 double amp, uint64_t seed, double *vxyz, int64_t *pid) {
+  /* Elements x-fastest (genmeshbox.f90:200-216), vertices in Neko's
+   * lexicographic hex order p(ix,iy,iz), point id
+   * 1 + ix + iy*(nelx+1) + iz*(nelx+1)*(nely+1).  Periodic directions merge
+   * the ids of matching points to the minimum (mesh.f90:1786-1790).
+   * Synthetic perturbation: every vertex strictly inside the box moves by
+   * amp*h*(2U-1) per coordinate, U = uniform(seed, 3*lattice_id + c). */
+  int e = 0;
+  const int64_t nx1 = nelx + 1, ny1 = nely + 1;
+  for (int ez = 0; ez < nelz; ez++)
+    for (int ey = 0; ey < nely; ey++)
+      for (int ex = 0; ex < nelx; ex++, e++) {
+        for (int iz = 0; iz < 2; iz++)
+          for (int iy = 0; iy < 2; iy++)
+            for (int ix = 0; ix < 2; ix++) {
+              int v = ix + 2 * iy + 4 * iz;
+              int gx = ex + ix, gy = ey + iy, gz = ez + iz;
+              int64_t lat = gx + nx1 * (gy + ny1 * (int64_t)gz);
+              double c[3] = {(double)gx / nelx, (double)gy / nely,
+                             (double)gz / nelz};
+              int interior = gx > 0 && gx < nelx && gy > 0 && gy < nely &&
+                             gz > 0 && gz < nelz;
+              if (interior && amp != 0.0) {
+                const double h[3] = {1.0 / nelx, 1.0 / nely, 1.0 / nelz};
+                for (int cc = 0; cc < 3; cc++)
+                  c[cc] += amp * h[cc] *
+                           (2.0 * or_rng_uniform(seed, 3 * (uint64_t)lat + cc) - 1.0);
+              }
+              for (int cc = 0; cc < 3; cc++) vxyz[((int64_t)e * 8 + v) * 3 + cc] = c[cc];
+              int px = gx, py = gy, pz = gz;
+              if (perx && px == nelx) px = 0;
+              if (pery && py == nely) py = 0;
+              if (perz && pz == nelz) pz = 0;
+              pid[(int64_t)e * 8 + v] = 1 + px + nx1 * (py + ny1 * (int64_t)pz);
+            }
+      }
+}
+
+/* Local vertex numbers (0-based, lexicographic) of the 6 faces in Neko's
+ * facet numbering; face_nodes table of mesh.f90:1756-1763. */
+static const int FACE_NODES[6][4] = {
+    {0, 4, 6, 2}, {1, 5, 7, 3}, {0, 1, 5, 4},
+    {2, 3, 7, 6}, {0, 1, 3, 2}, {4, 5, 7, 6}};
+
+/* ---- small open-addressing hash table on up to 4 int64 keys ---------- */
+typedef struct {
+  int64_t *keys; /* 4 per slot */
+  int64_t *vals;
+  unsigned char *used;
+  int64_t cap;
+} htab_t;
+
+static void ht_init(htab_t *t, int64_t n) {
+  int64_t cap = 16;
+  while (cap < 2 * n + 16) cap <<= 1;
+  t->cap = cap;
+  t->keys = (int64_t *)calloc((size_t)cap * 4, sizeof(int64_t));
+  t->vals = (int64_t *)calloc((size_t)cap, sizeof(int64_t));
+  t->used = (unsigned char *)calloc((size_t)cap, 1);
+}
+static void ht_free(htab_t *t) {
+  free(t->keys);
+  free(t->vals);
+  free(t->used);
+}
+static void ht_clear(htab_t *t) { memset(t->used, 0, (size_t)t->cap); }
+static uint64_t ht_hash(const int64_t *k) {
+  uint64_t h = 0x12345;
+  for (int i = 0; i < 4; i++) h = mix64(h ^ (uint64_t)k[i]) + 0x9e3779b97f4a7c15ULL;
+  return h;
+}
+/* returns pointer to value; *found set */
+static int64_t *ht_get(htab_t *t, const int64_t *k, int insert, int *found) {
+  uint64_t h = ht_hash(k) & (uint64_t)(t->cap - 1);
+  for (;;) {
+    if (!t->used[h]) {
+      *found = 0;
+      if (!insert) return NULL;
+      t->used[h] = 1;
+      memcpy(t->keys + 4 * h, k, 4 * sizeof(int64_t));
+      return t->vals + h;
+    }
+    if (memcmp(t->keys + 4 * h, k, 4 * sizeof(int64_t)) == 0) {
+      *found = 1;
+      return t->vals + h;
+    }
+    h = (h + 1) & (uint64_t)(t->cap - 1);
+  }
+}
+
+static void sort4(int64_t *a) {
+  for (int i = 1; i < 4; i++) {
+    int64_t v = a[i];
+    int j = i - 1;
+    while (j >= 0 && a[j] > v) {
+      a[j + 1] = a[j];
+      j--;
+    }
+    a[j + 1] = v;
+  }
+}
+
+void or_facet_neigh(int nel, const int64_t *pid, int *fneigh) {
+  /* mesh.f90:590-628 (single rank): a facet's neighbour is the other
+   * element holding the same facet (facet id = its 4 point ids); 0 on the
+   * boundary.  Elements are 1-based in fneigh, as in Neko. */
+  htab_t t;
+  ht_init(&t, (int64_t)nel * 6);
+  for (int e = 0; e < nel; e++)
+    for (int f = 0; f < 6; f++) {
+      int64_t k[4];
+      for (int i = 0; i < 4; i++) k[i] = pid[(int64_t)e * 8 + FACE_NODES[f][i]];
+      sort4(k);
+      int found;
+      int64_t *v = ht_get(&t, k, 1, &found);
+      if (!found) {
+        *v = (int64_t)(e + 1); /* first holder */
+        fneigh[e * 6 + f] = 0;
+      } else {
+        int other = (int)(*v);
+        fneigh[e * 6 + f] = other;
+        /* patch the first holder's facet with the same id */
+        for (int g = 0; g < 6; g++) {
+          int64_t k2[4];
+          for (int i = 0; i < 4; i++)
+            k2[i] = pid[(int64_t)(other - 1) * 8 + FACE_NODES[g][i]];
+          sort4(k2);
+          if (memcmp(k, k2, sizeof(k)) == 0) fneigh[(other - 1) * 6 + g] = e + 1;
+        }
+      }
+    }
+  ht_free(&t);
+}
+
+void or_dfrmd_el(int nel, const double *vxyz, int *dfrmd) {
+  /* mesh.f90:437-467 mesh_generate_flags (3-D branch), NEKO_EPS test */
+  const double eps = 2.220446049250313e-16;
+  for (int e = 0; e < nel; e++) {
+    const double *p = vxyz + (int64_t)e * 24;
+#define PT(n, c) p[((n) - 1) * 3 + (c)]
+    double u[3], v[3], w[3], t;
+    int d = 0;
+    for (int c = 0; c < 3; c++) {
+      u[c] = PT(2, c) - PT(1, c);
+      v[c] = PT(3, c) - PT(1, c);
+      w[c] = PT(5, c) - PT(1, c);
+    }
+    t = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    if (!(fabs(t - 0.0) < eps)) d = 1;
+    t = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    if (!(fabs(t - 0.0) < eps)) d = 1;
+    for (int c = 0; c < 3; c++) {
+      u[c] = PT(7, c) - PT(8, c);
+      v[c] = PT(6, c) - PT(8, c);
+      w[c] = PT(4, c) - PT(8, c);
+    }
+    t = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+    if (!(fabs(t - 0.0) < eps)) d = 1;
+    t = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    if (!(fabs(t - 0.0) < eps)) d = 1;
+#undef PT
+    dfrmd[e] = d;
+  }
+}
+
+/* The 12 edges as (start vertex, end vertex, direction), in the order
+ * Neko numbers them in dofmap.f90:305-620 (r-edges, s-edges, t-edges). */
+void or_dof_ids(int nel, int lx, const int64_t *pid, int64_t *dof) {
+  /* Global dof numbering (dofmap.f90:264-797).  Vertices take the point id;
+   * edge interiors are numbered along the edge from its lower-id end
+   * (dofmap.f90:316-330); face interiors from the face corner with the
+   * lowest id towards its lower-id neighbour corner (dofmap_facetidx,
+   * dofmap.f90:742-797).  Only which local dofs share an id matters to the
+   * gs maps; the id values are this restatement's own. */
+  const int L = lx - 2;
+  htab_t edges, faces;
+  ht_init(&edges, (int64_t)nel * 12);
+  ht_init(&faces, (int64_t)nel * 6);
+  int64_t nvert = 0;
+  for (int64_t i = 0; i < (int64_t)nel * 8; i++)
+    if (pid[i] > nvert) nvert = pid[i];
+  int64_t next_edge = 0, next_face = 0;
+  /* first pass: number edges and faces in element order */
+  int64_t *eidx = (int64_t *)malloc(sizeof(int64_t) * (size_t)nel * 12);
+  int64_t *fidx = (int64_t *)malloc(sizeof(int64_t) * (size_t)nel * 6);
+  static const int EDGES[12][2] = {
+      {0, 1}, {4, 5}, {2, 3}, {6, 7},  /* r-direction: (y,z)=(0,0),(0,1),(1,0),(1,1) */
+      {0, 2}, {4, 6}, {1, 3}, {5, 7},  /* s-direction */
+      {0, 4}, {1, 5}, {2, 6}, {3, 7}}; /* t-direction */
+  for (int e = 0; e < nel; e++) {
+    for (int q = 0; q < 12; q++) {
+      int64_t a = pid[(int64_t)e * 8 + EDGES[q][0]], b = pid[(int64_t)e * 8 + EDGES[q][1]];
+      int64_t k[4] = {a < b ? a : b, a < b ? b : a, 0, 0};
+      int found;
+      int64_t *v = ht_get(&edges, k, 1, &found);
+      if (!found) *v = next_edge++;
+      eidx[e * 12 + q] = *v;
+    }
+    for (int f = 0; f < 6; f++) {
+      int64_t k[4];
+      for (int i = 0; i < 4; i++) k[i] = pid[(int64_t)e * 8 + FACE_NODES[f][i]];
+      sort4(k);
+      int found;
+      int64_t *v = ht_get(&faces, k, 1, &found);
+      if (!found) *v = next_face++;
+      fidx[e * 6 + f] = *v;
+    }
+  }
+  const int64_t edge_base = nvert + 1;
+  const int64_t face_base = edge_base + next_edge * L;
+  const int64_t int_base = face_base + next_face * (int64_t)L * L;
+  for (int e = 0; e < nel; e++) {
+    const int64_t *p = pid + (int64_t)e * 8;
+    for (int k = 0; k < lx; k++)
+      for (int j = 0; j < lx; j++)
+        for (int i = 0; i < lx; i++) {
+          int bi = (i == 0 || i == lx - 1), bj = (j == 0 || j == lx - 1),
+              bk = (k == 0 || k == lx - 1);
+          int nb = bi + bj + bk;
+          int64_t id;
+          if (nb == 3) {
+            int v = (i == lx - 1) + 2 * (j == lx - 1) + 4 * (k == lx - 1);
+            id = p[v];
+          } else if (nb == 2) {
+            int q, pos, va, vb;
+            if (!bi) { /* r-edge */
+              q = (j == lx - 1) * 2 + (k == lx - 1);
+              pos = i;
+            } else if (!bj) { /* s-edge */
+              q = 4 + (i == lx - 1) * 2 + (k == lx - 1);
+              pos = j;
+            } else { /* t-edge */
+              q = 8 + (i == lx - 1) + 2 * (j == lx - 1);
+              pos = k;
+            }
+            va = EDGES[q][0];
+            vb = EDGES[q][1];
+            int t = (p[va] < p[vb]) ? (pos - 1) : (lx - 2 - pos);
+            id = edge_base + eidx[e * 12 + q] * L + t;
+          } else if (nb == 1) {
+            int f, a, b;
+            if (bi) {
+              f = (i == lx - 1);
+              a = j; b = k; /* corners order FACE_NODES: (0,0),(0,b),(a,b),(a,0)?? */
+            } else if (bj) {
+              f = 2 + (j == lx - 1);
+              a = i; b = k;
+            } else {
+              f = 4 + (k == lx - 1);
+              a = i; b = j;
+            }
+            /* face-local corner ids at (a,b) in {0,max}^2 */
+            int64_t c00, c10, c01, c11;
+            {
+              int v00, v10, v01, v11;
+              if (f < 2) {
+                int xo = (f == 1);
+                v00 = xo + 0; v10 = xo + 2; v01 = xo + 4; v11 = xo + 6;
+              } else if (f < 4) {
+                int yo = (f == 3) * 2;
+                v00 = yo + 0; v10 = yo + 1; v01 = yo + 4; v11 = yo + 5;
+              } else {
+                int zo = (f == 5) * 4;
+                v00 = zo + 0; v10 = zo + 1; v01 = zo + 2; v11 = zo + 3;
+              }
+              c00 = p[v00]; c10 = p[v10]; c01 = p[v01]; c11 = p[v11];
+            }
+            /* coordinates relative to the lowest-id corner, first axis
+             * towards its lower-id neighbour */
+            int s = a - 1, tt = b - 1; /* 0..L-1 from (0,0) */
+            int ca, cb;                 /* position from chosen origin */
+            int64_t m = c00;
+            int oa = 0, ob = 0;
+            if (c10 < m) { m = c10; oa = 1; ob = 0; }
+            if (c01 < m) { m = c01; oa = 0; ob = 1; }
+            if (c11 < m) { m = c11; oa = 1; ob = 1; }
+            int da = oa ? (L - 1 - s) : s;   /* distance along a from origin */
+            int db = ob ? (L - 1 - tt) : tt; /* distance along b from origin */
+            /* neighbours of the origin: across a (flip a) and across b */
+            int64_t na = (oa ? (ob ? c01 : c00) : (ob ? c11 : c10));
+            int64_t nbv = (ob ? (oa ? c10 : c00) : (oa ? c11 : c01));
+            if (na < nbv) { ca = da; cb = db; }
+            else { ca = db; cb = da; }
+            id = face_base + fidx[e * 6 + f] * (int64_t)L * L + ca + (int64_t)L * cb;
+          } else {
+            id = int_base + IDX4(i, j, k, e, lx);
+          }
+          dof[IDX4(i, j, k, e, lx)] = id;
+        }
+  }
+  free(eidx);
+  free(fidx);
+  ht_free(&edges);
+  ht_free(&faces);
+}
+
+void or_xyz(int nel, int lx, const double *zg, const double *vxyz, double *x,
+            double *y, double *z) {
+  /* dofmap.f90:866-939 dofmap_xyzlin: trilinear map of the 8 vertices to
+   * the GLL points, tensr3 applied r then s then t. */
+  double *jx = (double *)malloc(sizeof(double) * lx * 2);
+  for (int i = 0; i < lx; i++) {
+    jx[i * 2 + 0] = (1.0 - zg[i]) / 2.0;
+    jx[i * 2 + 1] = (1.0 + zg[i]) / 2.0;
+  }
+  for (int e = 0; e < nel; e++) {
+    for (int c = 0; c < 3; c++) {
+      double xb[2][2][2];
+      for (int v = 0; v < 8; v++)
+        xb[v >> 2][(v >> 1) & 1][v & 1] = vxyz[((int64_t)e * 8 + v) * 3 + c];
+      double *out = c == 0 ? x : (c == 1 ? y : z);
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++) {
+            double acc = 0.0;
+            for (int cz = 0; cz < 2; cz++) {
+              double ay = 0.0;
+              for (int cy = 0; cy < 2; cy++) {
+                double ax = jx[i * 2 + 0] * xb[cz][cy][0] + jx[i * 2 + 1] * xb[cz][cy][1];
+                ay += jx[j * 2 + cy] * ax;
+              }
+              acc += jx[k * 2 + cz] * ay;
+            }
+            out[IDX4(i, j, k, e, lx)] = acc;
+          }
+    }
+  }
+  free(jx);
+}
+
+/* ===================================================================== */
+/* a6: gs mapping (gather_scatter.f90:513-1375)                          */
+/* ===================================================================== */
+static int gs_add_dof(htab_t *t, int64_t dof, int *max_id) {
+  /* gather_scatter.f90:1276-1288 gs_mapping_add_dof */
+  int64_t k[4] = {dof, 0, 0, 0};
+  int found;
+  int64_t *v = ht_get(t, k, 1, &found);
+  if (!found) {
+    *max_id = *max_id + 1;
+    *v = *max_id;
+  }
+  return (int)*v;
+}
+
+static void gs_qsort(int *dg, int *gd, int lo, int hi) {
+  /* gather_scatter.f90:1291-1330 gs_qsort_dofmap: Hoare partition with the
+   * middle pivot dg((lo+hi)/2); 1-based lo/hi, arrays 1-based via -1. */
+  for (;;) {
+    int i = lo - 1, j = hi + 1;
+    int pivot = dg[(lo + hi) / 2 - 1];
+    for (;;) {
+      do { i = i + 1; } while (dg[i - 1] < pivot);
+      do { j = j - 1; } while (dg[j - 1] > pivot);
+      if (i < j) {
+        int t = dg[i - 1]; dg[i - 1] = dg[j - 1]; dg[j - 1] = t;
+        t = gd[i - 1]; gd[i - 1] = gd[j - 1]; gd[j - 1] = t;
+      } else if (i == j) {
+        i = i + 1;
+        break;
+      } else {
+        break;
+      }
+    }
+    /* recurse on the smaller side first is NOT done: order of the two
+     * recursive calls does not change the result (disjoint ranges). */
+    if (lo < j) gs_qsort(dg, gd, lo, j);
+    if (i < hi) {
+      lo = i; /* tail call: gs_qsort(dg, gd, i, hi) */
+      continue;
+    }
+    break;
+  }
+}
+
+int or_gs_map(int nel, int lx, const int64_t *dof, const int *fneigh,
+              int *dg, int *gd, int *nlocal_out, int *facet_offset_out,
+              int *blk_len, int *blk_off, int *nblks_out) {
+  /* Single rank: dofmap%shared_dof is .false. everywhere, so every point
+   * takes the "local" branch of gs_init_mapping.  Registration order:
+   * element loop over the 8 vertices (:555-650), table cleared (:652),
+   * element loop over the 12 edges (:656-853), table cleared (:856), element
+   * loop over faces with facet_neigh /= 0 (:960-1062).  Arrays dg (gs id)
+   * and gd (1-based linear local index) are then sorted by gs id (:1185)
+   * and runs before the facet offset become blocks (:1187). */
+  const int ly = lx, lz = lx;
+  htab_t dm;
+  ht_init(&dm, (int64_t)nel * lx * lx * 6 + 64);
+  int max_id = 0, n = 0;
+#define LIN(i, j, k, e) ((int)(IDX4((i) - 1, (j) - 1, (k) - 1, (e), lx) + 1))
+#define PUSH(i, j, k, e)                                                   \
+  do {                                                                     \
+    dg[n] = gs_add_dof(&dm, dof[IDX4((i) - 1, (j) - 1, (k) - 1, (e), lx)], &max_id); \
+    gd[n] = LIN(i, j, k, e);                                               \
+    n++;                                                                   \
+  } while (0)
+  for (int e = 0; e < nel; e++) {
+    PUSH(1, 1, 1, e);
+    PUSH(lx, 1, 1, e);
+    PUSH(1, ly, 1, e);
+    PUSH(lx, ly, 1, e);
+    PUSH(1, 1, lz, e);
+    PUSH(lx, 1, lz, e);
+    PUSH(1, ly, lz, e);
+    PUSH(lx, ly, lz, e);
+  }
+  ht_clear(&dm);
+  for (int e = 0; e < nel; e++) {
+    for (int j = 2; j <= lx - 1; j++) PUSH(j, 1, 1, e);
+    for (int j = 2; j <= lx - 1; j++) PUSH(j, 1, lz, e);
+    for (int j = 2; j <= lx - 1; j++) PUSH(j, ly, 1, e);
+    for (int j = 2; j <= lx - 1; j++) PUSH(j, ly, lz, e);
+    for (int k = 2; k <= ly - 1; k++) PUSH(1, k, 1, e);
+    for (int k = 2; k <= ly - 1; k++) PUSH(1, k, lz, e);
+    for (int k = 2; k <= ly - 1; k++) PUSH(lx, k, 1, e);
+    for (int k = 2; k <= ly - 1; k++) PUSH(lx, k, lz, e);
+    for (int l = 2; l <= lz - 1; l++) PUSH(1, 1, l, e);
+    for (int l = 2; l <= lz - 1; l++) PUSH(lx, 1, l, e);
+    for (int l = 2; l <= lz - 1; l++) PUSH(1, ly, l, e);
+    for (int l = 2; l <= lz - 1; l++) PUSH(lx, ly, l, e);
+  }
+  ht_clear(&dm);
+  const int nlocal_dof = n;
+  for (int e = 0; e < nel; e++) {
+    if (fneigh[e * 6 + 0] != 0)
+      for (int l = 2; l <= lz - 1; l++)
+        for (int k = 2; k <= ly - 1; k++) PUSH(1, k, l, e);
+    if (fneigh[e * 6 + 1] != 0)
+      for (int l = 2; l <= lz - 1; l++)
+        for (int k = 2; k <= ly - 1; k++) PUSH(lx, k, l, e);
+    if (fneigh[e * 6 + 2] != 0)
+      for (int l = 2; l <= lz - 1; l++)
+        for (int j = 2; j <= lx - 1; j++) PUSH(j, 1, l, e);
+    if (fneigh[e * 6 + 3] != 0)
+      for (int l = 2; l <= lz - 1; l++)
+        for (int j = 2; j <= lx - 1; j++) PUSH(j, ly, l, e);
+    if (fneigh[e * 6 + 4] != 0)
+      for (int k = 2; k <= ly - 1; k++)
+        for (int j = 2; j <= lx - 1; j++) PUSH(j, k, 1, e);
+    if (fneigh[e * 6 + 5] != 0)
+      for (int k = 2; k <= ly - 1; k++)
+        for (int j = 2; j <= lx - 1; j++) PUSH(j, k, lz, e);
+  }
+#undef PUSH
+#undef LIN
+  ht_free(&dm);
+  const int nlocal = n;
+  const int facet_offset = nlocal_dof + 1;
+  if (nlocal > 0) gs_qsort(dg, gd, 1, nlocal);
+  /* gs_find_blks (:1333-1373) */
+  int nb = 0, i = 1;
+  while (i < facet_offset) {
+    int id = dg[i - 1], count = 1, j = i;
+    while (j + 1 <= nlocal && dg[j] == id) {
+      j++;
+      count++;
+    }
+    blk_len[nb++] = count;
+    i = j + 1;
+  }
+  if (nb > 0) {
+    blk_off[0] = 0;
+    for (int q = 1; q < nb; q++) blk_off[q] = blk_off[q - 1] + blk_len[q - 1];
+  }
+  *nlocal_out = nlocal;
+  *facet_offset_out = facet_offset;
+  *nblks_out = nb;
+  return 0;
+}
+
+/* ===================================================================== */
+/* a5: gs ADD (gs_cpu.f90:98-138 gather, :292-322 scatter)               */
+/* ===================================================================== */
+void or_gs_op_add(double *u, int n, int m, int o, const int *dg,
+                  const int *gd, int nb, const int *b, const int *bo,
+                  double *v) {
+  (void)n;
+  /* gather */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nb; i++) {
+    int k = bo[i], len = b[i];
+    double tmp = u[gd[k] - 1];
+    for (int j = 2; j <= len; j++) tmp = tmp + u[gd[k + j - 1] - 1];
+    v[dg[k] - 1] = tmp;
+  }
+  if (o < 0) {
+    for (int i = -o; i <= m; i++) v[dg[i - 1] - 1] = u[gd[i - 1] - 1];
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int i = o; i <= m; i += 2)
+      v[dg[i - 1] - 1] = u[gd[i - 1] - 1] + u[gd[i] - 1];
+  }
+  /* scatter */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nb; i++) {
+    int k = bo[i], len = b[i];
+    double tmp = v[dg[k] - 1];
+    for (int j = 1; j <= len; j++) u[gd[k + j - 1] - 1] = tmp;
+  }
+  int start = nb > 0 ? bo[nb - 1] + b[nb - 1] + 1 : 1;
+#pragma omp parallel for schedule(static)
+  for (int i = start; i <= m; i++) u[gd[i - 1] - 1] = v[dg[i - 1] - 1];
+}
+
+/* ===================================================================== */
+/* a3: geometric factors (coef.f90:714-1030, mass :1112-1156)            */
+/* ===================================================================== */
+void or_coef(int nel, int lx, const double *x, const double *y,
+             const double *z, const double *d, const double *w3,
+             double *g11, double *g22, double *g33, double *g12,
+             double *g13, double *g23, double *jac, double *B) {
+  const int n3 = lx * lx * lx;
+#pragma omp parallel
+  {
+    double *buf = (double *)malloc(sizeof(double) * n3 * 9);
+#pragma omp for schedule(static)
+    for (int e = 0; e < nel; e++) {
+      const double *xyz[3] = {x + (int64_t)e * n3, y + (int64_t)e * n3, z + (int64_t)e * n3};
+      double *dr[3], *ds[3], *dt[3];
+      for (int c = 0; c < 3; c++) {
+        dr[c] = buf + c * n3;
+        ds[c] = buf + (3 + c) * n3;
+        dt[c] = buf + (6 + c) * n3;
+      }
+      for (int c = 0; c < 3; c++) {
+        const double *f = xyz[c];
+        for (int k = 0; k < lx; k++)
+          for (int j = 0; j < lx; j++)
+            for (int i = 0; i < lx; i++) {
+              double a = 0.0, b = 0.0, cc = 0.0;
+              for (int l = 0; l < lx; l++) {
+                a += d[i + lx * l] * f[l + lx * (j + lx * k)];
+                b += f[i + lx * (l + lx * k)] * d[j + lx * l];
+                cc += f[i + lx * (j + lx * l)] * d[k + lx * l];
+              }
+              dr[c][i + lx * (j + lx * k)] = a;
+              ds[c][i + lx * (j + lx * k)] = b;
+              dt[c][i + lx * (j + lx * k)] = cc;
+            }
+      }
+      for (int p = 0; p < n3; p++) {
+        const double xr = dr[0][p], yr = dr[1][p], zr = dr[2][p];
+        const double xs = ds[0][p], ys = ds[1][p], zs = ds[2][p];
+        const double xt = dt[0][p], yt = dt[1][p], zt = dt[2][p];
+        double jc = 0.0;
+        jc = jc + (xr * ys * zt);
+        jc = jc + (xt * yr * zs);
+        jc = jc + (xs * yt * zr);
+        jc = jc - (xr * yt * zs);
+        jc = jc - (xs * yr * zt);
+        jc = jc - (xt * ys * zr);
+        const double rx = ys * zt - yt * zs, ry = xt * zs - xs * zt,
+                     rz = xs * yt - xt * ys;
+        const double sx = yt * zr - yr * zt, sy = xr * zt - xt * zr,
+                     sz = xt * yr - xr * yt;
+        const double tx = yr * zs - ys * zr, ty = xs * zr - xr * zs,
+                     tz = xr * ys - xs * yr;
+        const double jinv = 1.0 / jc;
+        const int64_t q = (int64_t)e * n3 + p;
+        double a11 = rx * rx + ry * ry + rz * rz;
+        double a22 = sx * sx + sy * sy + sz * sz;
+        double a33 = tx * tx + ty * ty + tz * tz;
+        a11 = a11 * jinv;
+        a22 = a22 * jinv;
+        a33 = a33 * jinv;
+        double a12 = rx * sx + ry * sy + rz * sz;
+        double a13 = rx * tx + ry * ty + rz * tz;
+        double a23 = sx * tx + sy * ty + sz * tz;
+        a12 = a12 * jinv;
+        a13 = a13 * jinv;
+        a23 = a23 * jinv;
+        g11[q] = a11 * w3[p];
+        g22[q] = a22 * w3[p];
+        g12[q] = a12 * w3[p];
+        g33[q] = a33 * w3[p];
+        g13[q] = a13 * w3[p];
+        g23[q] = a23 * w3[p];
+        jac[q] = jc;
+        B[q] = jc * w3[p];
+      }
+    }
+    free(buf);
+  }
+}
+
+/* ===================================================================== */
+/* a4: ax_helm (ax_helm_cpu.f90:88-166, lx-specialised :1211-1341)       */
+/* ===================================================================== */
+void or_ax_helm(double *w, const double *u, const double *d, const double *dt,
+                const double *h1, const double *g11, const double *g22,
+                const double *g33, const double *g12, const double *g13,
+                const double *g23, int nel, int lx, const double *h2,
+                const double *B, int ifh2) {
+  const int n3 = lx * lx * lx, n2 = lx * lx;
+  /* lx 2..14 use the unrolled kernels, whose w accumulation is
+   * w = ((w + t1) + t2) + ...; other lx use ax_helm_lx (w = w + sum). */
+  const int unrolled = (lx >= 2 && lx <= 14);
+#pragma omp parallel
+  {
+    double *wk = (double *)malloc(sizeof(double) * n3 * 6);
+    double *wur = wk, *wus = wk + n3, *wut = wk + 2 * n3;
+    double *ur = wk + 3 * n3, *us = wk + 4 * n3, *ut = wk + 5 * n3;
+#pragma omp for schedule(static)
+    for (int e = 0; e < nel; e++) {
+      const double *ue = u + (int64_t)e * n3;
+      double *we = w + (int64_t)e * n3;
+      for (int jj = 0; jj < n2; jj++)
+        for (int i = 0; i < lx; i++) {
+          double t = 0.0;
+          for (int k = 0; k < lx; k++) t = t + d[i + lx * k] * ue[k + lx * jj];
+          wur[i + lx * jj] = t;
+        }
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++) {
+            double t = 0.0;
+            for (int l = 0; l < lx; l++) t = t + d[j + lx * l] * ue[i + lx * (l + lx * k)];
+            wus[i + lx * (j + lx * k)] = t;
+          }
+      for (int k = 0; k < lx; k++)
+        for (int i = 0; i < n2; i++) {
+          double t = 0.0;
+          for (int l = 0; l < lx; l++) t = t + d[k + lx * l] * ue[i + n2 * l];
+          wut[i + n2 * k] = t;
+        }
+      const int64_t o = (int64_t)e * n3;
+      for (int i = 0; i < n3; i++) {
+        ur[i] = h1[o + i] * (g11[o + i] * wur[i] + g12[o + i] * wus[i] + g13[o + i] * wut[i]);
+        us[i] = h1[o + i] * (g12[o + i] * wur[i] + g22[o + i] * wus[i] + g23[o + i] * wut[i]);
+        ut[i] = h1[o + i] * (g13[o + i] * wur[i] + g23[o + i] * wus[i] + g33[o + i] * wut[i]);
+      }
+      for (int jj = 0; jj < n2; jj++)
+        for (int i = 0; i < lx; i++) {
+          double t = 0.0;
+          for (int k = 0; k < lx; k++) t = t + dt[i + lx * k] * ur[k + lx * jj];
+          we[i + lx * jj] = t;
+        }
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++) {
+            const int p = i + lx * (j + lx * k);
+            if (unrolled) {
+              double acc = we[p];
+              for (int l = 0; l < lx; l++) acc = acc + dt[j + lx * l] * us[i + lx * (l + lx * k)];
+              we[p] = acc;
+            } else {
+              double t = 0.0;
+              for (int l = 0; l < lx; l++) t = t + dt[j + lx * l] * us[i + lx * (l + lx * k)];
+              we[p] = we[p] + t;
+            }
+          }
+      for (int k = 0; k < lx; k++)
+        for (int i = 0; i < n2; i++) {
+          const int p = i + n2 * k;
+          if (unrolled) {
+            double acc = we[p];
+            for (int l = 0; l < lx; l++) acc = acc + dt[k + lx * l] * ut[i + n2 * l];
+            we[p] = acc;
+          } else {
+            double t = 0.0;
+            for (int l = 0; l < lx; l++) t = t + dt[k + lx * l] * ut[i + n2 * l];
+            we[p] = we[p] + t;
+          }
+        }
+    }
+    free(wk);
+    if (ifh2) {
+      const int64_t ntot = (int64_t)nel * n3;
+#pragma omp for schedule(static)
+      for (int64_t i = 0; i < ntot; i++) w[i] = w[i] + h2[i] * B[i] * u[i];
+    }
+  }
+}
+
+/* ===================================================================== */
+/* a7: zero Dirichlet (zero_dirichlet.f90:92-118; mask on all points of   */
+/* facets with facet_neigh == 0, examples/poisson/setup.f90:2-30)        */
+/* ===================================================================== */
+int or_mask(int nel, int lx, const int *fneigh, int *msk) {
+  int m = 0;
+  for (int e = 0; e < nel; e++)
+    for (int f = 0; f < 6; f++) {
+      if (fneigh[e * 6 + f] != 0) continue;
+      for (int b = 0; b < lx; b++)
+        for (int a = 0; a < lx; a++) {
+          int i, j, k;
+          if (f < 2) { i = (f == 1) ? lx - 1 : 0; j = a; k = b; }
+          else if (f < 4) { j = (f == 3) ? lx - 1 : 0; i = a; k = b; }
+          else { k = (f == 5) ? lx - 1 : 0; i = a; j = b; }
+          if (msk) msk[m] = (int)IDX4(i, j, k, e, lx) + 1;
+          m++;
+        }
+    }
+  return m;
+}
+
+void or_mask_apply(double *x, const int *msk, int m) {
+  for (int i = 0; i < m; i++) x[msk[i] - 1] = 0.0;
+}
+
+/* ===================================================================== */
+/* a8: glsc3 (math.f90:1286-1306)                                        */
+/* ===================================================================== */
+double or_glsc3(const double *a, const double *b, const double *c, int64_t n) {
+  double tmp = 0.0;
+#pragma omp parallel for reduction(+ : tmp) schedule(static)
+  for (int64_t i = 0; i < n; i++) tmp = tmp + a[i] * b[i] * c[i];
+  return tmp;
+}
+
+/* ===================================================================== */
+/* a12: Jacobi diagonal (pc_jacobi.f90:93-264, CPU formula)              */
+/* ===================================================================== */
+void or_jacobi_diag(double *d, const double *dt, const double *g11,
+                    const double *g22, const double *g33, const double *g12,
+                    const double *g13, const double *g23, const int *dfrmd,
+                    int nel, int lx) {
+  /* d = diag of the element stiffness (before h1, h2, gs and inversion).
+   * dxt(i,l) = dt[i + lx*l]. */
+#define DT(a, b) dt[(a) + lx * (b)]
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < nel; e++) {
+    const int64_t o = (int64_t)e * lx * lx * lx;
+#define A(arr, i, j, k) arr[o + (i) + lx * ((j) + lx * (k))]
+    for (int q = 0; q < lx * lx * lx; q++) d[o + q] = 0.0;
+    for (int l = 0; l < lx; l++)
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++)
+            A(d, i, j, k) = A(d, i, j, k) + A(g11, l, j, k) * (DT(i, l) * DT(i, l));
+    for (int l = 0; l < lx; l++)
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++)
+            A(d, i, j, k) = A(d, i, j, k) + A(g22, i, l, k) * (DT(j, l) * DT(j, l));
+    for (int l = 0; l < lx; l++)
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++)
+            A(d, i, j, k) = A(d, i, j, k) + A(g33, i, j, l) * (DT(k, l) * DT(k, l));
+    if (dfrmd[e]) {
+      const int L = lx - 1;
+      const int step = (lx - 1) > 0 ? (lx - 1) : 1;
+      for (int j = 0; j <= L; j += step)
+        for (int k = 0; k <= L; k += step) {
+          A(d, 0, j, k) = A(d, 0, j, k) + A(g12, 0, j, k) * DT(0, 0) * DT(j, j) +
+                          A(g13, 0, j, k) * DT(0, 0) * DT(k, k);
+          A(d, L, j, k) = A(d, L, j, k) + A(g12, L, j, k) * DT(L, L) * DT(j, j) +
+                          A(g13, L, j, k) * DT(L, L) * DT(k, k);
+        }
+      for (int i = 0; i <= L; i += step)
+        for (int k = 0; k <= L; k += step) {
+          A(d, i, 0, k) = A(d, i, 0, k) + A(g12, i, 0, k) * DT(0, 0) * DT(i, i) +
+                          A(g23, i, 0, k) * DT(0, 0) * DT(k, k);
+          A(d, i, L, k) = A(d, i, L, k) + A(g12, i, L, k) * DT(L, L) * DT(i, i) +
+                          A(g23, i, L, k) * DT(L, L) * DT(k, k);
+        }
+      for (int i = 0; i <= L; i += step)
+        for (int j = 0; j <= L; j += step) {
+          A(d, i, j, 0) = A(d, i, j, 0) + A(g13, i, j, 0) * DT(0, 0) * DT(i, i) +
+                          A(g23, i, j, 0) * DT(0, 0) * DT(j, j);
+          A(d, i, j, L) = A(d, i, j, L) + A(g13, i, j, L) * DT(L, L) * DT(i, i) +
+                          A(g23, i, j, L) * DT(L, L) * DT(j, j);
+        }
+    }
+#undef A
+  }
+#undef DT
+}
+
+/* ===================================================================== */
+/* a10: Jacobi-PCG (cg.f90:140-279)                                       */
+/* ===================================================================== */
+void or_ax_full(const or_problem_t *p, double *w, const double *u) {
+  /* ax%compute, gs_h%op(ADD), blst%apply (the operator each Krylov step
+   * applies: cg.f90:207-209) */
+  const int64_t n = (int64_t)p->nel * p->lx * p->lx * p->lx;
+  or_ax_helm(w, u, p->d, p->dt, p->h1, p->g11, p->g22, p->g33, p->g12, p->g13,
+             p->g23, p->nel, p->lx, p->h2, p->B, p->ifh2);
+  or_gs_op_add(w, (int)n, p->gs_m, p->gs_o, p->dg, p->gd, p->gs_nb, p->blk_len,
+               p->blk_off, p->gs_buf);
+  or_mask_apply(w, p->msk, p->nmsk);
+}
+
+#define CG_P_SPACE 7
+#define BLK 1024
+
+or_monitor_t or_cg_solve(const or_problem_t *pb, const double *pcd, double *x,
+                         const double *f, int max_iter, double abs_tol) {
+  const int64_t n = (int64_t)pb->nel * pb->lx * pb->lx * pb->lx;
+  or_monitor_t res = {0, 0, 0, 0};
+  double *w = (double *)malloc(sizeof(double) * n);
+  double *r = (double *)malloc(sizeof(double) * n);
+  double *z = (double *)malloc(sizeof(double) * n);
+  double *p = (double *)malloc(sizeof(double) * n * CG_P_SPACE);
+  double alpha[CG_P_SPACE];
+  const double norm_fac = 1.0 / sqrt(pb->volume);
+  double rtz1 = 1.0, rtz2, rtr = 0.0, rnorm, beta, pap;
+  const double *mult = pb->mult;
+#pragma omp parallel for reduction(+ : rtr) schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    x[i] = 0.0;
+    p[(CG_P_SPACE - 1) * n + i] = 0.0;
+    r[i] = f[i];
+    rtr = rtr + (r[i] * mult[i] * r[i]);
+  }
+  rnorm = sqrt(rtr) * norm_fac;
+  res.res_start = rnorm;
+  res.res_final = rnorm;
+  res.iter = 0;
+  if (fabs(rnorm - 0.0) < 2.220446049250313e-16) {
+    res.converged = 1;
+    goto done;
+  }
+  int p_prev = CG_P_SPACE, p_cur = 1, iter;
+  for (iter = 1; iter <= max_iter; iter++) {
+    /* M%solve: jacobi z = r*d (pc_jacobi.f90:86-92) */
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) z[i] = r[i] * pcd[i];
+    rtz2 = rtz1;
+    rtz1 = or_glsc3(r, mult, z, n);
+    beta = rtz1 / rtz2;
+    if (iter == 1) beta = 0.0;
+    double *pc = p + (int64_t)(p_cur - 1) * n, *pp = p + (int64_t)(p_prev - 1) * n;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) pc[i] = z[i] + beta * pp[i];
+    or_ax_full(pb, w, pc);
+    pap = or_glsc3(w, mult, pc, n);
+    alpha[p_cur - 1] = rtz1 / pap;
+    {
+      /* second_cg_part (:261-279) */
+      double tmp = 0.0;
+      const double a = alpha[p_cur - 1];
+#pragma omp parallel for reduction(+ : tmp) schedule(static)
+      for (int64_t i = 0; i < n; i++) {
+        r[i] = r[i] - a * w[i];
+        tmp = tmp + r[i] * r[i] * mult[i];
+      }
+      rtr = tmp;
+    }
+    rnorm = sqrt(rtr) * norm_fac;
+    if (p_cur == CG_P_SPACE || rnorm < abs_tol || iter == max_iter) {
+#pragma omp parallel for schedule(static)
+      for (int64_t i0 = 0; i0 < n; i0 += BLK) {
+        if (i0 + BLK <= n) {
+          double xp[BLK];
+          for (int k = 0; k < BLK; k++) xp[k] = 0.0;
+          for (int j = 1; j <= p_cur; j++)
+            for (int k = 0; k < BLK; k++) xp[k] = xp[k] + alpha[j - 1] * p[(int64_t)(j - 1) * n + i0 + k];
+          for (int k = 0; k < BLK; k++) x[i0 + k] = x[i0 + k] + xp[k];
+        } else {
+          for (int64_t k = 0; k < n - i0; k++) {
+            double tmp = 0.0;
+            for (int j = 1; j <= p_cur; j++) tmp = tmp + alpha[j - 1] * p[(int64_t)(j - 1) * n + i0 + k];
+            x[i0 + k] = x[i0 + k] + tmp;
+          }
+        }
+      }
+      p_prev = p_cur;
+      p_cur = 1;
+      if (rnorm < abs_tol) break;
+    } else {
+      p_prev = p_cur;
+      p_cur = p_cur + 1;
+    }
+  }
+  res.res_final = rnorm;
+  res.iter = iter;
+  res.converged = !(iter >= max_iter) && !(rnorm > abs_tol);
+done:
+  free(w);
+  free(r);
+  free(z);
+  free(p);
+  return res;
+}
+
+/* ===================================================================== */
+/* a11: GMRES(lgmres) (gmres.f90:175-411), right preconditioned           */
+/* ===================================================================== */
+or_monitor_t or_gmres_solve(const or_problem_t *pb, const double *pcd,
+                            double *x, const double *f, int max_iter,
+                            double abs_tol, int lg) {
+  const int64_t n = (int64_t)pb->nel * pb->lx * pb->lx * pb->lx;
+  const double *mult = pb->mult;
+  or_monitor_t res = {0, 0, 0, 0};
+  double *w = (double *)malloc(sizeof(double) * n);
+  double *r = (double *)malloc(sizeof(double) * n);
+  double *v = (double *)malloc(sizeof(double) * n * lg);
+  double *z = (double *)malloc(sizeof(double) * n * lg);
+  double *h = (double *)calloc((size_t)lg * lg, sizeof(double));
+  double *c = (double *)malloc(sizeof(double) * lg);
+  double *s = (double *)malloc(sizeof(double) * lg);
+  double *gam = (double *)calloc((size_t)lg + 1, sizeof(double));
+  double *hp = (double *)malloc(sizeof(double) * lg);
+#define H(a, b) h[((a) - 1) + (int64_t)lg * ((b) - 1)]
+  const double norm_fac = 1.0 / sqrt(pb->volume);
+  int conv = 0, iter = 0, j = 0;
+  double rnorm = 0.0;
+  for (int64_t i = 0; i < n; i++) x[i] = 0.0;
+  for (int q = 0; q < lg; q++) { c[q] = 1.0; s[q] = 1.0; }
+  while (!conv && iter < max_iter) {
+    for (int64_t i = 0; i < n; i++) r[i] = f[i];
+    if (iter != 0) {
+      or_ax_full(pb, w, x);
+      for (int64_t i = 0; i < n; i++) r[i] = r[i] - w[i];
+    }
+    gam[0] = sqrt(or_glsc3(r, r, mult, n));
+    if (iter == 0) res.res_start = gam[0] * norm_fac;
+    if (fabs(gam[0]) < 2.220446049250313e-16) break;
+    rnorm = 0.0;
+    double temp = 1.0 / gam[0];
+    for (int64_t i = 0; i < n; i++) v[i] = r[i] * temp;
+    for (j = 1; j <= lg; j++) {
+      iter = iter + 1;
+      double *zj = z + (int64_t)(j - 1) * n;
+      const double *vj = v + (int64_t)(j - 1) * n;
+      if (pcd) {
+        for (int64_t i = 0; i < n; i++) zj[i] = vj[i] * pcd[i];
+      } else {
+        for (int64_t i = 0; i < n; i++) zj[i] = vj[i];
+      }
+      or_ax_full(pb, w, zj);
+      for (int l = 0; l < j; l++) hp[l] = 0.0;
+      for (int64_t i0 = 0; i0 < n; i0 += BLK) {
+        int64_t kk = (i0 + BLK <= n) ? BLK : n - i0;
+        for (int l = 1; l <= j; l++) {
+          double acc = hp[l - 1];
+          for (int64_t k = 0; k < kk; k++)
+            acc = acc + w[i0 + k] * v[(int64_t)(l - 1) * n + i0 + k] * mult[i0 + k];
+          hp[l - 1] = acc;
+        }
+      }
+      for (int l = 1; l <= j; l++) H(l, j) = hp[l - 1];
+      double alpha2 = 0.0;
+      for (int64_t i0 = 0; i0 < n; i0 += BLK) {
+        int64_t kk = (i0 + BLK <= n) ? BLK : n - i0;
+        if (kk == BLK) {
+          double wp[BLK];
+          for (int k = 0; k < BLK; k++) wp[k] = 0.0;
+          for (int l = 1; l <= j; l++)
+            for (int k = 0; k < BLK; k++) wp[k] = wp[k] - H(l, j) * v[(int64_t)(l - 1) * n + i0 + k];
+          for (int k = 0; k < BLK; k++) {
+            w[i0 + k] = w[i0 + k] + wp[k];
+            alpha2 = alpha2 + w[i0 + k] * w[i0 + k] * mult[i0 + k];
+          }
+        } else {
+          for (int64_t k = 0; k < kk; k++) {
+            double tmp = 0.0;
+            for (int l = 1; l <= j; l++) tmp = tmp - H(l, j) * v[(int64_t)(l - 1) * n + i0 + k];
+            w[i0 + k] = w[i0 + k] + tmp;
+            alpha2 = alpha2 + w[i0 + k] * w[i0 + k] * mult[i0 + k];
+          }
+        }
+      }
+      double alpha = sqrt(alpha2);
+      for (int i = 1; i <= j - 1; i++) {
+        temp = H(i, j);
+        H(i, j) = c[i - 1] * temp + s[i - 1] * H(i + 1, j);
+        H(i + 1, j) = -s[i - 1] * temp + c[i - 1] * H(i + 1, j);
+      }
+      rnorm = 0.0;
+      if (fabs(alpha) < 2.220446049250313e-16) {
+        conv = 1;
+        break;
+      }
+      double lr = sqrt(H(j, j) * H(j, j) + alpha2);
+      temp = 1.0 / lr;
+      c[j - 1] = H(j, j) * temp;
+      s[j - 1] = alpha * temp;
+      H(j, j) = lr;
+      gam[j] = -s[j - 1] * gam[j - 1];
+      gam[j - 1] = c[j - 1] * gam[j - 1];
+      rnorm = fabs(gam[j]) * norm_fac;
+      if (rnorm < abs_tol) {
+        conv = 1;
+        break;
+      }
+      if (iter + 1 > max_iter) break;
+      if (j < lg) {
+        temp = 1.0 / alpha;
+        double *vn = v + (int64_t)j * n;
+        for (int64_t i = 0; i < n; i++) vn[i] = w[i] * temp;
+      }
+    }
+    if (j > lg) j = lg;
+    for (int k = j; k >= 1; k--) {
+      temp = gam[k - 1];
+      for (int i = j; i >= k + 1; i--) temp = temp - H(k, i) * c[i - 1];
+      c[k - 1] = temp / H(k, k);
+    }
+    for (int64_t i0 = 0; i0 < n; i0 += BLK) {
+      int64_t kk = (i0 + BLK <= n) ? BLK : n - i0;
+      if (kk == BLK) {
+        double xp[BLK];
+        for (int k = 0; k < BLK; k++) xp[k] = 0.0;
+        for (int l = 1; l <= j; l++)
+          for (int k = 0; k < BLK; k++) xp[k] = xp[k] + c[l - 1] * z[(int64_t)(l - 1) * n + i0 + k];
+        for (int k = 0; k < BLK; k++) x[i0 + k] = x[i0 + k] + xp[k];
+      } else {
+        for (int64_t k = 0; k < kk; k++) {
+          double tmp = 0.0;
+          for (int l = 1; l <= j; l++) tmp = tmp + c[l - 1] * z[(int64_t)(l - 1) * n + i0 + k];
+          x[i0 + k] = x[i0 + k] + tmp;
+        }
+      }
+    }
+  }
+#undef H
+  res.res_final = rnorm;
+  res.iter = iter;
+  res.converged = !(iter >= max_iter) && !(rnorm > abs_tol);
+  free(w); free(r); free(v); free(z); free(h); free(c); free(s); free(gam); free(hp);
+  return res;
+}

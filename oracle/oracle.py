@@ -1,0 +1,283 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/neko_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP product is compared against.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product package (neko_amd) never does.
+
+`OracleCase` builds a whole problem the way Neko's setup does (box mesh ->
+facet connectivity -> dofmap -> gs maps -> coef -> mask -> Jacobi) and
+exposes Neko's solve-path operations on numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libneko_oracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _SO
+
+
+_lib = None
+
+
+class OrProblem(C.Structure):
+    _fields_ = [
+        ("nel", C.c_int), ("lx", C.c_int),
+        ("d", C.c_void_p), ("dt", C.c_void_p),
+        ("h1", C.c_void_p), ("h2", C.c_void_p), ("B", C.c_void_p),
+        ("ifh2", C.c_int),
+        ("g11", C.c_void_p), ("g22", C.c_void_p), ("g33", C.c_void_p),
+        ("g12", C.c_void_p), ("g13", C.c_void_p), ("g23", C.c_void_p),
+        ("mult", C.c_void_p),
+        ("gs_m", C.c_int), ("gs_o", C.c_int), ("gs_nb", C.c_int),
+        ("dg", C.c_void_p), ("gd", C.c_void_p),
+        ("blk_len", C.c_void_p), ("blk_off", C.c_void_p),
+        ("gs_buf", C.c_void_p),
+        ("msk", C.c_void_p), ("nmsk", C.c_int),
+        ("volume", C.c_double),
+    ]
+
+
+class OrMonitor(C.Structure):
+    _fields_ = [("iter", C.c_int), ("res_start", C.c_double),
+                ("res_final", C.c_double), ("converged", C.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_SO):
+        build()
+    L = C.CDLL(_SO)
+    L.or_rng_uniform.restype = C.c_double
+    L.or_rng_uniform.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_rng_normal.restype = C.c_double
+    L.or_rng_normal.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_zwgll.argtypes = [_dp, _dp, C.c_int]
+    L.or_dgll.argtypes = [_dp, _dp, _dp, C.c_int]
+    L.or_box_mesh.argtypes = [C.c_int] * 6 + [C.c_double, C.c_uint64, _dp, _lp]
+    L.or_facet_neigh.argtypes = [C.c_int, _lp, _ip]
+    L.or_dfrmd_el.argtypes = [C.c_int, _dp, _ip]
+    L.or_dof_ids.argtypes = [C.c_int, C.c_int, _lp, _lp]
+    L.or_xyz.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.or_gs_map.argtypes = [C.c_int, C.c_int, _lp, _ip, _ip, _ip,
+                            C.POINTER(C.c_int), C.POINTER(C.c_int), _ip, _ip,
+                            C.POINTER(C.c_int)]
+    L.or_gs_op_add.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, _ip,
+                               C.c_int, _ip, _ip, _dp]
+    L.or_coef.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp] + [_dp] * 8
+    L.or_ax_helm.argtypes = ([_dp, _dp, _dp, _dp, _dp] + [_dp] * 6 +
+                             [C.c_int, C.c_int, _dp, _dp, C.c_int])
+    L.or_mask.restype = C.c_int
+    L.or_mask.argtypes = [C.c_int, C.c_int, _ip, C.c_void_p]
+    L.or_mask_apply.argtypes = [_dp, _ip, C.c_int]
+    L.or_glsc3.restype = C.c_double
+    L.or_glsc3.argtypes = [_dp, _dp, _dp, C.c_int64]
+    L.or_jacobi_diag.argtypes = [_dp, _dp] + [_dp] * 6 + [_ip, C.c_int, C.c_int]
+    L.or_cg_solve.restype = OrMonitor
+    L.or_cg_solve.argtypes = [C.POINTER(OrProblem), C.c_void_p, _dp, _dp,
+                              C.c_int, C.c_double]
+    L.or_gmres_solve.restype = OrMonitor
+    L.or_gmres_solve.argtypes = [C.POINTER(OrProblem), C.c_void_p, _dp, _dp,
+                                 C.c_int, C.c_double, C.c_int]
+    L.or_ax_full.argtypes = [C.POINTER(OrProblem), _dp, _dp]
+    _lib = L
+    return L
+
+
+def zwgll(lx):
+    z = np.zeros(lx)
+    w = np.zeros(lx)
+    lib().or_zwgll(z, w, lx)
+    return z, w
+
+
+def dgll(z):
+    lx = len(z)
+    d = np.zeros(lx * lx)
+    dt = np.zeros(lx * lx)
+    lib().or_dgll(d, dt, np.ascontiguousarray(z, dtype=np.float64), lx)
+    return d, dt
+
+
+class OracleCase:
+    """A whole Neko-style problem on a (possibly perturbed/periodic) box."""
+
+    def __init__(self, nelx, nely, nelz, lx, amp=0.0, seed=0,
+                 periodic=(False, False, False), h1=1.0, h2=0.0):
+        L = lib()
+        self.nelx, self.nely, self.nelz, self.lx = nelx, nely, nelz, lx
+        nel = nelx * nely * nelz
+        self.nel = nel
+        self.n = n = nel * lx ** 3
+        self.vxyz = np.zeros(nel * 24)
+        self.pid = np.zeros(nel * 8, dtype=np.int64)
+        L.or_box_mesh(nelx, nely, nelz, int(periodic[0]), int(periodic[1]),
+                      int(periodic[2]), float(amp), int(seed), self.vxyz,
+                      self.pid)
+        self.fneigh = np.zeros(nel * 6, dtype=np.int32)
+        L.or_facet_neigh(nel, self.pid, self.fneigh)
+        self.dfrmd = np.zeros(nel, dtype=np.int32)
+        L.or_dfrmd_el(nel, self.vxyz, self.dfrmd)
+        self.z, self.w = zwgll(lx)
+        self.d, self.dt = dgll(self.z)
+        w = self.w
+        self.w3 = np.einsum("k,j,i->kji", w, w, w).reshape(-1).copy()
+        self.dof = np.zeros(n, dtype=np.int64)
+        L.or_dof_ids(nel, lx, self.pid, self.dof)
+        self.x = np.zeros(n)
+        self.y = np.zeros(n)
+        self.zc = np.zeros(n)
+        L.or_xyz(nel, lx, self.z, self.vxyz, self.x, self.y, self.zc)
+        # gs maps
+        cap = n
+        self.dg = np.zeros(cap, dtype=np.int32)
+        self.gd = np.zeros(cap, dtype=np.int32)
+        bl = np.zeros(cap, dtype=np.int32)
+        bo = np.zeros(cap, dtype=np.int32)
+        nl, fo, nb = C.c_int(), C.c_int(), C.c_int()
+        L.or_gs_map(nel, lx, self.dof, self.fneigh, self.dg, self.gd,
+                    C.byref(nl), C.byref(fo), bl, bo, C.byref(nb))
+        self.nlocal, self.facet_offset, self.nblks = nl.value, fo.value, nb.value
+        self.dg = self.dg[: self.nlocal].copy()
+        self.gd = self.gd[: self.nlocal].copy()
+        self.blk_len = bl[: self.nblks].copy()
+        self.blk_off = bo[: self.nblks].copy()
+        self.gs_buf = np.zeros(max(self.nlocal, 1))
+        # coef
+        self.g = [np.zeros(n) for _ in range(6)]
+        self.jac = np.zeros(n)
+        self.B = np.zeros(n)
+        L.or_coef(nel, lx, self.x, self.y, self.zc, self.d, self.w3, *self.g,
+                  self.jac, self.B)
+        self.mult = np.ones(n)
+        self.gs(self.mult)
+        self.mult = 1.0 / self.mult
+        self.volume = float(np.sum(self.B))
+        self.h1 = np.full(n, float(h1))
+        self.h2 = np.full(n, float(h2))
+        self.ifh2 = h2 != 0.0
+        m = L.or_mask(nel, lx, self.fneigh, None)
+        self.msk = np.zeros(max(m, 1), dtype=np.int32)
+        L.or_mask(nel, lx, self.fneigh, self.msk.ctypes.data_as(C.c_void_p))
+        self.nmsk = m
+        self._prob = None
+
+    # ---- operations -----------------------------------------------------
+    def gs(self, u):
+        lib().or_gs_op_add(u, self.n, self.nlocal, self.facet_offset, self.dg,
+                           self.gd, self.nblks, self.blk_len, self.blk_off,
+                           self.gs_buf)
+        return u
+
+    def ax_local(self, u, h1=None):
+        w = np.zeros(self.n)
+        lib().or_ax_helm(w, u, self.d, self.dt,
+                         self.h1 if h1 is None else h1, *self.g[:3],
+                         *self.g[3:], self.nel, self.lx, self.h2, self.B,
+                         int(self.ifh2))
+        return w
+
+    def mask_apply(self, u):
+        lib().or_mask_apply(u, self.msk, self.nmsk)
+        return u
+
+    def glsc3(self, a, b, c):
+        return lib().or_glsc3(a, b, c, len(a))
+
+    def jacobi(self):
+        dd = np.zeros(self.n)
+        lib().or_jacobi_diag(dd, self.dt, *self.g, self.dfrmd, self.nel,
+                             self.lx)
+        dd = dd * self.h1
+        if self.ifh2:
+            dd = dd + self.h2 * self.B
+        self.gs(dd)
+        return 1.0 / dd
+
+    def problem(self):
+        if self._prob is None:
+            p = OrProblem()
+            p.nel, p.lx = self.nel, self.lx
+            p.d = self.d.ctypes.data
+            p.dt = self.dt.ctypes.data
+            p.h1 = self.h1.ctypes.data
+            p.h2 = self.h2.ctypes.data
+            p.B = self.B.ctypes.data
+            p.ifh2 = int(self.ifh2)
+            for name, arr in zip(("g11", "g22", "g33", "g12", "g13", "g23"),
+                                 self.g):
+                setattr(p, name, arr.ctypes.data)
+            p.mult = self.mult.ctypes.data
+            p.gs_m, p.gs_o, p.gs_nb = self.nlocal, self.facet_offset, self.nblks
+            p.dg = self.dg.ctypes.data
+            p.gd = self.gd.ctypes.data
+            p.blk_len = self.blk_len.ctypes.data
+            p.blk_off = self.blk_off.ctypes.data
+            p.gs_buf = self.gs_buf.ctypes.data
+            p.msk = self.msk.ctypes.data
+            p.nmsk = self.nmsk
+            p.volume = self.volume
+            self._prob = p
+        return self._prob
+
+    def ax_full(self, u):
+        w = np.zeros(self.n)
+        lib().or_ax_full(C.byref(self.problem()), w,
+                         np.ascontiguousarray(u, dtype=np.float64))
+        return w
+
+    def rhs_normal(self, seed):
+        """i.i.d. N(0,1) per local dof, made consistent by one gs sum, then
+        masked (the synthetic RHS of the large Poisson workload)."""
+        L = lib()
+        f = np.array([L.or_rng_normal(seed, i) for i in range(self.n)])
+        self.gs(f)
+        self.mask_apply(f)
+        return f
+
+    def rhs_sine(self):
+        """b = mask(gs(B * 3 pi^2 sin(pi x) sin(pi y) sin(pi z)))."""
+        pi = np.pi
+        f = 3 * pi * pi * np.sin(pi * self.x) * np.sin(pi * self.y) * \
+            np.sin(pi * self.zc)
+        f = self.B * f
+        self.gs(f)
+        self.mask_apply(f)
+        return f
+
+    def res_norm(self, r):
+        return float(np.sqrt(self.glsc3(r, self.mult, r)) /
+                     np.sqrt(self.volume))
+
+    def cg(self, f, abs_tol, max_iter=10000, pc=None):
+        x = np.zeros(self.n)
+        if pc is None:
+            pc = self.jacobi()
+        self._pc = pc
+        mon = lib().or_cg_solve(C.byref(self.problem()),
+                                pc.ctypes.data_as(C.c_void_p), x,
+                                np.ascontiguousarray(f), max_iter,
+                                float(abs_tol))
+        return x, mon
+
+    def gmres(self, f, abs_tol, max_iter=10000, lgmres=30, pc=None):
+        x = np.zeros(self.n)
+        pcp = None if pc is None else pc.ctypes.data_as(C.c_void_p)
+        mon = lib().or_gmres_solve(C.byref(self.problem()), pcp, x,
+                                   np.ascontiguousarray(f), max_iter,
+                                   float(abs_tol), lgmres)
+        return x, mon
