@@ -26,7 +26,7 @@ namespace nm {
 // ax_helm_cpu.f90:180-290 (generic) / :1211-1341 (lx=8); device twin
 // ax_helm_kernel.h:149-355.
 // ---------------------------------------------------------------------------
-template <typename T, int LX, int EPB>
+template <typename T, int LX, int EPB, bool PF>
 __global__ void __launch_bounds__(LX *LX *EPB)
     ax_kstep_kernel(T *__restrict__ w, const T *__restrict__ u,
                     const T *__restrict__ dx, const T *__restrict__ h1, T h1c,
@@ -36,10 +36,11 @@ __global__ void __launch_bounds__(LX *LX *EPB)
                     const T *__restrict__ h2b, int64_t nel,
                     const int *__restrict__ skip) {
   constexpr int P = LX * LX;
-  __shared__ T sD[LX * LX];
-  __shared__ T su[EPB][P];
-  __shared__ T sr[EPB][P];
-  __shared__ T ss[EPB][P];
+  constexpr int N3 = LX * P;
+  __shared__ T su[EPB][N3];     // element u
+  __shared__ T sw[EPB][N3];     // r/s part of w, per point
+  __shared__ T sr[EPB][2][P];   // r-flux plane, double buffered
+  __shared__ T ss[EPB][2][P];   // s-flux plane, double buffered
 
   if (skip != nullptr && *skip) return;
 
@@ -49,104 +50,137 @@ __global__ void __launch_bounds__(LX *LX *EPB)
   const int j = t / LX;
   const int64_t e = (int64_t)blockIdx.x * EPB + q;
   const bool active = e < nel;
+  const int64_t base = e * (int64_t)N3 + t;
 
-  for (int s = t + q * P; s < LX * LX; s += P * EPB) sD[s] = dx[s];
-
-  const int64_t base = e * (int64_t)(LX * P) + t;
-  T uc[LX], ut[LX], wc[LX];
-#pragma unroll
-  for (int k = 0; k < LX; k++) uc[k] = active ? u[base + k * P] : T(0);
-
-  __syncthreads();
-
-  // t-derivative: ut(k) = sum_l D(k,l) u(i,j,l), all in registers.
+  // u column in registers (static indices only) and the whole element in
+  // LDS for the r/s derivatives; plane k is P consecutive values, so both
+  // the loads and the LDS stores are coalesced.
+  T uc[LX];
 #pragma unroll
   for (int k = 0; k < LX; k++) {
-    T acc = T(0);
-#pragma unroll
-    for (int l = 0; l < LX; l++) acc += sD[k + LX * l] * uc[l];
-    ut[k] = acc;
+    uc[k] = active ? u[base + k * P] : T(0);
+    su[q][k * P + t] = uc[k];
   }
 
+  // rows D(i,:), D(j,:) and columns D(:,i), D(:,j) of the thread
+  T Di[LX], Dj[LX], Dti[LX], Dtj[LX];
 #pragma unroll
+  for (int l = 0; l < LX; l++) {
+    Di[l] = dx[i + LX * l];
+    Dj[l] = dx[j + LX * l];
+    Dti[l] = dx[l + LX * i];
+    Dtj[l] = dx[l + LX * j];
+  }
+  // t-direction transpose accumulates in registers (scatter form)
+  T wc[LX];
+#pragma unroll
+  for (int m = 0; m < LX; m++) wc[m] = T(0);
+  __syncthreads();
+
+  // Plane loop kept rolled: the working set per iteration is one plane, so
+  // the register footprint stays small enough for several waves per SIMD.
+  T nG[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  T nh = h1c;
+  if (PF && active) {
+    nG[0] = g11[base]; nG[1] = g22[base]; nG[2] = g33[base];
+    nG[3] = g12[base]; nG[4] = g13[base]; nG[5] = g23[base];
+    if (h1 != nullptr) nh = h1[base];
+  }
+#pragma unroll 1
   for (int k = 0; k < LX; k++) {
-    su[q][t] = uc[k];
-    __syncthreads();
-    T ur = T(0), us = T(0);
-#pragma unroll
-    for (int l = 0; l < LX; l++) {
-      ur += sD[i + LX * l] * su[q][l + LX * j];
-      us += sD[j + LX * l] * su[q][i + LX * l];
-    }
-    const int64_t p = base + k * P;
+    const int b = k & 1;
+    const int64_t p = base + (int64_t)k * P;
     T G11 = T(0), G22 = T(0), G33 = T(0), G12 = T(0), G13 = T(0), G23 = T(0);
     T hh = h1c;
-    if (active) {
+    if (PF) {
+      // software pipelining: this plane's factors were loaded one plane
+      // ahead; issue the next plane's loads now
+      G11 = nG[0]; G22 = nG[1]; G33 = nG[2]; G12 = nG[3]; G13 = nG[4]; G23 = nG[5];
+      hh = nh;
+      if (active && k + 1 < LX) {
+        const int64_t pn = p + P;
+        nG[0] = g11[pn]; nG[1] = g22[pn]; nG[2] = g33[pn];
+        nG[3] = g12[pn]; nG[4] = g13[pn]; nG[5] = g23[pn];
+        if (h1 != nullptr) nh = h1[pn];
+      }
+    } else if (active) {
       G11 = g11[p]; G22 = g22[p]; G33 = g33[p];
       G12 = g12[p]; G13 = g13[p]; G23 = g23[p];
       if (h1 != nullptr) hh = h1[p];
     }
-    const T vt = ut[k];
+    // D(k,:) is wave-uniform (scalar loads)
+    T dk[LX];
+#pragma unroll
+    for (int l = 0; l < LX; l++) dk[l] = dx[k + LX * l];
+    T vt = T(0), ur = T(0), us = T(0);
+    const T *plane = &su[q][k * P];
+#pragma unroll
+    for (int l = 0; l < LX; l++) {
+      vt += dk[l] * uc[l];
+      ur += Di[l] * plane[l + LX * j];
+      us += Dj[l] * plane[i + LX * l];
+    }
     const T rr = hh * (G11 * ur + G12 * us + G13 * vt);
     const T rs = hh * (G12 * ur + G22 * us + G23 * vt);
     const T rt = hh * (G13 * ur + G23 * us + G33 * vt);
-    ut[k] = rt;
-    sr[q][t] = rr;
-    ss[q][t] = rs;
+    sr[q][b][t] = rr;
+    ss[q][b][t] = rs;
+#pragma unroll
+    for (int m = 0; m < LX; m++) wc[m] += dk[m] * rt;
     __syncthreads();
     T acc = T(0);
 #pragma unroll
-    for (int l = 0; l < LX; l++) acc += sD[l + LX * i] * sr[q][l + LX * j];
+    for (int l = 0; l < LX; l++) acc += Dti[l] * sr[q][b][l + LX * j];
 #pragma unroll
-    for (int l = 0; l < LX; l++) acc += sD[l + LX * j] * ss[q][i + LX * l];
-    wc[k] = acc;
-    __syncthreads();
+    for (int l = 0; l < LX; l++) acc += Dtj[l] * ss[q][b][i + LX * l];
+    sw[q][k * P + t] = acc;
   }
 
-#pragma unroll
-  for (int k = 0; k < LX; k++) {
-    T acc = wc[k];
-#pragma unroll
-    for (int l = 0; l < LX; l++) acc += sD[l + LX * k] * ut[l];
-    wc[k] = acc;
-  }
   if (active) {
-    if (h2b != nullptr) {
 #pragma unroll
-      for (int k = 0; k < LX; k++) wc[k] += h2b[base + k * P] * uc[k];
+    for (int m = 0; m < LX; m++) {
+      T r = wc[m] + sw[q][m * P + t];
+      if (h2b != nullptr) r += h2b[base + m * P] * uc[m];
+      w[base + m * P] = r;
     }
-#pragma unroll
-    for (int k = 0; k < LX; k++) w[base + k * P] = wc[k];
   }
 }
 
-template <typename T, int LX, int EPB>
+template <typename T, int LX, int EPB, bool PF>
 static int ax_go(const AxArgs<T> &a, hipStream_t s) {
   dim3 block(LX * LX, EPB);
   dim3 grid((unsigned)((a.nel + EPB - 1) / EPB));
   if (a.nel <= 0) return NM_OK;
-  hipLaunchKernelGGL((ax_kstep_kernel<T, LX, EPB>), grid, block, 0, s, a.w,
+  hipLaunchKernelGGL((ax_kstep_kernel<T, LX, EPB, PF>), grid, block, 0, s, a.w,
                      a.u, a.dx, a.h1, a.h1c, a.g[0], a.g[1], a.g[2], a.g[3],
                      a.g[4], a.g[5], a.h2b, a.nel, a.skip);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
 
+// Variant code: epb = elements per workgroup (1,2,4,8,16; 0 = default),
+// plus 100 for the G-prefetching (software-pipelined) variant.
 template <typename T, int LX>
-static int ax_epb(const AxArgs<T> &a, int epb, hipStream_t s) {
+static int ax_epb(const AxArgs<T> &a, int code, hipStream_t s) {
   constexpr int P = LX * LX;
+  const bool pf = code >= 100;
+  int epb = code % 100;
   if (epb <= 0) {
     epb = 1;
     while (P * epb * 2 <= 64 && epb < 16) epb *= 2;
   }
+#define NM_AX_CASE(E)                                         \
+  case E:                                                     \
+    if constexpr (P * E <= 256) {                             \
+      return pf ? ax_go<T, LX, E, true>(a, s)                 \
+                : ax_go<T, LX, E, false>(a, s);               \
+    }                                                         \
+    break;
   switch (epb) {
-    case 1: return ax_go<T, LX, 1>(a, s);
-    case 2: if constexpr (P * 2 <= 256) return ax_go<T, LX, 2>(a, s); break;
-    case 4: if constexpr (P * 4 <= 256) return ax_go<T, LX, 4>(a, s); break;
-    case 8: if constexpr (P * 8 <= 256) return ax_go<T, LX, 8>(a, s); break;
-    case 16: if constexpr (P * 16 <= 256) return ax_go<T, LX, 16>(a, s); break;
+    NM_AX_CASE(1) NM_AX_CASE(2) NM_AX_CASE(4) NM_AX_CASE(8) NM_AX_CASE(16)
     default: break;
   }
+#undef NM_AX_CASE
   set_error("ax: elements-per-workgroup %d unsupported for lx=%d", epb, LX);
   return NM_ERR_UNSUPPORTED;
 }
