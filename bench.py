@@ -78,6 +78,12 @@ def main():
     stream = torch.cuda.current_stream()
     u = p.rhs_normal(7)
     w = torch.empty_like(u)
+    tuned = p.autotune(reps=5)
+    log(f"[bench] autotuned variants {tuned}")
+    # reference copy kernel of known size (8n read + 8n write) for scaling
+    # the profiler's FETCH_SIZE / WRITE_SIZE counters
+    w.copy_(u)
+    torch.cuda.synchronize()
 
     # ---- Ax+gs steps -------------------------------------------------------
     for _ in range(args.warmup):
@@ -133,6 +139,7 @@ def main():
                         f"+-0.1h perturbed, Dirichlet",
             "elements": E ** 3, "lx": lx, "local_dof": n,
             "gs_entries": info.nlocal, "parallelism": "single GPU",
+            "kernel_variants": tuned,
         },
         "axgs": {"ms": round(ms_step, 4), "bytes_per_apply": axgs_bytes,
                  "GBs": round(axgs_bytes / (ms_step * 1e-3) / 1e9, 1),

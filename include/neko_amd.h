@@ -178,7 +178,15 @@ int nm_op_info(const nm_op_t *op, nm_op_info_t *info);
 const void *nm_op_field(const nm_op_t *op, int which);
 int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
                   int32_t *blk_len, int32_t *blk_off);
+/* Ax variant code: elements per workgroup (1,2,4,8,16; 0 = default) plus
+ * 100 for the G-prefetching variant; gs variant 0 (thread per block) or 1
+ * (lane-parallel, blocks <= 8 entries). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
+int nm_op_set_gs_variant(nm_op_t *op, int variant);
+/* autotuner: times all Ax (FP64, FP32) and gs variants on the problem's own
+ * data and keeps the fastest; returns the chosen codes */
+int nm_op_autotune(nm_op_t *op, int reps, int *best64, int *best32,
+                   int *best_gs, hipStream_t s);
 /* w = A_local u ; w = gs(A_local u) ; w = mask(gs(A_local u)) (FP64) */
 int nm_op_ax_local(const nm_op_t *op, double *w, const double *u,
                    hipStream_t s);

@@ -79,6 +79,8 @@ SIGNATURES = {
     "nm_op_field": (_vp, [_vp, _i]),
     "nm_op_gs_maps": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "nm_op_set_epb": (_i, [_vp, _i, _i]),
+    "nm_op_set_gs_variant": (_i, [_vp, _i]),
+    "nm_op_autotune": (_i, [_vp, _i, _pi, _pi, _pi, _vp]),
     "nm_op_ax_local": (_i, [_vp, _vp, _vp, _vp]),
     "nm_op_ax_gs": (_i, [_vp, _vp, _vp, _vp]),
     "nm_op_ax": (_i, [_vp, _vp, _vp, _vp]),

@@ -43,6 +43,7 @@ struct GsArgs {
   int64_t pair0;           // offset of the first facet pair in gidx
   int64_t npairs;
   const int *skip;
+  int variant = 0;         // 0 thread per block/pair, 1 lane-parallel (blocks <= 8)
 };
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s);

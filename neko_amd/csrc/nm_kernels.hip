@@ -209,6 +209,21 @@ int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s) {
 template int launch_ax<double>(const AxArgs<double> &, int, hipStream_t);
 template int launch_ax<float>(const AxArgs<float> &, int, hipStream_t);
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2), so blockIdx b runs on XCD b % 8.  Remapping
+// b -> (b % 8) * per + b / 8 hands every XCD one contiguous range of the
+// logical work, so the element cache lines a range of gs entries shares stay
+// in one L2 instead of being fetched by all eight.  The launch must use a
+// grid of 8 * per blocks; logical blocks >= the real count exit.
+constexpr int NM_NXCD = 8;
+__device__ inline int64_t xcd_block(int64_t per) {
+  const int64_t b = blockIdx.x;
+  return (b % NM_NXCD) * per + b / NM_NXCD;
+}
+static inline int64_t xcd_per(int64_t nblocks) {
+  return (nblocks + NM_NXCD - 1) / NM_NXCD;
+}
+
 // ---------------------------------------------------------------------------
 // Gather-scatter ADD, fused: one thread per gs block (a run of local dofs
 // with the same gs id) or per facet pair.  Block sums are taken in the
@@ -221,9 +236,9 @@ __global__ void __launch_bounds__(256)
     gs_fused_kernel(T *__restrict__ u, const int32_t *__restrict__ blk_ptr,
                     const int32_t *__restrict__ gidx, int64_t nblks,
                     int64_t pair0, int64_t npairs,
-                    const int *__restrict__ skip) {
+                    const int *__restrict__ skip, int64_t per) {
   if (skip != nullptr && *skip) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
   if (t < nblks) {
     const int32_t s0 = blk_ptr[t], s1 = blk_ptr[t + 1];
     T acc = u[gidx[s0]];
@@ -238,14 +253,79 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Lane-parallel variant: one 8-lane group per block (blocks of at most 8
+// entries) and one lane per facet-pair entry, so the index loads are
+// coalesced and every entry's value load is issued by its own lane.  The
+// group sums its values in map order through lane shuffles (same order as
+// the thread-per-block kernel, hence the same bits).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gs_blk8_kernel(T *__restrict__ u, const int32_t *__restrict__ blk_ptr,
+                   const int32_t *__restrict__ gidx, int64_t nblks,
+                   const int *__restrict__ skip, int64_t per) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
+  const int64_t g = t >> 3;
+  const int m = (int)(t & 7);
+  const int lane0 = (int)(threadIdx.x & 63) & ~7;
+  int32_t s0 = 0, len = 0, idx = 0;
+  T v = T(0);
+  if (g < nblks) {
+    s0 = blk_ptr[g];
+    len = blk_ptr[g + 1] - s0;
+    if (m < len) {
+      idx = gidx[s0 + m];
+      v = u[idx];
+    }
+  }
+  T acc = __shfl(v, lane0, 64);
+#pragma unroll
+  for (int q = 1; q < 8; q++) {
+    const T o = __shfl(v, lane0 + q, 64);
+    if (q < len) acc += o;
+  }
+  if (g < nblks && m < len) u[idx] = acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gs_pair_kernel(T *__restrict__ u, const int32_t *__restrict__ gidx,
+                   int64_t pair0, int64_t npairs, const int *__restrict__ skip,
+                   int64_t per) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
+  const bool act = t < 2 * npairs;
+  int32_t idx = 0;
+  T v = T(0);
+  if (act) {
+    idx = gidx[pair0 + t];
+    v = u[idx];
+  }
+  const T o = __shfl_xor(v, 1, 64);
+  if (act) u[idx] = v + o;  // a + b == b + a exactly
+}
+
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s) {
   const int64_t items = a.nblks + a.npairs;
   if (items <= 0) return NM_OK;
   const int bs = 256;
-  hipLaunchKernelGGL(gs_fused_kernel<T>, dim3((unsigned)((items + bs - 1) / bs)),
-                     dim3(bs), 0, s, a.u, a.blk_ptr, a.gidx, a.nblks, a.pair0,
-                     a.npairs, a.skip);
+  if (a.variant == 1) {
+    if (a.nblks > 0) {
+      const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
+      hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per);
+    }
+    if (a.npairs > 0) {
+      const int64_t per = xcd_per((2 * a.npairs + bs - 1) / bs);
+      hipLaunchKernelGGL(gs_pair_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                         a.u, a.gidx, a.pair0, a.npairs, a.skip, per);
+    }
+  } else {
+    const int64_t per = xcd_per((items + bs - 1) / bs);
+    hipLaunchKernelGGL(gs_fused_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                       a.u, a.blk_ptr, a.gidx, a.nblks, a.pair0, a.npairs, a.skip, per);
+  }
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }

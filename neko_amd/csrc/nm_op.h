@@ -44,7 +44,9 @@ struct Op {
   OpFields<double> f64;
   OpFields<float> f32;
   bool has32 = false;
-  int epb64 = 0, epb32 = 0;  // Ax elements per workgroup (0 = default)
+  int epb64 = 0, epb32 = 0;  // Ax variant codes (0 = default)
+  int gs_variant = 0;        // gather-scatter kernel variant
+  int max_blk = 0;           // longest gs block
   double *red_part = nullptr;   // RED_BLOCKS * 4 doubles
   double *red_out = nullptr;    // 4 doubles
   std::vector<void *> owned;    // every device allocation of this op

@@ -58,6 +58,7 @@ template int32_t *Op::alloc<int32_t>(int64_t);
 template <typename T>
 int Op::gs(T *u, const int *skip, hipStream_t s) const {
   GsArgs<T> a{u, blk_ptr, gidx, nblks, pair0, npairs, skip};
+  a.variant = (gs_variant == 1 && max_blk <= 8) ? 1 : 0;
   return launch_gs<T>(a, s);
 }
 template int Op::gs<double>(double *, const int *, hipStream_t) const;
@@ -132,7 +133,11 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
     return NM_ERR_ARG;
   }
   std::vector<int32_t> ptr(nblks + 1), gi(nlocal > 0 ? nlocal : 1);
-  for (int64_t b = 0; b < nblks; b++) ptr[b] = blk_off[b];
+  op->max_blk = 0;
+  for (int64_t b = 0; b < nblks; b++) {
+    ptr[b] = blk_off[b];
+    if (blk_len[b] > op->max_blk) op->max_blk = blk_len[b];
+  }
   ptr[nblks] = nblks > 0 ? blk_off[nblks - 1] + blk_len[nblks - 1] : 0;
   if (ptr[nblks] != facet_offset - 1) {
     set_error("gs: blocks end at %d, facet offset %lld", ptr[nblks],
@@ -464,6 +469,76 @@ int nm_op_gs_maps(const nm_op_t *h, int32_t *dg, int32_t *gd, int32_t *blk_len,
 int nm_op_set_epb(nm_op_t *h, int epb64, int epb32) {
   h->op.epb64 = epb64;
   h->op.epb32 = epb32;
+  return NM_OK;
+}
+
+int nm_op_set_gs_variant(nm_op_t *h, int variant) {
+  h->op.gs_variant = variant;
+  return NM_OK;
+}
+
+// Autotuner: time every Ax variant (elements per workgroup x G prefetch)
+// in FP64 and FP32 and every gs variant on this problem's own data, keep
+// the fastest.  Results (codes and ms) are reported through the arrays
+// (up to 16 entries each, may be NULL).  All variants compute identical
+// results (gs) or results within rounding of each other (Ax).
+int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
+                   int *best_gs, hipStream_t s) {
+  Op &op = h->op;
+  int st = op.ensure32(s);
+  if (st) return st;
+  if (reps <= 0) reps = 5;
+  const int64_t n = op.n;
+  double *u = nullptr, *w = nullptr;
+  NM_HIP_TRY(hipMalloc(&u, sizeof(double) * n));
+  NM_HIP_TRY(hipMalloc(&w, sizeof(double) * n));
+  hipEvent_t e0, e1;
+  NM_HIP_TRY(hipEventCreate(&e0));
+  NM_HIP_TRY(hipEventCreate(&e1));
+  launch_vec<double>(V_CFILL, u, nullptr, nullptr, nullptr, 1.0, n, s);
+  const int codes[] = {0, 2, 4, 100, 102, 104};
+  auto timeit = [&](auto fn) -> float {
+    for (int r = 0; r < 2; r++) fn();
+    hipEventRecord(e0, s);
+    for (int r = 0; r < reps; r++) fn();
+    hipEventRecord(e1, s);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms / reps;
+  };
+  float b64 = 1e30f, b32 = 1e30f;
+  int c64 = 0, c32 = 0;
+  for (int c : codes) {
+    op.epb64 = c;
+    op.epb32 = c;
+    if (op.apply<double>(w, u, nullptr, s, false, false) != NM_OK) continue;
+    const float t64 = timeit([&] { op.apply<double>(w, u, nullptr, s, false, false); });
+    const float t32 = timeit([&] {
+      op.apply<float>((float *)w, (const float *)u, nullptr, s, false, false);
+    });
+    if (t64 < b64) { b64 = t64; c64 = c; }
+    if (t32 < b32) { b32 = t32; c32 = c; }
+  }
+  op.epb64 = c64;
+  op.epb32 = c32;
+  float bg = 1e30f;
+  int cg = 0;
+  for (int v = 0; v < 2; v++) {
+    if (v == 1 && op.max_blk > 8) continue;
+    op.gs_variant = v;
+    const float tg = timeit([&] { op.gs<double>(w, nullptr, s); });
+    if (tg < bg) { bg = tg; cg = v; }
+  }
+  op.gs_variant = cg;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(u);
+  hipFree(w);
+  if (best64) *best64 = c64;
+  if (best32) *best32 = c32;
+  if (best_gs) *best_gs = cg;
+  NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
 

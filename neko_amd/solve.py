@@ -184,6 +184,16 @@ class Problem:
     def set_epb(self, epb64=0, epb32=0):
         check(lib().nm_op_set_epb(self.h, int(epb64), int(epb32)), "set_epb")
 
+    def set_gs_variant(self, v):
+        check(lib().nm_op_set_gs_variant(self.h, int(v)), "set_gs_variant")
+
+    def autotune(self, reps=5):
+        """Pick the fastest Ax / gs kernel variants on this problem."""
+        a, b, g = C.c_int(), C.c_int(), C.c_int()
+        check(lib().nm_op_autotune(self.h, int(reps), C.byref(a), C.byref(b),
+                                   C.byref(g), _stream()), "autotune")
+        return {"ax64": a.value, "ax32": b.value, "gs": g.value}
+
     # ---- operations (Neko names) ----------------------------------------
     def ax_local(self, w: torch.Tensor, u: torch.Tensor):
         """ax_t%compute: w = A_local u (no assembly)."""

@@ -80,16 +80,45 @@ def test_ax_h1_field_and_h2(lx):
     assert _relmax(_np(w), c.ax_local(u)) < 1e-12
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("case", CASES[:7])
-def test_gs_bit_exact(case):
+def test_gs_bit_exact(case, variant):
     nx, ny, nz, lx, amp, per = case
     c = O.OracleCase(nx, ny, nz, lx, amp=amp, seed=3, periodic=per)
     p = _problem_from_oracle(c)
+    p.set_gs_variant(variant)
     u = np.random.default_rng(1).standard_normal(c.n)
     ud = _t(u)
     p.gs_op(ud)
     ref = c.gs(u.copy())
     assert np.array_equal(_np(ud), ref)
+
+
+@pytest.mark.parametrize("code", [0, 2, 4, 100, 102, 104])
+def test_ax_variants_match_oracle(code):
+    c = O.OracleCase(3, 3, 2, 8, amp=0.1, seed=12)
+    p = _problem_from_oracle(c)
+    p.set_epb(code, code)
+    u = np.random.default_rng(2).standard_normal(c.n)
+    w = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    p.ax_local(w, _t(u))
+    assert _relmax(_np(w), c.ax_local(u)) < 1e-12
+    # FP32 variant: within single-precision rounding of the FP64 oracle
+    w32 = torch.zeros(c.n, dtype=torch.float32, device="cuda")
+    p.ax_local(w32, _t(u, torch.float32))
+    assert _relmax(_np(w32).astype(np.float64), c.ax_local(u)) < 2e-5
+
+
+def test_autotune_keeps_results():
+    c = O.OracleCase(4, 3, 3, 8, amp=0.1, seed=13)
+    p = _problem_from_oracle(c)
+    choice = p.autotune(reps=2)
+    assert set(choice) == {"ax64", "ax32", "gs"}
+    u = np.random.default_rng(3).standard_normal(c.n)
+    ud = _t(u)
+    w = torch.zeros_like(ud)
+    p.ax_full(w, ud)
+    assert _relmax(_np(w), c.ax_full(u)) < 1e-12
 
 
 def test_dropin_gather_scatter_bit_exact():
