@@ -106,6 +106,8 @@ void nm_jacobi_update(void *d, void *dxt, void *G11, void *G22, void *G33,
  * ====================================================================== */
 typedef struct nm_op_s nm_op_t;
 typedef struct nm_cg_s nm_cg_t;
+typedef struct nm_hsmg_s nm_hsmg_t;
+typedef struct nm_gmres_s nm_gmres_t;
 
 typedef struct {
   int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
@@ -114,6 +116,12 @@ typedef struct {
   double h1c;
   int ifh2;
 } nm_op_info_t;
+
+typedef struct {
+  int iter, outer;
+  double res_start, res_final;
+  int converged;
+} nm_gmres_monitor_t;
 
 typedef struct {
   int iter;          /* CG iterations (mixed: total inner FP32 iterations) */
@@ -196,6 +204,9 @@ int nm_op_ax_local32(nm_op_t *op, float *w, const float *u, hipStream_t s);
 int nm_op_ax_gs32(nm_op_t *op, float *w, const float *u, hipStream_t s);
 int nm_op_gs(const nm_op_t *op, double *u, hipStream_t s);
 int nm_op_mask(const nm_op_t *op, double *u, hipStream_t s);
+/* Jacobi diagonal with every element treated as deformed (the coarse grid
+ * of pc_hsmg.f90:312-318 sets msh%dfrmd_el = .true.) */
+int nm_op_set_all_deformed(nm_op_t *op, hipStream_t s);
 
 /* --- Jacobi-PCG (ksp_t 'cg' + pc_t 'jacobi') ----------------------------- */
 /* precision 0: FP64; 1: FP32 inner CG inside FP64 iterative refinement */
@@ -204,6 +215,32 @@ void nm_cg_free(nm_cg_t *cg);
 int nm_cg_solve(nm_cg_t *cg, double *x, const double *f, double abs_tol,
                 int max_iter, double inner_rel, nm_monitor_t *mon,
                 hipStream_t stream);
+
+/* --- hsmg preconditioner (pc_t 'hsmg', src/krylov/pc_hsmg.f90) ---------- */
+/* Three-level V-cycle lx -> lx_mid (4, or max(lx-1,3) below lx 5) -> 2 on
+ * the mesh of a box operator: overlapping Schwarz with FDM local solves
+ * (src/math/schwarz.f90:112-634, src/math/fdm.f90:110-665) on the fine and
+ * middle levels, `niter` Jacobi-CG iterations (abs. tol 1e-9) on the
+ * coarse level (pc_hsmg.f90:147-365 init, :481-627 solve, CPU branch).
+ * facet_bc: facet_type of the boundary facets (1 pressure, 2 velocity;
+ * src/fluid/fluid_pnpn.f90 boundary set-up). */
+int nm_hsmg_create(nm_hsmg_t **h, nm_op_t *op, int facet_bc, int niter,
+                   hipStream_t s);
+void nm_hsmg_free(nm_hsmg_t *h);
+/* z = M^{-1} r (precon.f90 pc_solve interface: solve(z, r, n)); r is used
+ * as scratch copy only, the input stays untouched */
+int nm_hsmg_apply(nm_hsmg_t *h, double *z, const double *r, hipStream_t s);
+/* fine-level Schwarz smoother alone: e = schwarz(r), r masked in place
+ * (schwarz_compute, schwarz.f90:565-595) */
+int nm_hsmg_schwarz_fine(nm_hsmg_t *h, double *e, double *r, hipStream_t s);
+
+/* --- GMRES(m) (ksp_t 'gmres', src/krylov/bcknd/cpu/gmres.f90:175-411) --- */
+/* pc: 0 Jacobi (the operator's diagonal), 1 hsmg (handle), 2 identity */
+int nm_gmres_create(nm_gmres_t **g, nm_op_t *op, int pc, nm_hsmg_t *hsmg,
+                    int lgmres);
+void nm_gmres_free(nm_gmres_t *g);
+int nm_gmres_solve(nm_gmres_t *g, double *x, const double *f, double abs_tol,
+                   int max_iter, nm_gmres_monitor_t *mon, hipStream_t s);
 
 /* ordered copy helper for host harnesses (kind: hipMemcpyKind) */
 int nm_memcpy(void *dst, const void *src, size_t bytes, int kind,

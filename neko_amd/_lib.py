@@ -88,6 +88,14 @@ SIGNATURES = {
     "nm_op_ax_gs32": (_i, [_vp, _vp, _vp, _vp]),
     "nm_op_gs": (_i, [_vp, _vp, _vp]),
     "nm_op_mask": (_i, [_vp, _vp, _vp]),
+    "nm_op_set_all_deformed": (_i, [_vp, _vp]),
+    "nm_hsmg_create": (_i, [C.POINTER(_vp), _vp, _i, _i, _vp]),
+    "nm_hsmg_free": (None, [_vp]),
+    "nm_hsmg_apply": (_i, [_vp, _vp, _vp, _vp]),
+    "nm_hsmg_schwarz_fine": (_i, [_vp, _vp, _vp, _vp]),
+    "nm_gmres_create": (_i, [C.POINTER(_vp), _vp, _i, _vp, _i]),
+    "nm_gmres_free": (None, [_vp]),
+    "nm_gmres_solve": (_i, [_vp, _vp, _vp, _d, _i, C.POINTER(Monitor), _vp]),
     "nm_cg_create": (_i, [C.POINTER(_vp), _vp, _i]),
     "nm_cg_free": (None, [_vp]),
     "nm_cg_solve": (_i, [_vp, _vp, _vp, _d, _i, _d, C.POINTER(Monitor), _vp]),

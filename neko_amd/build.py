@@ -24,10 +24,12 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-I", CSRC, "-I", os.path.join(ROOT, "include"),
 ]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-pthread", "-ffp-contract=off",
-             "-I", CSRC, "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
+             "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
+             "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["nm_kernels.hip", "nm_op.hip", "nm_cg.hip", "nm_abi.hip"]
-CXX_SOURCES = ["nm_setup.cpp"]
+HIP_SOURCES = ["nm_kernels.hip", "nm_op.hip", "nm_cg.hip", "nm_abi.hip", "nm_hsmg.hip",
+               "nm_gmres.hip"]
+CXX_SOURCES = ["nm_setup.cpp", "nm_fdm_setup.cpp"]
 
 
 def _run(cmd):

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "neko_amd.h"
 #include "nm_common.h"
 #include "nm_kernels.h"
 

@@ -272,6 +272,65 @@ static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
   return NM_OK;
 }
 
+// Fixed-budget Jacobi-PCG without any host synchronisation: init plus
+// `niter` iterations are enqueued; every kernel turns into a no-op once the
+// device flag is set (converged below abs_tol or niter reached).  This is
+// the coarse-grid solve of hsmg (pc_hsmg.f90:598-608: CG, Jacobi, 10
+// iterations, Neko's default absolute tolerance).
+struct CgFixed {
+  const Op *op = nullptr;
+  double *r = nullptr, *p = nullptr, *w = nullptr, *part = nullptr;
+  CgScal *sc = nullptr;
+  std::vector<void *> owned;
+  ~CgFixed() {
+    for (void *q : owned) (void)hipFree(q);
+  }
+};
+
+int cg_fixed_create(CgFixed **out, const Op *op) {
+  CgFixed *c = new CgFixed;
+  c->op = op;
+  auto al = [&](void **q, size_t bytes) {
+    if (hipMalloc(q, bytes) != hipSuccess) return false;
+    c->owned.push_back(*q);
+    return true;
+  };
+  const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(op->n, 1);
+  if (!al((void **)&c->r, nb) || !al((void **)&c->p, nb) || !al((void **)&c->w, nb) ||
+      !al((void **)&c->part, sizeof(double) * 2 * RED_BLOCKS) ||
+      !al((void **)&c->sc, sizeof(CgScal))) {
+    delete c;
+    set_error("cg_fixed_create: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  *out = c;
+  return NM_OK;
+}
+
+void cg_fixed_free(CgFixed *c) { delete c; }
+
+// x = CG(A, f) with at most niter iterations, all on stream s.
+int cg_fixed_solve(CgFixed *c, double *x, const double *f, int niter,
+                   double abs_tol, hipStream_t s) {
+  const Op &op = *c->op;
+  const int64_t n = op.n;
+  CgScal init{};
+  init.norm_fac = 1.0 / std::sqrt(op.volume);
+  init.scale = 1.0;
+  NM_HIP_TRY(hipMemcpyAsync(c->sc, &init, sizeof(CgScal), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((cg_init_kernel<double, double, double>), dim3(RED_BLOCKS),
+                     dim3(RED_THREADS), 0, s, x, c->r, c->p, f, op.f64.mult,
+                     op.f64.dinv, n, (const CgScal *)nullptr, c->part);
+  hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, c->part,
+                     c->sc, abs_tol, 0.0, niter);
+  for (int k = 0; k < niter; k++) {
+    int st = cg_iteration<double, double>(op, x, c->r, c->p, c->w, c->sc, c->part, s);
+    if (st) return st;
+  }
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
 struct CgSolver {
   Op *op = nullptr;
   int mixed = 0;
@@ -347,17 +406,6 @@ struct nm_cg_s {
 };
 
 extern "C" {
-
-typedef struct nm_op_s nm_op_t;
-typedef struct nm_cg_s nm_cg_t;
-
-typedef struct {
-  int iter;          // CG iterations (mixed: total inner FP32 iterations)
-  int outer;         // refinement sweeps (mixed), 0 for FP64
-  double res_start;  // Neko-normalised residual norms
-  double res_final;
-  int converged;
-} nm_monitor_t;
 
 // precision: 0 = FP64 throughout; 1 = FP32 inner CG inside FP64 refinement.
 int nm_cg_create(nm_cg_t **out, nm_op_t *oph, int precision) {

@@ -1,0 +1,604 @@
+// hsmg preconditioner on the device and GMRES(m) around it.
+//
+// hsmg: three-level p-multigrid V-cycle (lx -> lx_mid -> 2) with overlapping
+// Schwarz smoothing (overlap one GLL point, fast-diagonalisation local
+// solves on the (lx+2)^3 extended element) on the fine and middle levels,
+// and a fixed-budget Jacobi-CG on the coarse level (pc_hsmg.f90:147-627,
+// schwarz.f90:112-634, fdm.f90:110-665).  GMRES(m) with right
+// preconditioning and classical Gram-Schmidt (gmres.f90:175-411).
+//
+// Kernels: one workgroup per element for the element-local tensor work
+// (Schwarz extrusion, FDM, interpolation), LDS-staged; gs and BLAS-1 reuse
+// the operator kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nm_common.h"
+#include "nm_kernels.h"
+#include "nm_op.h"
+
+// host set-up helpers of nm_fdm_setup.cpp (internal)
+extern "C" {
+int nm_fdm_setup_host(int64_t nel, int lx, const double *x, const double *y,
+                      const double *z, const double *w, const double *d1,
+                      const int *facet_type, void (*gs_l)(void *, double *),
+                      void *gs_ctx, double *s, double *d);
+int nm_interp_matrix(const double *z_to, int n_to, const double *z_from,
+                     int n_from, double *j, double *jt);
+}
+
+namespace nm {
+
+// ------------------------------------------------------------------ kernels
+__device__ inline int64_t i3(int i, int j, int k, int n) {
+  return i + (int64_t)n * (j + (int64_t)n * k);
+}
+
+// a (ext, (n+2)^3) = 0 with the interior set to b (n^3)  (schwarz_toext3d)
+__global__ void toext_kernel(double *__restrict__ a, const double *__restrict__ b,
+                             int n, int64_t nel) {
+  const int m = n + 2;
+  const int64_t m3 = (int64_t)m * m * m;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * m3) return;
+  const int64_t e = idx / m3;
+  const int p = (int)(idx % m3);
+  const int i = p % m, j = (p / m) % m, k = p / (m * m);
+  double v = 0.0;
+  if (i >= 1 && i <= n && j >= 1 && j <= n && k >= 1 && k <= n)
+    v = b[e * (int64_t)n * n * n + i3(i - 1, j - 1, k - 1, n)];
+  a[idx] = v;
+}
+
+// b (n^3) = interior of a  (schwarz_toreg3d)
+__global__ void toreg_kernel(double *__restrict__ b, const double *__restrict__ a,
+                             int n, int64_t nel) {
+  const int m = n + 2;
+  const int64_t n3 = (int64_t)n * n * n;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * n3) return;
+  const int64_t e = idx / n3;
+  const int p = (int)(idx % n3);
+  const int i = p % n, j = (p / n) % n, k = p / (n * n);
+  b[idx] = a[e * (int64_t)m * m * m + i3(i + 1, j + 1, k + 1, m)];
+}
+
+// schwarz_extrude: face layers of arr1 at depth l1 = f1*arr1 + f2*arr2 at
+// depth l2, face interiors only; r-faces, then s-faces, then t-faces (the
+// reference's loop order, which matters where the layers of two
+// directions meet).  One workgroup per element.
+__global__ void __launch_bounds__(256)
+    extrude_kernel(double *arr1, int l1, double f1, const double *arr2, int l2,
+                   double f2, int nx, int64_t nel) {
+  const int64_t e = blockIdx.x;
+  if (e >= nel) return;
+  double *a = arr1 + e * (int64_t)nx * nx * nx;
+  const double *b = arr2 + e * (int64_t)nx * nx * nx;
+  const int w = nx - 2;  // face-interior width
+  const int nf = w * w;
+  const int h1 = nx - 1 - l1, h2 = nx - 1 - l2;
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    const int j = 1 + q % w, k = 1 + q / w;
+    a[i3(l1, j, k, nx)] = f1 * a[i3(l1, j, k, nx)] + f2 * b[i3(l2, j, k, nx)];
+    a[i3(h1, j, k, nx)] = f1 * a[i3(h1, j, k, nx)] + f2 * b[i3(h2, j, k, nx)];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    const int i = 1 + q % w, k = 1 + q / w;
+    a[i3(i, l1, k, nx)] = f1 * a[i3(i, l1, k, nx)] + f2 * b[i3(i, l2, k, nx)];
+    a[i3(i, h1, k, nx)] = f1 * a[i3(i, h1, k, nx)] + f2 * b[i3(i, h2, k, nx)];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    const int i = 1 + q % w, j = 1 + q / w;
+    a[i3(i, j, l1, nx)] = f1 * a[i3(i, j, l1, nx)] + f2 * b[i3(i, j, l2, nx)];
+    a[i3(i, j, h1, nx)] = f1 * a[i3(i, j, h1, nx)] + f2 * b[i3(i, j, h2, nx)];
+  }
+}
+
+// Element-local tensor product v = (C (x) B (x) A) u with per-call small
+// matrices: v(i,j,l) = sum A(i,a) Bt(b,j) Ct(c,l) u(a,b,c) (tnsr3d);
+// matrices may be per element (stride > 0).  One workgroup per element,
+// everything staged in LDS.
+template <int MAXN>
+__global__ void __launch_bounds__(256)
+    tnsr3d_kernel(double *__restrict__ v, int nv, const double *__restrict__ u,
+                  int nu, const double *__restrict__ A, const double *__restrict__ Bt,
+                  const double *__restrict__ Ct, int64_t mstride, int64_t nel) {
+  __shared__ double su[MAXN * MAXN * MAXN];
+  __shared__ double s1[MAXN * MAXN * MAXN];
+  __shared__ double sA[MAXN * MAXN], sB[MAXN * MAXN], sC[MAXN * MAXN];
+  const int64_t e = blockIdx.x;
+  if (e >= nel) return;
+  const double *ue = u + e * (int64_t)nu * nu * nu;
+  double *ve = v + e * (int64_t)nv * nv * nv;
+  const int nn = nv * nu;
+  for (int q = threadIdx.x; q < nn; q += blockDim.x) {
+    sA[q] = A[e * mstride + q];
+    sB[q] = Bt[e * mstride + q];
+    sC[q] = Ct[e * mstride + q];
+  }
+  for (int q = threadIdx.x; q < nu * nu * nu; q += blockDim.x) su[q] = ue[q];
+  __syncthreads();
+  // pass 1: w1(i,b,c) = sum_a A(i,a) u(a,b,c)        (nv x nu x nu)
+  for (int q = threadIdx.x; q < nv * nu * nu; q += blockDim.x) {
+    const int i = q % nv, bc = q / nv;
+    double t = 0.0;
+    for (int a = 0; a < nu; a++) t += sA[i + nv * a] * su[a + nu * bc];
+    s1[q] = t;
+  }
+  __syncthreads();
+  // pass 2: w2(i,j,c) = sum_b w1(i,b,c) Bt(b,j)      (nv x nv x nu) -> su
+  for (int q = threadIdx.x; q < nv * nv * nu; q += blockDim.x) {
+    const int i = q % nv, j = (q / nv) % nv, c = q / (nv * nv);
+    double t = 0.0;
+    for (int b = 0; b < nu; b++) t += s1[i + nv * (b + nu * c)] * sB[b + nu * j];
+    su[q] = t;
+  }
+  __syncthreads();
+  // pass 3: v(i,j,l) = sum_c w2(i,j,c) Ct(c,l)
+  for (int q = threadIdx.x; q < nv * nv * nv; q += blockDim.x) {
+    const int ij = q % (nv * nv), l = q / (nv * nv);
+    double t = 0.0;
+    for (int c = 0; c < nu; c++) t += su[ij + nv * nv * c] * sC[c + nu * l];
+    ve[q] = t;
+  }
+}
+
+// fdm_do_fast (3-D): e = (S_t (x) S_s (x) S_r) D (S_t^T (x) S_s^T (x) S_r^T) r
+// with per-element S (nl x nl) and D (nl^3).
+template <int NL>
+__global__ void __launch_bounds__(256)
+    fdm_kernel(double *__restrict__ e, const double *__restrict__ r,
+               const double *__restrict__ S, const double *__restrict__ D,
+               int64_t nel) {
+  constexpr int N3 = NL * NL * NL, N2 = NL * NL;
+  __shared__ double a[N3], b[N3];
+  __shared__ double s[6 * N2];
+  const int64_t el = blockIdx.x;
+  if (el >= nel) return;
+  const double *re = r + el * N3;
+  const double *se = S + el * 6 * N2;
+  for (int q = threadIdx.x; q < 6 * N2; q += blockDim.x) s[q] = se[q];
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) a[q] = re[q];
+  __syncthreads();
+  const double *S1 = s, *S1t = s + N2, *S2 = s + 2 * N2, *S2t = s + 3 * N2,
+               *S3 = s + 4 * N2, *S3t = s + 5 * N2;
+  // forward: A = S1^T (as stored in S1t: A(i,a) = S1t[i + NL a]),
+  //          Bt = S2, Ct = S3
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int i = q % NL, bc = q / NL;
+    double t = 0.0;
+    for (int x = 0; x < NL; x++) t += S1t[i + NL * x] * a[x + NL * bc];
+    b[q] = t;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int i = q % NL, j = (q / NL) % NL, c = q / N2;
+    double t = 0.0;
+    for (int y = 0; y < NL; y++) t += b[i + NL * (y + NL * c)] * S2[y + NL * j];
+    a[q] = t;
+  }
+  __syncthreads();
+  const double *de = D + el * N3;
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int ij = q % N2, l = q / N2;
+    double t = 0.0;
+    for (int z = 0; z < NL; z++) t += a[ij + N2 * z] * S3[z + NL * l];
+    b[q] = de[q] * t;
+  }
+  __syncthreads();
+  // backward: A = S1, Bt = S2^T, Ct = S3^T
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int i = q % NL, bc = q / NL;
+    double t = 0.0;
+    for (int x = 0; x < NL; x++) t += S1[i + NL * x] * b[x + NL * bc];
+    a[q] = t;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int i = q % NL, j = (q / NL) % NL, c = q / N2;
+    double t = 0.0;
+    for (int y = 0; y < NL; y++) t += a[i + NL * (y + NL * c)] * S2t[y + NL * j];
+    b[q] = t;
+  }
+  __syncthreads();
+  double *ee = e + el * N3;
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int ij = q % N2, l = q / N2;
+    double t = 0.0;
+    for (int z = 0; z < NL; z++) t += b[ij + N2 * z] * S3t[z + NL * l];
+    ee[q] = t;
+  }
+}
+
+// schwarz_wt3d: weights on the two outer layers of every face
+__global__ void wt3d_kernel(double *__restrict__ e, const double *__restrict__ wt,
+                            int n, int64_t nel) {
+  const int64_t n3 = (int64_t)n * n * n;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * n3) return;
+  const int64_t el = idx / n3;
+  const int p = (int)(idx % n3);
+  const int i = p % n, j = (p / n) % n, k = p / (n * n);
+  const double *we = wt + el * (int64_t)n * n * 12;
+  auto W = [&](int a, int b, int c, int d) { return we[a + n * (b + n * (c + 4 * d))]; };
+  double v = e[idx];
+  // r-faces for all (j,k); s-faces for i in 3..n-2 (1-based); t-faces for
+  // i,j in 3..n-2 (schwarz.f90:606-630).  A point on several layers (n = 3)
+  // takes every weight, in the reference's order.
+  auto apply = [&](int x, int a, int b, int dir) {
+    if (x == 0) v *= W(a, b, 0, dir);
+    if (x == 1) v *= W(a, b, 1, dir);
+    if (x == n - 2) v *= W(a, b, 2, dir);
+    if (x == n - 1) v *= W(a, b, 3, dir);
+  };
+  apply(i, j, k, 0);
+  const bool mid_i = (i >= 2 && i <= n - 3), mid_j = (j >= 2 && j <= n - 3);
+  if (mid_i) apply(j, i, k, 1);
+  if (mid_i && mid_j) apply(k, i, j, 2);
+  e[idx] = v;
+}
+
+// wt from the summed overlap counts (schwarz_setup_schwarz_wt3d_2)
+__global__ void wt_setup_kernel(double *__restrict__ wt, const double *__restrict__ c,
+                                int n, int64_t nel) {
+  const int64_t per = (int64_t)n * n * 12;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * per) return;
+  const int64_t el = idx / per;
+  const int q = (int)(idx % per);
+  const int a = q % n, b = (q / n) % n, lay = (q / (n * n)) % 4, dir = q / (n * n * 4);
+  const int pos = lay == 0 ? 0 : (lay == 1 ? 1 : (lay == 2 ? n - 2 : n - 1));
+  const double *ce = c + el * (int64_t)n * n * n;
+  double v;
+  if (dir == 0) v = ce[i3(pos, a, b, n)];
+  else if (dir == 1) v = ce[i3(a, pos, b, n)];
+  else v = ce[i3(a, b, pos, n)];
+  wt[idx] = 1.0 / v;
+}
+
+static inline unsigned blocks_for(int64_t n, int bs = 256) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+// --------------------------------------------------------------- structures
+struct GsDev {
+  int32_t *blk_ptr = nullptr, *gidx = nullptr;
+  int64_t nblks = 0, pair0 = 0, npairs = 0;
+  int max_blk = 0;
+};
+
+struct Schwarz {
+  int lx = 0, nl = 0;
+  int64_t nel = 0;
+  const Op *op = nullptr;
+  GsDev ext;
+  double *S = nullptr, *D = nullptr, *wt = nullptr;
+  double *work1 = nullptr, *work2 = nullptr;
+};
+
+struct Hsmg {
+  Op *fine = nullptr;
+  nm_op_t *mid_h = nullptr, *crs_h = nullptr;
+  Op *mid = nullptr, *crs = nullptr;
+  Schwarz sw_f, sw_m;
+  double *jfm = nullptr, *jfm_t = nullptr, *jmc = nullptr, *jmc_t = nullptr;
+  double *r = nullptr, *w = nullptr, *e_mid = nullptr, *e_crs = nullptr, *r_crs = nullptr;
+  CgFixed *cg = nullptr;
+  int niter = 10;
+  double crs_tol = 1e-9;
+  std::vector<void *> owned;
+  ~Hsmg() {
+    if (cg) cg_fixed_free(cg);
+    for (void *q : owned) (void)hipFree(q);
+    if (mid_h) nm_op_free(mid_h);
+    if (crs_h) nm_op_free(crs_h);
+  }
+  template <typename T>
+  T *alloc(int64_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+    owned.push_back(q);
+    return (T *)q;
+  }
+};
+
+static int gs_dev(const GsDev &g, double *u, hipStream_t s) {
+  GsArgs<double> a{u, g.blk_ptr, g.gidx, g.nblks, g.pair0, g.npairs, nullptr};
+  a.variant = g.max_blk <= 8 ? 1 : 0;
+  return launch_gs<double>(a, s);
+}
+
+static int extrude(double *a1, int l1, double f1, const double *a2, int l2,
+                   double f2, int nx, int64_t nel, hipStream_t s) {
+  hipLaunchKernelGGL(extrude_kernel, dim3((unsigned)nel), dim3(128), 0, s, a1, l1,
+                     f1, a2, l2, f2, nx, nel);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+static int tnsr3d(double *v, int nv, const double *u, int nu, const double *A,
+                  const double *Bt, const double *Ct, int64_t nel, hipStream_t s) {
+  if (std::max(nv, nu) > 14) {
+    set_error("tnsr3d: size %d unsupported", std::max(nv, nu));
+    return NM_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL((tnsr3d_kernel<14>), dim3((unsigned)nel), dim3(256), 0, s, v, nv,
+                     u, nu, A, Bt, Ct, (int64_t)0, nel);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+static int fdm(double *e, const double *r, const double *S, const double *D,
+               int nl, int64_t nel, hipStream_t s) {
+  switch (nl) {
+#define NM_FDM(N)                                                                  \
+  case N:                                                                          \
+    hipLaunchKernelGGL((fdm_kernel<N>), dim3((unsigned)nel), dim3(256), 0, s, e, r, \
+                       S, D, nel);                                                 \
+    break;
+    NM_FDM(5) NM_FDM(6) NM_FDM(7) NM_FDM(8) NM_FDM(9) NM_FDM(10) NM_FDM(11) NM_FDM(12)
+#undef NM_FDM
+    default:
+      set_error("fdm: extended size %d unsupported (lx 3..10)", nl);
+      return NM_ERR_UNSUPPORTED;
+  }
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// schwarz_compute (CPU branch semantics, schwarz.f90:565-595); r masked in
+// place as in the reference.
+static int schwarz_apply(const Schwarz &sw, double *e, double *r, hipStream_t s) {
+  const Op &op = *sw.op;
+  const int nx = sw.nl, lx = sw.lx;
+  const int64_t nel = sw.nel, ns = nel * nx * nx * nx;
+  int st;
+  if ((st = launch_mask<double>(r, op.msk, op.nmsk, nullptr, s))) return st;
+  hipLaunchKernelGGL(toext_kernel, dim3(blocks_for(ns)), dim3(256), 0, s, sw.work1, r, lx, nel);
+  if ((st = extrude(sw.work1, 0, 0.0, sw.work1, 2, 1.0, nx, nel, s))) return st;
+  if ((st = gs_dev(sw.ext, sw.work1, s))) return st;
+  if ((st = extrude(sw.work1, 0, 1.0, sw.work1, 2, -1.0, nx, nel, s))) return st;
+  if ((st = fdm(sw.work2, sw.work1, sw.S, sw.D, nx, nel, s))) return st;
+  if ((st = extrude(sw.work1, 0, 0.0, sw.work2, 0, 1.0, nx, nel, s))) return st;
+  if ((st = gs_dev(sw.ext, sw.work2, s))) return st;
+  if ((st = extrude(sw.work2, 0, 1.0, sw.work1, 0, -1.0, nx, nel, s))) return st;
+  if ((st = extrude(sw.work2, 2, 1.0, sw.work2, 0, 1.0, nx, nel, s))) return st;
+  hipLaunchKernelGGL(toreg_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.work2, lx, nel);
+  if ((st = op.gs<double>(e, nullptr, s))) return st;
+  if ((st = launch_mask<double>(e, op.msk, op.nmsk, nullptr, s))) return st;
+  hipLaunchKernelGGL(wt3d_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.wt, lx, nel);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// host gs ADD with Neko's 1-based maps (setup only)
+struct HostGs {
+  const Op *op;
+};
+static void host_gs(void *ctx, double *u) {
+  const Op &op = *((HostGs *)ctx)->op;
+  const int32_t *dg = op.h_dg.data(), *gd = op.h_gd.data();
+  const int64_t nb = (int64_t)op.h_blk_len.size(), m = (int64_t)op.h_dg.size();
+  for (int64_t b = 0; b < nb; b++) {
+    const int32_t k = op.h_blk_off[b], len = op.h_blk_len[b];
+    double t = u[gd[k] - 1];
+    for (int q = 1; q < len; q++) t += u[gd[k + q] - 1];
+    for (int q = 0; q < len; q++) u[gd[k + q] - 1] = t;
+  }
+  for (int64_t i = op.facet_offset; i + 1 <= m; i += 2) {
+    const double t = u[gd[i - 1] - 1] + u[gd[i] - 1];
+    u[gd[i - 1] - 1] = t;
+    u[gd[i] - 1] = t;
+  }
+  (void)dg;
+}
+
+template <typename H>
+static int gsdev_box(H *own, GsDev *g, const int box[3], const int per[3], int lx,
+                     hipStream_t s) {
+  int64_t nl, fo, nb;
+  int st = nm_box_gs_sizes(box[0], box[1], box[2], lx, per[0], per[1], per[2], &nl, &fo);
+  if (st) return st;
+  std::vector<int32_t> dg(nl), gd(nl), bl(fo), bo(fo);
+  st = nm_box_gs_map(box[0], box[1], box[2], lx, per[0], per[1], per[2], dg.data(),
+                     gd.data(), bl.data(), bo.data(), &nl, &fo, &nb);
+  if (st) return st;
+  std::vector<int32_t> ptr(nb + 1), gi(nl);
+  g->max_blk = 0;
+  for (int64_t b = 0; b < nb; b++) {
+    ptr[b] = bo[b];
+    g->max_blk = std::max<int>(g->max_blk, bl[b]);
+  }
+  ptr[nb] = nb > 0 ? bo[nb - 1] + bl[nb - 1] : 0;
+  for (int64_t i = 0; i < nl; i++) gi[i] = gd[i] - 1;
+  g->nblks = nb;
+  g->pair0 = fo - 1;
+  g->npairs = (nl - (fo - 1)) / 2;
+  if (!(g->blk_ptr = own->template alloc<int32_t>(nb + 1)) ||
+      !(g->gidx = own->template alloc<int32_t>(nl))) {
+    set_error("hsmg: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  NM_HIP_TRY(hipMemcpyAsync(g->blk_ptr, ptr.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipMemcpyAsync(g->gidx, gi.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+static int schwarz_setup(Hsmg *h, Schwarz *sw, const Op *op, int facet_bc, hipStream_t s) {
+  const int lx = op->lx, nl = lx + 2;
+  const int64_t nel = op->nel, n = op->n;
+  sw->lx = lx;
+  sw->nl = nl;
+  sw->nel = nel;
+  sw->op = op;
+  int st = gsdev_box(h, &sw->ext, op->box, op->per, nl, s);
+  if (st) return st;
+  const int64_t ns = nel * nl * nl * nl;
+  if (!(sw->S = h->alloc<double>(nel * 6 * nl * nl)) || !(sw->D = h->alloc<double>(ns)) ||
+      !(sw->wt = h->alloc<double>(nel * lx * lx * 12)) || !(sw->work1 = h->alloc<double>(ns)) ||
+      !(sw->work2 = h->alloc<double>(ns))) {
+    set_error("hsmg: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  // host geometry of the level + FDM pencils
+  std::vector<double> x(n), y(n), z(n), zg(lx), wg(lx), d1(lx * lx), dt(lx * lx);
+  NM_HIP_TRY(hipMemcpyAsync(x.data(), op->xyz[0], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NM_HIP_TRY(hipMemcpyAsync(y.data(), op->xyz[1], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NM_HIP_TRY(hipMemcpyAsync(z.data(), op->xyz[2], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  nm_gll(lx, zg.data(), wg.data(), d1.data(), dt.data());
+  // facet types: 0 where a neighbour exists, facet_bc on the boundary
+  std::vector<int> ft(nel * 6);
+  const int64_t nx_ = op->box[0], ny_ = op->box[1], nz_ = op->box[2];
+  for (int64_t e = 0; e < nel; e++) {
+    const int64_t ex = e % nx_, ey = (e / nx_) % ny_, ez = e / (nx_ * ny_);
+    const bool nb[6] = {op->per[0] || ex > 0, op->per[0] || ex < nx_ - 1,
+                        op->per[1] || ey > 0, op->per[1] || ey < ny_ - 1,
+                        op->per[2] || ez > 0, op->per[2] || ez < nz_ - 1};
+    for (int f = 0; f < 6; f++) ft[e * 6 + f] = nb[f] ? 0 : facet_bc;
+  }
+  std::vector<double> hS(nel * 6 * nl * nl), hD(ns);
+  HostGs hg{op};
+  st = nm_fdm_setup_host(nel, lx, x.data(), y.data(), z.data(), wg.data(), d1.data(),
+                         ft.data(), host_gs, &hg, hS.data(), hD.data());
+  if (st) return st;
+  NM_HIP_TRY(hipMemcpyAsync(sw->S, hS.data(), sizeof(double) * hS.size(), hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipMemcpyAsync(sw->D, hD.data(), sizeof(double) * hD.size(), hipMemcpyHostToDevice, s));
+  // weights (schwarz_setup_wt): counts of overlapping copies
+  double *e1;
+  if (!(e1 = h->alloc<double>(n))) return NM_ERR_NOMEM;
+  if ((st = launch_vec<double>(V_CFILL, sw->work2, nullptr, nullptr, nullptr, 1.0, ns, s))) return st;
+  if ((st = launch_vec<double>(V_RZERO, sw->work1, nullptr, nullptr, nullptr, 0.0, ns, s))) return st;
+  if ((st = extrude(sw->work1, 0, 0.0, sw->work2, 0, 1.0, nl, nel, s))) return st;
+  if ((st = gs_dev(sw->ext, sw->work2, s))) return st;
+  if ((st = extrude(sw->work2, 0, 1.0, sw->work1, 0, -1.0, nl, nel, s))) return st;
+  if ((st = extrude(sw->work2, 2, 1.0, sw->work2, 0, 1.0, nl, nel, s))) return st;
+  hipLaunchKernelGGL(toreg_kernel, dim3(blocks_for(n)), dim3(256), 0, s, e1, sw->work2, lx, nel);
+  if ((st = op->gs<double>(e1, nullptr, s))) return st;
+  hipLaunchKernelGGL(wt_setup_kernel, dim3(blocks_for(nel * lx * lx * 12)), dim3(256), 0, s,
+                     sw->wt, e1, lx, nel);
+  NM_HIP_TRY(hipGetLastError());
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+}  // namespace nm
+
+using namespace nm;
+
+struct nm_hsmg_s {
+  Hsmg h;
+};
+struct nm_gmres_s;
+
+extern "C" {
+
+
+// hsmg on a box operator created by nm_op_create_box.  facet_bc: Neko's
+// facet_type of the boundary facets (1 strong Dirichlet as set for
+// pressure, 2 as set for velocity; fluid_pnpn_bc_fctry.f90:162,259).
+int nm_hsmg_create(nm_hsmg_t **out, nm_op_t *fine_h, int facet_bc, int niter,
+                   hipStream_t s) {
+  *out = nullptr;
+  Op *fine = &fine_h->op;
+  if (fine->box[0] == 0) {
+    set_error("nm_hsmg_create: operator was not built by nm_op_create_box");
+    return NM_ERR_ARG;
+  }
+  const int lf = fine->lx;
+  if (lf < 3 || lf > 10) {
+    set_error("nm_hsmg_create: lx=%d outside 3..10", lf);
+    return NM_ERR_ARG;
+  }
+  const int lm = lf < 5 ? std::max(lf - 1, 3) : 4, lc = 2;
+  nm_hsmg_t *hh = new nm_hsmg_t;
+  Hsmg *h = &hh->h;
+  h->fine = fine;
+  h->niter = niter > 0 ? niter : 10;
+  auto fail = [&](int c) {
+    delete hh;
+    return c;
+  };
+  // level operators on the same mesh (the amplitude/seed live in the
+  // mesh: rebuild it with the same parameters)
+  const int *b = fine->box, *p = fine->per;
+  int st = nm_op_create_box(&h->mid_h, b[0], b[1], b[2], lm, p[0], p[1], p[2], fine->amp,
+                            fine->seed, 1.0, 0.0, s);
+  if (st) return fail(st);
+  st = nm_op_create_box(&h->crs_h, b[0], b[1], b[2], lc, p[0], p[1], p[2], fine->amp,
+                        fine->seed, fine->h1c, 0.0, s);
+  if (st) return fail(st);
+  h->mid = &h->mid_h->op;
+  h->crs = &h->crs_h->op;
+  if ((st = nm_op_set_all_deformed(h->crs_h, s))) return fail(st);
+  if ((st = schwarz_setup(h, &h->sw_f, fine, facet_bc, s))) return fail(st);
+  if ((st = schwarz_setup(h, &h->sw_m, h->mid, facet_bc, s))) return fail(st);
+  // interpolation matrices (mid -> fine: lf x lm, crs -> mid: lm x lc)
+  std::vector<double> zf(lf), zm(lm), zc(lc), tmp(lf * lf * 3);
+  std::vector<double> wdum(lf), ddum(lf * lf), dtdum(lf * lf);
+  nm_gll(lf, zf.data(), wdum.data(), ddum.data(), dtdum.data());
+  nm_gll(lm, zm.data(), wdum.data(), ddum.data(), dtdum.data());
+  nm_gll(lc, zc.data(), wdum.data(), ddum.data(), dtdum.data());
+  std::vector<double> jfm(lf * lm), jfmt(lf * lm), jmc(lm * lc), jmct(lm * lc);
+  nm_interp_matrix(zf.data(), lf, zm.data(), lm, jfm.data(), jfmt.data());
+  nm_interp_matrix(zm.data(), lm, zc.data(), lc, jmc.data(), jmct.data());
+  if (!(h->jfm = h->alloc<double>(lf * lm)) || !(h->jfm_t = h->alloc<double>(lf * lm)) ||
+      !(h->jmc = h->alloc<double>(lm * lc)) || !(h->jmc_t = h->alloc<double>(lm * lc)) ||
+      !(h->r = h->alloc<double>(fine->n)) || !(h->w = h->alloc<double>(fine->n)) ||
+      !(h->e_mid = h->alloc<double>(h->mid->n)) || !(h->e_crs = h->alloc<double>(h->crs->n)) ||
+      !(h->r_crs = h->alloc<double>(h->crs->n)))
+    return fail(NM_ERR_NOMEM);
+  if (hipMemcpyAsync(h->jfm, jfm.data(), sizeof(double) * jfm.size(), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(h->jfm_t, jfmt.data(), sizeof(double) * jfmt.size(), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(h->jmc, jmc.data(), sizeof(double) * jmc.size(), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(h->jmc_t, jmct.data(), sizeof(double) * jmct.size(), hipMemcpyHostToDevice, s))
+    return fail(NM_ERR_HIP);
+  if ((st = cg_fixed_create(&h->cg, h->crs))) return fail(st);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(NM_ERR_HIP);
+  *out = hh;
+  return NM_OK;
+}
+
+void nm_hsmg_free(nm_hsmg_t *h) { delete h; }
+
+// z = hsmg(r) (pc_hsmg.f90:601-626, CPU branch semantics).  Device FP64.
+int nm_hsmg_apply(nm_hsmg_t *hh, double *z, const double *rin, hipStream_t s) {
+  Hsmg *h = &hh->h;
+  const Op &F = *h->fine, &M = *h->mid, &Cc = *h->crs;
+  const int lf = F.lx, lm = M.lx, lc = Cc.lx;
+  const int64_t nel = F.nel;
+  int st;
+  NM_HIP_TRY(hipMemcpyAsync(h->r, rin, sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
+  if ((st = schwarz_apply(h->sw_f, z, h->r, s))) return st;
+  if ((st = launch_vec<double>(V_COL2, h->r, F.f64.mult, nullptr, nullptr, 0.0, F.n, s))) return st;
+  double *wm = h->w;  // mid-size work in the fine-size buffer
+  if ((st = tnsr3d(wm, lm, h->r, lf, h->jfm_t, h->jfm, h->jfm, nel, s))) return st;
+  if ((st = M.gs<double>(wm, nullptr, s))) return st;
+  if ((st = schwarz_apply(h->sw_m, h->e_mid, wm, s))) return st;
+  if ((st = launch_vec<double>(V_COL2, wm, M.f64.mult, nullptr, nullptr, 0.0, M.n, s))) return st;
+  if ((st = tnsr3d(h->r_crs, lc, wm, lm, h->jmc_t, h->jmc, h->jmc, nel, s))) return st;
+  if ((st = Cc.gs<double>(h->r_crs, nullptr, s))) return st;
+  if ((st = launch_mask<double>(h->r_crs, Cc.msk, Cc.nmsk, nullptr, s))) return st;
+  if ((st = cg_fixed_solve(h->cg, h->e_crs, h->r_crs, h->niter, h->crs_tol, s))) return st;
+  if ((st = launch_mask<double>(h->e_crs, Cc.msk, Cc.nmsk, nullptr, s))) return st;
+  if ((st = tnsr3d(wm, lm, h->e_crs, lc, h->jmc, h->jmc_t, h->jmc_t, nel, s))) return st;
+  if ((st = launch_vec<double>(V_ADD2, h->e_mid, wm, nullptr, nullptr, 0.0, M.n, s))) return st;
+  if ((st = tnsr3d(h->r, lf, h->e_mid, lm, h->jfm, h->jfm_t, h->jfm_t, nel, s))) return st;
+  if ((st = launch_vec<double>(V_ADD2, z, h->r, nullptr, nullptr, 0.0, F.n, s))) return st;
+  if ((st = F.gs<double>(z, nullptr, s))) return st;
+  return launch_vec<double>(V_COL2, z, F.f64.mult, nullptr, nullptr, 0.0, F.n, s);
+}
+
+// Schwarz smoother of the fine level alone (e = schwarz(r); r masked).
+int nm_hsmg_schwarz_fine(nm_hsmg_t *hh, double *e, double *r, hipStream_t s) {
+  return schwarz_apply(hh->h.sw_f, e, r, s);
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "neko_amd.h"
 #include "nm_common.h"
 #include "nm_kernels.h"
 
@@ -51,6 +52,8 @@ struct Op {
   double *red_out = nullptr;    // 4 doubles
   std::vector<void *> owned;    // every device allocation of this op
   int box[3] = {0, 0, 0}, per[3] = {0, 0, 0};
+  double amp = 0.0;     // box perturbation amplitude (box operators)
+  uint64_t seed = 0;    // box perturbation seed
 
   ~Op();
   template <typename T>
@@ -77,6 +80,14 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
               const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
               const int32_t *blk_off, hipStream_t s);
 int op_finish(Op *op, hipStream_t s);
+
+// Coarse-grid CG with a fixed budget and no host synchronisation
+// (nm_cg.hip).
+struct CgFixed;
+int cg_fixed_create(CgFixed **out, const Op *op);
+void cg_fixed_free(CgFixed *c);
+int cg_fixed_solve(CgFixed *c, double *x, const double *f, int niter,
+                   double abs_tol, hipStream_t s);
 
 }  // namespace nm
 

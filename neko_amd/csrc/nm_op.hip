@@ -228,7 +228,6 @@ using namespace nm;
 
 extern "C" {
 
-typedef struct nm_op_s nm_op_t;
 
 // Build a whole box problem: host mesh/gs maps/mask, device coordinates,
 // geometric factors, mass, multiplicity and Jacobi diagonal.
@@ -254,6 +253,8 @@ int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
   op->ifh2 = h2 != 0.0;
   op->box[0] = nelx; op->box[1] = nely; op->box[2] = nelz;
   op->per[0] = perx; op->per[1] = pery; op->per[2] = perz;
+  op->amp = amp;
+  op->seed = seed;
   const int64_t n = op->n, nel = op->nel;
   int st = NM_OK;
   auto fail = [&](int code) {
@@ -414,14 +415,6 @@ int nm_op_create(nm_op_t **out, int nel, int lx, const double *dx,
 
 void nm_op_free(nm_op_t *h) { delete h; }
 
-typedef struct {
-  int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
-  int lx;
-  double volume;
-  double h1c;
-  int ifh2;
-} nm_op_info_t;
-
 int nm_op_info(const nm_op_t *h, nm_op_info_t *info) {
   const Op &op = h->op;
   info->nel = op.nel;
@@ -470,6 +463,17 @@ int nm_op_set_epb(nm_op_t *h, int epb64, int epb32) {
   h->op.epb64 = epb64;
   h->op.epb32 = epb32;
   return NM_OK;
+}
+
+// msh%all_deformed() (mesh.f90:470-473): every element takes the Jacobi
+// cross terms; the derived fields (mult, volume, Jacobi) are rebuilt.
+int nm_op_set_all_deformed(nm_op_t *h, hipStream_t s) {
+  Op *op = &h->op;
+  std::vector<int> ones(op->nel, 1);
+  NM_HIP_TRY(hipMemcpyAsync(op->dfrmd, ones.data(), sizeof(int) * op->nel,
+                            hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return op_finish(op, s);
 }
 
 int nm_op_set_gs_variant(nm_op_t *h, int variant) {

@@ -3,6 +3,8 @@
 // the Dirichlet mask.  This plays the role of Neko's own setup (space_t,
 // mesh_t, dofmap_t, gs_init) for the standalone harnesses; when the library
 // runs behind Neko, Neko supplies these arrays itself.
+#include "neko_amd.h"
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>

@@ -284,6 +284,81 @@ class CG:
             pass
 
 
+class HSMG:
+    """hsmg preconditioner (pc_t 'hsmg', src/krylov/pc_hsmg.f90) on a box
+    problem: Schwarz/FDM smoothing on lx and lx_mid, Jacobi-CG on lx=2.
+    facet_bc: facet_type of boundary facets (1 pressure, 2 velocity)."""
+
+    def __init__(self, problem: Problem, facet_bc: int = 1, niter: int = 10):
+        self.problem = problem
+        h = C.c_void_p()
+        check(lib().nm_hsmg_create(C.byref(h), problem.h, int(facet_bc),
+                                   int(niter), _stream()), "nm_hsmg_create")
+        self.h = h
+
+    def apply(self, z: torch.Tensor, r: torch.Tensor):
+        """z = M^{-1} r (pc_hsmg.f90:481-627); r is left untouched."""
+        check(lib().nm_hsmg_apply(self.h, _ptr(z), _ptr(r), _stream()),
+              "nm_hsmg_apply")
+
+    def schwarz_fine(self, e: torch.Tensor, r: torch.Tensor):
+        """e = schwarz(r) on the fine level; r is masked in place."""
+        check(lib().nm_hsmg_schwarz_fine(self.h, _ptr(e), _ptr(r), _stream()),
+              "nm_hsmg_schwarz_fine")
+
+    def free(self):
+        if self.h:
+            lib().nm_hsmg_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GMRES:
+    """Restarted GMRES(m) with right preconditioning (ksp_t 'gmres',
+    src/krylov/bcknd/cpu/gmres.f90).  pc: 'jacobi', 'identity' or an HSMG."""
+
+    def __init__(self, problem: Problem, pc="jacobi", lgmres: int = 30):
+        self.problem = problem
+        self.pc = pc
+        h = C.c_void_p()
+        if isinstance(pc, HSMG):
+            code, ph = 1, pc.h
+        elif pc == "jacobi":
+            code, ph = 0, None
+        elif pc == "identity":
+            code, ph = 2, None
+        else:
+            raise ValueError(f"unknown preconditioner {pc!r}")
+        check(lib().nm_gmres_create(C.byref(h), problem.h, code, ph,
+                                    int(lgmres)), "nm_gmres_create")
+        self.h = h
+
+    def solve(self, x: torch.Tensor, f: torch.Tensor, abs_tol: float,
+              max_iter: int = 10000) -> KspMonitor:
+        mon = Monitor()
+        check(lib().nm_gmres_solve(self.h, _ptr(x), _ptr(f), float(abs_tol),
+                                   int(max_iter), C.byref(mon), _stream()),
+              "nm_gmres_solve")
+        return KspMonitor(mon.iter, mon.res_start, mon.res_final,
+                          bool(mon.converged), mon.outer)
+
+    def free(self):
+        if self.h:
+            lib().nm_gmres_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def _memcpy_d2d(dst: int, src: int, nbytes: int):
     check(lib().nm_memcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, 3,
                           _stream()), "nm_memcpy")

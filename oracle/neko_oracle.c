@@ -12,6 +12,7 @@
  * file) is listed in the header of tests/test_oracle.py.
  */
 #include "neko_oracle.h"
+#include "neko_oracle_hsmg.h"
 
 #include <math.h>
 #include <stdint.h>
@@ -1176,9 +1177,25 @@ done:
 /* ===================================================================== */
 /* a11: GMRES(lgmres) (gmres.f90:175-411), right preconditioned           */
 /* ===================================================================== */
+static void pc_jacobi_cb(void *ctx, double *z, const double *v, int64_t n) {
+  const double *pcd = (const double *)ctx;
+  if (pcd) {
+    for (int64_t i = 0; i < n; i++) z[i] = v[i] * pcd[i];
+  } else {
+    for (int64_t i = 0; i < n; i++) z[i] = v[i];
+  }
+}
+
 or_monitor_t or_gmres_solve(const or_problem_t *pb, const double *pcd,
                             double *x, const double *f, int max_iter,
                             double abs_tol, int lg) {
+  return or_gmres_solve_pc(pb, pc_jacobi_cb, (void *)pcd, x, f, max_iter,
+                           abs_tol, lg);
+}
+
+or_monitor_t or_gmres_solve_pc(const or_problem_t *pb, or_pc_fn pc, void *ctx,
+                               double *x, const double *f, int max_iter,
+                               double abs_tol, int lg) {
   const int64_t n = (int64_t)pb->nel * pb->lx * pb->lx * pb->lx;
   const double *mult = pb->mult;
   or_monitor_t res = {0, 0, 0, 0};
@@ -1213,11 +1230,7 @@ or_monitor_t or_gmres_solve(const or_problem_t *pb, const double *pcd,
       iter = iter + 1;
       double *zj = z + (int64_t)(j - 1) * n;
       const double *vj = v + (int64_t)(j - 1) * n;
-      if (pcd) {
-        for (int64_t i = 0; i < n; i++) zj[i] = vj[i] * pcd[i];
-      } else {
-        for (int64_t i = 0; i < n; i++) zj[i] = vj[i];
-      }
+      pc(ctx, zj, vj, n);
       or_ax_full(pb, w, zj);
       for (int l = 0; l < j; l++) hp[l] = 0.0;
       for (int64_t i0 = 0; i0 < n; i0 += BLK) {

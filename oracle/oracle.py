@@ -120,6 +120,7 @@ class OracleCase:
                  periodic=(False, False, False), h1=1.0, h2=0.0):
         L = lib()
         self.nelx, self.nely, self.nelz, self.lx = nelx, nely, nelz, lx
+        self.amp, self.seed, self.periodic = amp, seed, tuple(periodic)
         nel = nelx * nely * nelz
         self.nel = nel
         self.n = n = nel * lx ** 3
@@ -280,4 +281,214 @@ class OracleCase:
         mon = lib().or_gmres_solve(C.byref(self.problem()), pcp, x,
                                    np.ascontiguousarray(f), max_iter,
                                    float(abs_tol), lgmres)
+        return x, mon
+
+
+# --------------------------------------------------------------------------
+# hsmg (pc_hsmg.f90) on top of OracleCase levels
+# --------------------------------------------------------------------------
+class OrGs(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("o", C.c_int), ("nb", C.c_int),
+                ("dg", C.c_void_p), ("gd", C.c_void_p), ("b", C.c_void_p),
+                ("bo", C.c_void_p), ("buf", C.c_void_p)]
+
+
+class OrSchwarz(C.Structure):
+    _fields_ = [("lx", C.c_int), ("nel", C.c_int), ("gs", OrGs),
+                ("gs_ext", OrGs), ("msk", C.c_void_p), ("nmsk", C.c_int),
+                ("s", C.c_void_p), ("d", C.c_void_p), ("wt", C.c_void_p),
+                ("work1", C.c_void_p), ("work2", C.c_void_p)]
+
+
+class OrHsmg(C.Structure):
+    _fields_ = [("nel", C.c_int), ("lf", C.c_int), ("lm", C.c_int),
+                ("lc", C.c_int), ("nf", C.c_int64), ("nm", C.c_int64),
+                ("nc", C.c_int64), ("sw_fine", OrSchwarz), ("sw_mid", OrSchwarz),
+                ("gs_crs", OrGs), ("msk_c", C.c_void_p), ("nmsk_c", C.c_int),
+                ("mult_f", C.c_void_p), ("mult_m", C.c_void_p),
+                ("jfm", C.c_void_p), ("jfm_t", C.c_void_p), ("jmc", C.c_void_p),
+                ("jmc_t", C.c_void_p), ("crs", OrProblem), ("pc_crs", C.c_void_p),
+                ("niter", C.c_int), ("crs_tol", C.c_double), ("r", C.c_void_p),
+                ("w", C.c_void_p), ("e_mid", C.c_void_p), ("e_crs", C.c_void_p),
+                ("last_crs_iter", C.c_int)]
+
+
+def _gs_struct(c):
+    g = OrGs()
+    g.n, g.m, g.o, g.nb = c.n, c.nlocal, c.facet_offset, c.nblks
+    g.dg, g.gd = c.dg.ctypes.data, c.gd.ctypes.data
+    g.b, g.bo = c.blk_len.ctypes.data, c.blk_off.ctypes.data
+    g.buf = c.gs_buf.ctypes.data
+    return g
+
+
+def _bind_hsmg(L):
+    if getattr(L, "_hsmg_bound", False):
+        return
+    L.or_setup_intp.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+    L.or_fdm_setup.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip,
+                               C.POINTER(OrGs), _dp, _dp]
+    L.or_schwarz_setup_wt.argtypes = [C.c_int, C.c_int, C.POINTER(OrGs),
+                                      C.POINTER(OrGs), _dp]
+    L.or_schwarz_compute.argtypes = [C.POINTER(OrSchwarz), _dp, _dp]
+    L.or_hsmg_solve.argtypes = [C.POINTER(OrHsmg), _dp, _dp]
+    L.or_gmres_hsmg_solve.restype = OrMonitor
+    L.or_gmres_hsmg_solve.argtypes = [C.POINTER(OrProblem), C.POINTER(OrHsmg),
+                                      _dp, _dp, C.c_int, C.c_double, C.c_int]
+    L.or_tnsr3d.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int]
+    L.or_semhat.argtypes = [C.c_int, _dp, _dp]
+    L.or_fdm_1d.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                            C.c_double, C.c_double, C.c_int]
+    L.or_plane_space.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                 C.c_int]
+    L._hsmg_bound = True
+
+
+def semhat(n):
+    """1-D reference stiffness ah ((n+1)^2, column-major) and GLL mass bh
+    (fast3d.f90 semhat)."""
+    L = lib()
+    _bind_hsmg(L)
+    ah = np.zeros((n + 1) * (n + 1))
+    bh = np.zeros(n + 1)
+    L.or_semhat(n, ah, bh)
+    return ah.reshape(n + 1, n + 1).T.copy(), bh
+
+
+def fdm_1d(lbc, rbc, ll, lm, lr, n):
+    """Pencil (A, B) of the extended 1-D element (fdm.f90:478-560) and the
+    FDM eigenvectors S (boundary rows zeroed) and eigenvalues."""
+    L = lib()
+    _bind_hsmg(L)
+    nl = n + 3
+    a = np.zeros(nl * nl)
+    b = np.zeros(nl * nl)
+    s = np.zeros(2 * nl * nl)
+    lam = np.zeros(nl)
+    L.or_fdm_1d(a, b, s, lam, lbc, rbc, ll, lm, lr, n)
+    f = lambda m: m.reshape(nl, nl).T.copy()
+    return f(a), f(b), f(s[: nl * nl]), f(s[nl * nl:]), lam
+
+
+def plane_space(case):
+    """Per-element lengths between the first and last GLL planes in r, s, t
+    (fdm.f90 plane_space)."""
+    L = lib()
+    _bind_hsmg(L)
+    out = [np.zeros(case.nel) for _ in range(3)]
+    L.or_plane_space(*out, case.w, case.x, case.y, case.zc, case.lx, case.nel)
+    return out
+
+
+def tnsr3d(u, nu, nv, A, Bt, Ct, nel):
+    """v = (C (x) B (x) A) u per element (tnsr3d_el_nvnu); matrices
+    column-major numpy arrays (A: nv x nu, Bt, Ct: nu x nv)."""
+    L = lib()
+    _bind_hsmg(L)
+    v = np.zeros(nel * nv ** 3)
+    cm = lambda m: np.ascontiguousarray(np.asarray(m).T).ravel()
+    L.or_tnsr3d(v, nv, np.ascontiguousarray(u, dtype=np.float64), nu, cm(A),
+                cm(Bt), cm(Ct), nel)
+    return v
+
+
+def setup_intp(z_to, z_from):
+    L = lib()
+    _bind_hsmg(L)
+    nt, nf = len(z_to), len(z_from)
+    jh = np.zeros(nt * nf)
+    jht = np.zeros(nt * nf)
+    L.or_setup_intp(jh, jht, np.ascontiguousarray(z_to, dtype=np.float64),
+                    np.ascontiguousarray(z_from, dtype=np.float64), nt, nf, 0)
+    return jh, jht
+
+
+class OracleHsmg:
+    """hsmg with levels lx -> lx_mid -> 2 on the mesh of `case`
+    (pc_hsmg.f90:147-378).  facet_bc: facet_type of boundary facets
+    (1: Dirichlet as set for pressure, 2: as set for velocity)."""
+
+    def __init__(self, case, facet_bc=2, niter=10, crs_tol=1e-9):
+        L = lib()
+        _bind_hsmg(L)
+        self.case = case
+        lf = case.lx
+        if lf < 5:
+            lm = max(lf - 1, 3)
+        else:
+            lm = 4
+        lc = 2
+        kw = dict(amp=case.amp, seed=case.seed, periodic=case.periodic)
+        dims = (case.nelx, case.nely, case.nelz)
+        self.mid = OracleCase(*dims, lm, **kw)
+        self.crs = OracleCase(*dims, lc, h1=float(case.h1[0]), h2=0.0, **kw)
+        self.crs.dfrmd[:] = 1  # hsmg_init: msh%all_deformed()
+        self.ext_f = OracleCase(*dims, lf + 2, **kw)
+        self.ext_m = OracleCase(*dims, lm + 2, **kw)
+        nel = case.nel
+        self.lf, self.lm, self.lc = lf, lm, lc
+        ft = np.where(case.fneigh != 0, 0, facet_bc).astype(np.int32)
+        self.facet_type = ft
+        self._keep = []
+
+        def schwarz(c, ext):
+            nl = c.lx + 2
+            s = np.zeros(nel * nl * nl * 6)
+            d = np.zeros(nel * nl ** 3)
+            g = _gs_struct(c)
+            ge = _gs_struct(ext)
+            L.or_fdm_setup(nel, c.lx, c.x, c.y, c.zc, c.w, ft, C.byref(g), s, d)
+            wt = np.zeros(nel * c.lx * c.lx * 12)
+            L.or_schwarz_setup_wt(nel, c.lx, C.byref(ge), C.byref(g), wt)
+            w1 = np.zeros(nel * nl ** 3)
+            w2 = np.zeros(nel * nl ** 3)
+            sw = OrSchwarz()
+            sw.lx, sw.nel, sw.gs, sw.gs_ext = c.lx, nel, g, ge
+            sw.msk, sw.nmsk = c.msk.ctypes.data, c.nmsk
+            sw.s, sw.d, sw.wt = s.ctypes.data, d.ctypes.data, wt.ctypes.data
+            sw.work1, sw.work2 = w1.ctypes.data, w2.ctypes.data
+            self._keep += [s, d, wt, w1, w2]
+            return sw, s, d, wt
+
+        self.sw_f, self.s_f, self.d_f, self.wt_f = schwarz(case, self.ext_f)
+        self.sw_m, self.s_m, self.d_m, self.wt_m = schwarz(self.mid, self.ext_m)
+        self.jfm, self.jfm_t = setup_intp(case.z, self.mid.z)
+        self.jmc, self.jmc_t = setup_intp(self.mid.z, self.crs.z)
+        self.pc_crs = self.crs.jacobi()
+        h = OrHsmg()
+        h.nel, h.lf, h.lm, h.lc = nel, lf, lm, lc
+        h.nf, h.nm, h.nc = case.n, self.mid.n, self.crs.n
+        h.sw_fine, h.sw_mid = self.sw_f, self.sw_m
+        h.gs_crs = _gs_struct(self.crs)
+        h.msk_c, h.nmsk_c = self.crs.msk.ctypes.data, self.crs.nmsk
+        h.mult_f, h.mult_m = case.mult.ctypes.data, self.mid.mult.ctypes.data
+        h.jfm, h.jfm_t = self.jfm.ctypes.data, self.jfm_t.ctypes.data
+        h.jmc, h.jmc_t = self.jmc.ctypes.data, self.jmc_t.ctypes.data
+        h.crs = self.crs.problem()
+        h.pc_crs = self.pc_crs.ctypes.data
+        h.niter, h.crs_tol = niter, crs_tol
+        self.r = np.zeros(case.n)
+        self.w = np.zeros(case.n)
+        self.e_mid = np.zeros(self.mid.n)
+        self.e_crs = np.zeros(self.crs.n)
+        h.r, h.w = self.r.ctypes.data, self.w.ctypes.data
+        h.e_mid, h.e_crs = self.e_mid.ctypes.data, self.e_crs.ctypes.data
+        self.h = h
+
+    def apply(self, r):
+        z = np.zeros(self.case.n)
+        lib().or_hsmg_solve(C.byref(self.h), z, np.ascontiguousarray(r, dtype=np.float64))
+        return z
+
+    def schwarz_fine(self, r):
+        e = np.zeros(self.case.n)
+        rr = np.array(r, dtype=np.float64)
+        lib().or_schwarz_compute(C.byref(self.sw_f), e, rr)
+        return e
+
+    def gmres(self, f, abs_tol, max_iter=10000, lgmres=30):
+        x = np.zeros(self.case.n)
+        mon = lib().or_gmres_hsmg_solve(C.byref(self.case.problem()), C.byref(self.h),
+                                        x, np.ascontiguousarray(f), max_iter,
+                                        float(abs_tol), lgmres)
         return x, mon
