@@ -108,6 +108,7 @@ typedef struct nm_op_s nm_op_t;
 typedef struct nm_cg_s nm_cg_t;
 typedef struct nm_hsmg_s nm_hsmg_t;
 typedef struct nm_gmres_s nm_gmres_t;
+typedef struct nm_server_s nm_server_t;
 
 typedef struct {
   int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
@@ -204,6 +205,10 @@ int nm_op_ax_local32(nm_op_t *op, float *w, const float *u, hipStream_t s);
 int nm_op_ax_gs32(nm_op_t *op, float *w, const float *u, hipStream_t s);
 int nm_op_gs(const nm_op_t *op, double *u, hipStream_t s);
 int nm_op_mask(const nm_op_t *op, double *u, hipStream_t s);
+/* f = mask(gs(g)) with g_i ~ N(0,1) from (seed, i): the seeded right-hand
+ * side of the synthetic workloads, generated on the device */
+int nm_op_rhs_normal(const nm_op_t *op, uint64_t seed, double *f,
+                     hipStream_t s);
 /* Jacobi diagonal with every element treated as deformed (the coarse grid
  * of pc_hsmg.f90:312-318 sets msh%dfrmd_el = .true.) */
 int nm_op_set_all_deformed(nm_op_t *op, hipStream_t s);
@@ -241,6 +246,38 @@ int nm_gmres_create(nm_gmres_t **g, nm_op_t *op, int pc, nm_hsmg_t *hsmg,
 void nm_gmres_free(nm_gmres_t *g);
 int nm_gmres_solve(nm_gmres_t *g, double *x, const double *f, double abs_tol,
                    int max_iter, nm_gmres_monitor_t *mon, hipStream_t s);
+
+/* --- serving loop (no counterpart in Neko; drives the CG above) --------- */
+typedef struct {
+  int state;            /* 0 queued, 1 running, 2 done, 3 failed */
+  int iter, outer, converged;
+  double res_start, res_final;
+  double t_arrive, t_admit, t_done;  /* seconds from nm_server_run start */
+  int64_t n;            /* local dofs of the request's mesh */
+} nm_serve_result_t;
+/* precision 0 FP64 / 1 mixed; chunk_iters: CG iterations issued per
+ * request between host polls; keep_solutions: copy each x aside */
+int nm_server_create(nm_server_t **sv, double hbm_budget_bytes,
+                     int max_inflight, int precision, int chunk_iters,
+                     int keep_solutions);
+void nm_server_free(nm_server_t *sv);
+/* register a resident Dirichlet box mesh (Poisson, h1 = 1); prewarm: CG
+ * states to allocate up front */
+int nm_server_add_box_mesh(nm_server_t *sv, int nelx, int nely, int nelz,
+                           int lx, double amp, uint64_t seed, int prewarm,
+                           int *mesh_id, hipStream_t s);
+int nm_server_mesh_op(nm_server_t *sv, int mesh_id, nm_op_t **op);
+/* a request: mesh, RHS seed (nm_op_rhs_normal), relative tolerance,
+ * arrival time in seconds after the start of nm_server_run */
+int nm_server_submit(nm_server_t *sv, int mesh_id, uint64_t rhs_seed,
+                     double rel_tol, int max_iter, double t_arrive,
+                     int *req_id);
+int nm_server_run(nm_server_t *sv, double *wall_s);
+int nm_server_result(const nm_server_t *sv, int req_id,
+                     nm_serve_result_t *r);
+int nm_server_solution(const nm_server_t *sv, int req_id, double *x,
+                       hipStream_t s);
+int nm_server_memory(const nm_server_t *sv, int64_t *used, int64_t *peak);
 
 /* ordered copy helper for host harnesses (kind: hipMemcpyKind) */
 int nm_memcpy(void *dst, const void *src, size_t bytes, int kind,

@@ -25,6 +25,12 @@ class OpInfo(C.Structure):
                 ("lx", _i), ("volume", _d), ("h1c", _d), ("ifh2", _i)]
 
 
+class ServeResult(C.Structure):
+    _fields_ = [("state", _i), ("iter", _i), ("outer", _i), ("converged", _i),
+                ("res_start", _d), ("res_final", _d), ("t_arrive", _d),
+                ("t_admit", _d), ("t_done", _d), ("n", C.c_int64)]
+
+
 class Monitor(C.Structure):
     _fields_ = [("iter", _i), ("outer", _i), ("res_start", _d),
                 ("res_final", _d), ("converged", _i)]
@@ -89,6 +95,7 @@ SIGNATURES = {
     "nm_op_gs": (_i, [_vp, _vp, _vp]),
     "nm_op_mask": (_i, [_vp, _vp, _vp]),
     "nm_op_set_all_deformed": (_i, [_vp, _vp]),
+    "nm_op_rhs_normal": (_i, [_vp, C.c_uint64, _vp, _vp]),
     "nm_hsmg_create": (_i, [C.POINTER(_vp), _vp, _i, _i, _vp]),
     "nm_hsmg_free": (None, [_vp]),
     "nm_hsmg_apply": (_i, [_vp, _vp, _vp, _vp]),
@@ -96,6 +103,16 @@ SIGNATURES = {
     "nm_gmres_create": (_i, [C.POINTER(_vp), _vp, _i, _vp, _i]),
     "nm_gmres_free": (None, [_vp]),
     "nm_gmres_solve": (_i, [_vp, _vp, _vp, _d, _i, C.POINTER(Monitor), _vp]),
+    "nm_server_create": (_i, [C.POINTER(_vp), _d, _i, _i, _i, _i]),
+    "nm_server_free": (None, [_vp]),
+    "nm_server_add_box_mesh": (_i, [_vp, _i, _i, _i, _i, _d, C.c_uint64, _i,
+                                    C.POINTER(_i), _vp]),
+    "nm_server_mesh_op": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "nm_server_submit": (_i, [_vp, _i, C.c_uint64, _d, _i, _d, C.POINTER(_i)]),
+    "nm_server_run": (_i, [_vp, C.POINTER(_d)]),
+    "nm_server_result": (_i, [_vp, _i, C.POINTER(ServeResult)]),
+    "nm_server_solution": (_i, [_vp, _i, _vp, _vp]),
+    "nm_server_memory": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nm_cg_create": (_i, [C.POINTER(_vp), _vp, _i]),
     "nm_cg_free": (None, [_vp]),
     "nm_cg_solve": (_i, [_vp, _vp, _vp, _d, _i, _d, C.POINTER(Monitor), _vp]),

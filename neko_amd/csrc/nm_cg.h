@@ -1,0 +1,59 @@
+// Device Jacobi-PCG as a non-blocking state machine: begin() enqueues the
+// start of a solve on the solver's own stream, poll() advances it by one
+// chunk of graph-captured iterations whenever the previous chunk has
+// finished (never blocks), wait() drives it to the end.  The blocking C-ABI
+// solve and the serving loop share it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "nm_op.h"
+
+namespace nm {
+
+struct CgScal;
+
+struct CgSolver {
+  Op *op = nullptr;
+  int mixed = 0;
+  hipStream_t s = nullptr;
+  // FP64 state
+  double *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr, *f = nullptr;
+  // FP32 inner state
+  float *e32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
+  double *part = nullptr;
+  CgScal *sc = nullptr;      // device: [0] main/inner, [1] outer
+  CgScal *h_sc = nullptr;    // pinned host mirror
+  hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  int graph_iters = 4;
+  int64_t bytes = 0;         // device memory held by this solver
+  std::vector<void *> owned;
+  // run state
+  enum Phase { IDLE, RUN64, OUTER, INNER, DONE } phase = IDLE;
+  hipEvent_t ev = nullptr;
+  bool pending = false;      // an event is recorded and not yet consumed
+  double abs_tol = 0.0, rel_tol = 0.0, inner_rel = 1e-4, tol = 0.0;
+  int max_iter = 0, total = 0, sweeps = 0, launched = 0, chunk = 1;
+  int chunk_max = 16;        // graph launches per chunk (doubling up to this)
+  double res_start = -1.0, rnorm = 0.0;
+  int converged = 0;
+  ~CgSolver();
+  template <typename T>
+  T *alloc(int64_t cnt);
+};
+
+// precision 0: FP64; 1: FP32 inner CG inside FP64 iterative refinement
+int cg_solver_create(CgSolver **out, Op *op, int precision);
+void cg_solver_destroy(CgSolver *cg);
+// Start a solve of A x = f (f copied; f == nullptr: cg->f already holds
+// the right-hand side).  Stops when the Neko residual norm drops below
+// rel_tol * (initial norm) if rel_tol > 0, else below abs_tol.
+int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
+             int max_iter, double inner_rel);
+// Non-blocking: 1 when the solve has finished, 0 while running, < 0 error.
+int cg_poll(CgSolver *cg);
+int cg_wait(CgSolver *cg);
+
+}  // namespace nm

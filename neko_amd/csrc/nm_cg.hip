@@ -17,6 +17,7 @@
 
 #include "nm_common.h"
 #include "nm_kernels.h"
+#include "nm_cg.h"
 #include "nm_op.h"
 
 namespace nm {
@@ -331,35 +332,25 @@ int cg_fixed_solve(CgFixed *c, double *x, const double *f, int niter,
   return NM_OK;
 }
 
-struct CgSolver {
-  Op *op = nullptr;
-  int mixed = 0;
-  hipStream_t s = nullptr;
-  // FP64 state
-  double *x = nullptr, *r = nullptr, *p = nullptr, *w = nullptr, *f = nullptr;
-  // FP32 inner state
-  float *e32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
-  double *part = nullptr;
-  CgScal *sc = nullptr;      // device: [0] main/inner, [1] outer
-  CgScal *h_sc = nullptr;    // pinned host mirror
-  hipGraphExec_t g64 = nullptr, g32 = nullptr;
-  int graph_iters = 4;
-  std::vector<void *> owned;
-  ~CgSolver() {
-    if (g64) (void)hipGraphExecDestroy(g64);
-    if (g32) (void)hipGraphExecDestroy(g32);
-    for (void *q : owned) (void)hipFree(q);
-    if (h_sc) (void)hipHostFree(h_sc);
-    if (s) (void)hipStreamDestroy(s);
-  }
-  template <typename T>
-  T *alloc(int64_t cnt) {
-    void *q = nullptr;
-    if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(cnt, 1)) != hipSuccess) return nullptr;
-    owned.push_back(q);
-    return (T *)q;
-  }
-};
+CgSolver::~CgSolver() {
+  if (g64) (void)hipGraphExecDestroy(g64);
+  if (g32) (void)hipGraphExecDestroy(g32);
+  if (ev) (void)hipEventDestroy(ev);
+  if (s) (void)hipStreamSynchronize(s);
+  for (void *q : owned) (void)hipFree(q);
+  if (h_sc) (void)hipHostFree(h_sc);
+  if (s) (void)hipStreamDestroy(s);
+}
+
+template <typename T>
+T *CgSolver::alloc(int64_t cnt) {
+  void *q = nullptr;
+  const size_t b = sizeof(T) * (size_t)std::max<int64_t>(cnt, 1);
+  if (hipMalloc(&q, b) != hipSuccess) return nullptr;
+  owned.push_back(q);
+  bytes += (int64_t)b;
+  return (T *)q;
+}
 
 template <typename T, typename TX>
 static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
@@ -377,23 +368,222 @@ static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
   return NM_OK;
 }
 
-// Run iterations (graph batches) until the device flag is set.
-static int run_until_done(CgSolver *cg, hipGraphExec_t g, CgScal *sc_dev,
-                          int max_iter) {
-  int launched = 0, batch = 1;
-  for (;;) {
-    for (int b = 0; b < batch; b++) {
-      NM_HIP_TRY(hipGraphLaunch(g, cg->s));
-      launched += cg->graph_iters;
+int cg_solver_create(CgSolver **out, Op *op, int precision) {
+  *out = nullptr;
+  CgSolver *cg = new CgSolver;
+  cg->op = op;
+  cg->mixed = precision;
+  const int64_t n = op->n;
+  auto fail = [&](int code) {
+    delete cg;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&cg->s, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&cg->ev, hipEventDisableTiming) != hipSuccess) {
+    set_error("cg: stream/event creation failed");
+    return fail(NM_ERR_HIP);
+  }
+  if (!(cg->x = cg->alloc<double>(n)) || !(cg->r = cg->alloc<double>(n)) ||
+      !(cg->w = cg->alloc<double>(n)) || !(cg->f = cg->alloc<double>(n)) ||
+      !(cg->part = cg->alloc<double>(2 * RED_BLOCKS)) || !(cg->sc = cg->alloc<CgScal>(2))) {
+    set_error("cg: out of device memory");
+    return fail(NM_ERR_NOMEM);
+  }
+  if (!precision) {
+    if (!(cg->p = cg->alloc<double>(n))) {
+      set_error("cg: out of device memory");
+      return fail(NM_ERR_NOMEM);
     }
-    NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, sc_dev, sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
-    NM_HIP_TRY(hipStreamSynchronize(cg->s));
-    if (cg->h_sc->done) return NM_OK;
-    if (launched > max_iter + 8 * cg->graph_iters) {
-      set_error("cg: device did not stop after %d iterations", launched);
+  } else {
+    if (!(cg->e32 = cg->alloc<float>(n)) || !(cg->r32 = cg->alloc<float>(n)) ||
+        !(cg->p32 = cg->alloc<float>(n)) || !(cg->w32 = cg->alloc<float>(n))) {
+      set_error("cg: out of device memory");
+      return fail(NM_ERR_NOMEM);
+    }
+    int st = op->ensure32(cg->s);
+    if (st) return fail(st);
+  }
+  if (hipHostMalloc((void **)&cg->h_sc, sizeof(CgScal) * 2, hipHostMallocDefault) != hipSuccess) {
+    set_error("cg: pinned host allocation failed");
+    return fail(NM_ERR_NOMEM);
+  }
+  int st;
+  if (!precision)
+    st = capture_iters<double, double>(cg, cg->x, cg->r, cg->p, cg->w, cg->sc, &cg->g64);
+  else
+    st = capture_iters<float, float>(cg, cg->e32, cg->r32, cg->p32, cg->w32, cg->sc, &cg->g32);
+  if (st) return fail(st);
+  if (hipStreamSynchronize(cg->s) != hipSuccess) return fail(NM_ERR_HIP);
+  *out = cg;
+  return NM_OK;
+}
+
+void cg_solver_destroy(CgSolver *cg) { delete cg; }
+
+// enqueue `chunk` graph launches, the scalar read-back and the event
+static int enqueue_chunk(CgSolver *cg, hipGraphExec_t g) {
+  for (int b = 0; b < cg->chunk; b++) {
+    NM_HIP_TRY(hipGraphLaunch(g, cg->s));
+    cg->launched += cg->graph_iters;
+  }
+  if (cg->chunk < cg->chunk_max) cg->chunk *= 2;
+  NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
+  NM_HIP_TRY(hipEventRecord(cg->ev, cg->s));
+  cg->pending = true;
+  return NM_OK;
+}
+
+// outer step of the refinement: r = f - A x (FP64), its norm, read-back
+static int enqueue_outer(CgSolver *cg) {
+  const Op &op = *cg->op;
+  const int64_t n = op.n;
+  int st;
+  if ((st = op.apply<double>(cg->w, cg->x, nullptr, cg->s, true, true))) return st;
+  hipLaunchKernelGGL(resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->r,
+                     cg->f, cg->w, op.f64.mult, n, cg->part);
+  hipLaunchKernelGGL(resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                     cg->sc + 1, cg->sc);
+  NM_HIP_TRY(hipGetLastError());
+  NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
+  NM_HIP_TRY(hipEventRecord(cg->ev, cg->s));
+  cg->pending = true;
+  return NM_OK;
+}
+
+int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
+             int max_iter, double inner_rel) {
+  const Op &op = *cg->op;
+  const int64_t n = op.n;
+  if (cg->pending) {
+    set_error("cg: a solve is still in flight on this solver");
+    return NM_ERR_STATE;
+  }
+  cg->abs_tol = abs_tol;
+  cg->rel_tol = rel_tol;
+  cg->inner_rel = inner_rel > 0.0 ? inner_rel : 1e-4;
+  cg->max_iter = max_iter;
+  cg->total = cg->sweeps = cg->launched = 0;
+  cg->chunk = 1;
+  cg->res_start = -1.0;
+  cg->rnorm = 0.0;
+  cg->converged = 0;
+  CgScal init{};
+  init.norm_fac = 1.0 / std::sqrt(op.volume);
+  init.scale = 1.0;
+  std::memcpy(&cg->h_sc[0], &init, sizeof(CgScal));
+  std::memcpy(&cg->h_sc[1], &init, sizeof(CgScal));
+  NM_HIP_TRY(hipMemcpyAsync(cg->sc, cg->h_sc, 2 * sizeof(CgScal), hipMemcpyHostToDevice, cg->s));
+  if (f) NM_HIP_TRY(hipMemcpyAsync(cg->f, f, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
+  if (!cg->mixed) {
+    hipLaunchKernelGGL((cg_init_kernel<double, double, double>), dim3(RED_BLOCKS),
+                       dim3(RED_THREADS), 0, cg->s, cg->x, cg->r, cg->p, cg->f,
+                       op.f64.mult, op.f64.dinv, n, (const CgScal *)nullptr, cg->part);
+    hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                       cg->sc, abs_tol, rel_tol, max_iter);
+    NM_HIP_TRY(hipGetLastError());
+    cg->phase = CgSolver::RUN64;
+    return enqueue_chunk(cg, cg->g64);
+  }
+  int st = launch_vec<double>(V_RZERO, cg->x, nullptr, nullptr, nullptr, 0.0, n, cg->s);
+  if (st) return st;
+  cg->phase = CgSolver::OUTER;
+  return enqueue_outer(cg);
+}
+
+int cg_poll(CgSolver *cg) {
+  if (cg->phase == CgSolver::DONE) return 1;
+  if (cg->phase == CgSolver::IDLE) {
+    set_error("cg: poll without a solve");
+    return NM_ERR_STATE;
+  }
+  if (cg->pending) {
+    const hipError_t q = hipEventQuery(cg->ev);
+    if (q == hipErrorNotReady) return 0;
+    if (q != hipSuccess) {
+      set_error("cg: %s", hipGetErrorString(q));
+      return NM_ERR_HIP;
+    }
+    cg->pending = false;
+  }
+  const Op &op = *cg->op;
+  const int64_t n = op.n;
+  const CgScal &in = cg->h_sc[0], &out = cg->h_sc[1];
+  switch (cg->phase) {
+    case CgSolver::RUN64:
+      if (in.done) {
+        cg->total = in.iter;
+        cg->res_start = in.res_start;
+        cg->rnorm = in.rnorm;
+        cg->converged = in.converged;
+        cg->phase = CgSolver::DONE;
+        return 1;
+      }
+      if (cg->launched > cg->max_iter + 8 * cg->graph_iters) {
+        set_error("cg: device did not stop after %d iterations", cg->launched);
+        return NM_ERR_STATE;
+      }
+      {
+        const int st = enqueue_chunk(cg, cg->g64);
+        return st ? st : 0;
+      }
+    case CgSolver::OUTER: {
+      cg->rnorm = out.rnorm;
+      if (cg->res_start < 0) {
+        cg->res_start = cg->rnorm;
+        cg->tol = cg->rel_tol > 0.0 ? cg->rel_tol * cg->res_start : cg->abs_tol;
+      }
+      if (cg->rnorm < cg->tol || cg->total >= cg->max_iter ||
+          std::fabs(cg->rnorm) < 2.220446049250313e-16 || cg->sweeps > 64) {
+        cg->converged = (cg->rnorm < cg->tol) && cg->total < cg->max_iter;
+        cg->phase = CgSolver::DONE;
+        return 1;
+      }
+      // inner FP32 solve of A e = r / ||r|| to a relative tolerance
+      CgScal *inner = cg->sc;
+      hipLaunchKernelGGL((cg_init_kernel<float, float, double>), dim3(RED_BLOCKS),
+                         dim3(RED_THREADS), 0, cg->s, cg->e32, cg->r32, cg->p32, cg->r,
+                         op.f32.mult, op.f32.dinv, n, (const CgScal *)inner, cg->part);
+      hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                         inner, 0.0, cg->inner_rel, cg->max_iter - cg->total);
+      NM_HIP_TRY(hipGetLastError());
+      cg->phase = CgSolver::INNER;
+      cg->chunk = 1;
+      cg->launched = 0;
+      const int st = enqueue_chunk(cg, cg->g32);
+      return st ? st : 0;
+    }
+    case CgSolver::INNER: {
+      if (!in.done) {
+        if (cg->launched > cg->max_iter + 8 * cg->graph_iters) {
+          set_error("cg: device did not stop after %d inner iterations", cg->launched);
+          return NM_ERR_STATE;
+        }
+        const int st = enqueue_chunk(cg, cg->g32);
+        return st ? st : 0;
+      }
+      cg->total += in.iter;
+      cg->sweeps++;
+      hipLaunchKernelGGL(refine_add_kernel, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->x,
+                         cg->e32, n, cg->sc);
+      cg->phase = CgSolver::OUTER;
+      const int st = enqueue_outer(cg);
+      return st ? st : 0;
+    }
+    default:
+      set_error("cg: bad phase");
+      return NM_ERR_STATE;
+  }
+}
+
+int cg_wait(CgSolver *cg) {
+  for (;;) {
+    if (cg->pending) NM_HIP_TRY(hipEventSynchronize(cg->ev));
+    const int st = cg_poll(cg);
+    if (st < 0 || st > 0) return st < 0 ? st : NM_OK;
+    if (st == 0 && !cg->pending) {
+      set_error("cg: solver stalled");
       return NM_ERR_STATE;
     }
-    if (batch < 16) batch *= 2;
   }
 }
 
@@ -402,7 +592,10 @@ static int run_until_done(CgSolver *cg, hipGraphExec_t g, CgScal *sc_dev,
 using namespace nm;
 
 struct nm_cg_s {
-  CgSolver cg;
+  CgSolver *cg = nullptr;
+  ~nm_cg_s() {
+    if (cg) cg_solver_destroy(cg);
+  }
 };
 
 extern "C" {
@@ -411,39 +604,11 @@ extern "C" {
 int nm_cg_create(nm_cg_t **out, nm_op_t *oph, int precision) {
   *out = nullptr;
   nm_cg_t *h = new nm_cg_t;
-  CgSolver *cg = &h->cg;
-  cg->op = &oph->op;
-  cg->mixed = precision;
-  const int64_t n = cg->op->n;
-  auto fail = [&](int code) {
+  const int st = cg_solver_create(&h->cg, &oph->op, precision);
+  if (st) {
     delete h;
-    return code;
-  };
-  if (hipStreamCreateWithFlags(&cg->s, hipStreamNonBlocking) != hipSuccess) return fail(NM_ERR_HIP);
-  if (!(cg->x = cg->alloc<double>(n)) || !(cg->r = cg->alloc<double>(n)) ||
-      !(cg->w = cg->alloc<double>(n)) || !(cg->f = cg->alloc<double>(n)) ||
-      !(cg->part = cg->alloc<double>(2 * RED_BLOCKS)) || !(cg->sc = cg->alloc<CgScal>(2))) {
-    set_error("nm_cg_create: out of device memory");
-    return fail(NM_ERR_NOMEM);
+    return st;
   }
-  if (!precision) {
-    if (!(cg->p = cg->alloc<double>(n))) return fail(NM_ERR_NOMEM);
-  } else {
-    if (!(cg->e32 = cg->alloc<float>(n)) || !(cg->r32 = cg->alloc<float>(n)) ||
-        !(cg->p32 = cg->alloc<float>(n)) || !(cg->w32 = cg->alloc<float>(n)))
-      return fail(NM_ERR_NOMEM);
-    int st = cg->op->ensure32(cg->s);
-    if (st) return fail(st);
-  }
-  if (hipHostMalloc((void **)&cg->h_sc, sizeof(CgScal) * 2, hipHostMallocDefault) != hipSuccess)
-    return fail(NM_ERR_NOMEM);
-  int st;
-  if (!precision)
-    st = capture_iters<double, double>(cg, cg->x, cg->r, cg->p, cg->w, cg->sc, &cg->g64);
-  else
-    st = capture_iters<float, float>(cg, cg->e32, cg->r32, cg->p32, cg->w32, cg->sc, &cg->g32);
-  if (st) return fail(st);
-  if (hipStreamSynchronize(cg->s) != hipSuccess) return fail(NM_ERR_HIP);
   *out = h;
   return NM_OK;
 }
@@ -458,74 +623,25 @@ void nm_cg_free(nm_cg_t *h) { delete h; }
 int nm_cg_solve(nm_cg_t *h, double *x, const double *f, double abs_tol,
                 int max_iter, double inner_rel, nm_monitor_t *mon,
                 hipStream_t stream) {
-  CgSolver *cg = &h->cg;
-  const Op &op = *cg->op;
-  const int64_t n = op.n;
+  CgSolver *cg = h->cg;
+  const int64_t n = cg->op->n;
+  std::memset(mon, 0, sizeof(*mon));
   hipEvent_t ev;
   NM_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   NM_HIP_TRY(hipEventRecord(ev, stream));
   NM_HIP_TRY(hipStreamWaitEvent(cg->s, ev, 0));
-  CgScal init{};
-  init.norm_fac = 1.0 / std::sqrt(op.volume);
-  init.scale = 1.0;
-  std::memcpy(&cg->h_sc[0], &init, sizeof(CgScal));
-  std::memcpy(&cg->h_sc[1], &init, sizeof(CgScal));
-  NM_HIP_TRY(hipMemcpyAsync(cg->sc, cg->h_sc, 2 * sizeof(CgScal), hipMemcpyHostToDevice, cg->s));
-  NM_HIP_TRY(hipMemcpyAsync(cg->f, f, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
-  int st = NM_OK;
-  std::memset(mon, 0, sizeof(*mon));
-  if (!cg->mixed) {
-    hipLaunchKernelGGL((cg_init_kernel<double, double, double>), dim3(RED_BLOCKS),
-                       dim3(RED_THREADS), 0, cg->s, cg->x, cg->r, cg->p, cg->f,
-                       op.f64.mult, op.f64.dinv, n, (const CgScal *)nullptr, cg->part);
-    hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                       cg->sc, abs_tol, 0.0, max_iter);
-    st = run_until_done(cg, cg->g64, cg->sc, max_iter);
-    if (st) return st;
-    mon->iter = cg->h_sc->iter;
-    mon->res_start = cg->h_sc->res_start;
-    mon->res_final = cg->h_sc->rnorm;
-    mon->converged = cg->h_sc->converged;
-    NM_HIP_TRY(hipMemcpyAsync(x, cg->x, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
-  } else {
-    // outer FP64 refinement: x = 0, r = f
-    CgScal *inner = cg->sc, *outer = cg->sc + 1;
-    const double rel = inner_rel > 0.0 ? inner_rel : 1e-4;
-    NM_HIP_TRY(hipMemsetAsync(cg->x, 0, sizeof(double) * n, cg->s));
-    int total = 0, sweeps = 0;
-    double res_start = -1.0, rnorm = 0.0;
-    for (;;) {
-      // r = f - A x (FP64 operator, gs, mask) and its norm
-      if ((st = op.apply<double>(cg->w, cg->x, nullptr, cg->s, true, true))) return st;
-      hipLaunchKernelGGL(resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->r,
-                         cg->f, cg->w, op.f64.mult, n, cg->part);
-      hipLaunchKernelGGL(resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                         outer, inner);
-      NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
-      NM_HIP_TRY(hipStreamSynchronize(cg->s));
-      rnorm = cg->h_sc[1].rnorm;
-      if (res_start < 0) res_start = rnorm;
-      if (rnorm < abs_tol || total >= max_iter || std::fabs(rnorm) < 2.220446049250313e-16) break;
-      // inner FP32 solve of A e = r / ||r||, relative tolerance
-      hipLaunchKernelGGL((cg_init_kernel<float, float, double>), dim3(RED_BLOCKS),
-                         dim3(RED_THREADS), 0, cg->s, cg->e32, cg->r32, cg->p32, cg->r,
-                         op.f32.mult, op.f32.dinv, n, (const CgScal *)inner, cg->part);
-      hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                         inner, 0.0, rel, max_iter - total);
-      if ((st = run_until_done(cg, cg->g32, inner, max_iter))) return st;
-      total += cg->h_sc->iter;
-      sweeps++;
-      hipLaunchKernelGGL(refine_add_kernel, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->x,
-                         cg->e32, n, inner);
-      if (sweeps > 64) break;
-    }
-    mon->iter = total;
-    mon->outer = sweeps;
-    mon->res_start = res_start;
-    mon->res_final = rnorm;
-    mon->converged = (rnorm < abs_tol) && total < max_iter;
-    NM_HIP_TRY(hipMemcpyAsync(x, cg->x, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
+  int st = cg_begin(cg, f, abs_tol, 0.0, max_iter, inner_rel);
+  if (!st) st = cg_wait(cg);
+  if (st) {
+    (void)hipEventDestroy(ev);
+    return st;
   }
+  mon->iter = cg->total;
+  mon->outer = cg->sweeps;
+  mon->res_start = cg->res_start;
+  mon->res_final = cg->rnorm;
+  mon->converged = cg->converged;
+  NM_HIP_TRY(hipMemcpyAsync(x, cg->x, sizeof(double) * n, hipMemcpyDeviceToDevice, cg->s));
   NM_HIP_TRY(hipEventRecord(ev, cg->s));
   NM_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
   NM_HIP_TRY(hipEventDestroy(ev));

@@ -51,6 +51,7 @@ struct Op {
   double *red_part = nullptr;   // RED_BLOCKS * 4 doubles
   double *red_out = nullptr;    // 4 doubles
   std::vector<void *> owned;    // every device allocation of this op
+  int64_t bytes = 0;            // their total size
   int box[3] = {0, 0, 0}, per[3] = {0, 0, 0};
   double amp = 0.0;     // box perturbation amplitude (box operators)
   uint64_t seed = 0;    // box perturbation seed

@@ -148,15 +148,24 @@ class Problem:
         return cls(h.value, keep=(arrs, h1a, h2a, dfa, ints, m))
 
     def free(self):
-        if self.h:
+        if self.h and not getattr(self, "_borrowed", False):
             lib().nm_op_free(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
             self.free()
         except Exception:
             pass
+
+    def rhs_normal_device(self, seed: int) -> torch.Tensor:
+        """Same seeded RHS as rhs_normal, generated on the device
+        (nm_op_rhs_normal; equal to it up to last-ulp differences of the
+        device log/cos)."""
+        f = torch.empty(self.n, dtype=torch.float64, device="cuda")
+        check(lib().nm_op_rhs_normal(self.h, int(seed), _ptr(f), _stream()),
+              "nm_op_rhs_normal")
+        return f
 
     # ---- data -----------------------------------------------------------
     def field(self, which: int) -> torch.Tensor:
@@ -350,6 +359,94 @@ class GMRES:
     def free(self):
         if self.h:
             lib().nm_gmres_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop for streams of Poisson solves on resident box meshes
+    (no counterpart in Neko).  precision: 'fp64' or 'mixed'."""
+
+    def __init__(self, hbm_budget_bytes: float, max_inflight: int = 8,
+                 precision: str = "mixed", chunk_iters: int = 8,
+                 keep_solutions: bool = False):
+        from ._lib import ServeResult  # noqa: F401  (binding check)
+        h = C.c_void_p()
+        check(lib().nm_server_create(C.byref(h), float(hbm_budget_bytes),
+                                     int(max_inflight),
+                                     0 if precision == "fp64" else 1,
+                                     int(chunk_iters), int(bool(keep_solutions))),
+              "nm_server_create")
+        self.h = h
+        self.meshes = []
+
+    def add_box_mesh(self, nelx, nely, nelz, lx=8, amp=0.1, seed=0, prewarm=1):
+        mid = C.c_int()
+        check(lib().nm_server_add_box_mesh(self.h, nelx, nely, nelz, lx,
+                                           float(amp), int(seed), int(prewarm),
+                                           C.byref(mid), _stream()),
+              "nm_server_add_box_mesh")
+        self.meshes.append((nelx, nely, nelz, lx, amp, seed))
+        return mid.value
+
+    def mesh_problem(self, mesh_id: int) -> "Problem":
+        """The resident operator of a mesh (owned by the server)."""
+        oph = C.c_void_p()
+        check(lib().nm_server_mesh_op(self.h, int(mesh_id), C.byref(oph)),
+              "nm_server_mesh_op")
+        pr = Problem.__new__(Problem)
+        pr.h = oph
+        pr._keep = self
+        pr._borrowed = True
+        info = OpInfo()
+        check(lib().nm_op_info(oph, C.byref(info)), "nm_op_info")
+        pr.info, pr.n, pr.lx, pr.nel, pr.volume = (info, info.n, info.lx,
+                                                   info.nel, info.volume)
+        return pr
+
+    def submit(self, mesh_id, rhs_seed, rel_tol=1e-8, max_iter=20000,
+               t_arrive=0.0) -> int:
+        rid = C.c_int()
+        check(lib().nm_server_submit(self.h, int(mesh_id), int(rhs_seed),
+                                     float(rel_tol), int(max_iter),
+                                     float(t_arrive), C.byref(rid)),
+              "nm_server_submit")
+        return rid.value
+
+    def run(self) -> float:
+        torch.cuda.synchronize()
+        wall = C.c_double()
+        check(lib().nm_server_run(self.h, C.byref(wall)), "nm_server_run")
+        return wall.value
+
+    def result(self, rid: int):
+        from ._lib import ServeResult
+        r = ServeResult()
+        check(lib().nm_server_result(self.h, int(rid), C.byref(r)),
+              "nm_server_result")
+        return r
+
+    def solution(self, rid: int) -> torch.Tensor:
+        r = self.result(rid)
+        x = torch.empty(int(r.n), dtype=torch.float64, device="cuda")
+        check(lib().nm_server_solution(self.h, int(rid), _ptr(x), _stream()),
+              "nm_server_solution")
+        return x
+
+    def memory(self):
+        used, peak = C.c_int64(), C.c_int64()
+        check(lib().nm_server_memory(self.h, C.byref(used), C.byref(peak)),
+              "nm_server_memory")
+        return used.value, peak.value
+
+    def free(self):
+        if self.h:
+            lib().nm_server_free(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
