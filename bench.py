@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rates", default="200,800,2000")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -193,6 +196,16 @@ def main():
         solves["speedup_mixed"] = round(solves["fp64"]["seconds"] / solves["mixed"]["seconds"], 3)
         result["solves"] = solves
 
+    # ---- serving stream ------------------------------------------------------
+    if not args.no_serve:
+        del w, u
+        if not args.no_solve:
+            del f, x
+        p.free()
+        torch.cuda.empty_cache()
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        result["serving"] = serving(S, rates, args.serve_requests)
+
     # ---- CPU baseline (oracle) on a bounded sample ------------------------
     if not args.no_cpu:
         try:
@@ -201,6 +214,67 @@ def main():
             result["cpu_baseline"] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+def _box_dims(E):
+    """Box dimensions (a, b, c) with a*b*c close to E, as cubic as possible."""
+    a = max(1, round(E ** (1 / 3)))
+    b = max(1, round((E / a) ** 0.5))
+    c = max(1, round(E / (a * b)))
+    return a, b, c
+
+
+def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
+    """Stream of Poisson solves (lx=8, perturbed boxes with element counts
+    log-uniform in [512, 32768], N(0,1) RHS, tolerance 1e-8 relative) with
+    Poisson arrivals at each rate: mixed-precision serving loop (up to 8 in
+    flight, 200 GB HBM budget) vs the same stream run one at a time in
+    plain FP64.  Meshes are a resident pool built before the stream."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    Es = np.exp(rng.uniform(np.log(512), np.log(32768), pool))
+    dims = [_box_dims(E) for E in Es]
+    out = {"pool": [list(d) for d in dims], "budget_GB": budget / 1e9,
+           "tolerance": rel_tol, "requests_per_rate": nreq}
+
+    servers = {}
+    for label, precision, inflight in (("mixed_served", "mixed", 8),
+                                       ("fp64_sequential", "fp64", 1)):
+        sv = S.Server(budget, max_inflight=inflight, precision=precision,
+                      chunk_iters=8)
+        ids = [sv.add_box_mesh(*d, lx=8, amp=0.1, seed=100 + k,
+                               prewarm=min(inflight, 2))
+               for k, d in enumerate(dims)]
+        servers[label] = (sv, ids)
+
+    def run(label, arrivals, mesh_of, seed0):
+        sv, ids = servers[label]
+        rids = [sv.submit(ids[m], seed0 + q, rel_tol=rel_tol, t_arrive=float(t))
+                for q, (t, m) in enumerate(zip(arrivals, mesh_of))]
+        wall = sv.run()
+        res = [sv.result(r) for r in rids]
+        lat = np.array([r.t_done - r.t_arrive for r in res])
+        _, peak = sv.memory()
+        d = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+             "solves_per_s": round(len(res) / wall, 3),
+             "wall_s": round(wall, 3),
+             "mean_iter": round(float(np.mean([r.iter for r in res])), 1),
+             "mean_dof": int(sum(r.n for r in res) / len(res)),
+             "all_converged": all(r.state == 2 and r.converged for r in res),
+             "peak_GB": round(peak / 1e9, 2)}
+        log(f"[bench] serve {label}: {d}")
+        return d
+
+    for k, rate in enumerate(rates):
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        arrivals -= arrivals[0]
+        mesh_of = rng.integers(0, pool, nreq)
+        out[f"{int(rate)}_req_s"] = {
+            label: run(label, arrivals, mesh_of, 1000 * (k + 1)) for label in servers}
+    for sv, _ in servers.values():
+        sv.free()
+    return out
 
 
 def cpu_baseline(lx, seconds):

@@ -31,10 +31,10 @@ struct CgSolver {
   int64_t bytes = 0;         // device memory held by this solver
   std::vector<void *> owned;
   // run state
-  enum Phase { IDLE, RUN64, OUTER, INNER, DONE } phase = IDLE;
+  enum Phase { IDLE, RUN64, MIX, REPLACE, DONE } phase = IDLE;
   hipEvent_t ev = nullptr;
   bool pending = false;      // an event is recorded and not yet consumed
-  double abs_tol = 0.0, rel_tol = 0.0, inner_rel = 1e-4, tol = 0.0;
+  double abs_tol = 0.0, rel_tol = 0.0, inner_rel = 0.1, tol = 0.0;
   int max_iter = 0, total = 0, sweeps = 0, launched = 0, chunk = 1;
   int chunk_max = 16;        // graph launches per chunk (doubling up to this)
   double res_start = -1.0, rnorm = 0.0;
@@ -44,7 +44,7 @@ struct CgSolver {
   T *alloc(int64_t cnt);
 };
 
-// precision 0: FP64; 1: FP32 inner CG inside FP64 iterative refinement
+// precision 0: FP64; 1: FP32 CG with FP64 residual replacement
 int cg_solver_create(CgSolver **out, Op *op, int precision);
 void cg_solver_destroy(CgSolver *cg);
 // Start a solve of A x = f (f copied; f == nullptr: cg->f already holds

@@ -24,8 +24,11 @@ namespace nm {
 
 struct CgScal {
   double rtz, rtz_old, pap, rtr, alpha, beta, rnorm, tol, norm_fac, res_start;
-  double scale;  // refinement: inner rhs = scale * r
-  int iter, done, max_iter, converged;
+  double scale;       // start: r = scale * f
+  double tol_final;   // mixed: final tolerance on the true residual
+  double delta;       // mixed: recurrence reduction between replacements
+  double true_rnorm;  // mixed: norm of the last true residual
+  int iter, done, max_iter, converged, updates;
 };
 
 // ---- kernels ---------------------------------------------------------------
@@ -204,46 +207,134 @@ __global__ void cg_fin_kernel(const double *__restrict__ part, CgScal *__restric
   }
 }
 
-// refinement helpers ---------------------------------------------------------
-// r64 = f - w (w = A x), partial rtr
+// mixed precision: FP32 CG with FP64 residual replacement ----------------------
+// The FP32 recurrence runs on (x32, r32, p32, w32) with FP32 operator and
+// vectors and FP64 dot-product accumulation.  Whenever its residual has
+// dropped by `delta` below the last true residual (or below the final
+// tolerance), the FP64 solution absorbs x32 and the true residual
+// r = f - A x is recomputed with the FP64 operator and replaces r32; the
+// search direction p32 is kept, so the Krylov process continues instead of
+// restarting.
+
+// start: x = 0, x32 = 0, p32 = 0, r32 = (float) f; partial rtr (FP64 f) and
+// rtz (r32 with the FP32 Jacobi diagonal)
 __global__ void __launch_bounds__(RED_THREADS)
-    resid_kernel(double *__restrict__ r, const double *__restrict__ f,
-                 const double *__restrict__ w, const double *__restrict__ mult,
-                 int64_t n, double *__restrict__ part) {
+    mx_init_kernel(double *__restrict__ x, float *__restrict__ x32, float *__restrict__ r32,
+                   float *__restrict__ p32, const double *__restrict__ f,
+                   const double *__restrict__ mult, const float *__restrict__ dinv32,
+                   int64_t n, double *__restrict__ part) {
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double fi = f[i];
+    const float ri = (float)fi;
+    x[i] = 0.0;
+    x32[i] = 0.0f;
+    p32[i] = 0.0f;
+    r32[i] = ri;
+    a += fi * fi * mult[i];
+    b += (double)ri * (double)ri * mult[i] * (double)dinv32[i];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = b;
+  }
+}
+
+__global__ void mx_init_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ sc,
+                                   double abs_tol, double rel, double delta, int max_iter) {
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) {
+    a += part[q];
+    b += part[RED_BLOCKS + q];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    sc->rtr = a;
+    sc->rtz = b;
+    sc->rtz_old = 1.0;
+    sc->beta = 0.0;
+    sc->rnorm = sqrt(a) * sc->norm_fac;
+    sc->res_start = sc->rnorm;
+    sc->tol_final = rel > 0.0 ? rel * sc->rnorm : abs_tol;
+    sc->delta = delta;
+    sc->true_rnorm = sc->rnorm;
+    sc->tol = fmax(delta * sc->rnorm, sc->tol_final);
+    sc->iter = 0;
+    sc->updates = 0;
+    sc->max_iter = max_iter;
+    sc->converged = 0;
+    sc->done = (fabs(sc->rnorm) < 2.220446049250313e-16 || sc->rnorm < sc->tol_final) ? 1 : 0;
+    if (sc->done) sc->converged = 1;
+  }
+}
+
+// x += x32, x32 = 0
+__global__ void mx_absorb_kernel(double *__restrict__ x, float *__restrict__ x32, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// true residual r = f - w (w = A x, FP64), r32 = (float) r; partial rtr
+// (FP64) and rtz (r32, FP32 Jacobi)
+__global__ void __launch_bounds__(RED_THREADS)
+    mx_resid_kernel(float *__restrict__ r32, const double *__restrict__ f,
+                    const double *__restrict__ w, const double *__restrict__ mult,
+                    const float *__restrict__ dinv32, int64_t n, double *__restrict__ part) {
   __shared__ double sh[16];
   double a = 0.0, b = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const double ri = f[i] - w[i];
-    r[i] = ri;
+    const float r3 = (float)ri;
+    r32[i] = r3;
     a += ri * ri * mult[i];
+    b += (double)r3 * (double)r3 * mult[i] * (double)dinv32[i];
   }
-  bsum2(a, b, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = a;
-}
-
-// outer scalars: rnorm of r64, scale = 1/||r|| for the FP32 inner problem
-__global__ void resid_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ outer,
-                                 CgScal *__restrict__ inner) {
-  __shared__ double sh[16];
-  double a = 0.0, b = 0.0;
-  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) a += part[q];
   bsum2(a, b, sh);
   if (threadIdx.x == 0) {
-    outer->rtr = a;
-    outer->rnorm = sqrt(a) * outer->norm_fac;
-    const double nr = sqrt(a);
-    inner->scale = nr > 0.0 ? 1.0 / nr : 1.0;
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = b;
   }
 }
 
-// x64 += e32 / scale
-__global__ void refine_add_kernel(double *__restrict__ x, const float *__restrict__ e,
-                                  int64_t n, const CgScal *__restrict__ inner) {
-  const double inv = 1.0 / inner->scale;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    x[i] += (double)e[i] * inv;
+// replacement scalars: true norm, new rtz and beta (the next direction is
+// p = z + beta p with z from the replaced residual), next threshold
+__global__ void mx_resid_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ sc) {
+  __shared__ double sh[16];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) {
+    a += part[q];
+    b += part[RED_BLOCKS + q];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    sc->rtr = a;
+    sc->rnorm = sqrt(a) * sc->norm_fac;
+    sc->true_rnorm = sc->rnorm;
+    sc->updates = sc->updates + 1;
+    if (sc->iter > 0) {
+      sc->rtz = b;
+      sc->beta = b / sc->rtz_old;
+    }
+    sc->tol = fmax(sc->delta * sc->rnorm, sc->tol_final);
+    if (sc->rnorm < sc->tol_final || fabs(sc->rnorm) < 2.220446049250313e-16) {
+      sc->done = 1;
+      sc->converged = 1;
+    } else if (sc->iter >= sc->max_iter) {
+      sc->done = 1;
+      sc->converged = 0;
+    } else {
+      sc->done = 0;
+      sc->converged = 0;
+    }
+  }
 }
 
 static inline unsigned vec_blocks(int64_t n) {
@@ -433,16 +524,19 @@ static int enqueue_chunk(CgSolver *cg, hipGraphExec_t g) {
   return NM_OK;
 }
 
-// outer step of the refinement: r = f - A x (FP64), its norm, read-back
-static int enqueue_outer(CgSolver *cg) {
+// residual replacement: x += x32, r = f - A x (FP64 operator), r32 = r,
+// new scalars, read-back
+static int enqueue_replace(CgSolver *cg) {
   const Op &op = *cg->op;
   const int64_t n = op.n;
   int st;
+  hipLaunchKernelGGL(mx_absorb_kernel, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->x,
+                     cg->e32, n);
   if ((st = op.apply<double>(cg->w, cg->x, nullptr, cg->s, true, true))) return st;
-  hipLaunchKernelGGL(resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->r,
-                     cg->f, cg->w, op.f64.mult, n, cg->part);
-  hipLaunchKernelGGL(resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                     cg->sc + 1, cg->sc);
+  hipLaunchKernelGGL(mx_resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->r32,
+                     cg->f, cg->w, op.f64.mult, op.f32.dinv, n, cg->part);
+  hipLaunchKernelGGL(mx_resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                     cg->sc);
   NM_HIP_TRY(hipGetLastError());
   NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
   NM_HIP_TRY(hipEventRecord(cg->ev, cg->s));
@@ -460,7 +554,7 @@ int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
   }
   cg->abs_tol = abs_tol;
   cg->rel_tol = rel_tol;
-  cg->inner_rel = inner_rel > 0.0 ? inner_rel : 1e-4;
+  cg->inner_rel = inner_rel > 0.0 ? inner_rel : 0.1;
   cg->max_iter = max_iter;
   cg->total = cg->sweeps = cg->launched = 0;
   cg->chunk = 1;
@@ -484,10 +578,13 @@ int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
     cg->phase = CgSolver::RUN64;
     return enqueue_chunk(cg, cg->g64);
   }
-  int st = launch_vec<double>(V_RZERO, cg->x, nullptr, nullptr, nullptr, 0.0, n, cg->s);
-  if (st) return st;
-  cg->phase = CgSolver::OUTER;
-  return enqueue_outer(cg);
+  hipLaunchKernelGGL(mx_init_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->x,
+                     cg->e32, cg->r32, cg->p32, cg->f, op.f64.mult, op.f32.dinv, n, cg->part);
+  hipLaunchKernelGGL(mx_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
+                     cg->sc, abs_tol, rel_tol, cg->inner_rel, max_iter);
+  NM_HIP_TRY(hipGetLastError());
+  cg->phase = CgSolver::MIX;
+  return enqueue_chunk(cg, cg->g32);
 }
 
 int cg_poll(CgSolver *cg) {
@@ -507,7 +604,7 @@ int cg_poll(CgSolver *cg) {
   }
   const Op &op = *cg->op;
   const int64_t n = op.n;
-  const CgScal &in = cg->h_sc[0], &out = cg->h_sc[1];
+  const CgScal &in = cg->h_sc[0];
   switch (cg->phase) {
     case CgSolver::RUN64:
       if (in.done) {
@@ -526,47 +623,32 @@ int cg_poll(CgSolver *cg) {
         const int st = enqueue_chunk(cg, cg->g64);
         return st ? st : 0;
       }
-    case CgSolver::OUTER: {
-      cg->rnorm = out.rnorm;
-      if (cg->res_start < 0) {
-        cg->res_start = cg->rnorm;
-        cg->tol = cg->rel_tol > 0.0 ? cg->rel_tol * cg->res_start : cg->abs_tol;
-      }
-      if (cg->rnorm < cg->tol || cg->total >= cg->max_iter ||
-          std::fabs(cg->rnorm) < 2.220446049250313e-16 || cg->sweeps > 64) {
-        cg->converged = (cg->rnorm < cg->tol) && cg->total < cg->max_iter;
-        cg->phase = CgSolver::DONE;
-        return 1;
-      }
-      // inner FP32 solve of A e = r / ||r|| to a relative tolerance
-      CgScal *inner = cg->sc;
-      hipLaunchKernelGGL((cg_init_kernel<float, float, double>), dim3(RED_BLOCKS),
-                         dim3(RED_THREADS), 0, cg->s, cg->e32, cg->r32, cg->p32, cg->r,
-                         op.f32.mult, op.f32.dinv, n, (const CgScal *)inner, cg->part);
-      hipLaunchKernelGGL(cg_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                         inner, 0.0, cg->inner_rel, cg->max_iter - cg->total);
-      NM_HIP_TRY(hipGetLastError());
-      cg->phase = CgSolver::INNER;
-      cg->chunk = 1;
-      cg->launched = 0;
-      const int st = enqueue_chunk(cg, cg->g32);
-      return st ? st : 0;
-    }
-    case CgSolver::INNER: {
+    case CgSolver::MIX: {
+      // FP32 recurrence stopped (or still running): replace the residual
       if (!in.done) {
         if (cg->launched > cg->max_iter + 8 * cg->graph_iters) {
-          set_error("cg: device did not stop after %d inner iterations", cg->launched);
+          set_error("cg: device did not stop after %d iterations", cg->launched);
           return NM_ERR_STATE;
         }
         const int st = enqueue_chunk(cg, cg->g32);
         return st ? st : 0;
       }
-      cg->total += in.iter;
-      cg->sweeps++;
-      hipLaunchKernelGGL(refine_add_kernel, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->x,
-                         cg->e32, n, cg->sc);
-      cg->phase = CgSolver::OUTER;
-      const int st = enqueue_outer(cg);
+      cg->phase = CgSolver::REPLACE;
+      const int st = enqueue_replace(cg);
+      return st ? st : 0;
+    }
+    case CgSolver::REPLACE: {
+      cg->res_start = in.res_start;
+      cg->rnorm = in.true_rnorm;
+      cg->total = in.iter;
+      cg->sweeps = in.updates;
+      if (in.done) {
+        cg->converged = in.converged;
+        cg->phase = CgSolver::DONE;
+        return 1;
+      }
+      cg->phase = CgSolver::MIX;
+      const int st = enqueue_chunk(cg, cg->g32);
       return st ? st : 0;
     }
     default:
@@ -617,9 +699,11 @@ void nm_cg_free(nm_cg_t *h) { delete h; }
 
 // Solve A x = f.  x, f: device FP64 arrays of op->n entries (caller owned).
 // abs_tol: Neko absolute tolerance on sqrt(r.mult.r)/sqrt(volume).
-// Mixed mode: inner FP32 CG to `inner_rel` of its start residual (<= 0:
-// default 1e-4), outer FP64 residual r = f - A x until the absolute
-// tolerance is met.  `stream` is ordered before and after the solve.
+// Mixed mode: FP32 CG (FP32 operator and vectors, FP64 dot products) whose
+// residual is replaced by the true FP64 residual f - A x each time the FP32
+// recurrence has dropped by `inner_rel` (<= 0: default 0.1) below the last
+// true residual; stops on the true residual.  mon->outer counts the
+// replacements.  `stream` is ordered before and after the solve.
 int nm_cg_solve(nm_cg_t *h, double *x, const double *f, double abs_tol,
                 int max_iter, double inner_rel, nm_monitor_t *mon,
                 hipStream_t stream) {

@@ -262,7 +262,7 @@ class Problem:
 
 class CG:
     """Jacobi-preconditioned CG (ksp_t 'cg' with pc_t 'jacobi').
-    precision='fp64' or 'mixed' (FP32 inner CG in FP64 refinement)."""
+    precision="fp64" or "mixed" (FP32 CG with FP64 residual replacement)."""
 
     def __init__(self, problem: Problem, precision: str = "fp64"):
         self.problem = problem
