@@ -24,6 +24,8 @@ struct CgSolver {
   // FP32 inner state
   float *e32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
   double *part = nullptr;
+  double *dpart = nullptr;   // fused p.Ap partials (trilinear Ax), npart
+  int64_t npart = 0;
   CgScal *sc = nullptr;      // device: [0] main/inner, [1] outer
   CgScal *h_sc = nullptr;    // pinned host mirror
   hipGraphExec_t g64 = nullptr, g32 = nullptr;

@@ -345,17 +345,43 @@ static inline unsigned vec_blocks(int64_t n) {
 }
 
 // ---- solver object -----------------------------------------------------------
+// fold `npart` per-workgroup partials of the fused p.Ap into RED_BLOCKS
+// partials (block b sums one contiguous chunk; fixed order)
+__global__ void __launch_bounds__(RED_THREADS)
+    fold_partials_kernel(const double *__restrict__ in, int64_t npart,
+                         double *__restrict__ out, const CgScal *__restrict__ sc) {
+  if (sc->done) return;
+  __shared__ double sh[16];
+  const int64_t chunk = (npart + RED_BLOCKS - 1) / RED_BLOCKS;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < npart ? lo + chunk : npart;
+  double a = 0.0, b = 0.0;
+  for (int64_t q = lo + threadIdx.x; q < hi; q += blockDim.x) a += in[q];
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = a;
+}
+
 // One CG iteration (T = vector type, TX = solution type) on stream s.
+// With dpart (npart > 0) p.Ap comes fused out of the Ax kernel as
+// sum_i p_i (A_local p)_i, which equals glsc3(w, mult, p) for the
+// continuous, masked p of CG (the multiplicity weights undo the gs sum);
+// otherwise it is a separate pass over w, mult and p.
 template <typename T, typename TX>
 static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
-                        double *part, hipStream_t s) {
+                        double *part, hipStream_t s, double *dpart = nullptr,
+                        int64_t npart = 0) {
   const OpFields<T> &F = op.fields<T>();
   const int64_t n = op.n;
   hipLaunchKernelGGL(cg_p_kernel<T>, dim3(vec_blocks(n)), dim3(256), 0, s, p, r, F.dinv, n, sc);
-  int st = op.apply<T>(w, p, &sc->done, s, true, true);
+  const bool fused = npart > 0 && dpart != nullptr;
+  int st = op.apply<T>(w, p, &sc->done, s, true, true, fused ? dpart : nullptr);
   if (st) return st;
-  hipLaunchKernelGGL(cg_pap_kernel<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, w, F.mult, p,
-                     n, sc, part);
+  if (fused)
+    hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, dpart,
+                       npart, part, sc);
+  else
+    hipLaunchKernelGGL(cg_pap_kernel<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, w, F.mult,
+                       p, n, sc, part);
   hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
   hipLaunchKernelGGL((cg_update_kernel<T, TX>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x,
                      r, p, w, F.mult, F.dinv, n, sc, part);
@@ -449,8 +475,10 @@ static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
   hipGraph_t graph;
   NM_HIP_TRY(hipStreamBeginCapture(cg->s, hipStreamCaptureModeThreadLocal));
   int st = NM_OK;
+  const bool fused = cg->op->fused_dot<T>();
   for (int k = 0; k < cg->graph_iters && st == NM_OK; k++)
-    st = cg_iteration<T, TX>(*cg->op, x, r, p, w, sc, cg->part, cg->s);
+    st = cg_iteration<T, TX>(*cg->op, x, r, p, w, sc, cg->part, cg->s,
+                             fused ? cg->dpart : nullptr, fused ? cg->npart : 0);
   hipError_t e = hipStreamEndCapture(cg->s, &graph);
   if (st) return st;
   NM_HIP_TRY(e);
@@ -497,6 +525,15 @@ int cg_solver_create(CgSolver **out, Op *op, int precision) {
   if (hipHostMalloc((void **)&cg->h_sc, sizeof(CgScal) * 2, hipHostMallocDefault) != hipSuccess) {
     set_error("cg: pinned host allocation failed");
     return fail(NM_ERR_NOMEM);
+  }
+  // per-workgroup partials of the fused p.Ap (trilinear Ax variants)
+  const int code = precision ? op->ax_code<float>() : op->ax_code<double>();
+  if (code >= 200) {
+    cg->npart = ax_grid(op->nel, op->lx, code);
+    if (!(cg->dpart = cg->alloc<double>(cg->npart))) {
+      set_error("cg: out of device memory");
+      return fail(NM_ERR_NOMEM);
+    }
   }
   int st;
   if (!precision)

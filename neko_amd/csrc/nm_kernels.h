@@ -26,7 +26,19 @@ struct AxArgs {
   int64_t nel;
   int lx;
   const int *skip;     // device flag: launch is a no-op when *skip != 0
+  // trilinear geometry (elements with straight edges): G and B evaluated
+  // in the kernel from 36 half edge vectors per element instead of read
+  const T *edges = nullptr;
+  const T *zg = nullptr;   // GLL nodes
+  const T *wg = nullptr;   // GLL weights
+  T h2c = T(0);            // constant h2 (mass term h2*B*u when nonzero)
+  // optional fused partial dot sum_i u_i (A_local u)_i, one double per
+  // workgroup (grid size returned by ax_grid)
+  double *dot_part = nullptr;
 };
+
+// Workgroups of an Ax launch for a variant code (for sizing dot_part)
+int64_t ax_grid(int64_t nel, int lx, int code);
 
 template <typename T>
 int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s);

@@ -146,6 +146,189 @@ __global__ void __launch_bounds__(LX *LX *EPB)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Ax with the geometry evaluated in the kernel ("trilinear" variant).
+//
+// For an element with straight edges Neko's GLL coordinates are the
+// trilinear interpolant of its 8 vertices (dofmap_xyzlin, dofmap.f90:
+// 799-856), so the spectral derivatives coef.f90 takes of them are the
+// exact derivatives of that map: dx/dr is bilinear in (s,t), dx/ds in
+// (r,t), dx/dt in (r,s).  From 36 half edge vectors per element the kernel
+// forms the Jacobian, its cofactors (Neko's drdx.. = cofactor entries),
+// jac, G = h1 w3 C C^T / jac and B = jac w3 at every point (the formulas of
+// coef_generate_geo / coef_generate_mass, coef.f90:903-1030,1112-1156), so
+// per point it streams only u in and w out (16 B FP64) instead of 64 B.
+// Same thread layout as ax_kstep_kernel.
+//
+// edges per element: [0,12) r-edges Er[c][cy][cz] at c*4 + cy + 2*cz,
+// [12,24) s-edges Es[c][cx][cz] at 12 + c*4 + cx + 2*cz, [24,36) t-edges
+// Et[c][cx][cy] at 24 + c*4 + cx + 2*cy, each (X(hi) - X(lo)) / 2.
+// DOT: also writes the workgroup's partial sum_i u_i w_i (local w, before
+// gs), which equals the assembled p.A p of CG for a continuous, masked u.
+// ---------------------------------------------------------------------------
+template <typename T, int LX, int EPB, bool DOT>
+__global__ void __launch_bounds__(LX *LX *EPB)
+    ax_tri_kernel(T *__restrict__ w, const T *__restrict__ u,
+                  const T *__restrict__ dx, const T *__restrict__ zg,
+                  const T *__restrict__ wg, const T *__restrict__ edges, T h1c,
+                  T h2c, int64_t nel, const int *__restrict__ skip,
+                  double *__restrict__ dot_part) {
+  constexpr int P = LX * LX;
+  constexpr int N3 = LX * P;
+  __shared__ T su[EPB][N3];
+  __shared__ T sw[EPB][N3];
+  __shared__ T sr[EPB][2][P];
+  __shared__ T ss[EPB][2][P];
+  __shared__ T sedge[EPB][36];
+  __shared__ double sdot[DOT ? LX * LX * EPB : 1];
+
+  if (skip != nullptr && *skip) return;
+
+  const int t = threadIdx.x;
+  const int q = threadIdx.y;
+  const int i = t % LX;
+  const int j = t / LX;
+  const int64_t e = (int64_t)blockIdx.x * EPB + q;
+  const bool active = e < nel;
+  const int64_t base = e * (int64_t)N3 + t;
+
+  T uc[LX];
+#pragma unroll
+  for (int k = 0; k < LX; k++) {
+    uc[k] = active ? u[base + k * P] : T(0);
+    su[q][k * P + t] = uc[k];
+  }
+  for (int x = t; x < 36; x += P) sedge[q][x] = active ? edges[e * 36 + x] : T(0);
+
+  T Di[LX], Dj[LX], Dti[LX], Dtj[LX];
+#pragma unroll
+  for (int l = 0; l < LX; l++) {
+    Di[l] = dx[i + LX * l];
+    Dj[l] = dx[j + LX * l];
+    Dti[l] = dx[l + LX * i];
+    Dtj[l] = dx[l + LX * j];
+  }
+  T wc[LX];
+#pragma unroll
+  for (int m = 0; m < LX; m++) wc[m] = T(0);
+  __syncthreads();
+
+  // per-thread geometry: d/dt constant along the column, d/dr and d/ds
+  // linear in t
+  const T ri1 = (T(1) + zg[i]) * T(0.5), ri0 = T(1) - ri1;
+  const T sj1 = (T(1) + zg[j]) * T(0.5), sj0 = T(1) - sj1;
+  const T wij = wg[i] * wg[j];
+  const T *E = sedge[q];
+  T ct[3], ar0[3], ar1[3], as0[3], as1[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    ct[c] = (E[24 + c * 4 + 0] * ri0 + E[24 + c * 4 + 1] * ri1) * sj0 +
+            (E[24 + c * 4 + 2] * ri0 + E[24 + c * 4 + 3] * ri1) * sj1;
+    ar0[c] = E[c * 4 + 0] * sj0 + E[c * 4 + 1] * sj1;        // cz = 0
+    ar1[c] = E[c * 4 + 2] * sj0 + E[c * 4 + 3] * sj1;        // cz = 1
+    as0[c] = E[12 + c * 4 + 0] * ri0 + E[12 + c * 4 + 1] * ri1;
+    as1[c] = E[12 + c * 4 + 2] * ri0 + E[12 + c * 4 + 3] * ri1;
+  }
+
+#pragma unroll 1
+  for (int k = 0; k < LX; k++) {
+    const int b = k & 1;
+    const T tk1 = (T(1) + zg[k]) * T(0.5), tk0 = T(1) - tk1;
+    T a[3], bb[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      a[c] = ar0[c] * tk0 + ar1[c] * tk1;    // dX_c/dr
+      bb[c] = as0[c] * tk0 + as1[c] * tk1;   // dX_c/ds
+    }
+    const T *cc = ct;                         // dX_c/dt
+    const T rx = bb[1] * cc[2] - cc[1] * bb[2];
+    const T ry = cc[0] * bb[2] - bb[0] * cc[2];
+    const T rz = bb[0] * cc[1] - cc[0] * bb[1];
+    const T sx = cc[1] * a[2] - a[1] * cc[2];
+    const T sy = a[0] * cc[2] - cc[0] * a[2];
+    const T sz = cc[0] * a[1] - a[0] * cc[1];
+    const T tx = a[1] * bb[2] - bb[1] * a[2];
+    const T ty = bb[0] * a[2] - a[0] * bb[2];
+    const T tz = a[0] * bb[1] - bb[0] * a[1];
+    const T jac = a[0] * rx + a[1] * ry + a[2] * rz;
+    const T w3 = wij * wg[k];
+    const T sc = h1c * w3 / jac;
+    const T G11 = sc * (rx * rx + ry * ry + rz * rz);
+    const T G22 = sc * (sx * sx + sy * sy + sz * sz);
+    const T G33 = sc * (tx * tx + ty * ty + tz * tz);
+    const T G12 = sc * (rx * sx + ry * sy + rz * sz);
+    const T G13 = sc * (rx * tx + ry * ty + rz * tz);
+    const T G23 = sc * (sx * tx + sy * ty + sz * tz);
+    const T hB = h2c * jac * w3;
+
+    T dk[LX];
+#pragma unroll
+    for (int l = 0; l < LX; l++) dk[l] = dx[k + LX * l];
+    T vt = T(0), ur = T(0), us = T(0);
+    const T *plane = &su[q][k * P];
+#pragma unroll
+    for (int l = 0; l < LX; l++) {
+      vt += dk[l] * uc[l];
+      ur += Di[l] * plane[l + LX * j];
+      us += Dj[l] * plane[i + LX * l];
+    }
+    const T rr = G11 * ur + G12 * us + G13 * vt;
+    const T rs = G12 * ur + G22 * us + G23 * vt;
+    const T rt = G13 * ur + G23 * us + G33 * vt;
+    sr[q][b][t] = rr;
+    ss[q][b][t] = rs;
+#pragma unroll
+    for (int m = 0; m < LX; m++) wc[m] += dk[m] * rt;
+    __syncthreads();
+    T acc = hB * plane[t];
+#pragma unroll
+    for (int l = 0; l < LX; l++) acc += Dti[l] * sr[q][b][l + LX * j];
+#pragma unroll
+    for (int l = 0; l < LX; l++) acc += Dtj[l] * ss[q][b][i + LX * l];
+    sw[q][k * P + t] = acc;
+  }
+
+  double dsum = 0.0;
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < LX; m++) {
+      const T r = wc[m] + sw[q][m * P + t];
+      w[base + m * P] = r;
+      if (DOT) dsum += (double)r * (double)uc[m];
+    }
+  }
+  if (DOT) {
+    // fixed-order tree in LDS (workgroups need not be whole waves)
+    constexpr int NT = LX * LX * EPB;
+    constexpr int H = NT <= 1 ? 1 : (NT <= 2 ? 1 : (NT <= 4 ? 2 : (NT <= 8 ? 4 :
+                      (NT <= 16 ? 8 : (NT <= 32 ? 16 : (NT <= 64 ? 32 :
+                      (NT <= 128 ? 64 : 128)))))));
+    const int flat = q * P + t;
+    sdot[flat] = dsum;
+    __syncthreads();
+    for (int h = H; h > 0; h >>= 1) {
+      if (flat < h && flat + h < NT) sdot[flat] += sdot[flat + h];
+      __syncthreads();
+    }
+    if (flat == 0) dot_part[blockIdx.x] = sdot[0];
+  }
+}
+
+template <typename T, int LX, int EPB>
+static int ax_tri_go(const AxArgs<T> &a, hipStream_t s) {
+  dim3 block(LX * LX, EPB);
+  dim3 grid((unsigned)((a.nel + EPB - 1) / EPB));
+  if (a.nel <= 0) return NM_OK;
+  if (a.dot_part)
+    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, true>), grid, block, 0, s, a.w, a.u, a.dx,
+                       a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part);
+  else
+    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, false>), grid, block, 0, s, a.w, a.u, a.dx,
+                       a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
 template <typename T, int LX, int EPB, bool PF>
 static int ax_go(const AxArgs<T> &a, hipStream_t s) {
   dim3 block(LX * LX, EPB);
@@ -160,14 +343,46 @@ static int ax_go(const AxArgs<T> &a, hipStream_t s) {
 
 // Variant code: epb = elements per workgroup (1,2,4,8,16; 0 = default),
 // plus 100 for the G-prefetching (software-pipelined) variant.
-template <typename T, int LX>
-static int ax_epb(const AxArgs<T> &a, int code, hipStream_t s) {
-  constexpr int P = LX * LX;
-  const bool pf = code >= 100;
+static inline int ax_epb_of(int lx, int code) {
   int epb = code % 100;
   if (epb <= 0) {
     epb = 1;
-    while (P * epb * 2 <= 64 && epb < 16) epb *= 2;
+    while (lx * lx * epb * 2 <= 64 && epb < 16) epb *= 2;
+  }
+  return epb;
+}
+
+int64_t ax_grid(int64_t nel, int lx, int code) {
+  const int epb = ax_epb_of(lx, code);
+  return (nel + epb - 1) / epb;
+}
+
+template <typename T, int LX>
+static int ax_epb(const AxArgs<T> &a, int code, hipStream_t s) {
+  constexpr int P = LX * LX;
+  const bool pf = code >= 100 && code < 200;
+  const bool tri = code >= 200;
+  const int epb = ax_epb_of(LX, code);
+  if (tri != (a.edges != nullptr)) {
+    set_error("ax: variant %d needs %s geometry", code, tri ? "trilinear" : "stored");
+    return NM_ERR_UNSUPPORTED;
+  }
+  if (a.dot_part && !tri) {
+    set_error("ax: fused dot product only in the trilinear variants");
+    return NM_ERR_UNSUPPORTED;
+  }
+  if (tri) {
+    switch (epb) {
+#define NM_AXT_CASE(E)                                 \
+  case E:                                              \
+    if constexpr (P * E <= 256) return ax_tri_go<T, LX, E>(a, s); \
+    break;
+      NM_AXT_CASE(1) NM_AXT_CASE(2) NM_AXT_CASE(4) NM_AXT_CASE(8) NM_AXT_CASE(16)
+#undef NM_AXT_CASE
+      default: break;
+    }
+    set_error("ax: elements-per-workgroup %d unsupported for lx=%d", epb, LX);
+    return NM_ERR_UNSUPPORTED;
   }
 #define NM_AX_CASE(E)                                         \
   case E:                                                     \

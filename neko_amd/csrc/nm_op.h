@@ -21,6 +21,8 @@ struct OpFields {
   T *h2b = nullptr;       // h2*B (nullptr: no h2 term)
   T *mult = nullptr;
   T *dinv = nullptr;      // Jacobi: 1/gs(h1 diag(K) + h2 B)
+  T *edges = nullptr;     // trilinear geometry: 36 half edge vectors/element
+  T *zg = nullptr, *wg = nullptr;  // GLL nodes and weights
 };
 
 struct Op {
@@ -53,6 +55,8 @@ struct Op {
   std::vector<void *> owned;    // every device allocation of this op
   int64_t bytes = 0;            // their total size
   int box[3] = {0, 0, 0}, per[3] = {0, 0, 0};
+  bool trilinear = false;  // straight-edged elements: geometry in-kernel
+  double h2c = 0.0;        // constant h2 (with trilinear geometry)
   double amp = 0.0;     // box perturbation amplitude (box operators)
   uint64_t seed = 0;    // box perturbation seed
 
@@ -65,7 +69,13 @@ struct Op {
   // w = mask(gs(A_local u))
   template <typename T>
   int apply(T *w, const T *u, const int *skip, hipStream_t s, bool gs = true,
-            bool mask = true) const;
+            bool mask = true, double *dot_part = nullptr) const;
+  // variant code of the Ax kernel used for precision T
+  template <typename T>
+  int ax_code() const;
+  // can the fused p.Ap dot run (trilinear variant active)?
+  template <typename T>
+  bool fused_dot() const { return ax_code<T>() >= 200; }
   template <typename T>
   int gs(T *u, const int *skip, hipStream_t s) const;
 };

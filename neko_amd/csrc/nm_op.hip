@@ -2,6 +2,7 @@
 // C-ABI handle functions around it.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -89,8 +90,15 @@ template int Op::gs<double>(double *, const int *, hipStream_t) const;
 template int Op::gs<float>(float *, const int *, hipStream_t) const;
 
 template <typename T>
+int Op::ax_code() const {
+  return std::is_same<T, double>::value ? epb64 : epb32;
+}
+template int Op::ax_code<double>() const;
+template int Op::ax_code<float>() const;
+
+template <typename T>
 int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
-              bool do_mask) const {
+              bool do_mask, double *dot_part) const {
   const OpFields<T> &F = fields<T>();
   AxArgs<T> a;
   a.w = w;
@@ -103,7 +111,18 @@ int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
   a.nel = nel;
   a.lx = lx;
   a.skip = skip;
-  int st = launch_ax<T>(a, std::is_same<T, double>::value ? epb64 : epb32, s);
+  const int code = ax_code<T>();
+  if (code >= 200) {
+    a.edges = F.edges;
+    a.zg = F.zg;
+    a.wg = F.wg;
+    a.h2c = (T)(ifh2 ? h2c : 0.0);
+    a.dot_part = dot_part;
+  } else if (dot_part) {
+    set_error("apply: fused dot product needs the trilinear Ax variant");
+    return NM_ERR_STATE;
+  }
+  int st = launch_ax<T>(a, code, s);
   if (st) return st;
   if (do_gs) {
     st = gs<T>(w, skip, s);
@@ -113,9 +132,9 @@ int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
   return st;
 }
 template int Op::apply<double>(double *, const double *, const int *,
-                               hipStream_t, bool, bool) const;
+                               hipStream_t, bool, bool, double *) const;
 template int Op::apply<float>(float *, const float *, const int *, hipStream_t,
-                              bool, bool) const;
+                              bool, bool, double *) const;
 
 int Op::ensure32(hipStream_t s) {
   Op *op = this;
@@ -133,6 +152,14 @@ int Op::ensure32(hipStream_t s) {
   if (f64.h2b) {
     NM_ALLOC(f32.h2b, float, n);
     if (launch_d2f(f64.h2b, f32.h2b, n, s)) return NM_ERR_HIP;
+  }
+  if (f64.edges) {
+    NM_ALLOC(f32.edges, float, nel * 36);
+    if (launch_d2f(f64.edges, f32.edges, nel * 36, s)) return NM_ERR_HIP;
+    NM_ALLOC(f32.zg, float, lx);
+    if (launch_d2f(f64.zg, f32.zg, lx, s)) return NM_ERR_HIP;
+    NM_ALLOC(f32.wg, float, lx);
+    if (launch_d2f(f64.wg, f32.wg, lx, s)) return NM_ERR_HIP;
   }
   NM_ALLOC(f32.mult, float, n);
   if (launch_d2f(f64.mult, f32.mult, n, s)) return NM_ERR_HIP;
@@ -169,6 +196,53 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
     return NM_ERR_ARG;
   }
   for (int64_t i = 0; i < nlocal; i++) gi[i] = gd[i] - 1;
+  // Processing order for locality: blocks and facet pairs are independent
+  // of each other, so the device layout lists them grouped by the element
+  // of their first (lowest) local dof -- a stable counting sort that keeps
+  // Neko's order inside each group and never touches the order of the
+  // entries inside a block (the summation order of a block is unchanged).
+  {
+    const int64_t n3 = (int64_t)op->lx * op->lx * op->lx;
+    const int64_t ne = op->nel > 0 ? op->nel : 1;
+    auto regroup = [&](int64_t count, auto first, auto len, auto emit) {
+      std::vector<int64_t> start(ne + 1, 0);
+      std::vector<int32_t> key(count);
+      for (int64_t b = 0; b < count; b++) {
+        int32_t m = INT32_MAX;
+        const int64_t f = first(b);
+        for (int q = 0; q < len(b); q++) m = std::min(m, gi[f + q]);
+        key[b] = (int32_t)(m / n3);
+        start[key[b] + 1]++;
+      }
+      for (int64_t e = 0; e < ne; e++) start[e + 1] += start[e];
+      std::vector<int64_t> order(count);
+      for (int64_t b = 0; b < count; b++) order[start[key[b]]++] = b;
+      for (int64_t k = 0; k < count; k++) emit(order[k]);
+    };
+    std::vector<int32_t> g2(gi.size()), p2(nblks + 1);
+    int64_t o = 0, nb = 0;
+    regroup(
+        nblks, [&](int64_t b) { return (int64_t)ptr[b]; },
+        [&](int64_t b) { return ptr[b + 1] - ptr[b]; },
+        [&](int64_t b) {
+          p2[nb++] = (int32_t)o;
+          for (int32_t q = ptr[b]; q < ptr[b + 1]; q++) g2[o++] = gi[q];
+        });
+    p2[nblks] = (int32_t)o;
+    const int64_t p0 = op->pair0;
+    regroup(
+        op->npairs, [&](int64_t pp) { return p0 + 2 * pp; }, [&](int64_t) { return 2; },
+        [&](int64_t pp) {
+          g2[o++] = gi[p0 + 2 * pp];
+          g2[o++] = gi[p0 + 2 * pp + 1];
+        });
+    if (o != nlocal) {
+      set_error("gs: regrouping lost entries (%lld of %lld)", (long long)o, (long long)nlocal);
+      return NM_ERR_STATE;
+    }
+    gi.swap(g2);
+    ptr.swap(p2);
+  }
   NM_ALLOC(op->blk_ptr, int32_t, nblks + 1);
   NM_ALLOC(op->gidx, int32_t, nlocal);
   NM_HIP_TRY(hipMemcpyAsync(op->blk_ptr, ptr.data(), sizeof(int32_t) * (nblks + 1),
@@ -323,6 +397,37 @@ int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
   if (op->ifh2) {
     if (!(op->f64.h2b = op->alloc<double>(n))) return fail(NM_ERR_NOMEM);
     launch_vec<double>(V_CMULT2, op->f64.h2b, op->B, nullptr, nullptr, h2, n, s);
+  }
+  // trilinear geometry for the in-kernel G / B variants: half edge vectors
+  // of every element (corner (cx,cy,cz) at vxyz[(8e + 4cz + 2cy + cx)*3])
+  {
+    std::vector<double> ed(nel * 36);
+    for (int64_t e = 0; e < nel; e++) {
+      auto V = [&](int cx, int cy, int cz, int c) {
+        return vxyz[(e * 8 + 4 * cz + 2 * cy + cx) * 3 + c];
+      };
+      double *E = ed.data() + e * 36;
+      for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++) {
+            E[c * 4 + a + 2 * b] = 0.5 * (V(1, a, b, c) - V(0, a, b, c));
+            E[12 + c * 4 + a + 2 * b] = 0.5 * (V(a, 1, b, c) - V(a, 0, b, c));
+            E[24 + c * 4 + a + 2 * b] = 0.5 * (V(a, b, 1, c) - V(a, b, 0, c));
+          }
+    }
+    if (!(op->f64.edges = op->alloc<double>(nel * 36)) || !(op->f64.zg = op->alloc<double>(lx)) ||
+        !(op->f64.wg = op->alloc<double>(lx)))
+      return fail(NM_ERR_NOMEM);
+    if (hipMemcpyAsync(op->f64.edges, ed.data(), sizeof(double) * nel * 36,
+                       hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(op->f64.zg, z.data(), sizeof(double) * lx, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(op->f64.wg, w.data(), sizeof(double) * lx, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail(NM_ERR_HIP);
+    op->trilinear = true;
+    op->h2c = h2;
   }
   // gs maps + mask on the host (Neko's layout), then device layout
   int64_t nlocal, foff, nb;
@@ -484,6 +589,13 @@ int nm_op_gs_maps(const nm_op_t *h, int32_t *dg, int32_t *gd, int32_t *blk_len,
 }
 
 int nm_op_set_epb(nm_op_t *h, int epb64, int epb32) {
+  const Op &op = h->op;
+  const bool tri_ok = op.trilinear && !op.f64.h1;
+  if ((epb64 >= 200 || epb32 >= 200) && !tri_ok) {
+    set_error("nm_op_set_epb: trilinear variants need a straight-edged mesh "
+              "(box operator) and constant h1");
+    return NM_ERR_UNSUPPORTED;
+  }
   h->op.epb64 = epb64;
   h->op.epb32 = epb32;
   return NM_OK;
@@ -524,7 +636,8 @@ int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
   NM_HIP_TRY(hipEventCreate(&e0));
   NM_HIP_TRY(hipEventCreate(&e1));
   launch_vec<double>(V_CFILL, u, nullptr, nullptr, nullptr, 1.0, n, s);
-  const int codes[] = {0, 2, 4, 100, 102, 104};
+  std::vector<int> codes = {0, 2, 4, 100, 102, 104};
+  if (op.trilinear && !op.f64.h1) codes.insert(codes.end(), {201, 202, 204});
   auto timeit = [&](auto fn) -> float {
     for (int r = 0; r < 2; r++) fn();
     hipEventRecord(e0, s);
