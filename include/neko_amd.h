@@ -109,6 +109,7 @@ typedef struct nm_cg_s nm_cg_t;
 typedef struct nm_hsmg_s nm_hsmg_t;
 typedef struct nm_gmres_s nm_gmres_t;
 typedef struct nm_server_s nm_server_t;
+typedef struct nm_batch_s nm_batch_t;
 
 typedef struct {
   int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
@@ -246,6 +247,26 @@ int nm_gmres_create(nm_gmres_t **g, nm_op_t *op, int pc, nm_hsmg_t *hsmg,
 void nm_gmres_free(nm_gmres_t *g);
 int nm_gmres_solve(nm_gmres_t *g, double *x, const double *f, double abs_tol,
                    int max_iter, nm_gmres_monitor_t *mon, hipStream_t s);
+
+/* --- batched solves (no counterpart in Neko; same CG semantics) -------- */
+/* Many box operators (nm_op_create_box, constant h1) of uneven size and
+ * order solved together: one FP32 CG iteration of every problem per
+ * captured graph step (one batched Ax launch per lx, one merged gs),
+ * FP64 residual replacement per problem. */
+int nm_batch_create(nm_batch_t **b, int nproblems, nm_op_t *const *ops,
+                    hipStream_t s);
+void nm_batch_free(nm_batch_t *b);
+/* per-lx elements-per-workgroup autotuner (recaptures the graph) */
+int nm_batch_autotune(nm_batch_t *b, int reps, int *lx_out, int *epb_out,
+                      int *ngroups);
+/* x[i], f[i]: device FP64 arrays of problem i; rel_tol relative to each
+ * problem's initial residual; delta: FP32 reduction between residual
+ * replacements (<= 0: 0.1); mon[i] per problem */
+int nm_batch_solve(nm_batch_t *b, double *const *x, const double *const *f,
+                   double rel_tol, int max_iter, double delta,
+                   nm_monitor_t *mon, hipStream_t stream);
+int nm_batch_info(const nm_batch_t *b, int64_t *total_dof,
+                  int64_t *device_bytes);
 
 /* --- serving loop (no counterpart in Neko; drives the CG above) --------- */
 typedef struct {

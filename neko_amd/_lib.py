@@ -103,6 +103,12 @@ SIGNATURES = {
     "nm_gmres_create": (_i, [C.POINTER(_vp), _vp, _i, _vp, _i]),
     "nm_gmres_free": (None, [_vp]),
     "nm_gmres_solve": (_i, [_vp, _vp, _vp, _d, _i, C.POINTER(Monitor), _vp]),
+    "nm_batch_create": (_i, [C.POINTER(_vp), _i, C.POINTER(_vp), _vp]),
+    "nm_batch_free": (None, [_vp]),
+    "nm_batch_autotune": (_i, [_vp, _i, _pi, _pi, _pi]),
+    "nm_batch_solve": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), _d, _i, _d,
+                            C.POINTER(Monitor), _vp]),
+    "nm_batch_info": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nm_server_create": (_i, [C.POINTER(_vp), _d, _i, _i, _i, _i]),
     "nm_server_free": (None, [_vp]),
     "nm_server_add_box_mesh": (_i, [_vp, _i, _i, _i, _i, _d, C.c_uint64, _i,

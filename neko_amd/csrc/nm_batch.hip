@@ -1,0 +1,577 @@
+// Batched solves: many independent box problems of uneven size and order
+// advance together, one FP32 CG iteration of all of them per captured
+// graph step, with FP64 residual replacement per problem (the mixed
+// scheme of nm_cg.hip).
+//
+// Layout: every problem's vectors are one slice of shared batch vectors,
+// padded to whole 256-dof chunks (padding stays zero: dinv = mult = 0), so
+// a chunk belongs to exactly one problem.  Per iteration:
+//   p = dinv r + beta_i p            one launch over all chunks
+//   w = A_i p (FP32, trilinear)      one launch per lx, workgroups tagged
+//                                    with their problem; fused p.Ap partials
+//   w = mask(gs(w))                  merged gs maps (offsets added), one
+//                                    gs + one mask launch for all problems
+//   alpha_i, x32 += a p, r -= a w, rtr_i, rtz_i, beta_i, stop test
+// Problems that have stopped skip their work chunk by chunk.  Reductions
+// are per problem in a fixed order (chunk partials, then one workgroup per
+// problem), so results do not depend on the other problems of the batch.
+// Neko solves one system at a time (krylov.f90 ksp_t%solve); this has no
+// counterpart there.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
+#include "nm_cg.h"
+#include "nm_common.h"
+#include "nm_kernels.h"
+#include "nm_op.h"
+
+namespace nm {
+
+constexpr int CHUNK = 256;
+
+__device__ inline double bwsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// block of CHUNK threads: sums of a and b to thread 0
+__device__ inline void bsum_pair(double &a, double &b) {
+  __shared__ double sh[2][CHUNK / 64];
+  a = bwsum(a);
+  b = bwsum(b);
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ra = 0.0, rb = 0.0;
+    for (int q = 0; q < CHUNK / 64; q++) {
+      ra += sh[0][q];
+      rb += sh[1][q];
+    }
+    a = ra;
+    b = rb;
+  }
+  __syncthreads();
+}
+
+// p = dinv r + beta p
+__global__ void __launch_bounds__(CHUNK)
+    bp_kernel(float *__restrict__ p, const float *__restrict__ r,
+              const float *__restrict__ dinv, const int32_t *__restrict__ chunk_inst,
+              int64_t nchunk, const CgScal *__restrict__ sc) {
+  for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const CgScal &S = sc[chunk_inst[c]];
+    if (S.done) continue;
+    const float beta = (float)S.beta;
+    const int64_t i = c * CHUNK + threadIdx.x;
+    p[i] = dinv[i] * r[i] + beta * p[i];
+  }
+}
+
+// alpha_i from the fused p.Ap partials of the problem's workgroups
+__global__ void __launch_bounds__(CHUNK)
+    balpha_kernel(const double *__restrict__ dot_part, const BatchInst *__restrict__ inst,
+                  CgScal *__restrict__ sc) {
+  CgScal &S = sc[blockIdx.x];
+  if (S.done) return;
+  const BatchInst I = inst[blockIdx.x];
+  double a = 0.0, b = 0.0;
+  for (int64_t q = I.blk0 + threadIdx.x; q < I.blk0 + I.nblk; q += CHUNK) a += dot_part[q];
+  bsum_pair(a, b);
+  if (threadIdx.x == 0) {
+    S.pap = a;
+    S.alpha = S.rtz / a;
+  }
+}
+
+// x32 += alpha p; r -= alpha w; chunk partials of rtr and rtz
+__global__ void __launch_bounds__(CHUNK)
+    bupd_kernel(float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
+                const float *__restrict__ w, const float *__restrict__ mult,
+                const float *__restrict__ dinv, const int32_t *__restrict__ chunk_inst,
+                int64_t nchunk, const CgScal *__restrict__ sc, double *__restrict__ part) {
+  for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const CgScal &S = sc[chunk_inst[c]];
+    if (S.done) continue;  // uniform over the block
+    const float alpha = (float)S.alpha;
+    const int64_t i = c * CHUNK + threadIdx.x;
+    x[i] = x[i] + alpha * p[i];
+    const float ri = r[i] - alpha * w[i];
+    r[i] = ri;
+    double a = (double)ri * (double)ri * (double)mult[i];
+    double b = a * (double)dinv[i];
+    bsum_pair(a, b);
+    if (threadIdx.x == 0) {
+      part[c] = a;
+      part[nchunk + c] = b;
+    }
+  }
+}
+
+// per problem: rtr, rtz, beta, norm, stop test (cg_fin semantics)
+__global__ void __launch_bounds__(CHUNK)
+    bfin_kernel(const double *__restrict__ part, int64_t nchunk,
+                const BatchInst *__restrict__ inst, CgScal *__restrict__ sc) {
+  CgScal &S = sc[blockIdx.x];
+  if (S.done) return;
+  const BatchInst I = inst[blockIdx.x];
+  double a = 0.0, b = 0.0;
+  for (int64_t c = I.c0 + threadIdx.x; c < I.c1; c += CHUNK) {
+    a += part[c];
+    b += part[nchunk + c];
+  }
+  bsum_pair(a, b);
+  if (threadIdx.x == 0) {
+    S.rtr = a;
+    S.rtz_old = S.rtz;
+    S.rtz = b;
+    S.beta = b / S.rtz_old;
+    S.rnorm = sqrt(a) * S.norm_fac;
+    S.iter = S.iter + 1;
+    if (S.rnorm < S.tol || S.iter >= S.max_iter) S.done = 1;
+  }
+}
+
+// float copy of a double field into a batch slice
+__global__ void d2f_slice_kernel(float *__restrict__ dst, const double *__restrict__ src,
+                                 int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = (float)src[i];
+}
+
+struct LxGroup {
+  int lx = 0, epb = 1;
+  int64_t blk_base = 0, nblocks = 0;
+  std::vector<int> members;
+  const float *dx = nullptr, *zg = nullptr, *wg = nullptr;
+};
+
+struct Batch {
+  std::vector<Op *> ops;
+  std::vector<int64_t> voff, vlen;   // slice offsets / padded lengths
+  int64_t ntot = 0, nchunk = 0, nblk_total = 0;
+  std::vector<LxGroup> groups;
+  std::vector<BatchInst> h_inst;
+  hipStream_t s = nullptr;
+  // device
+  BatchInst *inst = nullptr;
+  int32_t *blk_inst = nullptr, *chunk_inst = nullptr;
+  int32_t *gs_ptr = nullptr, *gs_idx = nullptr, *msk = nullptr;
+  int64_t gs_nblks = 0, gs_pair0 = 0, gs_npairs = 0, nmsk = 0;
+  int gs_maxblk = 0;
+  double *x = nullptr, *f = nullptr, *w64 = nullptr, *part = nullptr, *dot_part = nullptr;
+  double *rpart = nullptr;  // scratch of the per-problem replacement
+  float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
+  float *dinv = nullptr, *mult = nullptr;
+  CgScal *sc = nullptr, *h_sc = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_iters = 8;
+  int64_t bytes = 0;
+  std::vector<void *> owned;
+  ~Batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (s) (void)hipStreamSynchronize(s);
+    for (void *q : owned) (void)hipFree(q);
+    if (h_sc) (void)hipHostFree(h_sc);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  template <typename T>
+  T *alloc(int64_t cnt) {
+    void *q = nullptr;
+    const size_t b = sizeof(T) * (size_t)std::max<int64_t>(cnt, 1);
+    if (hipMalloc(&q, b) != hipSuccess) return nullptr;
+    owned.push_back(q);
+    bytes += (int64_t)b;
+    return (T *)q;
+  }
+};
+
+static int batch_iteration(Batch *B, hipStream_t s) {
+  const unsigned vb = (unsigned)std::min<int64_t>(B->nchunk, 65536);
+  hipLaunchKernelGGL(bp_kernel, dim3(vb), dim3(CHUNK), 0, s, B->p32, B->r32, B->dinv,
+                     B->chunk_inst, B->nchunk, B->sc);
+  for (const LxGroup &g : B->groups) {
+    AxBatchArgs a;
+    a.w = B->w32;
+    a.u = B->p32;
+    a.dx = g.dx;
+    a.zg = g.zg;
+    a.wg = g.wg;
+    a.inst = B->inst;
+    a.blk_inst = B->blk_inst;
+    a.blk_base = g.blk_base;
+    a.nblocks = g.nblocks;
+    a.lx = g.lx;
+    a.epb = g.epb;
+    a.scal = B->sc;
+    a.scal_size = (int)sizeof(CgScal);
+    a.done_off = (int)offsetof(CgScal, done);
+    a.dot_part = B->dot_part;
+    int st = launch_ax_batch(a, s);
+    if (st) return st;
+  }
+  GsArgs<float> ga{B->w32, B->gs_ptr, B->gs_idx, B->gs_nblks, B->gs_pair0, B->gs_npairs,
+                   nullptr};
+  ga.variant = B->gs_maxblk <= 8 ? 1 : 0;
+  int st = launch_gs<float>(ga, s);
+  if (st) return st;
+  if ((st = launch_mask<float>(B->w32, B->msk, B->nmsk, nullptr, s))) return st;
+  const unsigned ni = (unsigned)B->ops.size();
+  hipLaunchKernelGGL(balpha_kernel, dim3(ni), dim3(CHUNK), 0, s, B->dot_part, B->inst, B->sc);
+  hipLaunchKernelGGL(bupd_kernel, dim3(vb), dim3(CHUNK), 0, s, B->x32, B->r32, B->p32, B->w32,
+                     B->mult, B->dinv, B->chunk_inst, B->nchunk, B->sc, B->part);
+  hipLaunchKernelGGL(bfin_kernel, dim3(ni), dim3(CHUNK), 0, s, B->part, B->nchunk, B->inst,
+                     B->sc);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+static int batch_capture(Batch *B) {
+  if (B->graph) {
+    NM_HIP_TRY(hipGraphExecDestroy(B->graph));
+    B->graph = nullptr;
+  }
+  hipGraph_t g;
+  NM_HIP_TRY(hipStreamBeginCapture(B->s, hipStreamCaptureModeThreadLocal));
+  int st = NM_OK;
+  for (int k = 0; k < B->graph_iters && st == NM_OK; k++) st = batch_iteration(B, B->s);
+  hipError_t e = hipStreamEndCapture(B->s, &g);
+  if (st) return st;
+  NM_HIP_TRY(e);
+  NM_HIP_TRY(hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0));
+  NM_HIP_TRY(hipGraphDestroy(g));
+  return NM_OK;
+}
+
+// Build the per-group workgroup tables for the current elements-per-
+// workgroup choices.
+static int batch_tables(Batch *B) {
+  const int ni = (int)B->ops.size();
+  int64_t gb = 0;
+  std::vector<int32_t> bi;
+  for (LxGroup &g : B->groups) {
+    g.blk_base = gb;
+    for (int m : g.members) {
+      BatchInst &I = B->h_inst[m];
+      I.blk0 = gb;
+      I.nblk = (I.nel + g.epb - 1) / g.epb;
+      for (int64_t q = 0; q < I.nblk; q++) bi.push_back(m);
+      gb += I.nblk;
+    }
+    g.nblocks = gb - g.blk_base;
+  }
+  if (gb > B->nblk_total) {
+    set_error("batch: workgroup table grew beyond its allocation");
+    return NM_ERR_STATE;
+  }
+  NM_HIP_TRY(hipMemcpyAsync(B->blk_inst, bi.data(), sizeof(int32_t) * bi.size(),
+                            hipMemcpyHostToDevice, B->s));
+  NM_HIP_TRY(hipMemcpyAsync(B->inst, B->h_inst.data(), sizeof(BatchInst) * ni,
+                            hipMemcpyHostToDevice, B->s));
+  NM_HIP_TRY(hipStreamSynchronize(B->s));
+  return NM_OK;
+}
+
+}  // namespace nm
+
+using namespace nm;
+
+struct nm_batch_s {
+  Batch b;
+};
+
+extern "C" {
+
+int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_t s) {
+  *out = nullptr;
+  if (ninst < 1) {
+    set_error("nm_batch_create: no problems");
+    return NM_ERR_ARG;
+  }
+  nm_batch_t *h = new nm_batch_t;
+  Batch *B = &h->b;
+  auto fail = [&](int c) {
+    delete h;
+    return c;
+  };
+  if (hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking) != hipSuccess) return fail(NM_ERR_HIP);
+  int64_t off = 0, ms = 0, nblk_max = 0;
+  int64_t gs_entries = 0, gs_blocks = 0, gs_pairs = 0;
+  for (int i = 0; i < ninst; i++) {
+    Op *op = &ops[i]->op;
+    if (!op->trilinear || op->f64.h1) {
+      set_error("nm_batch_create: problem %d is not a box operator with constant h1", i);
+      return fail(NM_ERR_ARG);
+    }
+    int st = op->ensure32(s);
+    if (st) return fail(st);
+    B->ops.push_back(op);
+    B->voff.push_back(off);
+    const int64_t len = (op->n + CHUNK - 1) / CHUNK * CHUNK;
+    B->vlen.push_back(len);
+    off += len;
+    ms += op->nmsk;
+    nblk_max += op->nel;  // epb >= 1
+    gs_entries += op->nlocal;
+    gs_blocks += op->nblks;
+    gs_pairs += op->npairs;
+  }
+  if (off >= ((int64_t)1 << 31)) {
+    set_error("nm_batch_create: %lld dofs exceed the 32-bit gs index range", (long long)off);
+    return fail(NM_ERR_ARG);
+  }
+  B->ntot = off;
+  B->nchunk = off / CHUNK;
+  B->nblk_total = nblk_max;
+  // groups by lx
+  for (int i = 0; i < ninst; i++) {
+    const int lx = B->ops[i]->lx;
+    auto it = std::find_if(B->groups.begin(), B->groups.end(),
+                           [&](const LxGroup &g) { return g.lx == lx; });
+    if (it == B->groups.end()) {
+      LxGroup g;
+      g.lx = lx;
+      g.dx = B->ops[i]->f32.dx;
+      g.zg = B->ops[i]->f32.zg;
+      g.wg = B->ops[i]->f32.wg;
+      g.epb = 1;
+      B->groups.push_back(g);
+      it = B->groups.end() - 1;
+    }
+    it->members.push_back(i);
+  }
+  // device vectors
+  const int64_t n = B->ntot;
+  if (!(B->x = B->alloc<double>(n)) || !(B->f = B->alloc<double>(n)) ||
+      !(B->w64 = B->alloc<double>(n)) || !(B->x32 = B->alloc<float>(n)) ||
+      !(B->r32 = B->alloc<float>(n)) || !(B->p32 = B->alloc<float>(n)) ||
+      !(B->w32 = B->alloc<float>(n)) || !(B->dinv = B->alloc<float>(n)) ||
+      !(B->mult = B->alloc<float>(n)) || !(B->part = B->alloc<double>(2 * B->nchunk)) ||
+      !(B->dot_part = B->alloc<double>(nblk_max)) ||
+      !(B->rpart = B->alloc<double>(2 * RED_BLOCKS)) || !(B->sc = B->alloc<CgScal>(ninst)) ||
+      !(B->inst = B->alloc<BatchInst>(ninst)) || !(B->blk_inst = B->alloc<int32_t>(nblk_max)) ||
+      !(B->chunk_inst = B->alloc<int32_t>(B->nchunk)) ||
+      !(B->gs_ptr = B->alloc<int32_t>(gs_blocks + 1)) ||
+      !(B->gs_idx = B->alloc<int32_t>(gs_entries)) || !(B->msk = B->alloc<int32_t>(ms))) {
+    set_error("nm_batch_create: out of device memory");
+    return fail(NM_ERR_NOMEM);
+  }
+  if (hipHostMalloc((void **)&B->h_sc, sizeof(CgScal) * ninst, hipHostMallocDefault) != hipSuccess)
+    return fail(NM_ERR_NOMEM);
+  // zero every vector (padding must stay zero), then the FP32 fields
+  for (void *v : {(void *)B->x, (void *)B->f, (void *)B->w64})
+    if (launch_vec<double>(V_RZERO, (double *)v, nullptr, nullptr, nullptr, 0.0, n, B->s))
+      return fail(NM_ERR_HIP);
+  for (void *v : {(void *)B->x32, (void *)B->r32, (void *)B->p32, (void *)B->w32,
+                  (void *)B->dinv, (void *)B->mult})
+    if (launch_vec<float>(V_RZERO, (float *)v, nullptr, nullptr, nullptr, 0.0f, n, B->s))
+      return fail(NM_ERR_HIP);
+  // instance table, chunk table, merged gs and mask (offsets added)
+  B->h_inst.resize(ninst);
+  std::vector<int32_t> cinst(B->nchunk), gptr, gidx, gpairs, msk;
+  gptr.reserve(gs_blocks + 1);
+  gidx.reserve(gs_entries);
+  gpairs.reserve(2 * gs_pairs);
+  msk.reserve(ms);
+  int maxblk = 0;
+  for (int i = 0; i < ninst; i++) {
+    Op *op = B->ops[i];
+    BatchInst &I = B->h_inst[i];
+    I.voff = B->voff[i];
+    I.nel = op->nel;
+    I.edges = op->f32.edges;
+    I.h1c = (float)op->h1c;
+    I.h2c = (float)(op->ifh2 ? op->h2c : 0.0);
+    I.c0 = B->voff[i] / CHUNK;
+    I.c1 = (B->voff[i] + B->vlen[i]) / CHUNK;
+    for (int64_t c = I.c0; c < I.c1; c++) cinst[c] = i;
+    std::vector<int32_t> ptr(op->nblks + 1), gi(op->nlocal), mk(op->nmsk);
+    if (hipMemcpy(ptr.data(), op->blk_ptr, sizeof(int32_t) * (op->nblks + 1),
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(gi.data(), op->gidx, sizeof(int32_t) * op->nlocal, hipMemcpyDeviceToHost) !=
+            hipSuccess ||
+        (op->nmsk > 0 && hipMemcpy(mk.data(), op->msk, sizeof(int32_t) * op->nmsk,
+                                   hipMemcpyDeviceToHost) != hipSuccess))
+      return fail(NM_ERR_HIP);
+    const int32_t base = (int32_t)gidx.size();
+    for (int64_t b = 0; b < op->nblks; b++) gptr.push_back(base + ptr[b]);
+    for (int64_t q = 0; q < ptr[op->nblks]; q++) gidx.push_back(gi[q] + (int32_t)I.voff);
+    for (int64_t q = op->pair0; q < op->nlocal; q++) gpairs.push_back(gi[q] + (int32_t)I.voff);
+    for (int64_t q = 0; q < op->nmsk; q++) msk.push_back(mk[q] + (int32_t)I.voff);
+    maxblk = std::max(maxblk, op->max_blk);
+    // FP32 Jacobi and multiplicity into the slices
+    const unsigned g = (unsigned)std::min<int64_t>((op->n + 255) / 256, 65536);
+    hipLaunchKernelGGL(d2f_slice_kernel, dim3(g), dim3(256), 0, B->s, B->dinv + I.voff,
+                       op->f64.dinv, op->n);
+    hipLaunchKernelGGL(d2f_slice_kernel, dim3(g), dim3(256), 0, B->s, B->mult + I.voff,
+                       op->f64.mult, op->n);
+  }
+  gptr.push_back((int32_t)gidx.size());
+  B->gs_nblks = gs_blocks;
+  B->gs_pair0 = (int64_t)gidx.size();
+  B->gs_npairs = (int64_t)gpairs.size() / 2;
+  B->gs_maxblk = maxblk;
+  gidx.insert(gidx.end(), gpairs.begin(), gpairs.end());
+  B->nmsk = (int64_t)msk.size();
+  if (hipMemcpyAsync(B->chunk_inst, cinst.data(), sizeof(int32_t) * cinst.size(),
+                     hipMemcpyHostToDevice, B->s) != hipSuccess ||
+      hipMemcpyAsync(B->gs_ptr, gptr.data(), sizeof(int32_t) * gptr.size(),
+                     hipMemcpyHostToDevice, B->s) != hipSuccess ||
+      hipMemcpyAsync(B->gs_idx, gidx.data(), sizeof(int32_t) * gidx.size(),
+                     hipMemcpyHostToDevice, B->s) != hipSuccess ||
+      (B->nmsk > 0 && hipMemcpyAsync(B->msk, msk.data(), sizeof(int32_t) * msk.size(),
+                                     hipMemcpyHostToDevice, B->s) != hipSuccess) ||
+      hipStreamSynchronize(B->s) != hipSuccess)
+    return fail(NM_ERR_HIP);
+  int st = batch_tables(B);
+  if (st) return fail(st);
+  if ((st = batch_capture(B))) return fail(st);
+  *out = h;
+  return NM_OK;
+}
+
+void nm_batch_free(nm_batch_t *h) { delete h; }
+
+// Autotuner: per lx group, time the batched FP32 Ax for every
+// elements-per-workgroup choice on the batch's own data, keep the fastest
+// and recapture the iteration graph.  epb_out[g] receives the choice of
+// group g (groups in order of first appearance; up to 16).
+int nm_batch_autotune(nm_batch_t *h, int reps, int *lx_out, int *epb_out, int *ngroups) {
+  Batch *B = &h->b;
+  if (reps <= 0) reps = 3;
+  std::vector<CgScal> zero(B->ops.size());
+  std::memset(zero.data(), 0, sizeof(CgScal) * zero.size());
+  NM_HIP_TRY(hipMemcpyAsync(B->sc, zero.data(), sizeof(CgScal) * zero.size(),
+                            hipMemcpyHostToDevice, B->s));
+  hipEvent_t e0, e1;
+  NM_HIP_TRY(hipEventCreate(&e0));
+  NM_HIP_TRY(hipEventCreate(&e1));
+  for (size_t gi = 0; gi < B->groups.size(); gi++) {
+    LxGroup &g = B->groups[gi];
+    float best = 1e30f;
+    int bepb = 1;
+    for (int epb : {1, 2, 4, 8}) {
+      if (g.lx * g.lx * epb > 256) continue;
+      g.epb = epb;
+      int st = batch_tables(B);
+      if (st) return st;
+      AxBatchArgs a;
+      a.w = B->w32; a.u = B->p32; a.dx = g.dx; a.zg = g.zg; a.wg = g.wg;
+      a.inst = B->inst; a.blk_inst = B->blk_inst; a.blk_base = g.blk_base;
+      a.nblocks = g.nblocks; a.lx = g.lx; a.epb = epb; a.scal = B->sc;
+      a.scal_size = (int)sizeof(CgScal); a.done_off = (int)offsetof(CgScal, done);
+      a.dot_part = B->dot_part;
+      for (int r = 0; r < 2; r++)
+        if ((st = launch_ax_batch(a, B->s))) return st;
+      NM_HIP_TRY(hipEventRecord(e0, B->s));
+      for (int r = 0; r < reps; r++) launch_ax_batch(a, B->s);
+      NM_HIP_TRY(hipEventRecord(e1, B->s));
+      NM_HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.f;
+      NM_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bepb = epb;
+      }
+    }
+    g.epb = bepb;
+    if (gi < 16) {
+      if (lx_out) lx_out[gi] = g.lx;
+      if (epb_out) epb_out[gi] = g.epb;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ngroups) *ngroups = (int)B->groups.size();
+  int st = batch_tables(B);
+  if (st) return st;
+  return batch_capture(B);
+}
+
+// Solve every problem i: A_i x_i = f_i to rel_tol of its initial residual
+// (mixed precision, FP64 residual replacement with reduction `delta` per
+// segment; <= 0: 0.1).  x, f: arrays of per-problem device pointers (host
+// arrays).  mon: per-problem monitors.  Ordered after / before `stream`.
+int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, double rel_tol,
+                   int max_iter, double delta, nm_monitor_t *mon, hipStream_t stream) {
+  Batch *B = &h->b;
+  const int ni = (int)B->ops.size();
+  if (delta <= 0.0) delta = 0.1;
+  hipEvent_t ev;
+  NM_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  NM_HIP_TRY(hipEventRecord(ev, stream));
+  NM_HIP_TRY(hipStreamWaitEvent(B->s, ev, 0));
+  for (int i = 0; i < ni; i++) {
+    CgScal init{};
+    init.norm_fac = 1.0 / std::sqrt(B->ops[i]->volume);
+    init.scale = 1.0;
+    B->h_sc[i] = init;
+  }
+  NM_HIP_TRY(hipMemcpyAsync(B->sc, B->h_sc, sizeof(CgScal) * ni, hipMemcpyHostToDevice, B->s));
+  int st;
+  for (int i = 0; i < ni; i++) {
+    const int64_t o = B->voff[i];
+    NM_HIP_TRY(hipMemcpyAsync(B->f + o, f[i], sizeof(double) * B->ops[i]->n,
+                              hipMemcpyDeviceToDevice, B->s));
+    if ((st = mx_slice_begin(*B->ops[i], B->x + o, B->x32 + o, B->r32 + o, B->p32 + o,
+                             B->f + o, B->sc + i, B->rpart, 0.0, rel_tol, delta, max_iter,
+                             B->s)))
+      return st;
+  }
+  std::vector<char> state(ni, 0);  // 0 running, 1 finished
+  int left = ni, launches = 0;
+  const int limit = max_iter / B->graph_iters + 64;
+  while (left > 0) {
+    NM_HIP_TRY(hipGraphLaunch(B->graph, B->s));
+    launches++;
+    NM_HIP_TRY(hipMemcpyAsync(B->h_sc, B->sc, sizeof(CgScal) * ni, hipMemcpyDeviceToHost, B->s));
+    NM_HIP_TRY(hipStreamSynchronize(B->s));
+    for (int i = 0; i < ni; i++) {
+      if (state[i]) continue;
+      const CgScal &S = B->h_sc[i];
+      if (S.fin) {
+        state[i] = 1;
+        left--;
+      } else if (S.done) {
+        const int64_t o = B->voff[i];
+        if ((st = mx_slice_replace(*B->ops[i], B->x + o, B->x32 + o, B->r32 + o, B->f + o,
+                                   B->w64 + o, B->sc + i, B->rpart, B->s)))
+          return st;
+      }
+    }
+    if (launches > limit) {
+      set_error("nm_batch_solve: problems did not stop after %d graph launches", launches);
+      return NM_ERR_STATE;
+    }
+  }
+  for (int i = 0; i < ni; i++) {
+    const CgScal &S = B->h_sc[i];
+    mon[i].iter = S.iter;
+    mon[i].outer = S.updates;
+    mon[i].res_start = S.res_start;
+    mon[i].res_final = S.true_rnorm;
+    mon[i].converged = S.converged;
+    NM_HIP_TRY(hipMemcpyAsync(x[i], B->x + B->voff[i], sizeof(double) * B->ops[i]->n,
+                              hipMemcpyDeviceToDevice, B->s));
+  }
+  NM_HIP_TRY(hipEventRecord(ev, B->s));
+  NM_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
+  NM_HIP_TRY(hipEventDestroy(ev));
+  return NM_OK;
+}
+
+int nm_batch_info(const nm_batch_t *h, int64_t *total_dof, int64_t *device_bytes) {
+  *total_dof = h->b.ntot;
+  *device_bytes = h->b.bytes;
+  return NM_OK;
+}
+
+}  // extern "C"

@@ -13,7 +13,16 @@
 
 namespace nm {
 
-struct CgScal;
+// Device-resident scalars of one CG solve.
+struct CgScal {
+  double rtz, rtz_old, pap, rtr, alpha, beta, rnorm, tol, norm_fac, res_start;
+  double scale;       // start: r = scale * f
+  double tol_final;   // mixed: final tolerance on the true residual
+  double delta;       // mixed: recurrence reduction between replacements
+  double true_rnorm;  // mixed: norm of the last true residual
+  int iter, done, max_iter, converged, updates;
+  int fin;            // mixed: solve finished (set with the true residual)
+};
 
 struct CgSolver {
   Op *op = nullptr;
@@ -57,5 +66,14 @@ int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
 // Non-blocking: 1 when the solve has finished, 0 while running, < 0 error.
 int cg_poll(CgSolver *cg);
 int cg_wait(CgSolver *cg);
+
+// Mixed-precision building blocks on one problem's slice of (possibly
+// shared) vectors: start (x = 0, r32 = f, scalars) and residual
+// replacement (x += x32, r32 = f - A x with the FP64 operator).
+int mx_slice_begin(const Op &op, double *x, float *x32, float *r32, float *p32,
+                   const double *f, CgScal *sc, double *part, double abs_tol, double rel,
+                   double delta, int max_iter, hipStream_t s);
+int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
+                     double *w, CgScal *sc, double *part, hipStream_t s);
 
 }  // namespace nm

@@ -22,15 +22,6 @@
 
 namespace nm {
 
-struct CgScal {
-  double rtz, rtz_old, pap, rtr, alpha, beta, rnorm, tol, norm_fac, res_start;
-  double scale;       // start: r = scale * f
-  double tol_final;   // mixed: final tolerance on the true residual
-  double delta;       // mixed: recurrence reduction between replacements
-  double true_rnorm;  // mixed: norm of the last true residual
-  int iter, done, max_iter, converged, updates;
-};
-
 // ---- kernels ---------------------------------------------------------------
 __device__ inline double wsum(double v) {
 #pragma unroll
@@ -269,6 +260,7 @@ __global__ void mx_init_fin_kernel(const double *__restrict__ part, CgScal *__re
     sc->converged = 0;
     sc->done = (fabs(sc->rnorm) < 2.220446049250313e-16 || sc->rnorm < sc->tol_final) ? 1 : 0;
     if (sc->done) sc->converged = 1;
+    sc->fin = sc->done;
   }
 }
 
@@ -327,9 +319,11 @@ __global__ void mx_resid_fin_kernel(const double *__restrict__ part, CgScal *__r
     if (sc->rnorm < sc->tol_final || fabs(sc->rnorm) < 2.220446049250313e-16) {
       sc->done = 1;
       sc->converged = 1;
+      sc->fin = 1;
     } else if (sc->iter >= sc->max_iter) {
       sc->done = 1;
       sc->converged = 0;
+      sc->fin = 1;
     } else {
       sc->done = 0;
       sc->converged = 0;
@@ -563,18 +557,34 @@ static int enqueue_chunk(CgSolver *cg, hipGraphExec_t g) {
 
 // residual replacement: x += x32, r = f - A x (FP64 operator), r32 = r,
 // new scalars, read-back
-static int enqueue_replace(CgSolver *cg) {
-  const Op &op = *cg->op;
-  const int64_t n = op.n;
-  int st;
-  hipLaunchKernelGGL(mx_absorb_kernel, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->x,
-                     cg->e32, n);
-  if ((st = op.apply<double>(cg->w, cg->x, nullptr, cg->s, true, true))) return st;
-  hipLaunchKernelGGL(mx_resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->r32,
-                     cg->f, cg->w, op.f64.mult, op.f32.dinv, n, cg->part);
-  hipLaunchKernelGGL(mx_resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                     cg->sc);
+int mx_slice_begin(const Op &op, double *x, float *x32, float *r32, float *p32,
+                   const double *f, CgScal *sc, double *part, double abs_tol, double rel,
+                   double delta, int max_iter, hipStream_t s) {
+  hipLaunchKernelGGL(mx_init_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x, x32, r32,
+                     p32, f, op.f64.mult, op.f32.dinv, op.n, part);
+  hipLaunchKernelGGL(mx_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc, abs_tol,
+                     rel, delta, max_iter);
   NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
+                     double *w, CgScal *sc, double *part, hipStream_t s) {
+  const int64_t n = op.n;
+  hipLaunchKernelGGL(mx_absorb_kernel, dim3(vec_blocks(n)), dim3(256), 0, s, x, x32, n);
+  int st = op.apply<double>(w, x, nullptr, s, true, true);
+  if (st) return st;
+  hipLaunchKernelGGL(mx_resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, r32, f, w,
+                     op.f64.mult, op.f32.dinv, n, part);
+  hipLaunchKernelGGL(mx_resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+static int enqueue_replace(CgSolver *cg) {
+  int st = mx_slice_replace(*cg->op, cg->x, cg->e32, cg->r32, cg->f, cg->w, cg->sc, cg->part,
+                            cg->s);
+  if (st) return st;
   NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
   NM_HIP_TRY(hipEventRecord(cg->ev, cg->s));
   cg->pending = true;
@@ -615,11 +625,9 @@ int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
     cg->phase = CgSolver::RUN64;
     return enqueue_chunk(cg, cg->g64);
   }
-  hipLaunchKernelGGL(mx_init_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, cg->s, cg->x,
-                     cg->e32, cg->r32, cg->p32, cg->f, op.f64.mult, op.f32.dinv, n, cg->part);
-  hipLaunchKernelGGL(mx_init_fin_kernel, dim3(1), dim3(RED_THREADS), 0, cg->s, cg->part,
-                     cg->sc, abs_tol, rel_tol, cg->inner_rel, max_iter);
-  NM_HIP_TRY(hipGetLastError());
+  int st = mx_slice_begin(op, cg->x, cg->e32, cg->r32, cg->p32, cg->f, cg->sc, cg->part,
+                          abs_tol, rel_tol, cg->inner_rel, max_iter, cg->s);
+  if (st) return st;
   cg->phase = CgSolver::MIX;
   return enqueue_chunk(cg, cg->g32);
 }

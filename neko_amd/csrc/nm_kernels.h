@@ -40,6 +40,30 @@ struct AxArgs {
 // Workgroups of an Ax launch for a variant code (for sizing dot_part)
 int64_t ax_grid(int64_t nel, int lx, int code);
 
+// Batched trilinear Ax (FP32) over many box problems of one lx: instance
+// table entry and launch arguments.
+struct BatchInst {
+  int64_t voff;          // offset of the instance in the batch vectors
+  int64_t nel;
+  const float *edges;    // its 36 half edge vectors per element
+  float h1c, h2c;
+  int64_t blk0, nblk;    // its Ax workgroups (global numbering)
+  int64_t c0, c1;        // its 256-dof chunks [c0, c1)
+};
+struct AxBatchArgs {
+  float *w;
+  const float *u;
+  const float *dx, *zg, *wg;  // of this lx
+  const BatchInst *inst;
+  const int32_t *blk_inst;    // instance of every global workgroup
+  int64_t blk_base, nblocks;  // this launch: global workgroups [base, base+n)
+  int lx, epb;
+  const void *scal;           // per-instance scalar records
+  int scal_size, done_off;    // record size, offset of the int `done`
+  double *dot_part;           // per global workgroup: sum u.w
+};
+int launch_ax_batch(const AxBatchArgs &a, hipStream_t s);
+
 template <typename T>
 int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s);
 

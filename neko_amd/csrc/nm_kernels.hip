@@ -166,13 +166,17 @@ __global__ void __launch_bounds__(LX *LX *EPB)
 // DOT: also writes the workgroup's partial sum_i u_i w_i (local w, before
 // gs), which equals the assembled p.A p of CG for a continuous, masked u.
 // ---------------------------------------------------------------------------
+// Body shared by the single-problem and the batched kernels.  Element q of
+// the workgroup: `active`, its first dof at w/u + `ebase`, its 36 edge
+// values at `ed`.  With DOT the workgroup sum lands in *dot_out.
 template <typename T, int LX, int EPB, bool DOT>
-__global__ void __launch_bounds__(LX *LX *EPB)
-    ax_tri_kernel(T *__restrict__ w, const T *__restrict__ u,
-                  const T *__restrict__ dx, const T *__restrict__ zg,
-                  const T *__restrict__ wg, const T *__restrict__ edges, T h1c,
-                  T h2c, int64_t nel, const int *__restrict__ skip,
-                  double *__restrict__ dot_part) {
+__device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restrict__ u,
+                                            const T *__restrict__ dx,
+                                            const T *__restrict__ zg,
+                                            const T *__restrict__ wg,
+                                            const T *__restrict__ ed, T h1c, T h2c,
+                                            bool active, int64_t ebase,
+                                            double *__restrict__ dot_out) {
   constexpr int P = LX * LX;
   constexpr int N3 = LX * P;
   __shared__ T su[EPB][N3];
@@ -182,15 +186,11 @@ __global__ void __launch_bounds__(LX *LX *EPB)
   __shared__ T sedge[EPB][36];
   __shared__ double sdot[DOT ? LX * LX * EPB : 1];
 
-  if (skip != nullptr && *skip) return;
-
   const int t = threadIdx.x;
   const int q = threadIdx.y;
   const int i = t % LX;
   const int j = t / LX;
-  const int64_t e = (int64_t)blockIdx.x * EPB + q;
-  const bool active = e < nel;
-  const int64_t base = e * (int64_t)N3 + t;
+  const int64_t base = ebase + t;
 
   T uc[LX];
 #pragma unroll
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(LX *LX *EPB)
     uc[k] = active ? u[base + k * P] : T(0);
     su[q][k * P + t] = uc[k];
   }
-  for (int x = t; x < 36; x += P) sedge[q][x] = active ? edges[e * 36 + x] : T(0);
+  for (int x = t; x < 36; x += P) sedge[q][x] = active ? ed[x] : T(0);
 
   T Di[LX], Dj[LX], Dti[LX], Dtj[LX];
 #pragma unroll
@@ -310,7 +310,88 @@ __global__ void __launch_bounds__(LX *LX *EPB)
       if (flat < h && flat + h < NT) sdot[flat] += sdot[flat + h];
       __syncthreads();
     }
-    if (flat == 0) dot_part[blockIdx.x] = sdot[0];
+    if (flat == 0) *dot_out = sdot[0];
+  }
+}
+
+template <typename T, int LX, int EPB, bool DOT>
+__global__ void __launch_bounds__(LX *LX *EPB)
+    ax_tri_kernel(T *__restrict__ w, const T *__restrict__ u,
+                  const T *__restrict__ dx, const T *__restrict__ zg,
+                  const T *__restrict__ wg, const T *__restrict__ edges, T h1c,
+                  T h2c, int64_t nel, const int *__restrict__ skip,
+                  double *__restrict__ dot_part) {
+  if (skip != nullptr && *skip) return;
+  const int64_t e = (int64_t)blockIdx.x * EPB + threadIdx.y;
+  const bool active = e < nel;
+  ax_tri_body<T, LX, EPB, DOT>(w, u, dx, zg, wg, edges + (active ? e : 0) * 36, h1c, h2c,
+                               active, e * (int64_t)(LX * LX * LX),
+                               DOT ? dot_part + blockIdx.x : nullptr);
+}
+
+// Batched: workgroup b of a launch is global workgroup blk_base + b, which
+// belongs to instance blk_inst[.] (instances never share a workgroup); the
+// instance's flag `done` (CgScal-compatible layout: int at done_off bytes)
+// turns its workgroups into no-ops.
+template <int LX, int EPB>
+__global__ void __launch_bounds__(LX *LX *EPB)
+    ax_tri_batch_kernel(float *__restrict__ w, const float *__restrict__ u,
+                        const float *__restrict__ dx, const float *__restrict__ zg,
+                        const float *__restrict__ wg, const BatchInst *__restrict__ inst,
+                        const int32_t *__restrict__ blk_inst, int64_t blk_base,
+                        const char *__restrict__ scal, int scal_size, int done_off,
+                        double *__restrict__ dot_part) {
+  const int64_t gb = blk_base + blockIdx.x;
+  const int ii = blk_inst[gb];
+  if (*(const int *)(scal + (int64_t)ii * scal_size + done_off)) return;
+  const BatchInst I = inst[ii];
+  const int64_t e = (gb - I.blk0) * EPB + threadIdx.y;
+  const bool active = e < I.nel;
+  ax_tri_body<float, LX, EPB, true>(w, u, dx, zg, wg, I.edges + (active ? e : 0) * 36,
+                                    I.h1c, I.h2c, active,
+                                    I.voff + e * (int64_t)(LX * LX * LX), dot_part + gb);
+}
+
+template <int LX>
+static int ax_batch_lx(const AxBatchArgs &a, hipStream_t s) {
+  constexpr int P = LX * LX;
+#define NM_AXB_CASE(E)                                                                   \
+  case E:                                                                                \
+    if constexpr (P * E <= 256) {                                                        \
+      hipLaunchKernelGGL((ax_tri_batch_kernel<LX, E>), dim3((unsigned)a.nblocks),        \
+                         dim3(P, E), 0, s, a.w, a.u, a.dx, a.zg, a.wg, a.inst,           \
+                         a.blk_inst, a.blk_base, (const char *)a.scal, a.scal_size,      \
+                         a.done_off, a.dot_part);                                        \
+      NM_HIP_TRY(hipGetLastError());                                                     \
+      return NM_OK;                                                                      \
+    }                                                                                    \
+    break;
+  switch (a.epb) {
+    NM_AXB_CASE(1) NM_AXB_CASE(2) NM_AXB_CASE(4) NM_AXB_CASE(8) NM_AXB_CASE(16)
+    default: break;
+  }
+#undef NM_AXB_CASE
+  set_error("ax batch: elements-per-workgroup %d unsupported for lx=%d", a.epb, LX);
+  return NM_ERR_UNSUPPORTED;
+}
+
+int launch_ax_batch(const AxBatchArgs &a, hipStream_t s) {
+  if (a.nblocks <= 0) return NM_OK;
+  switch (a.lx) {
+    case 2: return ax_batch_lx<2>(a, s);
+    case 3: return ax_batch_lx<3>(a, s);
+    case 4: return ax_batch_lx<4>(a, s);
+    case 5: return ax_batch_lx<5>(a, s);
+    case 6: return ax_batch_lx<6>(a, s);
+    case 7: return ax_batch_lx<7>(a, s);
+    case 8: return ax_batch_lx<8>(a, s);
+    case 9: return ax_batch_lx<9>(a, s);
+    case 10: return ax_batch_lx<10>(a, s);
+    case 11: return ax_batch_lx<11>(a, s);
+    case 12: return ax_batch_lx<12>(a, s);
+    default:
+      set_error("ax batch: lx=%d unsupported (2..12)", a.lx);
+      return NM_ERR_UNSUPPORTED;
   }
 }
 

@@ -368,6 +368,53 @@ class GMRES:
             pass
 
 
+class Batch:
+    """Many box problems of uneven size and order solved together (FP32 CG
+    with FP64 residual replacement per problem)."""
+
+    def __init__(self, problems):
+        self.problems = list(problems)
+        arr = (C.c_void_p * len(self.problems))(*[p.h.value for p in self.problems])
+        h = C.c_void_p()
+        check(lib().nm_batch_create(C.byref(h), len(self.problems), arr, _stream()),
+              "nm_batch_create")
+        self.h = h
+
+    def autotune(self, reps=3):
+        lx = (C.c_int * 16)()
+        epb = (C.c_int * 16)()
+        ng = C.c_int()
+        check(lib().nm_batch_autotune(self.h, int(reps), lx, epb, C.byref(ng)),
+              "nm_batch_autotune")
+        return {int(lx[g]): int(epb[g]) for g in range(ng.value)}
+
+    def solve(self, xs, fs, rel_tol=1e-10, max_iter=20000, delta=0.0):
+        k = len(self.problems)
+        xa = (C.c_void_p * k)(*[_ptr(x) for x in xs])
+        fa = (C.c_void_p * k)(*[_ptr(f) for f in fs])
+        mons = (Monitor * k)()
+        check(lib().nm_batch_solve(self.h, xa, fa, float(rel_tol), int(max_iter),
+                                   float(delta), mons, _stream()), "nm_batch_solve")
+        return [KspMonitor(m.iter, m.res_start, m.res_final, bool(m.converged),
+                           m.outer) for m in mons]
+
+    def info(self):
+        n, b = C.c_int64(), C.c_int64()
+        check(lib().nm_batch_info(self.h, C.byref(n), C.byref(b)), "nm_batch_info")
+        return n.value, b.value
+
+    def free(self):
+        if self.h:
+            lib().nm_batch_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Server:
     """Serving loop for streams of Poisson solves on resident box meshes
     (no counterpart in Neko).  precision: 'fp64' or 'mixed'."""
