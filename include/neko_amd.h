@@ -110,6 +110,7 @@ typedef struct nm_hsmg_s nm_hsmg_t;
 typedef struct nm_gmres_s nm_gmres_t;
 typedef struct nm_server_s nm_server_t;
 typedef struct nm_batch_s nm_batch_t;
+typedef struct nm_phmg_s nm_phmg_t;
 
 typedef struct {
   int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
@@ -240,9 +241,29 @@ int nm_hsmg_apply(nm_hsmg_t *h, double *z, const double *r, hipStream_t s);
  * (schwarz_compute, schwarz.f90:565-595) */
 int nm_hsmg_schwarz_fine(nm_hsmg_t *h, double *e, double *r, hipStream_t s);
 
+/* --- phmg-style V-cycle with Chebyshev smoothing -------------------------
+ * Chebyshev smoother cheby_t (src/krylov/bcknd/cpu/cheby.f90: power
+ * estimate :136-231 with 150 iterations, boost 1.1, lambda_max/30; solve
+ * cheby_impl :317-397) accelerated by the level's Schwarz smoother, in the
+ * V-cycle of phmg_mg_cycle (src/multigrid/phmg.f90:605-757) on hsmg's
+ * levels lx -> lx_mid -> 2 with hsmg's coarse Jacobi-CG (10 iterations) in
+ * place of phmg's tree-AMG.  smooth_its: Chebyshev iterations per
+ * smoothing (phmg 'smoother_iterations'; 4 = Chebyshev(4)).  Level
+ * operators take the fine h1 and ifh2; their h2 is Neko's coef default 1
+ * (level_h2_fine = 0, phmg.f90:252-254) or the fine level's h2 (1).  The
+ * power method starts from this project's generator (seed 3901), not
+ * gfortran's random_number. */
+int nm_phmg_create(nm_phmg_t **h, nm_op_t *op, int facet_bc, int smooth_its,
+                   int level_h2_fine, hipStream_t s);
+void nm_phmg_free(nm_phmg_t *h);
+int nm_phmg_apply(nm_phmg_t *h, double *z, const double *r, hipStream_t s);
+/* (tha, dlt) of the fine and the middle level smoothers */
+int nm_phmg_eigs(const nm_phmg_t *h, double *tha_dlt);
+
 /* --- GMRES(m) (ksp_t 'gmres', src/krylov/bcknd/cpu/gmres.f90:175-411) --- */
-/* pc: 0 Jacobi (the operator's diagonal), 1 hsmg (handle), 2 identity */
-int nm_gmres_create(nm_gmres_t **g, nm_op_t *op, int pc, nm_hsmg_t *hsmg,
+/* pc: 0 Jacobi (the operator's diagonal), 1 hsmg (nm_hsmg_t handle),
+ * 2 identity, 3 phmg (nm_phmg_t handle) */
+int nm_gmres_create(nm_gmres_t **g, nm_op_t *op, int pc, void *pc_handle,
                     int lgmres);
 void nm_gmres_free(nm_gmres_t *g);
 int nm_gmres_solve(nm_gmres_t *g, double *x, const double *f, double abs_tol,

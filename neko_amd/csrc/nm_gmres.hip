@@ -93,8 +93,8 @@ static inline unsigned vb(int64_t n) {
 struct Gmres {
   Op *op = nullptr;
   int lg = 30;
-  int pc = 0;  // 0 Jacobi, 1 hsmg, 2 none
-  nm_hsmg_s *hsmg = nullptr;
+  int pc = 0;  // 0 Jacobi, 1 hsmg, 2 none, 3 phmg
+  void *pch = nullptr;
   double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr, *f = nullptr;
   double *x = nullptr;
   const double **vptr = nullptr, **zptr = nullptr;
@@ -116,7 +116,8 @@ struct Gmres {
 
 static int gmres_pc(Gmres *g, double *z, const double *v, hipStream_t s) {
   const Op &op = *g->op;
-  if (g->pc == 1) return nm_hsmg_apply(g->hsmg, z, v, s);
+  if (g->pc == 1) return nm_hsmg_apply((nm_hsmg_t *)g->pch, z, v, s);
+  if (g->pc == 3) return nm_phmg_apply((nm_phmg_t *)g->pch, z, v, s);
   if (g->pc == 0)
     return launch_vec<double>(V_COL3, z, v, op.f64.dinv, nullptr, 0.0, op.n, s);
   NM_HIP_TRY(hipMemcpyAsync(z, v, sizeof(double) * op.n, hipMemcpyDeviceToDevice, s));
@@ -133,16 +134,20 @@ struct nm_gmres_s {
 
 extern "C" {
 
-// pc: 0 Jacobi (the operator's diagonal), 1 hsmg (pass its handle), 2 none.
-int nm_gmres_create(nm_gmres_t **out, nm_op_t *oph, int pc, nm_hsmg_s *hsmg,
-                    int lgmres) {
+// pc: 0 Jacobi (the operator's diagonal), 1 hsmg, 3 phmg (pass the
+// handle), 2 none.
+int nm_gmres_create(nm_gmres_t **out, nm_op_t *oph, int pc, void *pch, int lgmres) {
   *out = nullptr;
   if (lgmres < 1 || lgmres > 64) {
     set_error("nm_gmres_create: restart length %d outside 1..64", lgmres);
     return NM_ERR_ARG;
   }
-  if (pc == 1 && !hsmg) {
-    set_error("nm_gmres_create: hsmg preconditioner without handle");
+  if ((pc == 1 || pc == 3) && !pch) {
+    set_error("nm_gmres_create: multigrid preconditioner without handle");
+    return NM_ERR_ARG;
+  }
+  if (pc < 0 || pc > 3) {
+    set_error("nm_gmres_create: unknown preconditioner %d", pc);
     return NM_ERR_ARG;
   }
   nm_gmres_t *h = new nm_gmres_t;
@@ -150,7 +155,7 @@ int nm_gmres_create(nm_gmres_t **out, nm_op_t *oph, int pc, nm_hsmg_s *hsmg,
   g->op = &oph->op;
   g->lg = lgmres;
   g->pc = pc;
-  g->hsmg = hsmg;
+  g->pch = pch;
   const int64_t n = g->op->n;
   if (!(g->V = g->alloc<double>(n * lgmres)) || !(g->Z = g->alloc<double>(n * lgmres)) ||
       !(g->w = g->alloc<double>(n)) || !(g->r = g->alloc<double>(n)) ||

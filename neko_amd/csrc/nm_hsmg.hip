@@ -20,6 +20,7 @@
 #include "nm_common.h"
 #include "nm_kernels.h"
 #include "nm_op.h"
+#include "nm_rng.h"
 
 // host set-up helpers of nm_fdm_setup.cpp (internal)
 extern "C" {
@@ -431,7 +432,8 @@ static int gsdev_box(H *own, GsDev *g, const int box[3], const int per[3], int l
   return NM_OK;
 }
 
-static int schwarz_setup(Hsmg *h, Schwarz *sw, const Op *op, int facet_bc, hipStream_t s) {
+template <typename H>
+static int schwarz_setup(H *h, Schwarz *sw, const Op *op, int facet_bc, hipStream_t s) {
   const int lx = op->lx, nl = lx + 2;
   const int64_t nel = op->nel, n = op->n;
   sw->lx = lx;
@@ -441,9 +443,9 @@ static int schwarz_setup(Hsmg *h, Schwarz *sw, const Op *op, int facet_bc, hipSt
   int st = gsdev_box(h, &sw->ext, op->box, op->per, nl, s);
   if (st) return st;
   const int64_t ns = nel * nl * nl * nl;
-  if (!(sw->S = h->alloc<double>(nel * 6 * nl * nl)) || !(sw->D = h->alloc<double>(ns)) ||
-      !(sw->wt = h->alloc<double>(nel * lx * lx * 12)) || !(sw->work1 = h->alloc<double>(ns)) ||
-      !(sw->work2 = h->alloc<double>(ns))) {
+  if (!(sw->S = h->template alloc<double>(nel * 6 * nl * nl)) || !(sw->D = h->template alloc<double>(ns)) ||
+      !(sw->wt = h->template alloc<double>(nel * lx * lx * 12)) || !(sw->work1 = h->template alloc<double>(ns)) ||
+      !(sw->work2 = h->template alloc<double>(ns))) {
     set_error("hsmg: out of device memory");
     return NM_ERR_NOMEM;
   }
@@ -473,7 +475,7 @@ static int schwarz_setup(Hsmg *h, Schwarz *sw, const Op *op, int facet_bc, hipSt
   NM_HIP_TRY(hipMemcpyAsync(sw->D, hD.data(), sizeof(double) * hD.size(), hipMemcpyHostToDevice, s));
   // weights (schwarz_setup_wt): counts of overlapping copies
   double *e1;
-  if (!(e1 = h->alloc<double>(n))) return NM_ERR_NOMEM;
+  if (!(e1 = h->template alloc<double>(n))) return NM_ERR_NOMEM;
   if ((st = launch_vec<double>(V_CFILL, sw->work2, nullptr, nullptr, nullptr, 1.0, ns, s))) return st;
   if ((st = launch_vec<double>(V_RZERO, sw->work1, nullptr, nullptr, nullptr, 0.0, ns, s))) return st;
   if ((st = extrude(sw->work1, 0, 0.0, sw->work2, 0, 1.0, nl, nel, s))) return st;
@@ -486,6 +488,192 @@ static int schwarz_setup(Hsmg *h, Schwarz *sw, const Op *op, int facet_bc, hipSt
                      sw->wt, e1, lx, nel);
   NM_HIP_TRY(hipGetLastError());
   NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+// ------------------------------------------------------------- Chebyshev
+// cheby_t (cheby.f90): power estimate of lambda_max of M^-1 A (M = the
+// level's Schwarz smoother), then the bound solve cheby_impl.
+__global__ void cheby_start_kernel(double *__restrict__ d, int64_t n, uint64_t seed) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    d[i] = drng_uniform(seed, (uint64_t)i) + 10.0;
+}
+// r = f - w
+__global__ void sub3_kernel(double *__restrict__ r, const double *__restrict__ f,
+                            const double *__restrict__ w, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    r[i] = f[i] - w[i];
+}
+// d = a d + b w; x += d   (a = 0, b = 1/tha for the first step)
+__global__ void cheby_step_kernel(double *__restrict__ d, double *__restrict__ x,
+                                  const double *__restrict__ w, double a, double b,
+                                  int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double di = a * d[i] + b * w[i];
+    d[i] = di;
+    x[i] = x[i] + di;
+  }
+}
+// w = (r - w) * mult
+__global__ void resid_mult_kernel(double *__restrict__ w, const double *__restrict__ r,
+                                  const double *__restrict__ mult, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    w[i] = (r[i] - w[i]) * mult[i];
+}
+
+struct Cheby {
+  const Op *op = nullptr;
+  const Schwarz *sw = nullptr;
+  double tha = 0.0, dlt = 0.0;
+  double *d = nullptr, *w = nullptr, *r = nullptr;
+};
+
+static unsigned vblocks(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+}
+
+static int dot_host(const Op &op, const double *a, const double *b, double *part, double *out,
+                    double *h, hipStream_t s) {
+  int st = launch_glsc3_partial(a, op.f64.mult, b, op.n, part, s);
+  if (st) return st;
+  if ((st = launch_combine(part, 1, out, s))) return st;
+  NM_HIP_TRY(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+// cheby_power (cheby.f90:136-231); part/out: reduction scratch
+static int cheby_power(Cheby &c, uint64_t seed, int its, double *part, double *out,
+                       hipStream_t s) {
+  const Op &op = *c.op;
+  const int64_t n = op.n;
+  int st;
+  hipLaunchKernelGGL(cheby_start_kernel, dim3(vblocks(n)), dim3(256), 0, s, c.d, n, seed);
+  if ((st = op.gs<double>(c.d, nullptr, s))) return st;
+  if ((st = launch_mask<double>(c.d, op.msk, op.nmsk, nullptr, s))) return st;
+  double wtw = 0.0, dtw = 0.0, dtd = 0.0;
+  for (int it = 0; it <= its; it++) {
+    if ((st = op.apply<double>(c.w, c.d, nullptr, s, true, true))) return st;
+    if ((st = schwarz_apply(*c.sw, c.r, c.w, s))) return st;
+    NM_HIP_TRY(hipMemcpyAsync(c.w, c.r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (it == its) break;
+    if ((st = dot_host(op, c.w, c.w, part, out, &wtw, s))) return st;
+    if ((st = launch_vec<double>(V_CMULT2, c.d, c.w, nullptr, nullptr, 1.0 / std::sqrt(wtw), n,
+                                 s)))
+      return st;
+    if ((st = launch_mask<double>(c.d, op.msk, op.nmsk, nullptr, s))) return st;
+  }
+  if ((st = dot_host(op, c.d, c.w, part, out, &dtw, s))) return st;
+  if ((st = dot_host(op, c.d, c.d, part, out, &dtd, s))) return st;
+  const double lam = dtw / dtd;
+  const double b = lam * 1.1, a = lam / 30.0;
+  c.tha = (b + a) / 2.0;
+  c.dlt = (b - a) / 2.0;
+  return NM_OK;
+}
+
+// cheby_impl (cheby.f90:317-397): niter steps on A x = f, all on stream s
+static int cheby_solve(Cheby &c, double *x, const double *f, int niter, bool zero_init,
+                       hipStream_t s) {
+  const Op &op = *c.op;
+  const int64_t n = op.n;
+  const unsigned g = vblocks(n);
+  int st;
+  if (!zero_init) {
+    if ((st = op.apply<double>(c.w, x, nullptr, s, true, true))) return st;
+    hipLaunchKernelGGL(sub3_kernel, dim3(g), dim3(256), 0, s, c.r, f, c.w, n);
+  } else {
+    NM_HIP_TRY(hipMemcpyAsync(c.r, f, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  }
+  if ((st = schwarz_apply(*c.sw, c.w, c.r, s))) return st;
+  hipLaunchKernelGGL(cheby_step_kernel, dim3(g), dim3(256), 0, s, c.d, x, c.w, 0.0,
+                     1.0 / c.tha, n);
+  const double sig1 = c.tha / c.dlt;
+  double rhok = 1.0 / sig1;
+  for (int iter = 2; iter <= niter; iter++) {
+    const double rhokp1 = 1.0 / (2.0 * sig1 - rhok);
+    const double tmp1 = rhokp1 * rhok;
+    const double tmp2 = 2.0 * rhokp1 / c.dlt;
+    rhok = rhokp1;
+    if ((st = op.apply<double>(c.w, x, nullptr, s, true, true))) return st;
+    hipLaunchKernelGGL(sub3_kernel, dim3(g), dim3(256), 0, s, c.r, f, c.w, n);
+    if ((st = schwarz_apply(*c.sw, c.w, c.r, s))) return st;
+    hipLaunchKernelGGL(cheby_step_kernel, dim3(g), dim3(256), 0, s, c.d, x, c.w, tmp1, tmp2, n);
+  }
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// ------------------------------------------------------------------ phmg
+// phmg_mg_cycle (phmg.f90:605-757) on hsmg's levels lx -> lx_mid -> 2 with
+// Chebyshev smoothing accelerated by Schwarz on the two finer levels and
+// Jacobi-CG on the coarse level.
+struct Phmg {
+  Op *lev[3] = {nullptr, nullptr, nullptr};
+  nm_op_t *mid_h = nullptr, *crs_h = nullptr;
+  Schwarz sw[2];
+  Cheby ch[2];
+  double *jfm = nullptr, *jfm_t = nullptr, *jmc = nullptr, *jmc_t = nullptr;
+  double *r[3] = {}, *z[3] = {}, *w[3] = {};
+  double *part = nullptr, *out = nullptr;
+  CgFixed *cg = nullptr;
+  int smooth_its = 4, crs_niter = 10;
+  double crs_tol = 1e-9;
+  std::vector<void *> owned;
+  ~Phmg() {
+    if (cg) cg_fixed_free(cg);
+    for (void *q : owned) (void)hipFree(q);
+    if (mid_h) nm_op_free(mid_h);
+    if (crs_h) nm_op_free(crs_h);
+  }
+  template <typename T>
+  T *alloc(int64_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+    owned.push_back(q);
+    return (T *)q;
+  }
+};
+
+static int phmg_cycle(Phmg *h, double *zout, const double *rin, hipStream_t s) {
+  Op **L = h->lev;
+  const int64_t nel = L[0]->nel;
+  const double *jt[2] = {h->jfm_t, h->jmc_t}, *jj[2] = {h->jfm, h->jmc};
+  int st;
+  NM_HIP_TRY(hipMemcpyAsync(h->r[0], rin, sizeof(double) * L[0]->n, hipMemcpyDeviceToDevice, s));
+  if ((st = launch_vec<double>(V_RZERO, h->z[0], nullptr, nullptr, nullptr, 0.0, L[0]->n, s)))
+    return st;
+  for (int l = 0; l < 2; l++) {
+    const Op &op = *L[l];
+    if ((st = cheby_solve(h->ch[l], h->z[l], h->r[l], h->smooth_its, true, s))) return st;
+    if ((st = op.apply<double>(h->w[l], h->z[l], nullptr, s, true, true))) return st;
+    hipLaunchKernelGGL(resid_mult_kernel, dim3(vblocks(op.n)), dim3(256), 0, s, h->w[l],
+                       h->r[l], op.f64.mult, op.n);
+    if ((st = tnsr3d(h->r[l + 1], L[l + 1]->lx, h->w[l], op.lx, jt[l], jj[l], jj[l], nel, s)))
+      return st;
+    if ((st = L[l + 1]->gs<double>(h->r[l + 1], nullptr, s))) return st;
+    if ((st = launch_mask<double>(h->r[l + 1], L[l + 1]->msk, L[l + 1]->nmsk, nullptr, s)))
+      return st;
+    if ((st = launch_vec<double>(V_RZERO, h->z[l + 1], nullptr, nullptr, nullptr, 0.0,
+                                 L[l + 1]->n, s)))
+      return st;
+  }
+  if ((st = cg_fixed_solve(h->cg, h->z[2], h->r[2], h->crs_niter, h->crs_tol, s))) return st;
+  for (int l = 1; l >= 0; l--) {
+    const Op &op = *L[l];
+    if ((st = tnsr3d(h->w[l], op.lx, h->z[l + 1], L[l + 1]->lx, jj[l], jt[l], jt[l], nel, s)))
+      return st;
+    if ((st = op.gs<double>(h->w[l], nullptr, s))) return st;
+    if ((st = launch_vec<double>(V_ADDCOL3, h->z[l], h->w[l], op.f64.mult, nullptr, 0.0, op.n,
+                                 s)))
+      return st;
+    if ((st = cheby_solve(h->ch[l], h->z[l], h->r[l], h->smooth_its, false, s))) return st;
+  }
+  NM_HIP_TRY(hipMemcpyAsync(zout, h->z[0], sizeof(double) * L[0]->n, hipMemcpyDeviceToDevice, s));
   return NM_OK;
 }
 
@@ -594,6 +782,101 @@ int nm_hsmg_apply(nm_hsmg_t *hh, double *z, const double *rin, hipStream_t s) {
   if ((st = launch_vec<double>(V_ADD2, z, h->r, nullptr, nullptr, 0.0, F.n, s))) return st;
   if ((st = F.gs<double>(z, nullptr, s))) return st;
   return launch_vec<double>(V_COL2, z, F.f64.mult, nullptr, nullptr, 0.0, F.n, s);
+}
+
+struct nm_phmg_s {
+  Phmg h;
+};
+
+// phmg-style preconditioner on a box operator (see include/neko_amd.h).
+int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_its,
+                   int level_h2_fine, hipStream_t s) {
+  *out = nullptr;
+  Op *fine = &fine_h->op;
+  if (fine->box[0] == 0) {
+    set_error("nm_phmg_create: operator was not built by nm_op_create_box");
+    return NM_ERR_ARG;
+  }
+  const int lf = fine->lx;
+  if (lf < 3 || lf > 10) {
+    set_error("nm_phmg_create: lx=%d outside 3..10", lf);
+    return NM_ERR_ARG;
+  }
+  if (fine->f64.h1) {
+    set_error("nm_phmg_create: pointwise h1 unsupported (the levels take a constant h1)");
+    return NM_ERR_UNSUPPORTED;
+  }
+  const int lm = lf < 5 ? std::max(lf - 1, 3) : 4, lc = 2;
+  nm_phmg_t *hh = new nm_phmg_t;
+  Phmg *h = &hh->h;
+  h->smooth_its = smooth_its > 0 ? smooth_its : 4;
+  auto fail = [&](int c) {
+    delete hh;
+    return c;
+  };
+  // level operators: h1 and ifh2 of the fine level; h2 = 1 (Neko's coef
+  // default, phmg.f90:252-254 copies h1 and ifh2 only) or the fine h2
+  const double h2l = fine->ifh2 ? (level_h2_fine ? fine->h2c : 1.0) : 0.0;
+  const int *b = fine->box, *p = fine->per;
+  int st = nm_op_create_box(&h->mid_h, b[0], b[1], b[2], lm, p[0], p[1], p[2], fine->amp,
+                            fine->seed, fine->h1c, h2l, s);
+  if (st) return fail(st);
+  if ((st = nm_op_create_box(&h->crs_h, b[0], b[1], b[2], lc, p[0], p[1], p[2], fine->amp,
+                             fine->seed, fine->h1c, h2l, s)))
+    return fail(st);
+  h->lev[0] = fine;
+  h->lev[1] = &h->mid_h->op;
+  h->lev[2] = &h->crs_h->op;
+  for (int l = 0; l < 2; l++)
+    if ((st = schwarz_setup(h, &h->sw[l], h->lev[l], facet_bc, s))) return fail(st);
+  std::vector<double> zf(lf), zm(lm), zc(lc), wd(lf), dd(lf * lf), dtd(lf * lf);
+  nm_gll(lf, zf.data(), wd.data(), dd.data(), dtd.data());
+  nm_gll(lm, zm.data(), wd.data(), dd.data(), dtd.data());
+  nm_gll(lc, zc.data(), wd.data(), dd.data(), dtd.data());
+  std::vector<double> jfm(lf * lm), jfmt(lf * lm), jmc(lm * lc), jmct(lm * lc);
+  nm_interp_matrix(zf.data(), lf, zm.data(), lm, jfm.data(), jfmt.data());
+  nm_interp_matrix(zm.data(), lm, zc.data(), lc, jmc.data(), jmct.data());
+  if (!(h->jfm = h->alloc<double>(lf * lm)) || !(h->jfm_t = h->alloc<double>(lf * lm)) ||
+      !(h->jmc = h->alloc<double>(lm * lc)) || !(h->jmc_t = h->alloc<double>(lm * lc)) ||
+      !(h->part = h->alloc<double>(RED_BLOCKS)) || !(h->out = h->alloc<double>(1)))
+    return fail(NM_ERR_NOMEM);
+  for (int l = 0; l < 3; l++)
+    if (!(h->r[l] = h->alloc<double>(h->lev[l]->n)) || !(h->z[l] = h->alloc<double>(h->lev[l]->n)) ||
+        !(h->w[l] = h->alloc<double>(h->lev[l]->n)))
+      return fail(NM_ERR_NOMEM);
+  if (hipMemcpyAsync(h->jfm, jfm.data(), sizeof(double) * jfm.size(), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(h->jfm_t, jfmt.data(), sizeof(double) * jfmt.size(), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(h->jmc, jmc.data(), sizeof(double) * jmc.size(), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(h->jmc_t, jmct.data(), sizeof(double) * jmct.size(), hipMemcpyHostToDevice, s))
+    return fail(NM_ERR_HIP);
+  for (int l = 0; l < 2; l++) {
+    Cheby &c = h->ch[l];
+    c.op = h->lev[l];
+    c.sw = &h->sw[l];
+    const int64_t n = c.op->n;
+    if (!(c.d = h->alloc<double>(n)) || !(c.w = h->alloc<double>(n)) ||
+        !(c.r = h->alloc<double>(n)))
+      return fail(NM_ERR_NOMEM);
+    if ((st = cheby_power(c, 3901, 150, h->part, h->out, s))) return fail(st);
+  }
+  if ((st = cg_fixed_create(&h->cg, h->lev[2]))) return fail(st);
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(NM_ERR_HIP);
+  *out = hh;
+  return NM_OK;
+}
+
+void nm_phmg_free(nm_phmg_t *h) { delete h; }
+
+int nm_phmg_apply(nm_phmg_t *h, double *z, const double *r, hipStream_t s) {
+  return phmg_cycle(&h->h, z, r, s);
+}
+
+int nm_phmg_eigs(const nm_phmg_t *h, double *tha_dlt) {
+  for (int l = 0; l < 2; l++) {
+    tha_dlt[2 * l] = h->h.ch[l].tha;
+    tha_dlt[2 * l + 1] = h->h.ch[l].dlt;
+  }
+  return NM_OK;
 }
 
 // Schwarz smoother of the fine level alone (e = schwarz(r); r masked).

@@ -10,6 +10,7 @@
 #include "nm_common.h"
 #include "nm_kernels.h"
 #include "nm_op.h"
+#include "nm_rng.h"
 
 extern "C" {
 int nm_gll(int lx, double *z, double *w, double *d, double *dt);
@@ -34,16 +35,6 @@ namespace nm {
 // (64-bit mix of seed and index, Box-Muller on two uniforms), evaluated on
 // the device; agrees with the host generator to the last few ulps of the
 // transcendental functions.
-__device__ inline uint64_t dmix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-__device__ inline double drng_uniform(uint64_t seed, uint64_t idx) {
-  const uint64_t k = dmix64(seed + 0x9e3779b97f4a7c15ULL);
-  const uint64_t u = dmix64(k ^ (idx * 0xd1b54a32d192ed03ULL + 0x9e3779b97f4a7c15ULL));
-  return (double)(u >> 11) * 0x1.0p-53;
-}
 __global__ void rng_normal_kernel(double *__restrict__ f, int64_t n, uint64_t seed) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {

@@ -327,9 +327,46 @@ class HSMG:
             pass
 
 
+class PHMG:
+    """phmg-style V-cycle (Chebyshev(smooth_its) smoothing accelerated by
+    Schwarz on hsmg's levels, coarse Jacobi-CG) on a box problem.
+    level_h2: "default" (Neko's coef default h2 = 1 on the coarser levels)
+    or "fine" (the fine level's h2 on every level)."""
+
+    def __init__(self, problem: Problem, facet_bc: int = 2, smooth_its: int = 4,
+                 level_h2: str = "default"):
+        self.problem = problem
+        h = C.c_void_p()
+        check(lib().nm_phmg_create(C.byref(h), problem.h, int(facet_bc),
+                                   int(smooth_its), 1 if level_h2 == "fine" else 0,
+                                   _stream()), "nm_phmg_create")
+        self.h = h
+
+    def apply(self, z: torch.Tensor, r: torch.Tensor):
+        check(lib().nm_phmg_apply(self.h, _ptr(z), _ptr(r), _stream()),
+              "nm_phmg_apply")
+
+    def eigs(self):
+        out = (C.c_double * 4)()
+        check(lib().nm_phmg_eigs(self.h, out), "nm_phmg_eigs")
+        return [(out[0], out[1]), (out[2], out[3])]
+
+    def free(self):
+        if self.h:
+            lib().nm_phmg_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class GMRES:
     """Restarted GMRES(m) with right preconditioning (ksp_t 'gmres',
-    src/krylov/bcknd/cpu/gmres.f90).  pc: 'jacobi', 'identity' or an HSMG."""
+    src/krylov/bcknd/cpu/gmres.f90).  pc: 'jacobi', 'identity', an HSMG or
+    a PHMG."""
 
     def __init__(self, problem: Problem, pc="jacobi", lgmres: int = 30):
         self.problem = problem
@@ -337,6 +374,8 @@ class GMRES:
         h = C.c_void_p()
         if isinstance(pc, HSMG):
             code, ph = 1, pc.h
+        elif isinstance(pc, PHMG):
+            code, ph = 3, pc.h
         elif pc == "jacobi":
             code, ph = 0, None
         elif pc == "identity":

@@ -492,3 +492,135 @@ class OracleHsmg:
                                         x, np.ascontiguousarray(f), max_iter,
                                         float(abs_tol), lgmres)
         return x, mon
+
+
+# --------------------------------------------------------------------------
+# Chebyshev smoother and phmg-style V-cycle (neko_oracle_phmg.c)
+# --------------------------------------------------------------------------
+class OrCheby(C.Structure):
+    _fields_ = [("pb", C.c_void_p), ("sw", C.c_void_p), ("pcd", C.c_void_p),
+                ("tha", C.c_double), ("dlt", C.c_double),
+                ("d", C.c_void_p), ("w", C.c_void_p), ("r", C.c_void_p)]
+
+
+class OrPhmg(C.Structure):
+    _fields_ = [("nel", C.c_int), ("lf", C.c_int), ("lm", C.c_int), ("lc", C.c_int),
+                ("pb", C.c_void_p * 3), ("ch", OrCheby * 2),
+                ("smooth_its", C.c_int),
+                ("jfm", C.c_void_p), ("jfm_t", C.c_void_p),
+                ("jmc", C.c_void_p), ("jmc_t", C.c_void_p),
+                ("pc_crs", C.c_void_p), ("crs_niter", C.c_int),
+                ("crs_tol", C.c_double),
+                ("r", C.c_void_p * 3), ("z", C.c_void_p * 3), ("w", C.c_void_p * 3)]
+
+
+def _bind_phmg(L):
+    if getattr(L, "_phmg_bound", False):
+        return
+    L.or_cheby_power.argtypes = [C.POINTER(OrCheby), C.c_uint64, C.c_int]
+    L.or_cheby_solve.argtypes = [C.POINTER(OrCheby), _dp, _dp, C.c_int, C.c_int]
+    L.or_phmg_solve.argtypes = [C.POINTER(OrPhmg), _dp, _dp]
+    L.or_gmres_phmg_solve.restype = OrMonitor
+    L.or_gmres_phmg_solve.argtypes = [C.POINTER(OrProblem), C.POINTER(OrPhmg),
+                                      _dp, _dp, C.c_int, C.c_double, C.c_int]
+    L._phmg_bound = True
+
+
+class OracleCheby:
+    """Chebyshev smoother (cheby_t) on `case`: accelerated by a Schwarz
+    smoother (an OracleHsmg's fine level) or by Jacobi."""
+
+    def __init__(self, case, schwarz=None, seed=3901, power_its=150):
+        L = lib()
+        _bind_hsmg(L)
+        _bind_phmg(L)
+        self.case = case
+        n = case.n
+        self.bufs = [np.zeros(n) for _ in range(3)]
+        c = OrCheby()
+        c.pb = C.cast(C.pointer(case.problem()), C.c_void_p)
+        self._sw = schwarz
+        if schwarz is not None:
+            c.sw = C.cast(C.pointer(schwarz), C.c_void_p)
+            self.pcd = None
+        else:
+            self.pcd = case.jacobi()
+            c.pcd = self.pcd.ctypes.data
+        c.d, c.w, c.r = (b.ctypes.data for b in self.bufs)
+        self.c = c
+        L.or_cheby_power(C.byref(c), int(seed), int(power_its))
+
+    @property
+    def tha(self):
+        return self.c.tha
+
+    @property
+    def dlt(self):
+        return self.c.dlt
+
+    def solve(self, f, niter, x0=None):
+        x = np.zeros(self.case.n) if x0 is None else np.array(x0, dtype=np.float64)
+        lib().or_cheby_solve(C.byref(self.c), x, np.ascontiguousarray(f), int(niter),
+                             1 if x0 is None else 0)
+        return x
+
+
+class OraclePhmg:
+    """phmg-style V-cycle on `case`: levels lx -> lx_mid -> 2 (hsmg's),
+    Chebyshev(smooth_its) smoothing accelerated by the levels' Schwarz
+    smoothers, coarse Jacobi-CG (niter iterations).  The level operators
+    follow phmg's set-up: h1 of the fine level, ifh2 of the fine level and
+    coef's default h2 = 1 on the coarser levels (level_h2="default"), or
+    the fine level's h2 (level_h2="fine")."""
+
+    def __init__(self, case, facet_bc=2, smooth_its=4, seed=3901, power_its=150,
+                 crs_niter=10, crs_tol=1e-9, level_h2="default"):
+        L = lib()
+        _bind_hsmg(L)
+        _bind_phmg(L)
+        self.case = case
+        self.hs = OracleHsmg(case, facet_bc=facet_bc)   # Schwarz + transfers
+        hs = self.hs
+        h1c = float(case.h1[0])
+        # level_h2 "default": coef's h2 = 1 (phmg.f90 copies h1 and ifh2
+        # only); "fine": the fine level's h2 on every level
+        if level_h2 == "fine":
+            h2l = float(case.h2[0]) if case.ifh2 else 0.0
+        else:
+            h2l = 1.0 if case.ifh2 else 0.0
+        kw = dict(amp=case.amp, seed=case.seed, periodic=case.periodic, h1=h1c, h2=h2l)
+        dims = (case.nelx, case.nely, case.nelz)
+        self.mid = OracleCase(*dims, hs.lm, **kw)
+        self.crs = OracleCase(*dims, hs.lc, **kw)
+        self.pc_crs = self.crs.jacobi()
+        self.ch = [OracleCheby(case, hs.sw_f, seed, power_its),
+                   OracleCheby(self.mid, hs.sw_m, seed, power_its)]
+        h = OrPhmg()
+        h.nel, h.lf, h.lm, h.lc = case.nel, hs.lf, hs.lm, hs.lc
+        cases = (case, self.mid, self.crs)
+        for l, c in enumerate(cases):
+            h.pb[l] = C.cast(C.pointer(c.problem()), C.c_void_p)
+        h.ch[0], h.ch[1] = self.ch[0].c, self.ch[1].c
+        h.smooth_its = smooth_its
+        h.jfm, h.jfm_t = hs.jfm.ctypes.data, hs.jfm_t.ctypes.data
+        h.jmc, h.jmc_t = hs.jmc.ctypes.data, hs.jmc_t.ctypes.data
+        h.pc_crs = self.pc_crs.ctypes.data
+        h.crs_niter, h.crs_tol = crs_niter, crs_tol
+        self.work = [[np.zeros(c.n) for c in cases] for _ in range(3)]
+        for l in range(3):
+            h.r[l] = self.work[0][l].ctypes.data
+            h.z[l] = self.work[1][l].ctypes.data
+            h.w[l] = self.work[2][l].ctypes.data
+        self.h = h
+
+    def apply(self, r):
+        z = np.zeros(self.case.n)
+        lib().or_phmg_solve(C.byref(self.h), z, np.ascontiguousarray(r, dtype=np.float64))
+        return z
+
+    def gmres(self, f, abs_tol, max_iter=10000, lgmres=30):
+        x = np.zeros(self.case.n)
+        mon = lib().or_gmres_phmg_solve(C.byref(self.case.problem()), C.byref(self.h),
+                                        x, np.ascontiguousarray(f), max_iter,
+                                        float(abs_tol), lgmres)
+        return x, mon
