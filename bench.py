@@ -31,9 +31,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def ax_bytes_per_dof(lx, h1_field=False, h2=False):
-    # read u + G11..G23 (6) + write w, FP64; +h1 field, +h2*B field
+FP64_PEAK_TFS = 78.6   # MI355X FP64 (vector and matrix) peak
+
+
+def ax_bytes_per_dof(lx, code, h1_field=False, h2=False):
+    """Algorithmic HBM bytes per local DOF of one FP64 Ax launch.
+    Stored geometry (codes < 200): read u + G11..G23 + write w (+ h1 field,
+    + h2*B field).  In-kernel trilinear geometry (codes >= 200): read u +
+    write w + 36 edge values per element."""
+    if code >= 200:
+        return 8 * 2 + 36 * 8 / lx ** 3
     return 8 * (1 + 6 + 1) + (8 if h1_field else 0) + (8 if h2 else 0)
+
+
+def ax_flops_per_dof(lx, code):
+    """FP64 flops per local DOF of one Ax launch (FMA = 2), counted from
+    the kernel source: derivatives 6 lx, geometric flux 15, transposes
+    6 lx, final sum 2 (113 at lx = 8; Neko counts 15 + 12 lx); the
+    trilinear variant adds its Jacobian, cofactors, jac, G and B (91)."""
+    base = 2 * 3 * lx + 15 + 2 * lx + 2 * 2 * lx + 2
+    return base + (91 if code >= 200 else 0)
 
 
 def gs_bytes(info):
@@ -54,6 +71,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-helmholtz", action="store_true")
+    ap.add_argument("--max-iter", type=int, default=20000,
+                    help="iteration cap of the solves (profiling runs)")
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rates", default="200,800,2000")
@@ -112,8 +132,10 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     ms_ax = e0.elapsed_time(e1) / args.steps
-    ax_bytes = ax_bytes_per_dof(lx) * n
+    code64 = tuned["ax64"]
+    ax_bytes = ax_bytes_per_dof(lx, code64) * n
     ax_gbs = ax_bytes / (ms_ax * 1e-3) / 1e9
+    ax_tfs = ax_flops_per_dof(lx, code64) * n / (ms_ax * 1e-3) / 1e12
     # gs alone
     e0.record(stream)
     for _ in range(args.steps):
@@ -122,7 +144,7 @@ def main():
     torch.cuda.synchronize()
     ms_gs = e0.elapsed_time(e1) / args.steps
     gsb = gs_bytes(info)
-    log(f"[bench] Ax {ms_ax:.3f} ms ({ax_gbs:.0f} GB/s)  gs {ms_gs:.3f} ms "
+    log(f"[bench] Ax {ms_ax:.3f} ms ({ax_gbs:.0f} GB/s, {ax_tfs:.1f} TFLOP/s)  gs {ms_gs:.3f} ms "
         f"({gsb / (ms_gs * 1e-3) / 1e9:.0f} GB/s)")
     axgs_bytes = ax_bytes + gsb
 
@@ -149,17 +171,7 @@ def main():
         "axgs": {"ms": round(ms_step, 4), "bytes_per_apply": axgs_bytes,
                  "GBs": round(axgs_bytes / (ms_step * 1e-3) / 1e9, 1),
                  "roofline_frac": round(axgs_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "roofline": {
-            "kernel": "ax_kstep_kernel<double,8,1>",
-            "bound": "hbm",
-            "achieved": round(ax_gbs, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(ax_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "algorithmic_bytes_per_launch": ax_bytes,
-            "ms_per_launch": round(ms_ax, 4),
-        },
+        "roofline": roofline_entry(code64, lx, ax_gbs, ax_tfs, ax_bytes, ms_ax, n),
         "gs": {"ms": round(ms_gs, 4), "bytes": gsb,
                "GBs": round(gsb / (ms_gs * 1e-3) / 1e9, 1)},
         "setup_s": round(t_setup, 2),
@@ -180,7 +192,7 @@ def main():
             tol = 1e-10 * res_start
             torch.cuda.synchronize()
             t1 = time.time()
-            mon = cg.solve(x, f, tol)
+            mon = cg.solve(x, f, tol, max_iter=args.max_iter)
             torch.cuda.synchronize()
             dt = time.time() - t1
             solves[prec] = {"seconds": round(dt, 3), "iterations": mon.iter,
@@ -197,6 +209,10 @@ def main():
         solves["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(xm - xd) / torch.linalg.norm(xd))
         solves["speedup_mixed"] = round(solves["fp64"]["seconds"] / solves["mixed"]["seconds"], 3)
         result["solves"] = solves
+
+    # ---- Helmholtz with the Chebyshev/Schwarz p-multigrid ---------------------
+    if not args.no_helmholtz:
+        result["helmholtz_pmg"] = helmholtz_pmg(S)
 
     # ---- batched uneven instances -------------------------------------------
     if not args.no_batch:
@@ -280,6 +296,105 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
             label: run(label, arrivals, mesh_of, 1000 * (k + 1)) for label in servers}
     for sv, _ in servers.values():
         sv.free()
+    return out
+
+
+def roofline_entry(code, lx, gbs, tfs, nbytes, ms, n):
+    """Roofline object of the dominant kernel (the FP64 Ax launch): the
+    bound is the one the kernel comes closest to (HBM bytes, or the FP64
+    peak for the in-kernel-geometry variant, which reads 16.6 B/DOF and
+    executes ~204 flop/DOF); both fractions are reported."""
+    name = (f"ax_tri_kernel<double,{lx},{code % 100 or 1}>" if code >= 200 else
+            f"ax_kstep_kernel<double,{lx},{code % 100 or 1}>")
+    hbm_frac = gbs / HBM_PEAK_GBS
+    fp_frac = tfs / FP64_PEAK_TFS
+    common = {"kernel": name, "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+              "flops_per_dof": ax_flops_per_dof(lx, code), "ms_per_launch": round(ms, 4),
+              "hbm_GBs": round(gbs, 1), "hbm_frac": round(hbm_frac, 4),
+              "fp64_TFs": round(tfs, 2), "fp64_frac": round(fp_frac, 4)}
+    if fp_frac > hbm_frac:
+        return {"bound": "mfma", "achieved": round(tfs, 2), "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": round(fp_frac, 4), **common}
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(hbm_frac, 4), **common}
+
+
+def fourier_rhs(p, seed, kmax=4):
+    """b = mask(gs(B f)), f = sum over |k| <= kmax of a_k cos(2 pi k.x)
+    + b_k sin(2 pi k.x) with a_k, b_k ~ N(0,1) seeded (k in Z^3, k_x >= 0)."""
+    import numpy as np
+    import torch
+    from neko_amd import solve as S
+    rng = np.random.default_rng(seed)
+    x, y, z = (p.field(c) for c in (S.FIELD_X, S.FIELD_Y, S.FIELD_Z))
+    f = torch.zeros_like(x)
+    two_pi = 2.0 * np.pi
+    for kx in range(0, kmax + 1):
+        for ky in range(-kmax, kmax + 1):
+            for kz in range(-kmax, kmax + 1):
+                if kx * kx + ky * ky + kz * kz > kmax * kmax:
+                    continue
+                a, b = rng.standard_normal(2)
+                ph = two_pi * (kx * x + ky * y + kz * z)
+                f += a * torch.cos(ph) + b * torch.sin(ph)
+    f = (f * p.field(S.FIELD_B)).contiguous()
+    p.gs_op(f)
+    p.bc_apply(f)
+    return f
+
+
+def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10):
+    """Velocity-type Helmholtz: 48^3 elements, lx = 8, +-0.08h, periodic in
+    x, h1 = 1/1600, h2 = 1.5/dt = 1500 (dt = 1e-3), Fourier-mode RHS
+    (|k| <= 4, N(0,1) amplitudes).  GMRES(30) preconditioned by the
+    p-multigrid V-cycle on hsmg's levels 8 -> 4 -> 2 with Chebyshev(4)
+    smoothing accelerated by overlapping Schwarz and the coarse Jacobi-CG.
+    Two level-operator settings: the fine h2 on every level, and Neko's
+    coef default h2 = 1 on the coarser levels (phmg's own set-up, capped
+    at 300 iterations: its coarse correction is mis-scaled for this h2)."""
+    import numpy as np
+    import torch
+    t0 = time.time()
+    p = S.Problem.box(E, E, E, lx, periodic=(1, 0, 0), amp=0.08, seed=77,
+                      h1=1.0 / 1600, h2=1500.0)
+    tuned = p.autotune(reps=3)
+    f = fourier_rhs(p, 88)
+    mult = p.field(S.FIELD_MULT)
+    r0 = float(torch.sqrt(torch.sum(f * f * mult)) / np.sqrt(p.volume))
+    out = {"workload": f"Helmholtz box {E}^3 lx={lx} periodic x, h1=1/1600, h2=1500",
+           "local_dof": p.n, "setup_s": round(time.time() - t0, 2), "kernel_variants": tuned}
+    for lh2, cap in (("fine", 2000), ("default", 300)):
+        t1 = time.time()
+        ph = S.PHMG(p, facet_bc=2, smooth_its=4, level_h2=lh2)
+        torch.cuda.synchronize()
+        t_pc = time.time() - t1
+        g = S.GMRES(p, pc=ph, lgmres=30)
+        x = torch.zeros_like(f)
+        torch.cuda.synchronize()
+        t2 = time.time()
+        m = g.solve(x, f, rel_tol * r0, max_iter=cap)
+        torch.cuda.synchronize()
+        dt = time.time() - t2
+        out[f"gmres_phmg_level_h2_{lh2}"] = {
+            "iterations": m.iter, "converged": m.converged, "seconds": round(dt, 3),
+            "solves_per_s": round(1.0 / dt, 4) if m.converged else None,
+            "ms_per_iteration": round(dt / max(m.iter, 1) * 1e3, 3),
+            "res_final_rel": m.res_final / m.res_start, "pc_setup_s": round(t_pc, 2),
+            "chebyshev_bounds": ph.eigs()}
+        log(f"[bench] helmholtz phmg({lh2}): {out[f'gmres_phmg_level_h2_{lh2}']}")
+        g.free()
+        ph.free()
+    # Jacobi-PCG on the same system for reference
+    cg = S.CG(p, "fp64")
+    x = torch.zeros_like(f)
+    torch.cuda.synchronize()
+    t3 = time.time()
+    m = cg.solve(x, f, rel_tol * r0)
+    torch.cuda.synchronize()
+    out["cg_jacobi_fp64"] = {"iterations": m.iter, "seconds": round(time.time() - t3, 4)}
+    log(f"[bench] helmholtz cg jacobi: {out['cg_jacobi_fp64']}")
+    cg.free()
+    p.free()
     return out
 
 
