@@ -35,6 +35,7 @@ struct CgSolver {
   double *part = nullptr;
   double *dpart = nullptr;   // fused p.Ap partials (trilinear Ax), npart
   int64_t npart = 0;
+  bool cgp = false;          // CG prologue fused into Ax (x one step behind)
   CgScal *sc = nullptr;      // device: [0] main/inner, [1] outer
   CgScal *h_sc = nullptr;    // pinned host mirror
   hipGraphExec_t g64 = nullptr, g32 = nullptr;

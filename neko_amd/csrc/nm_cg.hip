@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "nm_common.h"
@@ -198,6 +199,43 @@ __global__ void cg_fin_kernel(const double *__restrict__ part, CgScal *__restric
   }
 }
 
+// fused schedule (CG prologue inside the Ax kernel): r -= alpha w; partial
+// rtr and rtz of the new r
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS)
+    cg_update2_kernel(T *__restrict__ r, const T *__restrict__ w, const T *__restrict__ mult,
+                      const T *__restrict__ dinv, int64_t n, const CgScal *__restrict__ sc,
+                      double *__restrict__ part) {
+  if (sc->done) return;
+  __shared__ double sh[16];
+  const T alpha_t = (T)sc->alpha;
+  double a = 0.0, b = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T ri = r[i] - alpha_t * w[i];
+    r[i] = ri;
+    const double rm = (double)ri * (double)ri * (double)mult[i];
+    a += rm;
+    b += rm * (double)dinv[i];
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[RED_BLOCKS + blockIdx.x] = b;
+  }
+}
+
+// the x update the fused schedule still owes: x += alpha p
+template <typename T>
+__global__ void cg_flush_kernel(T *__restrict__ x, const T *__restrict__ p, int64_t n,
+                                const CgScal *__restrict__ sc) {
+  const T alpha_t = (T)sc->alpha;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = x[i] + alpha_t * p[i];
+}
+__global__ void cg_clear_alpha_kernel(CgScal *sc) { sc->alpha = 0.0; }
+
 // mixed precision: FP32 CG with FP64 residual replacement ----------------------
 // The FP32 recurrence runs on (x32, r32, p32, w32) with FP32 operator and
 // vectors and FP64 dot-product accumulation.  Whenever its residual has
@@ -363,9 +401,24 @@ __global__ void __launch_bounds__(RED_THREADS)
 template <typename T, typename TX>
 static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
                         double *part, hipStream_t s, double *dpart = nullptr,
-                        int64_t npart = 0) {
+                        int64_t npart = 0, bool cgp = false) {
   const OpFields<T> &F = op.fields<T>();
   const int64_t n = op.n;
+  if constexpr (std::is_same<T, TX>::value) {
+    if (cgp) {
+      // p update and the previous x update ride in the Ax kernel
+      int st = op.apply_cgp<T>(w, p, r, x, &sc->beta, &sc->alpha, &sc->done, s, dpart);
+      if (st) return st;
+      hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, dpart,
+                         npart, part, sc);
+      hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+      hipLaunchKernelGGL(cg_update2_kernel<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, r, w,
+                         F.mult, F.dinv, n, sc, part);
+      hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+      NM_HIP_TRY(hipGetLastError());
+      return NM_OK;
+    }
+  }
   hipLaunchKernelGGL(cg_p_kernel<T>, dim3(vec_blocks(n)), dim3(256), 0, s, p, r, F.dinv, n, sc);
   const bool fused = npart > 0 && dpart != nullptr;
   int st = op.apply<T>(w, p, &sc->done, s, true, true, fused ? dpart : nullptr);
@@ -470,9 +523,10 @@ static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
   NM_HIP_TRY(hipStreamBeginCapture(cg->s, hipStreamCaptureModeThreadLocal));
   int st = NM_OK;
   const bool fused = cg->op->fused_dot<T>();
+  cg->cgp = fused && std::is_same<T, TX>::value;
   for (int k = 0; k < cg->graph_iters && st == NM_OK; k++)
     st = cg_iteration<T, TX>(*cg->op, x, r, p, w, sc, cg->part, cg->s,
-                             fused ? cg->dpart : nullptr, fused ? cg->npart : 0);
+                             fused ? cg->dpart : nullptr, fused ? cg->npart : 0, cg->cgp);
   hipError_t e = hipStreamEndCapture(cg->s, &graph);
   if (st) return st;
   NM_HIP_TRY(e);
@@ -582,6 +636,13 @@ int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const doub
 }
 
 static int enqueue_replace(CgSolver *cg) {
+  if (cg->cgp) {
+    const int64_t n = cg->op->n;
+    hipLaunchKernelGGL(cg_flush_kernel<float>, dim3(vec_blocks(n)), dim3(256), 0, cg->s, cg->e32,
+                       cg->p32, n, cg->sc);
+    hipLaunchKernelGGL(cg_clear_alpha_kernel, dim3(1), dim3(1), 0, cg->s, cg->sc);
+    NM_HIP_TRY(hipGetLastError());
+  }
   int st = mx_slice_replace(*cg->op, cg->x, cg->e32, cg->r32, cg->f, cg->w, cg->sc, cg->part,
                             cg->s);
   if (st) return st;
@@ -653,6 +714,12 @@ int cg_poll(CgSolver *cg) {
   switch (cg->phase) {
     case CgSolver::RUN64:
       if (in.done) {
+        if (cg->cgp) {
+          hipLaunchKernelGGL(cg_flush_kernel<double>, dim3(vec_blocks(n)), dim3(256), 0, cg->s,
+                             cg->x, cg->p, n, cg->sc);
+          hipLaunchKernelGGL(cg_clear_alpha_kernel, dim3(1), dim3(1), 0, cg->s, cg->sc);
+          NM_HIP_TRY(hipGetLastError());
+        }
         cg->total = in.iter;
         cg->res_start = in.res_start;
         cg->rnorm = in.rnorm;

@@ -35,6 +35,12 @@ struct AxArgs {
   // optional fused partial dot sum_i u_i (A_local u)_i, one double per
   // workgroup (grid size returned by ax_grid)
   double *dot_part = nullptr;
+  // optional fused CG prologue (trilinear variants, with dot_part): u is
+  // the search direction p, replaced by dinv r + beta p while loading, and
+  // x += alpha p_old; beta/alpha read from the device
+  const T *cg_r = nullptr, *cg_dinv = nullptr;
+  T *cg_x = nullptr;
+  const double *cg_beta = nullptr, *cg_alpha = nullptr;
 };
 
 // Workgroups of an Ax launch for a variant code (for sizing dot_part)

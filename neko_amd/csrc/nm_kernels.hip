@@ -169,14 +169,27 @@ __global__ void __launch_bounds__(LX *LX *EPB)
 // Body shared by the single-problem and the batched kernels.  Element q of
 // the workgroup: `active`, its first dof at w/u + `ebase`, its 36 edge
 // values at `ed`.  With DOT the workgroup sum lands in *dot_out.
-template <typename T, int LX, int EPB, bool DOT>
+// CG prologue fused into the operator (CGP): the operand is the new search
+// direction p = dinv r + beta p_old, formed while loading and written back
+// over p_old, and the previous step's x += alpha p_old is applied from the
+// same loads.
+template <typename T>
+struct CgPro {
+  T *p;
+  const T *r, *dinv;
+  T *x;
+  T beta, alpha;
+};
+
+template <typename T, int LX, int EPB, bool DOT, bool CGP = false>
 __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restrict__ u,
                                             const T *__restrict__ dx,
                                             const T *__restrict__ zg,
                                             const T *__restrict__ wg,
                                             const T *__restrict__ ed, T h1c, T h2c,
                                             bool active, int64_t ebase,
-                                            double *__restrict__ dot_out) {
+                                            double *__restrict__ dot_out,
+                                            CgPro<T> cp = CgPro<T>{}) {
   constexpr int P = LX * LX;
   constexpr int N3 = LX * P;
   __shared__ T su[EPB][N3];
@@ -195,7 +208,19 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   T uc[LX];
 #pragma unroll
   for (int k = 0; k < LX; k++) {
-    uc[k] = active ? u[base + k * P] : T(0);
+    if (CGP) {
+      T v = T(0);
+      if (active) {
+        const int64_t ix = base + k * P;
+        const T po = cp.p[ix];
+        v = cp.dinv[ix] * cp.r[ix] + cp.beta * po;
+        cp.p[ix] = v;
+        cp.x[ix] = cp.x[ix] + cp.alpha * po;
+      }
+      uc[k] = v;
+    } else {
+      uc[k] = active ? u[base + k * P] : T(0);
+    }
     su[q][k * P + t] = uc[k];
   }
   for (int x = t; x < 36; x += P) sedge[q][x] = active ? ed[x] : T(0);
@@ -314,19 +339,24 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   }
 }
 
-template <typename T, int LX, int EPB, bool DOT>
+template <typename T, int LX, int EPB, bool DOT, bool CGP>
 __global__ void __launch_bounds__(LX *LX *EPB)
     ax_tri_kernel(T *__restrict__ w, const T *__restrict__ u,
                   const T *__restrict__ dx, const T *__restrict__ zg,
                   const T *__restrict__ wg, const T *__restrict__ edges, T h1c,
                   T h2c, int64_t nel, const int *__restrict__ skip,
-                  double *__restrict__ dot_part) {
+                  double *__restrict__ dot_part, const T *__restrict__ cg_r,
+                  const T *__restrict__ cg_dinv, T *__restrict__ cg_x,
+                  const double *__restrict__ cg_beta, const double *__restrict__ cg_alpha) {
   if (skip != nullptr && *skip) return;
   const int64_t e = (int64_t)blockIdx.x * EPB + threadIdx.y;
   const bool active = e < nel;
-  ax_tri_body<T, LX, EPB, DOT>(w, u, dx, zg, wg, edges + (active ? e : 0) * 36, h1c, h2c,
-                               active, e * (int64_t)(LX * LX * LX),
-                               DOT ? dot_part + blockIdx.x : nullptr);
+  CgPro<T> cp{};
+  if (CGP) cp = CgPro<T>{const_cast<T *>(u), cg_r, cg_dinv, cg_x, (T)*cg_beta, (T)*cg_alpha};
+  ax_tri_body<T, LX, EPB, DOT, CGP>(w, CGP ? nullptr : u, dx, zg, wg,
+                                    edges + (active ? e : 0) * 36, h1c,
+                                    h2c, active, e * (int64_t)(LX * LX * LX),
+                                    DOT ? dot_part + blockIdx.x : nullptr, cp);
 }
 
 // Batched: workgroup b of a launch is global workgroup blk_base + b, which
@@ -400,12 +430,23 @@ static int ax_tri_go(const AxArgs<T> &a, hipStream_t s) {
   dim3 block(LX * LX, EPB);
   dim3 grid((unsigned)((a.nel + EPB - 1) / EPB));
   if (a.nel <= 0) return NM_OK;
-  if (a.dot_part)
-    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, true>), grid, block, 0, s, a.w, a.u, a.dx,
-                       a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part);
-  else
-    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, false>), grid, block, 0, s, a.w, a.u, a.dx,
-                       a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part);
+  if (a.cg_r) {
+    if (!a.dot_part || !a.cg_dinv || !a.cg_x || !a.cg_beta || !a.cg_alpha) {
+      set_error("ax: fused CG prologue needs dot partials, dinv, x and scalars");
+      return NM_ERR_ARG;
+    }
+    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, true, true>), grid, block, 0, s, a.w, a.u,
+                       a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part,
+                       a.cg_r, a.cg_dinv, a.cg_x, a.cg_beta, a.cg_alpha);
+  } else if (a.dot_part) {
+    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, true, false>), grid, block, 0, s, a.w, a.u,
+                       a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part,
+                       nullptr, nullptr, nullptr, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL((ax_tri_kernel<T, LX, EPB, false, false>), grid, block, 0, s, a.w, a.u,
+                       a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip, a.dot_part,
+                       nullptr, nullptr, nullptr, nullptr, nullptr);
+  }
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }

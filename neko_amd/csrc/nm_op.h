@@ -70,6 +70,11 @@ struct Op {
   template <typename T>
   int apply(T *w, const T *u, const int *skip, hipStream_t s, bool gs = true,
             bool mask = true, double *dot_part = nullptr) const;
+  // w = mask(gs(A p)) with the CG prologue fused into the trilinear Ax:
+  // p <- dinv r + (*beta) p and x += (*alpha) p_old while loading p
+  template <typename T>
+  int apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double *alpha,
+                const int *skip, hipStream_t s, double *dot_part) const;
   // variant code of the Ax kernel used for precision T
   template <typename T>
   int ax_code() const;

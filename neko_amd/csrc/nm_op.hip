@@ -122,6 +122,47 @@ int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
   if (do_mask) st = launch_mask<T>(w, msk, nmsk, skip, s);
   return st;
 }
+template <typename T>
+int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double *alpha,
+                  const int *skip, hipStream_t s, double *dot_part) const {
+  const OpFields<T> &F = fields<T>();
+  const int code = ax_code<T>();
+  if (code < 200 || !dot_part) {
+    set_error("apply_cgp: needs the trilinear Ax variant and dot partials");
+    return NM_ERR_STATE;
+  }
+  AxArgs<T> a;
+  a.w = w;
+  a.u = p;
+  a.dx = F.dx;
+  a.h1 = F.h1;
+  a.h1c = (T)h1c;
+  for (int c = 0; c < 6; c++) a.g[c] = F.g[c];
+  a.h2b = nullptr;
+  a.nel = nel;
+  a.lx = lx;
+  a.skip = skip;
+  a.edges = F.edges;
+  a.zg = F.zg;
+  a.wg = F.wg;
+  a.h2c = (T)(ifh2 ? h2c : 0.0);
+  a.dot_part = dot_part;
+  a.cg_r = r;
+  a.cg_dinv = F.dinv;
+  a.cg_x = x;
+  a.cg_beta = beta;
+  a.cg_alpha = alpha;
+  int st = launch_ax<T>(a, code, s);
+  if (st) return st;
+  if ((st = gs<T>(w, skip, s))) return st;
+  return launch_mask<T>(w, msk, nmsk, skip, s);
+}
+template int Op::apply_cgp<double>(double *, double *, const double *, double *,
+                                   const double *, const double *, const int *, hipStream_t,
+                                   double *) const;
+template int Op::apply_cgp<float>(float *, float *, const float *, float *, const double *,
+                                  const double *, const int *, hipStream_t, double *) const;
+
 template int Op::apply<double>(double *, const double *, const int *,
                                hipStream_t, bool, bool, double *) const;
 template int Op::apply<float>(float *, const float *, const int *, hipStream_t,
