@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-helmholtz", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="iteration cap of the solves (profiling runs)")
+    ap.add_argument("--pmg-default-cap", type=int, default=300,
+                    help="iteration cap of GMRES with the default-h2 level operators")
     ap.add_argument("--batch-size", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-rates", default="200,800,2000")
@@ -212,7 +214,7 @@ def main():
 
     # ---- Helmholtz with the Chebyshev/Schwarz p-multigrid ---------------------
     if not args.no_helmholtz:
-        result["helmholtz_pmg"] = helmholtz_pmg(S)
+        result["helmholtz_pmg"] = helmholtz_pmg(S, default_cap=args.pmg_default_cap)
 
     # ---- batched uneven instances -------------------------------------------
     if not args.no_batch:
@@ -299,6 +301,22 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
     return out
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE passes, scaled by the reference copy), or None
+    when that summary does not cover the kernel."""
+    path = os.path.join(ROOT, "profiles", "axgs_poisson_64cube_lx8.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                val, name = line.split(":", 1)[1].split()
+                if name == f"kernel={kernel}":
+                    return int(val)
+    except OSError:
+        pass
+    return None
+
+
 def roofline_entry(code, lx, gbs, tfs, nbytes, ms, n):
     """Roofline object of the dominant kernel (the FP64 Ax launch): the
     bound is the one the kernel comes closest to (HBM bytes, or the FP64
@@ -308,7 +326,8 @@ def roofline_entry(code, lx, gbs, tfs, nbytes, ms, n):
             f"ax_kstep_kernel<double,{lx},{code % 100 or 1}>")
     hbm_frac = gbs / HBM_PEAK_GBS
     fp_frac = tfs / FP64_PEAK_TFS
-    common = {"kernel": name, "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+    common = {"kernel": name, "traffic": measured_traffic(name),
+              "algorithmic_bytes_per_launch": nbytes,
               "flops_per_dof": ax_flops_per_dof(lx, code), "ms_per_launch": round(ms, 4),
               "hbm_GBs": round(gbs, 1), "hbm_frac": round(hbm_frac, 4),
               "fp64_TFs": round(tfs, 2), "fp64_frac": round(fp_frac, 4)}
@@ -343,7 +362,7 @@ def fourier_rhs(p, seed, kmax=4):
     return f
 
 
-def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10):
+def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
     """Velocity-type Helmholtz: 48^3 elements, lx = 8, +-0.08h, periodic in
     x, h1 = 1/1600, h2 = 1.5/dt = 1500 (dt = 1e-3), Fourier-mode RHS
     (|k| <= 4, N(0,1) amplitudes).  GMRES(30) preconditioned by the
@@ -363,7 +382,7 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10):
     r0 = float(torch.sqrt(torch.sum(f * f * mult)) / np.sqrt(p.volume))
     out = {"workload": f"Helmholtz box {E}^3 lx={lx} periodic x, h1=1/1600, h2=1500",
            "local_dof": p.n, "setup_s": round(time.time() - t0, 2), "kernel_variants": tuned}
-    for lh2, cap in (("fine", 2000), ("default", 300)):
+    for lh2, cap in (("fine", 2000), ("default", default_cap)):
         t1 = time.time()
         ph = S.PHMG(p, facet_bc=2, smooth_its=4, level_h2=lh2)
         torch.cuda.synchronize()
