@@ -267,7 +267,7 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
         sv = S.Server(budget, max_inflight=inflight, precision=precision,
                       chunk_iters=8)
         ids = [sv.add_box_mesh(*d, lx=8, amp=0.1, seed=100 + k,
-                               prewarm=min(inflight, 2))
+                               prewarm=min(inflight, 2), tune_reps=3)
                for k, d in enumerate(dims)]
         servers[label] = (sv, ids)
 

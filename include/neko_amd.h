@@ -303,11 +303,12 @@ int nm_server_create(nm_server_t **sv, double hbm_budget_bytes,
                      int max_inflight, int precision, int chunk_iters,
                      int keep_solutions);
 void nm_server_free(nm_server_t *sv);
-/* register a resident Dirichlet box mesh (Poisson, h1 = 1); prewarm: CG
- * states to allocate up front */
+/* register a resident Dirichlet box mesh (Poisson, h1 = 1); tune_reps > 0
+ * runs the kernel-variant autotuner on it first; prewarm: CG states to
+ * allocate up front */
 int nm_server_add_box_mesh(nm_server_t *sv, int nelx, int nely, int nelz,
-                           int lx, double amp, uint64_t seed, int prewarm,
-                           int *mesh_id, hipStream_t s);
+                           int lx, double amp, uint64_t seed, int tune_reps,
+                           int prewarm, int *mesh_id, hipStream_t s);
 int nm_server_mesh_op(nm_server_t *sv, int mesh_id, nm_op_t **op);
 /* a request: mesh, RHS seed (nm_op_rhs_normal), relative tolerance,
  * arrival time in seconds after the start of nm_server_run */

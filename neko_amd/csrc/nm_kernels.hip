@@ -244,26 +244,27 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   const T sj1 = (T(1) + zg[j]) * T(0.5), sj0 = T(1) - sj1;
   const T wij = wg[i] * wg[j];
   const T *E = sedge[q];
-  T ct[3], ar0[3], ar1[3], as0[3], as1[3];
+  T ct[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
+  for (int c = 0; c < 3; c++)
     ct[c] = (E[24 + c * 4 + 0] * ri0 + E[24 + c * 4 + 1] * ri1) * sj0 +
             (E[24 + c * 4 + 2] * ri0 + E[24 + c * 4 + 3] * ri1) * sj1;
-    ar0[c] = E[c * 4 + 0] * sj0 + E[c * 4 + 1] * sj1;        // cz = 0
-    ar1[c] = E[c * 4 + 2] * sj0 + E[c * 4 + 3] * sj1;        // cz = 1
-    as0[c] = E[12 + c * 4 + 0] * ri0 + E[12 + c * 4 + 1] * ri1;
-    as1[c] = E[12 + c * 4 + 2] * ri0 + E[12 + c * 4 + 3] * ri1;
-  }
 
 #pragma unroll 1
   for (int k = 0; k < LX; k++) {
     const int b = k & 1;
     const T tk1 = (T(1) + zg[k]) * T(0.5), tk0 = T(1) - tk1;
+    // dX/dr and dX/ds from the element's edges in LDS (broadcast reads;
+    // recomputed per plane to keep them out of registers)
     T a[3], bb[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      a[c] = ar0[c] * tk0 + ar1[c] * tk1;    // dX_c/dr
-      bb[c] = as0[c] * tk0 + as1[c] * tk1;   // dX_c/ds
+      const T e0 = E[c * 4 + 0] * tk0 + E[c * 4 + 2] * tk1;
+      const T e1 = E[c * 4 + 1] * tk0 + E[c * 4 + 3] * tk1;
+      const T f0 = E[12 + c * 4 + 0] * tk0 + E[12 + c * 4 + 2] * tk1;
+      const T f1 = E[12 + c * 4 + 1] * tk0 + E[12 + c * 4 + 3] * tk1;
+      a[c] = e0 * sj0 + e1 * sj1;     // dX_c/dr
+      bb[c] = f0 * ri0 + f1 * ri1;    // dX_c/ds
     }
     const T *cc = ct;                         // dX_c/dt
     const T rx = bb[1] * cc[2] - cc[1] * bb[2];

@@ -194,12 +194,17 @@ int nm_server_create(nm_server_t **out, double hbm_budget_bytes, int max_infligh
 void nm_server_free(nm_server_t *h) { delete h; }
 
 int nm_server_add_box_mesh(nm_server_t *h, int nelx, int nely, int nelz, int lx,
-                           double amp, uint64_t seed, int prewarm, int *mesh_id,
-                           hipStream_t s) {
+                           double amp, uint64_t seed, int tune_reps, int prewarm,
+                           int *mesh_id, hipStream_t s) {
   Server &sv = h->sv;
   ServeMesh m;
   int st = nm_op_create_box(&m.op, nelx, nely, nelz, lx, 0, 0, 0, amp, seed, 1.0, 0.0, s);
   if (st) return st;
+  // kernel variants are fixed before any CG state captures its graphs
+  if (tune_reps > 0 && (st = nm_op_autotune(m.op, tune_reps, nullptr, nullptr, nullptr, s))) {
+    nm_op_free(m.op);
+    return st;
+  }
   if (sv.used + m.op->op.bytes > sv.budget) {
     nm_op_free(m.op);
     set_error("nm_server_add_box_mesh: mesh does not fit the HBM budget");

@@ -471,11 +471,12 @@ class Server:
         self.h = h
         self.meshes = []
 
-    def add_box_mesh(self, nelx, nely, nelz, lx=8, amp=0.1, seed=0, prewarm=1):
+    def add_box_mesh(self, nelx, nely, nelz, lx=8, amp=0.1, seed=0, prewarm=1,
+                     tune_reps=0):
         mid = C.c_int()
         check(lib().nm_server_add_box_mesh(self.h, nelx, nely, nelz, lx,
-                                           float(amp), int(seed), int(prewarm),
-                                           C.byref(mid), _stream()),
+                                           float(amp), int(seed), int(tune_reps),
+                                           int(prewarm), C.byref(mid), _stream()),
               "nm_server_add_box_mesh")
         self.meshes.append((nelx, nely, nelz, lx, amp, seed))
         return mid.value

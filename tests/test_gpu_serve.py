@@ -46,7 +46,8 @@ def test_serve_interleaved_equals_alone(precision):
     print("serve", precision, flush=True)
     sv = Server(8e9, max_inflight=4, precision=precision, chunk_iters=8,
                 keep_solutions=True)
-    ids = [sv.add_box_mesh(*m, lx=8, amp=0.1, seed=10 + k, prewarm=1)
+    ids = [sv.add_box_mesh(*m, lx=8, amp=0.1, seed=10 + k, prewarm=1,
+                           tune_reps=2 if k == 0 else 0)
            for k, m in enumerate(MESHES)]
     reqs = []
     arrivals = [0.0, 0.0, 0.001, 0.002, 0.002, 0.004, 0.02]
