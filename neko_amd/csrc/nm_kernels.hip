@@ -225,13 +225,21 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   }
   for (int x = t; x < 36; x += P) sedge[q][x] = active ? ed[x] : T(0);
 
-  T Di[LX], Dj[LX], Dti[LX], Dtj[LX];
+  // rows of D in registers; columns too in FP64, read from LDS in FP32
+  // (there it buys a fifth wave per SIMD, in FP64 the compiler does better
+  // with them in registers)
+  constexpr bool kDtLds = std::is_same<T, float>::value;
+  __shared__ T sD[LX * LX];
+  if (kDtLds && q == 0) sD[t] = dx[t];
+  T Di[LX], Dj[LX], Dti[kDtLds ? 1 : LX], Dtj[kDtLds ? 1 : LX];
 #pragma unroll
   for (int l = 0; l < LX; l++) {
     Di[l] = dx[i + LX * l];
     Dj[l] = dx[j + LX * l];
-    Dti[l] = dx[l + LX * i];
-    Dtj[l] = dx[l + LX * j];
+    if (!kDtLds) {
+      Dti[l] = dx[l + LX * i];
+      Dtj[l] = dx[l + LX * j];
+    }
   }
   T wc[LX];
 #pragma unroll
@@ -308,9 +316,11 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
     __syncthreads();
     T acc = hB * plane[t];
 #pragma unroll
-    for (int l = 0; l < LX; l++) acc += Dti[l] * sr[q][b][l + LX * j];
+    for (int l = 0; l < LX; l++)
+      acc += (kDtLds ? sD[l + LX * i] : Dti[kDtLds ? 0 : l]) * sr[q][b][l + LX * j];
 #pragma unroll
-    for (int l = 0; l < LX; l++) acc += Dtj[l] * ss[q][b][i + LX * l];
+    for (int l = 0; l < LX; l++)
+      acc += (kDtLds ? sD[l + LX * j] : Dtj[kDtLds ? 0 : l]) * ss[q][b][i + LX * l];
     sw[q][k * P + t] = acc;
   }
 
