@@ -174,6 +174,7 @@ struct Batch {
   CgScal *sc = nullptr, *h_sc = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_iters = 8;
+  bool compacted = false;   // tables currently cover a subset
   int64_t bytes = 0;
   std::vector<void *> owned;
   ~Batch() {
@@ -253,13 +254,17 @@ static int batch_capture(Batch *B) {
 
 // Build the per-group workgroup tables for the current elements-per-
 // workgroup choices.
-static int batch_tables(Batch *B) {
+// With `finished`, problems marked there get no workgroups (compaction of
+// the Ax launches once part of the batch has converged; a problem's own
+// workgroups keep their relative order, so its results do not change).
+static int batch_tables(Batch *B, const std::vector<char> *finished = nullptr) {
   const int ni = (int)B->ops.size();
   int64_t gb = 0;
   std::vector<int32_t> bi;
   for (LxGroup &g : B->groups) {
     g.blk_base = gb;
     for (int m : g.members) {
+      if (finished && (*finished)[m]) continue;
       BatchInst &I = B->h_inst[m];
       I.blk0 = gb;
       I.nblk = (I.nel + g.epb - 1) / g.epb;
@@ -526,8 +531,12 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
                              B->s)))
       return st;
   }
+  if (B->compacted) {
+    if ((st = batch_tables(B)) || (st = batch_capture(B))) return st;
+    B->compacted = false;
+  }
   std::vector<char> state(ni, 0);  // 0 running, 1 finished
-  int left = ni, launches = 0;
+  int left = ni, launches = 0, covered = ni;
   const int limit = max_iter / B->graph_iters + 64;
   while (left > 0) {
     NM_HIP_TRY(hipGraphLaunch(B->graph, B->s));
@@ -550,6 +559,12 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
     if (launches > limit) {
       set_error("nm_batch_solve: problems did not stop after %d graph launches", launches);
       return NM_ERR_STATE;
+    }
+    // drop finished problems from the Ax launches once half are done
+    if (left > 0 && 2 * left <= covered) {
+      if ((st = batch_tables(B, &state)) || (st = batch_capture(B))) return st;
+      B->compacted = true;
+      covered = left;
     }
   }
   for (int i = 0; i < ni; i++) {
