@@ -152,21 +152,43 @@ __global__ void __launch_bounds__(256)
 
 // fdm_do_fast (3-D): e = (S_t (x) S_s (x) S_r) D (S_t^T (x) S_s^T (x) S_r^T) r
 // with per-element S (nl x nl) and D (nl^3).
+// face-interior point q of the 6 faces of an NL^3 block: face f (0,1: r low
+// / high, 2,3: s, 4,5: t), (u,v) in [1, NL-2]^2; returns the point index at
+// layer `depth` from that face
 template <int NL>
+__device__ inline int face_point(int f, int u, int v, int depth) {
+  const int l = (f & 1) ? NL - 1 - depth : depth;
+  switch (f >> 1) {
+    case 0: return l + NL * (u + NL * v);
+    case 1: return u + NL * (l + NL * v);
+    default: return u + NL * (v + NL * l);
+  }
+}
+
+template <int NL, bool FUSED>
 __global__ void __launch_bounds__(256)
-    fdm_kernel(double *__restrict__ e, const double *__restrict__ r,
-               const double *__restrict__ S, const double *__restrict__ D,
-               int64_t nel) {
-  constexpr int N3 = NL * NL * NL, N2 = NL * NL;
+    fdm_kernel(double *__restrict__ e, double *r, const double *__restrict__ S,
+               const double *__restrict__ D, int64_t nel) {
+  constexpr int N3 = NL * NL * NL, N2 = NL * NL, W = NL - 2, NF = 6 * W * W;
   __shared__ double a[N3], b[N3];
   __shared__ double s[6 * N2];
   const int64_t el = blockIdx.x;
   if (el >= nel) return;
-  const double *re = r + el * N3;
+  double *re = r + el * N3;
   const double *se = S + el * 6 * N2;
   for (int q = threadIdx.x; q < 6 * N2; q += blockDim.x) s[q] = se[q];
   for (int q = threadIdx.x; q < N3; q += blockDim.x) a[q] = re[q];
   __syncthreads();
+  if (FUSED) {
+    // the neighbour's overlap only: face layer 0 -= layer 2 (face interiors;
+    // the six faces are disjoint and read interior layers only)
+    for (int q = threadIdx.x; q < NF; q += blockDim.x) {
+      const int f = q / (W * W), u = 1 + (q % (W * W)) % W, v = 1 + (q % (W * W)) / W;
+      const int p0 = face_point<NL>(f, u, v, 0), p2 = face_point<NL>(f, u, v, 2);
+      a[p0] = a[p0] - a[p2];
+    }
+    __syncthreads();
+  }
   const double *S1 = s, *S1t = s + N2, *S2 = s + 2 * N2, *S2t = s + 3 * N2,
                *S3 = s + 4 * N2, *S3t = s + 5 * N2;
   // forward: A = S1^T (as stored in S1t: A(i,a) = S1t[i + NL a]),
@@ -214,7 +236,68 @@ __global__ void __launch_bounds__(256)
     double t = 0.0;
     for (int z = 0; z < NL; z++) t += b[ij + N2 * z] * S3t[z + NL * l];
     ee[q] = t;
+    if (FUSED) a[q] = t;
   }
+  if (FUSED) {
+    // keep this element's own face layer 0 (to subtract after the gs)
+    __syncthreads();
+    for (int q = threadIdx.x; q < NF; q += blockDim.x) {
+      const int f = q / (W * W), u = 1 + (q % (W * W)) % W, v = 1 + (q % (W * W)) / W;
+      const int p0 = face_point<NL>(f, u, v, 0);
+      re[p0] = a[p0];
+    }
+  }
+}
+
+// ext (NL^3, zero) <- interior r (lx^3) plus face layer 0 = interior layer
+// 2 (schwarz_toext3d + extrude(.., 0, 0, .., 2, 1)); one thread per point
+__global__ void toext_fused_kernel(double *__restrict__ a, const double *__restrict__ r, int n,
+                                   int64_t nel) {
+  const int m = n + 2;
+  const int64_t m3 = (int64_t)m * m * m;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * m3) return;
+  const int64_t e = idx / m3;
+  const int p = (int)(idx % m3);
+  int c[3] = {p % m, (p / m) % m, p / (m * m)};
+  int nb = 0, dir = -1;
+  for (int d = 0; d < 3; d++)
+    if (c[d] == 0 || c[d] == m - 1) {
+      nb++;
+      dir = d;
+    }
+  double v = 0.0;
+  if (nb == 0) {
+    v = r[e * (int64_t)n * n * n + i3(c[0] - 1, c[1] - 1, c[2] - 1, n)];
+  } else if (nb == 1) {
+    c[dir] = c[dir] == 0 ? 2 : m - 3;
+    v = r[e * (int64_t)n * n * n + i3(c[0] - 1, c[1] - 1, c[2] - 1, n)];
+  }
+  a[idx] = v;
+}
+
+// e (lx^3) <- interior of work2 after the overlap exchange: layer 2 of
+// every face += (work2 - work1) at layer 0 (extrude(w2, 0, 1, w1, 0, -1)
+// then extrude(w2, 2, 1, w2, 0, 1), r faces then s then t, low before
+// high), then schwarz_toreg3d
+__global__ void toreg_fused_kernel(double *__restrict__ e, const double *__restrict__ w2,
+                                   const double *__restrict__ w1, int n, int64_t nel) {
+  const int m = n + 2;
+  const int64_t n3 = (int64_t)n * n * n, m3 = (int64_t)m * m * m;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nel * n3) return;
+  const int64_t el = idx / n3;
+  const int p = (int)(idx % n3);
+  const int I = 1 + p % n, J = 1 + (p / n) % n, K = 1 + p / (n * n);
+  const double *A = w2 + el * m3, *B = w1 + el * m3;
+  double v = A[i3(I, J, K, m)];
+  if (I == 2) v += A[i3(0, J, K, m)] - B[i3(0, J, K, m)];
+  if (I == m - 3) v += A[i3(m - 1, J, K, m)] - B[i3(m - 1, J, K, m)];
+  if (J == 2) v += A[i3(I, 0, K, m)] - B[i3(I, 0, K, m)];
+  if (J == m - 3) v += A[i3(I, m - 1, K, m)] - B[i3(I, m - 1, K, m)];
+  if (K == 2) v += A[i3(I, J, 0, m)] - B[i3(I, J, 0, m)];
+  if (K == m - 3) v += A[i3(I, J, m - 1, m)] - B[i3(I, J, m - 1, m)];
+  e[idx] = v;
 }
 
 // schwarz_wt3d: weights on the two outer layers of every face
@@ -335,13 +418,19 @@ static int tnsr3d(double *v, int nv, const double *u, int nu, const double *A,
   return NM_OK;
 }
 
-static int fdm(double *e, const double *r, const double *S, const double *D,
-               int nl, int64_t nel, hipStream_t s) {
+// fused: r is the extended work array after its gs (face layer 0 is
+// corrected on load and receives e's layer 0 afterwards)
+static int fdm(double *e, double *r, const double *S, const double *D, int nl, int64_t nel,
+               hipStream_t s, bool fused = false) {
   switch (nl) {
-#define NM_FDM(N)                                                                  \
-  case N:                                                                          \
-    hipLaunchKernelGGL((fdm_kernel<N>), dim3((unsigned)nel), dim3(256), 0, s, e, r, \
-                       S, D, nel);                                                 \
+#define NM_FDM(N)                                                                       \
+  case N:                                                                               \
+    if (fused)                                                                          \
+      hipLaunchKernelGGL((fdm_kernel<N, true>), dim3((unsigned)nel), dim3(256), 0, s, e, \
+                         r, S, D, nel);                                                 \
+    else                                                                                \
+      hipLaunchKernelGGL((fdm_kernel<N, false>), dim3((unsigned)nel), dim3(256), 0, s,   \
+                         e, r, S, D, nel);                                              \
     break;
     NM_FDM(5) NM_FDM(6) NM_FDM(7) NM_FDM(8) NM_FDM(9) NM_FDM(10) NM_FDM(11) NM_FDM(12)
 #undef NM_FDM
@@ -360,17 +449,16 @@ static int schwarz_apply(const Schwarz &sw, double *e, double *r, hipStream_t s)
   const int nx = sw.nl, lx = sw.lx;
   const int64_t nel = sw.nel, ns = nel * nx * nx * nx;
   int st;
+  // the reference's sequence mask, toext3d, extrude x4, fdm, toreg3d runs
+  // here as three fused passes around the two extended-layout gs
   if ((st = launch_mask<double>(r, op.msk, op.nmsk, nullptr, s))) return st;
-  hipLaunchKernelGGL(toext_kernel, dim3(blocks_for(ns)), dim3(256), 0, s, sw.work1, r, lx, nel);
-  if ((st = extrude(sw.work1, 0, 0.0, sw.work1, 2, 1.0, nx, nel, s))) return st;
+  hipLaunchKernelGGL(toext_fused_kernel, dim3(blocks_for(ns)), dim3(256), 0, s, sw.work1, r, lx,
+                     nel);
   if ((st = gs_dev(sw.ext, sw.work1, s))) return st;
-  if ((st = extrude(sw.work1, 0, 1.0, sw.work1, 2, -1.0, nx, nel, s))) return st;
-  if ((st = fdm(sw.work2, sw.work1, sw.S, sw.D, nx, nel, s))) return st;
-  if ((st = extrude(sw.work1, 0, 0.0, sw.work2, 0, 1.0, nx, nel, s))) return st;
+  if ((st = fdm(sw.work2, sw.work1, sw.S, sw.D, nx, nel, s, true))) return st;
   if ((st = gs_dev(sw.ext, sw.work2, s))) return st;
-  if ((st = extrude(sw.work2, 0, 1.0, sw.work1, 0, -1.0, nx, nel, s))) return st;
-  if ((st = extrude(sw.work2, 2, 1.0, sw.work2, 0, 1.0, nx, nel, s))) return st;
-  hipLaunchKernelGGL(toreg_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.work2, lx, nel);
+  hipLaunchKernelGGL(toreg_fused_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.work2,
+                     sw.work1, lx, nel);
   if ((st = op.gs<double>(e, nullptr, s))) return st;
   if ((st = launch_mask<double>(e, op.msk, op.nmsk, nullptr, s))) return st;
   hipLaunchKernelGGL(wt3d_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.wt, lx, nel);
@@ -421,6 +509,12 @@ static int gsdev_box(H *own, GsDev *g, const int box[3], const int per[3], int l
   g->nblks = nb;
   g->pair0 = fo - 1;
   g->npairs = (nl - (fo - 1)) / 2;
+  ptr.resize(nb + 1);
+  {
+    const int64_t nel = (int64_t)box[0] * box[1] * box[2];
+    const int st2 = gs_regroup(nel, (int64_t)lx * lx * lx, nb, g->pair0, g->npairs, ptr, gi);
+    if (st2) return st2;
+  }
   if (!(g->blk_ptr = own->template alloc<int32_t>(nb + 1)) ||
       !(g->gidx = own->template alloc<int32_t>(nl))) {
     set_error("hsmg: out of device memory");

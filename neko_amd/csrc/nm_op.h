@@ -96,6 +96,10 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
               const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
               const int32_t *blk_off, hipStream_t s);
 int op_finish(Op *op, hipStream_t s);
+// Regroup a device gs layout (blk_ptr/gidx, pairs from pair0) by the element
+// of each block's / pair's lowest dof; the order inside blocks is kept.
+int gs_regroup(int64_t nel, int64_t n3, int64_t nblks, int64_t pair0, int64_t npairs,
+               std::vector<int32_t> &ptr, std::vector<int32_t> &gi);
 
 // Coarse-grid CG with a fixed budget and no host synchronisation
 // (nm_cg.hip).

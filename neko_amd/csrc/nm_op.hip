@@ -201,6 +201,54 @@ int Op::ensure32(hipStream_t s) {
   return NM_OK;
 }
 
+// Processing order for locality: blocks and facet pairs are independent of
+// each other, so the device layout lists them grouped by the element of
+// their first (lowest) local dof -- a stable counting sort that keeps
+// Neko's order inside each group and never touches the order of the
+// entries inside a block (the summation order of a block is unchanged).
+int gs_regroup(int64_t nel, int64_t n3, int64_t nblks, int64_t pair0, int64_t npairs,
+               std::vector<int32_t> &ptr, std::vector<int32_t> &gi) {
+  const int64_t ne = nel > 0 ? nel : 1;
+  auto regroup = [&](int64_t count, auto first, auto len, auto emit) {
+    std::vector<int64_t> start(ne + 1, 0);
+    std::vector<int32_t> key(count);
+    for (int64_t b = 0; b < count; b++) {
+      int32_t m = INT32_MAX;
+      const int64_t f = first(b);
+      for (int q = 0; q < len(b); q++) m = std::min(m, gi[f + q]);
+      key[b] = (int32_t)(m / n3);
+      start[key[b] + 1]++;
+    }
+    for (int64_t e = 0; e < ne; e++) start[e + 1] += start[e];
+    std::vector<int64_t> order(count);
+    for (int64_t b = 0; b < count; b++) order[start[key[b]]++] = b;
+    for (int64_t k = 0; k < count; k++) emit(order[k]);
+  };
+  std::vector<int32_t> g2(gi.size()), p2(nblks + 1);
+  int64_t o = 0, nb = 0;
+  regroup(
+      nblks, [&](int64_t b) { return (int64_t)ptr[b]; },
+      [&](int64_t b) { return ptr[b + 1] - ptr[b]; },
+      [&](int64_t b) {
+        p2[nb++] = (int32_t)o;
+        for (int32_t q = ptr[b]; q < ptr[b + 1]; q++) g2[o++] = gi[q];
+      });
+  p2[nblks] = (int32_t)o;
+  regroup(
+      npairs, [&](int64_t pp) { return pair0 + 2 * pp; }, [&](int64_t) { return 2; },
+      [&](int64_t pp) {
+        g2[o++] = gi[pair0 + 2 * pp];
+        g2[o++] = gi[pair0 + 2 * pp + 1];
+      });
+  if (o != (int64_t)gi.size()) {
+    set_error("gs: regrouping lost entries (%lld of %lld)", (long long)o, (long long)gi.size());
+    return NM_ERR_STATE;
+  }
+  gi.swap(g2);
+  ptr.swap(p2);
+  return NM_OK;
+}
+
 int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
               const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
               const int32_t *blk_off, hipStream_t s) {
@@ -228,52 +276,10 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
     return NM_ERR_ARG;
   }
   for (int64_t i = 0; i < nlocal; i++) gi[i] = gd[i] - 1;
-  // Processing order for locality: blocks and facet pairs are independent
-  // of each other, so the device layout lists them grouped by the element
-  // of their first (lowest) local dof -- a stable counting sort that keeps
-  // Neko's order inside each group and never touches the order of the
-  // entries inside a block (the summation order of a block is unchanged).
   {
-    const int64_t n3 = (int64_t)op->lx * op->lx * op->lx;
-    const int64_t ne = op->nel > 0 ? op->nel : 1;
-    auto regroup = [&](int64_t count, auto first, auto len, auto emit) {
-      std::vector<int64_t> start(ne + 1, 0);
-      std::vector<int32_t> key(count);
-      for (int64_t b = 0; b < count; b++) {
-        int32_t m = INT32_MAX;
-        const int64_t f = first(b);
-        for (int q = 0; q < len(b); q++) m = std::min(m, gi[f + q]);
-        key[b] = (int32_t)(m / n3);
-        start[key[b] + 1]++;
-      }
-      for (int64_t e = 0; e < ne; e++) start[e + 1] += start[e];
-      std::vector<int64_t> order(count);
-      for (int64_t b = 0; b < count; b++) order[start[key[b]]++] = b;
-      for (int64_t k = 0; k < count; k++) emit(order[k]);
-    };
-    std::vector<int32_t> g2(gi.size()), p2(nblks + 1);
-    int64_t o = 0, nb = 0;
-    regroup(
-        nblks, [&](int64_t b) { return (int64_t)ptr[b]; },
-        [&](int64_t b) { return ptr[b + 1] - ptr[b]; },
-        [&](int64_t b) {
-          p2[nb++] = (int32_t)o;
-          for (int32_t q = ptr[b]; q < ptr[b + 1]; q++) g2[o++] = gi[q];
-        });
-    p2[nblks] = (int32_t)o;
-    const int64_t p0 = op->pair0;
-    regroup(
-        op->npairs, [&](int64_t pp) { return p0 + 2 * pp; }, [&](int64_t) { return 2; },
-        [&](int64_t pp) {
-          g2[o++] = gi[p0 + 2 * pp];
-          g2[o++] = gi[p0 + 2 * pp + 1];
-        });
-    if (o != nlocal) {
-      set_error("gs: regrouping lost entries (%lld of %lld)", (long long)o, (long long)nlocal);
-      return NM_ERR_STATE;
-    }
-    gi.swap(g2);
-    ptr.swap(p2);
+    const int st = gs_regroup(op->nel, (int64_t)op->lx * op->lx * op->lx, nblks, op->pair0,
+                              op->npairs, ptr, gi);
+    if (st) return st;
   }
   NM_ALLOC(op->blk_ptr, int32_t, nblks + 1);
   NM_ALLOC(op->gidx, int32_t, nlocal);
