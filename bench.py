@@ -368,9 +368,10 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
     (|k| <= 4, N(0,1) amplitudes).  GMRES(30) preconditioned by the
     p-multigrid V-cycle on hsmg's levels 8 -> 4 -> 2 with Chebyshev(4)
     smoothing accelerated by overlapping Schwarz and the coarse Jacobi-CG.
-    Two level-operator settings: the fine h2 on every level, and Neko's
-    coef default h2 = 1 on the coarser levels (phmg's own set-up, capped
-    at 300 iterations: its coarse correction is mis-scaled for this h2)."""
+    Level operators with the fine h2 on every level (smoothers in FP32 and
+    in FP64), and with Neko's coef default h2 = 1 on the coarser levels
+    (phmg's own set-up, capped at 300 iterations: its coarse correction is
+    mis-scaled for this h2)."""
     import numpy as np
     import torch
     t0 = time.time()
@@ -382,9 +383,10 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
     r0 = float(torch.sqrt(torch.sum(f * f * mult)) / np.sqrt(p.volume))
     out = {"workload": f"Helmholtz box {E}^3 lx={lx} periodic x, h1=1/1600, h2=1500",
            "local_dof": p.n, "setup_s": round(time.time() - t0, 2), "kernel_variants": tuned}
-    for lh2, cap in (("fine", 2000), ("default", default_cap)):
+    for lh2, smoother, cap in (("fine", "fp32", 2000), ("fine", "fp64", 2000),
+                               ("default", "fp64", default_cap)):
         t1 = time.time()
-        ph = S.PHMG(p, facet_bc=2, smooth_its=4, level_h2=lh2)
+        ph = S.PHMG(p, facet_bc=2, smooth_its=4, level_h2=lh2, smoother=smoother)
         torch.cuda.synchronize()
         t_pc = time.time() - t1
         g = S.GMRES(p, pc=ph, lgmres=30)
@@ -394,13 +396,14 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
         m = g.solve(x, f, rel_tol * r0, max_iter=cap)
         torch.cuda.synchronize()
         dt = time.time() - t2
-        out[f"gmres_phmg_level_h2_{lh2}"] = {
+        key = f"gmres_phmg_level_h2_{lh2}_smoother_{smoother}"
+        out[key] = {
             "iterations": m.iter, "converged": m.converged, "seconds": round(dt, 3),
             "solves_per_s": round(1.0 / dt, 4) if m.converged else None,
             "ms_per_iteration": round(dt / max(m.iter, 1) * 1e3, 3),
             "res_final_rel": m.res_final / m.res_start, "pc_setup_s": round(t_pc, 2),
             "chebyshev_bounds": ph.eigs()}
-        log(f"[bench] helmholtz phmg({lh2}): {out[f'gmres_phmg_level_h2_{lh2}']}")
+        log(f"[bench] helmholtz phmg({lh2}, {smoother}): {out[key]}")
         g.free()
         ph.free()
     # Jacobi-PCG on the same system for reference

@@ -252,9 +252,11 @@ int nm_hsmg_schwarz_fine(nm_hsmg_t *h, double *e, double *r, hipStream_t s);
  * operators take the fine h1 and ifh2; their h2 is Neko's coef default 1
  * (level_h2_fine = 0, phmg.f90:252-254) or the fine level's h2 (1).  The
  * power method starts from this project's generator (seed 3901), not
- * gfortran's random_number. */
+ * gfortran's random_number.  smoother_fp32: the Chebyshev/Schwarz
+ * smoothing (operator, gs, FDM) runs in FP32 on FP32 copies of the level
+ * vectors; transfers, eigenvalue estimates and the coarse CG stay FP64. */
 int nm_phmg_create(nm_phmg_t **h, nm_op_t *op, int facet_bc, int smooth_its,
-                   int level_h2_fine, hipStream_t s);
+                   int level_h2_fine, int smoother_fp32, hipStream_t s);
 void nm_phmg_free(nm_phmg_t *h);
 int nm_phmg_apply(nm_phmg_t *h, double *z, const double *r, hipStream_t s);
 /* (tha, dlt) of the fine and the middle level smoothers */

@@ -101,7 +101,7 @@ SIGNATURES = {
     "nm_hsmg_apply": (_i, [_vp, _vp, _vp, _vp]),
     "nm_hsmg_schwarz_fine": (_i, [_vp, _vp, _vp, _vp]),
     "nm_gmres_create": (_i, [C.POINTER(_vp), _vp, _i, _vp, _i]),
-    "nm_phmg_create": (_i, [C.POINTER(_vp), _vp, _i, _i, _i, _vp]),
+    "nm_phmg_create": (_i, [C.POINTER(_vp), _vp, _i, _i, _i, _i, _vp]),
     "nm_phmg_free": (None, [_vp]),
     "nm_phmg_apply": (_i, [_vp, _vp, _vp, _vp]),
     "nm_phmg_eigs": (_i, [_vp, C.POINTER(_d)]),

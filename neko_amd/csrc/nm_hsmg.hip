@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "nm_common.h"
@@ -150,8 +151,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// fdm_do_fast (3-D): e = (S_t (x) S_s (x) S_r) D (S_t^T (x) S_s^T (x) S_r^T) r
-// with per-element S (nl x nl) and D (nl^3).
 // face-interior point q of the 6 faces of an NL^3 block: face f (0,1: r low
 // / high, 2,3: s, 4,5: t), (u,v) in [1, NL-2]^2; returns the point index at
 // layer `depth` from that face
@@ -165,17 +164,20 @@ __device__ inline int face_point(int f, int u, int v, int depth) {
   }
 }
 
-template <int NL, bool FUSED>
+// fdm_do_fast (3-D): e = (S_t (x) S_s (x) S_r) D (S_t^T (x) S_s^T (x) S_r^T) r
+// with per-element S (nl x nl) and D (nl^3); T = double, or float for the
+// FP32 smoothers.
+template <int NL, bool FUSED, typename T>
 __global__ void __launch_bounds__(256)
-    fdm_kernel(double *__restrict__ e, double *r, const double *__restrict__ S,
-               const double *__restrict__ D, int64_t nel) {
+    fdm_kernel(T *__restrict__ e, T *r, const T *__restrict__ S, const T *__restrict__ D,
+               int64_t nel) {
   constexpr int N3 = NL * NL * NL, N2 = NL * NL, W = NL - 2, NF = 6 * W * W;
-  __shared__ double a[N3], b[N3];
-  __shared__ double s[6 * N2];
+  __shared__ T a[N3], b[N3];
+  __shared__ T s[6 * N2];
   const int64_t el = blockIdx.x;
   if (el >= nel) return;
-  double *re = r + el * N3;
-  const double *se = S + el * 6 * N2;
+  T *re = r + el * N3;
+  const T *se = S + el * 6 * N2;
   for (int q = threadIdx.x; q < 6 * N2; q += blockDim.x) s[q] = se[q];
   for (int q = threadIdx.x; q < N3; q += blockDim.x) a[q] = re[q];
   __syncthreads();
@@ -189,28 +191,28 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  const double *S1 = s, *S1t = s + N2, *S2 = s + 2 * N2, *S2t = s + 3 * N2,
+  const T *S1 = s, *S1t = s + N2, *S2 = s + 2 * N2, *S2t = s + 3 * N2,
                *S3 = s + 4 * N2, *S3t = s + 5 * N2;
   // forward: A = S1^T (as stored in S1t: A(i,a) = S1t[i + NL a]),
   //          Bt = S2, Ct = S3
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int i = q % NL, bc = q / NL;
-    double t = 0.0;
+    T t = T(0);
     for (int x = 0; x < NL; x++) t += S1t[i + NL * x] * a[x + NL * bc];
     b[q] = t;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int i = q % NL, j = (q / NL) % NL, c = q / N2;
-    double t = 0.0;
+    T t = T(0);
     for (int y = 0; y < NL; y++) t += b[i + NL * (y + NL * c)] * S2[y + NL * j];
     a[q] = t;
   }
   __syncthreads();
-  const double *de = D + el * N3;
+  const T *de = D + el * N3;
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int ij = q % N2, l = q / N2;
-    double t = 0.0;
+    T t = T(0);
     for (int z = 0; z < NL; z++) t += a[ij + N2 * z] * S3[z + NL * l];
     b[q] = de[q] * t;
   }
@@ -218,22 +220,22 @@ __global__ void __launch_bounds__(256)
   // backward: A = S1, Bt = S2^T, Ct = S3^T
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int i = q % NL, bc = q / NL;
-    double t = 0.0;
+    T t = T(0);
     for (int x = 0; x < NL; x++) t += S1[i + NL * x] * b[x + NL * bc];
     a[q] = t;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int i = q % NL, j = (q / NL) % NL, c = q / N2;
-    double t = 0.0;
+    T t = T(0);
     for (int y = 0; y < NL; y++) t += a[i + NL * (y + NL * c)] * S2t[y + NL * j];
     b[q] = t;
   }
   __syncthreads();
-  double *ee = e + el * N3;
+  T *ee = e + el * N3;
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int ij = q % N2, l = q / N2;
-    double t = 0.0;
+    T t = T(0);
     for (int z = 0; z < NL; z++) t += b[ij + N2 * z] * S3t[z + NL * l];
     ee[q] = t;
     if (FUSED) a[q] = t;
@@ -251,7 +253,8 @@ __global__ void __launch_bounds__(256)
 
 // ext (NL^3, zero) <- interior r (lx^3) plus face layer 0 = interior layer
 // 2 (schwarz_toext3d + extrude(.., 0, 0, .., 2, 1)); one thread per point
-__global__ void toext_fused_kernel(double *__restrict__ a, const double *__restrict__ r, int n,
+template <typename T>
+__global__ void toext_fused_kernel(T *__restrict__ a, const T *__restrict__ r, int n,
                                    int64_t nel) {
   const int m = n + 2;
   const int64_t m3 = (int64_t)m * m * m;
@@ -266,7 +269,7 @@ __global__ void toext_fused_kernel(double *__restrict__ a, const double *__restr
       nb++;
       dir = d;
     }
-  double v = 0.0;
+  T v = T(0);
   if (nb == 0) {
     v = r[e * (int64_t)n * n * n + i3(c[0] - 1, c[1] - 1, c[2] - 1, n)];
   } else if (nb == 1) {
@@ -280,8 +283,9 @@ __global__ void toext_fused_kernel(double *__restrict__ a, const double *__restr
 // every face += (work2 - work1) at layer 0 (extrude(w2, 0, 1, w1, 0, -1)
 // then extrude(w2, 2, 1, w2, 0, 1), r faces then s then t, low before
 // high), then schwarz_toreg3d
-__global__ void toreg_fused_kernel(double *__restrict__ e, const double *__restrict__ w2,
-                                   const double *__restrict__ w1, int n, int64_t nel) {
+template <typename T>
+__global__ void toreg_fused_kernel(T *__restrict__ e, const T *__restrict__ w2,
+                                   const T *__restrict__ w1, int n, int64_t nel) {
   const int m = n + 2;
   const int64_t n3 = (int64_t)n * n * n, m3 = (int64_t)m * m * m;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -289,8 +293,8 @@ __global__ void toreg_fused_kernel(double *__restrict__ e, const double *__restr
   const int64_t el = idx / n3;
   const int p = (int)(idx % n3);
   const int I = 1 + p % n, J = 1 + (p / n) % n, K = 1 + p / (n * n);
-  const double *A = w2 + el * m3, *B = w1 + el * m3;
-  double v = A[i3(I, J, K, m)];
+  const T *A = w2 + el * m3, *B = w1 + el * m3;
+  T v = A[i3(I, J, K, m)];
   if (I == 2) v += A[i3(0, J, K, m)] - B[i3(0, J, K, m)];
   if (I == m - 3) v += A[i3(m - 1, J, K, m)] - B[i3(m - 1, J, K, m)];
   if (J == 2) v += A[i3(I, 0, K, m)] - B[i3(I, 0, K, m)];
@@ -301,17 +305,17 @@ __global__ void toreg_fused_kernel(double *__restrict__ e, const double *__restr
 }
 
 // schwarz_wt3d: weights on the two outer layers of every face
-__global__ void wt3d_kernel(double *__restrict__ e, const double *__restrict__ wt,
-                            int n, int64_t nel) {
+template <typename T>
+__global__ void wt3d_kernel(T *__restrict__ e, const T *__restrict__ wt, int n, int64_t nel) {
   const int64_t n3 = (int64_t)n * n * n;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nel * n3) return;
   const int64_t el = idx / n3;
   const int p = (int)(idx % n3);
   const int i = p % n, j = (p / n) % n, k = p / (n * n);
-  const double *we = wt + el * (int64_t)n * n * 12;
+  const T *we = wt + el * (int64_t)n * n * 12;
   auto W = [&](int a, int b, int c, int d) { return we[a + n * (b + n * (c + 4 * d))]; };
-  double v = e[idx];
+  T v = e[idx];
   // r-faces for all (j,k); s-faces for i in 3..n-2 (1-based); t-faces for
   // i,j in 3..n-2 (schwarz.f90:606-630).  A point on several layers (n = 3)
   // takes every weight, in the reference's order.
@@ -364,6 +368,17 @@ struct Schwarz {
   GsDev ext;
   double *S = nullptr, *D = nullptr, *wt = nullptr;
   double *work1 = nullptr, *work2 = nullptr;
+  // FP32 copies for the FP32 smoothers (allocated on request)
+  float *S32 = nullptr, *D32 = nullptr, *wt32 = nullptr;
+  float *work1_32 = nullptr, *work2_32 = nullptr;
+  template <typename T>
+  void arrays(const T *&s, const T *&d, const T *&w, T *&w1, T *&w2) const {
+    if constexpr (std::is_same<T, double>::value) {
+      s = S; d = D; w = wt; w1 = work1; w2 = work2;
+    } else {
+      s = S32; d = D32; w = wt32; w1 = work1_32; w2 = work2_32;
+    }
+  }
 };
 
 struct Hsmg {
@@ -392,10 +407,11 @@ struct Hsmg {
   }
 };
 
-static int gs_dev(const GsDev &g, double *u, hipStream_t s) {
-  GsArgs<double> a{u, g.blk_ptr, g.gidx, g.nblks, g.pair0, g.npairs, nullptr};
+template <typename T>
+static int gs_dev(const GsDev &g, T *u, hipStream_t s) {
+  GsArgs<T> a{u, g.blk_ptr, g.gidx, g.nblks, g.pair0, g.npairs, nullptr};
   a.variant = g.max_blk <= 8 ? 1 : 0;
-  return launch_gs<double>(a, s);
+  return launch_gs<T>(a, s);
 }
 
 static int extrude(double *a1, int l1, double f1, const double *a2, int l2,
@@ -420,16 +436,17 @@ static int tnsr3d(double *v, int nv, const double *u, int nu, const double *A,
 
 // fused: r is the extended work array after its gs (face layer 0 is
 // corrected on load and receives e's layer 0 afterwards)
-static int fdm(double *e, double *r, const double *S, const double *D, int nl, int64_t nel,
-               hipStream_t s, bool fused = false) {
+template <typename T>
+static int fdm(T *e, T *r, const T *S, const T *D, int nl, int64_t nel, hipStream_t s,
+               bool fused = false) {
   switch (nl) {
 #define NM_FDM(N)                                                                       \
   case N:                                                                               \
     if (fused)                                                                          \
-      hipLaunchKernelGGL((fdm_kernel<N, true>), dim3((unsigned)nel), dim3(256), 0, s, e, \
-                         r, S, D, nel);                                                 \
+      hipLaunchKernelGGL((fdm_kernel<N, true, T>), dim3((unsigned)nel), dim3(256), 0, s,  \
+                         e, r, S, D, nel);                                              \
     else                                                                                \
-      hipLaunchKernelGGL((fdm_kernel<N, false>), dim3((unsigned)nel), dim3(256), 0, s,   \
+      hipLaunchKernelGGL((fdm_kernel<N, false, T>), dim3((unsigned)nel), dim3(256), 0, s,\
                          e, r, S, D, nel);                                              \
     break;
     NM_FDM(5) NM_FDM(6) NM_FDM(7) NM_FDM(8) NM_FDM(9) NM_FDM(10) NM_FDM(11) NM_FDM(12)
@@ -444,24 +461,32 @@ static int fdm(double *e, double *r, const double *S, const double *D, int nl, i
 
 // schwarz_compute (CPU branch semantics, schwarz.f90:565-595); r masked in
 // place as in the reference.
-static int schwarz_apply(const Schwarz &sw, double *e, double *r, hipStream_t s) {
+template <typename T>
+static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
   const Op &op = *sw.op;
   const int nx = sw.nl, lx = sw.lx;
   const int64_t nel = sw.nel, ns = nel * nx * nx * nx;
+  const T *S, *D, *wt;
+  T *w1, *w2;
+  sw.arrays<T>(S, D, wt, w1, w2);
+  if (!S) {
+    set_error("schwarz: FP32 arrays not set up");
+    return NM_ERR_STATE;
+  }
   int st;
   // the reference's sequence mask, toext3d, extrude x4, fdm, toreg3d runs
   // here as three fused passes around the two extended-layout gs
-  if ((st = launch_mask<double>(r, op.msk, op.nmsk, nullptr, s))) return st;
-  hipLaunchKernelGGL(toext_fused_kernel, dim3(blocks_for(ns)), dim3(256), 0, s, sw.work1, r, lx,
+  if ((st = launch_mask<T>(r, op.msk, op.nmsk, nullptr, s))) return st;
+  hipLaunchKernelGGL(toext_fused_kernel<T>, dim3(blocks_for(ns)), dim3(256), 0, s, w1, r, lx,
                      nel);
-  if ((st = gs_dev(sw.ext, sw.work1, s))) return st;
-  if ((st = fdm(sw.work2, sw.work1, sw.S, sw.D, nx, nel, s, true))) return st;
-  if ((st = gs_dev(sw.ext, sw.work2, s))) return st;
-  hipLaunchKernelGGL(toreg_fused_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.work2,
-                     sw.work1, lx, nel);
-  if ((st = op.gs<double>(e, nullptr, s))) return st;
-  if ((st = launch_mask<double>(e, op.msk, op.nmsk, nullptr, s))) return st;
-  hipLaunchKernelGGL(wt3d_kernel, dim3(blocks_for(op.n)), dim3(256), 0, s, e, sw.wt, lx, nel);
+  if ((st = gs_dev<T>(sw.ext, w1, s))) return st;
+  if ((st = fdm<T>(w2, w1, S, D, nx, nel, s, true))) return st;
+  if ((st = gs_dev<T>(sw.ext, w2, s))) return st;
+  hipLaunchKernelGGL(toreg_fused_kernel<T>, dim3(blocks_for(op.n)), dim3(256), 0, s, e, w2, w1,
+                     lx, nel);
+  if ((st = op.gs<T>(e, nullptr, s))) return st;
+  if ((st = launch_mask<T>(e, op.msk, op.nmsk, nullptr, s))) return st;
+  hipLaunchKernelGGL(wt3d_kernel<T>, dim3(blocks_for(op.n)), dim3(256), 0, s, e, wt, lx, nel);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -594,22 +619,30 @@ __global__ void cheby_start_kernel(double *__restrict__ d, int64_t n, uint64_t s
     d[i] = drng_uniform(seed, (uint64_t)i) + 10.0;
 }
 // r = f - w
-__global__ void sub3_kernel(double *__restrict__ r, const double *__restrict__ f,
-                            const double *__restrict__ w, int64_t n) {
+template <typename T>
+__global__ void sub3_kernel(T *__restrict__ r, const T *__restrict__ f, const T *__restrict__ w,
+                            int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     r[i] = f[i] - w[i];
 }
-// d = a d + b w; x += d   (a = 0, b = 1/tha for the first step)
-__global__ void cheby_step_kernel(double *__restrict__ d, double *__restrict__ x,
-                                  const double *__restrict__ w, double a, double b,
-                                  int64_t n) {
+// d = a d + b w; x += d.  The first step (first != 0) sets d = b w without
+// reading d (which holds no value yet).
+template <typename T>
+__global__ void cheby_step_kernel(T *__restrict__ d, T *__restrict__ x, const T *__restrict__ w,
+                                  T a, T b, int64_t n, int first) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const double di = a * d[i] + b * w[i];
+    const T di = first ? b * w[i] : a * d[i] + b * w[i];
     d[i] = di;
     x[i] = x[i] + di;
   }
+}
+// FP64 <-> FP32 copies at the smoother boundary
+__global__ void f2d_kernel(double *__restrict__ y, const float *__restrict__ x, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    y[i] = (double)x[i];
 }
 // w = (r - w) * mult
 __global__ void resid_mult_kernel(double *__restrict__ w, const double *__restrict__ r,
@@ -624,6 +657,15 @@ struct Cheby {
   const Schwarz *sw = nullptr;
   double tha = 0.0, dlt = 0.0;
   double *d = nullptr, *w = nullptr, *r = nullptr;
+  float *d32 = nullptr, *w32 = nullptr, *r32 = nullptr;  // FP32 smoothing
+  template <typename T>
+  void work(T *&dd, T *&ww, T *&rr) {
+    if constexpr (std::is_same<T, double>::value) {
+      dd = d; ww = w; rr = r;
+    } else {
+      dd = d32; ww = w32; rr = r32;
+    }
+  }
 };
 
 static unsigned vblocks(int64_t n) {
@@ -671,21 +713,25 @@ static int cheby_power(Cheby &c, uint64_t seed, int its, double *part, double *o
 }
 
 // cheby_impl (cheby.f90:317-397): niter steps on A x = f, all on stream s
-static int cheby_solve(Cheby &c, double *x, const double *f, int niter, bool zero_init,
-                       hipStream_t s) {
+// cheby_impl (cheby.f90:317-397): niter steps on A x = f, all on stream s;
+// T = float runs the whole smoother (operator, gs, Schwarz) in FP32
+template <typename T>
+static int cheby_solve(Cheby &c, T *x, const T *f, int niter, bool zero_init, hipStream_t s) {
   const Op &op = *c.op;
   const int64_t n = op.n;
   const unsigned g = vblocks(n);
+  T *d, *w, *r;
+  c.work<T>(d, w, r);
   int st;
   if (!zero_init) {
-    if ((st = op.apply<double>(c.w, x, nullptr, s, true, true))) return st;
-    hipLaunchKernelGGL(sub3_kernel, dim3(g), dim3(256), 0, s, c.r, f, c.w, n);
+    if ((st = op.apply<T>(w, x, nullptr, s, true, true))) return st;
+    hipLaunchKernelGGL(sub3_kernel<T>, dim3(g), dim3(256), 0, s, r, f, w, n);
   } else {
-    NM_HIP_TRY(hipMemcpyAsync(c.r, f, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    NM_HIP_TRY(hipMemcpyAsync(r, f, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
   }
-  if ((st = schwarz_apply(*c.sw, c.w, c.r, s))) return st;
-  hipLaunchKernelGGL(cheby_step_kernel, dim3(g), dim3(256), 0, s, c.d, x, c.w, 0.0,
-                     1.0 / c.tha, n);
+  if ((st = schwarz_apply<T>(*c.sw, w, r, s))) return st;
+  hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(g), dim3(256), 0, s, d, x, w, T(0),
+                     T(1.0 / c.tha), n, 1);
   const double sig1 = c.tha / c.dlt;
   double rhok = 1.0 / sig1;
   for (int iter = 2; iter <= niter; iter++) {
@@ -693,10 +739,11 @@ static int cheby_solve(Cheby &c, double *x, const double *f, int niter, bool zer
     const double tmp1 = rhokp1 * rhok;
     const double tmp2 = 2.0 * rhokp1 / c.dlt;
     rhok = rhokp1;
-    if ((st = op.apply<double>(c.w, x, nullptr, s, true, true))) return st;
-    hipLaunchKernelGGL(sub3_kernel, dim3(g), dim3(256), 0, s, c.r, f, c.w, n);
-    if ((st = schwarz_apply(*c.sw, c.w, c.r, s))) return st;
-    hipLaunchKernelGGL(cheby_step_kernel, dim3(g), dim3(256), 0, s, c.d, x, c.w, tmp1, tmp2, n);
+    if ((st = op.apply<T>(w, x, nullptr, s, true, true))) return st;
+    hipLaunchKernelGGL(sub3_kernel<T>, dim3(g), dim3(256), 0, s, r, f, w, n);
+    if ((st = schwarz_apply<T>(*c.sw, w, r, s))) return st;
+    hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(g), dim3(256), 0, s, d, x, w, (T)tmp1,
+                       (T)tmp2, n, 0);
   }
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
@@ -717,6 +764,8 @@ struct Phmg {
   CgFixed *cg = nullptr;
   int smooth_its = 4, crs_niter = 10;
   double crs_tol = 1e-9;
+  bool fp32 = false;                      // FP32 smoothers
+  float *z32[2] = {}, *f32[2] = {};       // level solution / rhs in FP32
   std::vector<void *> owned;
   ~Phmg() {
     if (cg) cg_fixed_free(cg);
@@ -733,6 +782,26 @@ struct Phmg {
   }
 };
 
+// Chebyshev smoothing of level l (pre: from z = 0).  FP32 smoothers take
+// FP32 copies of the level's rhs and iterate and return z in FP64.
+static int phmg_smooth(Phmg *h, int l, bool pre, hipStream_t s) {
+  if (!h->fp32) return cheby_solve<double>(h->ch[l], h->z[l], h->r[l], h->smooth_its, pre, s);
+  const int64_t n = h->lev[l]->n;
+  const unsigned g = vblocks(n);
+  int st;
+  if (pre) {
+    if ((st = launch_d2f(h->r[l], h->f32[l], n, s))) return st;
+    if ((st = launch_vec<float>(V_RZERO, h->z32[l], nullptr, nullptr, nullptr, 0.0f, n, s)))
+      return st;
+  } else if ((st = launch_d2f(h->z[l], h->z32[l], n, s))) {
+    return st;
+  }
+  if ((st = cheby_solve<float>(h->ch[l], h->z32[l], h->f32[l], h->smooth_its, pre, s))) return st;
+  hipLaunchKernelGGL(f2d_kernel, dim3(g), dim3(256), 0, s, h->z[l], h->z32[l], n);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
 static int phmg_cycle(Phmg *h, double *zout, const double *rin, hipStream_t s) {
   Op **L = h->lev;
   const int64_t nel = L[0]->nel;
@@ -743,7 +812,7 @@ static int phmg_cycle(Phmg *h, double *zout, const double *rin, hipStream_t s) {
     return st;
   for (int l = 0; l < 2; l++) {
     const Op &op = *L[l];
-    if ((st = cheby_solve(h->ch[l], h->z[l], h->r[l], h->smooth_its, true, s))) return st;
+    if ((st = phmg_smooth(h, l, true, s))) return st;
     if ((st = op.apply<double>(h->w[l], h->z[l], nullptr, s, true, true))) return st;
     hipLaunchKernelGGL(resid_mult_kernel, dim3(vblocks(op.n)), dim3(256), 0, s, h->w[l],
                        h->r[l], op.f64.mult, op.n);
@@ -765,7 +834,7 @@ static int phmg_cycle(Phmg *h, double *zout, const double *rin, hipStream_t s) {
     if ((st = launch_vec<double>(V_ADDCOL3, h->z[l], h->w[l], op.f64.mult, nullptr, 0.0, op.n,
                                  s)))
       return st;
-    if ((st = cheby_solve(h->ch[l], h->z[l], h->r[l], h->smooth_its, false, s))) return st;
+    if ((st = phmg_smooth(h, l, false, s))) return st;
   }
   NM_HIP_TRY(hipMemcpyAsync(zout, h->z[0], sizeof(double) * L[0]->n, hipMemcpyDeviceToDevice, s));
   return NM_OK;
@@ -884,7 +953,7 @@ struct nm_phmg_s {
 
 // phmg-style preconditioner on a box operator (see include/neko_amd.h).
 int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_its,
-                   int level_h2_fine, hipStream_t s) {
+                   int level_h2_fine, int smoother_fp32, hipStream_t s) {
   *out = nullptr;
   Op *fine = &fine_h->op;
   if (fine->box[0] == 0) {
@@ -954,6 +1023,26 @@ int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_it
     if ((st = cheby_power(c, 3901, 150, h->part, h->out, s))) return fail(st);
   }
   if ((st = cg_fixed_create(&h->cg, h->lev[2]))) return fail(st);
+  if (smoother_fp32) {
+    h->fp32 = true;
+    for (int l = 0; l < 2; l++) {
+      Op *op = h->lev[l];
+      if ((st = op->ensure32(s))) return fail(st);
+      Schwarz &w = h->sw[l];
+      const int64_t n = op->n, nl = w.nl, ns = w.nel * nl * nl * nl;
+      const int64_t nS = w.nel * 6 * nl * nl, nW = w.nel * w.lx * w.lx * 12;
+      Cheby &c = h->ch[l];
+      if (!(w.S32 = h->alloc<float>(nS)) || !(w.D32 = h->alloc<float>(ns)) ||
+          !(w.wt32 = h->alloc<float>(nW)) || !(w.work1_32 = h->alloc<float>(ns)) ||
+          !(w.work2_32 = h->alloc<float>(ns)) || !(c.d32 = h->alloc<float>(n)) ||
+          !(c.w32 = h->alloc<float>(n)) || !(c.r32 = h->alloc<float>(n)) ||
+          !(h->z32[l] = h->alloc<float>(n)) || !(h->f32[l] = h->alloc<float>(n)))
+        return fail(NM_ERR_NOMEM);
+      if ((st = launch_d2f(w.S, w.S32, nS, s)) || (st = launch_d2f(w.D, w.D32, ns, s)) ||
+          (st = launch_d2f(w.wt, w.wt32, nW, s)))
+        return fail(st);
+    }
+  }
   if (hipStreamSynchronize(s) != hipSuccess) return fail(NM_ERR_HIP);
   *out = hh;
   return NM_OK;

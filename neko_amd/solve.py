@@ -331,15 +331,17 @@ class PHMG:
     """phmg-style V-cycle (Chebyshev(smooth_its) smoothing accelerated by
     Schwarz on hsmg's levels, coarse Jacobi-CG) on a box problem.
     level_h2: "default" (Neko's coef default h2 = 1 on the coarser levels)
-    or "fine" (the fine level's h2 on every level)."""
+    or "fine" (the fine level's h2 on every level).  smoother: "fp64", or
+    "fp32" (smoothing in FP32, transfers and coarse solve FP64)."""
 
     def __init__(self, problem: Problem, facet_bc: int = 2, smooth_its: int = 4,
-                 level_h2: str = "default"):
+                 level_h2: str = "default", smoother: str = "fp64"):
         self.problem = problem
         h = C.c_void_p()
         check(lib().nm_phmg_create(C.byref(h), problem.h, int(facet_bc),
                                    int(smooth_its), 1 if level_h2 == "fine" else 0,
-                                   _stream()), "nm_phmg_create")
+                                   1 if smoother == "fp32" else 0, _stream()),
+              "nm_phmg_create")
         self.h = h
 
     def apply(self, z: torch.Tensor, r: torch.Tensor):

@@ -219,3 +219,30 @@ def test_gmres_phmg_matches_oracle(case):
     assert abs(mon.iter - m_or.iter) <= 1, (mon.iter, m_or.iter)
     err = np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or)
     assert err < 1e-9
+
+
+@pytest.mark.parametrize("case", [PH_CASES[0], PH_CASES[2]])
+def test_phmg_fp32_smoothers(case):
+    """FP32 Chebyshev/Schwarz smoothing (FP64 transfers, coarse CG and
+    outer GMRES): one V-cycle within 1e-5 of the FP64 oracle's; GMRES to
+    1e-10 within 2 iterations of the oracle's FP64-smoother count and
+    solution within 1e-9."""
+    from neko_amd.solve import GMRES, PHMG
+    print("phmg fp32", case, flush=True)
+    dims, lx, amp, per, h1, h2, fb, lh2 = case
+    c, p = _pair(dims, lx, amp, per, h1=h1, h2=h2)
+    P = O.OraclePhmg(c, facet_bc=fb, level_h2=lh2)
+    ph = PHMG(p, facet_bc=fb, smooth_its=4, level_h2=lh2, smoother="fp32")
+    r = c.rhs_normal(21)
+    z = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    ph.apply(z, _t(r))
+    assert _relmax(_np(z), P.apply(r)) < 1e-5
+    f = c.rhs_normal(5)
+    tol = 1e-10 * c.res_norm(f)
+    x_or, m_or = P.gmres(f, tol)
+    g = GMRES(p, pc=ph, lgmres=30)
+    x = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    mon = g.solve(x, _t(f), tol)
+    assert mon.converged
+    assert abs(mon.iter - m_or.iter) <= 2, (mon.iter, m_or.iter)
+    assert np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or) < 1e-9
