@@ -181,6 +181,18 @@ struct CgPro {
   T beta, alpha;
 };
 
+// 1/a for the Jacobian.  FP64: the hardware reciprocal estimate refined by
+// two Newton-Raphson steps y <- y (2 - a y), each of which squares the
+// relative error, so the result is within an ulp or two of 1/a (a > 0 and
+// normal here); it skips the scaling and fix-up steps of a full IEEE
+// division.  FP32: the ordinary division.
+__device__ __forceinline__ double recip(double a) {
+  double y = __builtin_amdgcn_rcp(a);
+  y = y * (2.0 - a * y);
+  return y * (2.0 - a * y);
+}
+__device__ __forceinline__ float recip(float a) { return 1.0f / a; }
+
 template <typename T, int LX, int EPB, bool DOT, bool CGP = false>
 __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restrict__ u,
                                             const T *__restrict__ dx,
@@ -258,7 +270,11 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
     ct[c] = (E[24 + c * 4 + 0] * ri0 + E[24 + c * 4 + 1] * ri1) * sj0 +
             (E[24 + c * 4 + 2] * ri0 + E[24 + c * 4 + 3] * ri1) * sj1;
 
-#pragma unroll 1
+  // FP64: two planes per trip lets the scheduler overlap one plane's
+  // geometry with the other's LDS traffic (168 VGPRs, still 3 waves/SIMD);
+  // FP32 keeps one plane per trip (82 VGPRs, 5 waves/SIMD)
+  constexpr int kPlaneUnroll = kDtLds ? 1 : 2;
+#pragma unroll kPlaneUnroll
   for (int k = 0; k < LX; k++) {
     const int b = k & 1;
     const T tk1 = (T(1) + zg[k]) * T(0.5), tk0 = T(1) - tk1;
@@ -286,7 +302,7 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
     const T tz = a[0] * bb[1] - bb[0] * a[1];
     const T jac = a[0] * rx + a[1] * ry + a[2] * rz;
     const T w3 = wij * wg[k];
-    const T sc = h1c * w3 / jac;
+    const T sc = h1c * w3 * recip(jac);
     const T G11 = sc * (rx * rx + ry * ry + rz * rz);
     const T G22 = sc * (sx * sx + sy * sy + sz * sz);
     const T G33 = sc * (tx * tx + ty * ty + tz * tz);
