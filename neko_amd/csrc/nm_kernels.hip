@@ -37,8 +37,7 @@ __global__ void __launch_bounds__(LX *LX *EPB)
                     const int *__restrict__ skip) {
   constexpr int P = LX * LX;
   constexpr int N3 = LX * P;
-  __shared__ T su[EPB][N3];     // element u
-  __shared__ T sw[EPB][N3];     // r/s part of w, per point
+  __shared__ T su[EPB][N3];     // element u; plane k then holds the r/s part of w
   __shared__ T sr[EPB][2][P];   // r-flux plane, double buffered
   __shared__ T ss[EPB][2][P];   // s-flux plane, double buffered
 
@@ -133,13 +132,13 @@ __global__ void __launch_bounds__(LX *LX *EPB)
     for (int l = 0; l < LX; l++) acc += Dti[l] * sr[q][b][l + LX * j];
 #pragma unroll
     for (int l = 0; l < LX; l++) acc += Dtj[l] * ss[q][b][i + LX * l];
-    sw[q][k * P + t] = acc;
+    su[q][k * P + t] = acc;  // plane k is dead: its r/s reads precede the barrier
   }
 
   if (active) {
 #pragma unroll
     for (int m = 0; m < LX; m++) {
-      T r = wc[m] + sw[q][m * P + t];
+      T r = wc[m] + su[q][m * P + t];
       if (h2b != nullptr) r += h2b[base + m * P] * uc[m];
       w[base + m * P] = r;
     }
@@ -181,17 +180,21 @@ struct CgPro {
   T beta, alpha;
 };
 
-// 1/a for the Jacobian.  FP64: the hardware reciprocal estimate refined by
-// two Newton-Raphson steps y <- y (2 - a y), each of which squares the
-// relative error, so the result is within an ulp or two of 1/a (a > 0 and
+// 1/a for the Jacobian: the hardware reciprocal estimate refined by
+// Newton-Raphson steps y <- y (2 - a y), each of which squares the relative
+// error -- two in FP64, one in FP32 (whose estimate is already good to
+// about an ulp) -- so the result is within an ulp or two of 1/a (a > 0 and
 // normal here); it skips the scaling and fix-up steps of a full IEEE
-// division.  FP32: the ordinary division.
+// division.
 __device__ __forceinline__ double recip(double a) {
   double y = __builtin_amdgcn_rcp(a);
   y = y * (2.0 - a * y);
   return y * (2.0 - a * y);
 }
-__device__ __forceinline__ float recip(float a) { return 1.0f / a; }
+__device__ __forceinline__ float recip(float a) {
+  const float y = __builtin_amdgcn_rcpf(a);
+  return y * (2.0f - a * y);
+}
 
 template <typename T, int LX, int EPB, bool DOT, bool CGP = false>
 __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restrict__ u,
@@ -205,7 +208,6 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   constexpr int P = LX * LX;
   constexpr int N3 = LX * P;
   __shared__ T su[EPB][N3];
-  __shared__ T sw[EPB][N3];
   __shared__ T sr[EPB][2][P];
   __shared__ T ss[EPB][2][P];
   __shared__ T sedge[EPB][36];
@@ -237,79 +239,75 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   }
   for (int x = t; x < 36; x += P) sedge[q][x] = active ? ed[x] : T(0);
 
-  // rows of D in registers; columns too in FP64, read from LDS in FP32
-  // (there it buys a fifth wave per SIMD, in FP64 the compiler does better
-  // with them in registers)
-  constexpr bool kDtLds = std::is_same<T, float>::value;
+  // rows D(i,:), D(j,:) in registers; the columns D(:,i), D(:,j) of the
+  // transposed contraction are read from LDS (in registers they would cost
+  // a wave per SIMD, which is worth more than the LDS reads)
   __shared__ T sD[LX * LX];
-  if (kDtLds && q == 0) sD[t] = dx[t];
-  T Di[LX], Dj[LX], Dti[kDtLds ? 1 : LX], Dtj[kDtLds ? 1 : LX];
+  if (q == 0) sD[t] = dx[t];
+  T Di[LX], Dj[LX];
 #pragma unroll
   for (int l = 0; l < LX; l++) {
     Di[l] = dx[i + LX * l];
     Dj[l] = dx[j + LX * l];
-    if (!kDtLds) {
-      Dti[l] = dx[l + LX * i];
-      Dtj[l] = dx[l + LX * j];
-    }
   }
   T wc[LX];
 #pragma unroll
   for (int m = 0; m < LX; m++) wc[m] = T(0);
   __syncthreads();
 
-  // per-thread geometry: d/dt constant along the column, d/dr and d/ds
-  // linear in t
+  // Geometry.  dX/dt is bilinear in (r,s), so constant along a thread's
+  // column: kept in registers.  dX/dr is bilinear in (s,t) and dX/ds in
+  // (r,t): tabulated per element in LDS, one (plane, row) pair per thread,
+  // sa[plane k][row j] = dX/dr(s_j, t_k), sb[plane k][row i] = dX/ds(r_i,
+  // t_k), and read per plane (broadcast reads).
+  __shared__ T sa[EPB][LX][LX][3];
+  __shared__ T sb[EPB][LX][LX][3];
+  const T *E = sedge[q];
+  {
+    const T p1 = (T(1) + zg[j]) * T(0.5), p0 = T(1) - p1;  // row coordinate
+    const T t1 = (T(1) + zg[i]) * T(0.5), t0 = T(1) - t1;  // plane coordinate
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      sa[q][i][j][c] = (E[c * 4 + 0] * p0 + E[c * 4 + 1] * p1) * t0 +
+                       (E[c * 4 + 2] * p0 + E[c * 4 + 3] * p1) * t1;
+      sb[q][i][j][c] = (E[12 + c * 4 + 0] * p0 + E[12 + c * 4 + 1] * p1) * t0 +
+                       (E[12 + c * 4 + 2] * p0 + E[12 + c * 4 + 3] * p1) * t1;
+    }
+  }
   const T ri1 = (T(1) + zg[i]) * T(0.5), ri0 = T(1) - ri1;
   const T sj1 = (T(1) + zg[j]) * T(0.5), sj0 = T(1) - sj1;
   const T wij = wg[i] * wg[j];
-  const T *E = sedge[q];
+  const T h1w = h1c * wij, h2w = h2c * wij;
+  const bool mass = h2c != T(0);
   T ct[3];
 #pragma unroll
   for (int c = 0; c < 3; c++)
     ct[c] = (E[24 + c * 4 + 0] * ri0 + E[24 + c * 4 + 1] * ri1) * sj0 +
             (E[24 + c * 4 + 2] * ri0 + E[24 + c * 4 + 3] * ri1) * sj1;
+  __syncthreads();
 
   // FP64: two planes per trip lets the scheduler overlap one plane's
-  // geometry with the other's LDS traffic (168 VGPRs, still 3 waves/SIMD);
-  // FP32 keeps one plane per trip (82 VGPRs, 5 waves/SIMD)
-  constexpr int kPlaneUnroll = kDtLds ? 1 : 2;
+  // geometry with the other's LDS traffic (168 VGPRs, 3 waves/SIMD); FP32
+  // keeps one plane per trip (83 VGPRs, 5 waves/SIMD)
+  constexpr int kPlaneUnroll = std::is_same<T, float>::value ? 1 : 2;
 #pragma unroll kPlaneUnroll
   for (int k = 0; k < LX; k++) {
     const int b = k & 1;
-    const T tk1 = (T(1) + zg[k]) * T(0.5), tk0 = T(1) - tk1;
-    // dX/dr and dX/ds from the element's edges in LDS (broadcast reads;
-    // recomputed per plane to keep them out of registers)
-    T a[3], bb[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const T e0 = E[c * 4 + 0] * tk0 + E[c * 4 + 2] * tk1;
-      const T e1 = E[c * 4 + 1] * tk0 + E[c * 4 + 3] * tk1;
-      const T f0 = E[12 + c * 4 + 0] * tk0 + E[12 + c * 4 + 2] * tk1;
-      const T f1 = E[12 + c * 4 + 1] * tk0 + E[12 + c * 4 + 3] * tk1;
-      a[c] = e0 * sj0 + e1 * sj1;     // dX_c/dr
-      bb[c] = f0 * ri0 + f1 * ri1;    // dX_c/ds
-    }
-    const T *cc = ct;                         // dX_c/dt
-    const T rx = bb[1] * cc[2] - cc[1] * bb[2];
-    const T ry = cc[0] * bb[2] - bb[0] * cc[2];
-    const T rz = bb[0] * cc[1] - cc[0] * bb[1];
-    const T sx = cc[1] * a[2] - a[1] * cc[2];
-    const T sy = a[0] * cc[2] - cc[0] * a[2];
-    const T sz = cc[0] * a[1] - a[0] * cc[1];
-    const T tx = a[1] * bb[2] - bb[1] * a[2];
-    const T ty = bb[0] * a[2] - a[0] * bb[2];
-    const T tz = a[0] * bb[1] - bb[0] * a[1];
-    const T jac = a[0] * rx + a[1] * ry + a[2] * rz;
-    const T w3 = wij * wg[k];
-    const T sc = h1c * w3 * recip(jac);
-    const T G11 = sc * (rx * rx + ry * ry + rz * rz);
-    const T G22 = sc * (sx * sx + sy * sy + sz * sz);
-    const T G33 = sc * (tx * tx + ty * ty + tz * tz);
-    const T G12 = sc * (rx * sx + ry * sy + rz * sz);
-    const T G13 = sc * (rx * tx + ry * ty + rz * tz);
-    const T G23 = sc * (sx * tx + sy * ty + sz * tz);
-    const T hB = h2c * jac * w3;
+    const T a0 = sa[q][k][j][0], a1 = sa[q][k][j][1], a2 = sa[q][k][j][2];
+    const T b0 = sb[q][k][i][0], b1 = sb[q][k][i][1], b2 = sb[q][k][i][2];
+    // rows of the cofactor matrix (Neko's drdx,drdy,drdz / dsdx.. / dtdx..):
+    // r = X_s x X_t, s = X_t x X_r, t = X_r x X_s; jac = X_t . (X_r x X_s)
+    const T rx = b1 * ct[2] - ct[1] * b2;
+    const T ry = ct[0] * b2 - b0 * ct[2];
+    const T rz = b0 * ct[1] - ct[0] * b1;
+    const T sx = ct[1] * a2 - a1 * ct[2];
+    const T sy = a0 * ct[2] - ct[0] * a2;
+    const T sz = ct[0] * a1 - a0 * ct[1];
+    const T tx = a1 * b2 - b1 * a2;
+    const T ty = b0 * a2 - a0 * b2;
+    const T tz = a0 * b1 - b0 * a1;
+    const T jac = ct[0] * tx + ct[1] * ty + ct[2] * tz;
+    const T sc = h1w * wg[k] * recip(jac);
 
     T dk[LX];
 #pragma unroll
@@ -322,29 +320,35 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
       ur += Di[l] * plane[l + LX * j];
       us += Dj[l] * plane[i + LX * l];
     }
-    const T rr = G11 * ur + G12 * us + G13 * vt;
-    const T rs = G12 * ur + G22 * us + G23 * vt;
-    const T rt = G13 * ur + G23 * us + G33 * vt;
+    // G (ur,us,ut) with G = sc C C^T: g = sc C^T (ur,us,ut), then C g
+    const T gx = sc * (ur * rx + us * sx + vt * tx);
+    const T gy = sc * (ur * ry + us * sy + vt * ty);
+    const T gz = sc * (ur * rz + us * sz + vt * tz);
+    const T rr = rx * gx + ry * gy + rz * gz;
+    const T rs = sx * gx + sy * gy + sz * gz;
+    const T rt = tx * gx + ty * gy + tz * gz;
     sr[q][b][t] = rr;
     ss[q][b][t] = rs;
 #pragma unroll
     for (int m = 0; m < LX; m++) wc[m] += dk[m] * rt;
     __syncthreads();
-    T acc = hB * plane[t];
+    T acc = mass ? h2w * wg[k] * jac * plane[t] : T(0);
 #pragma unroll
     for (int l = 0; l < LX; l++)
-      acc += (kDtLds ? sD[l + LX * i] : Dti[kDtLds ? 0 : l]) * sr[q][b][l + LX * j];
+      acc += sD[l + LX * i] * sr[q][b][l + LX * j];
 #pragma unroll
     for (int l = 0; l < LX; l++)
-      acc += (kDtLds ? sD[l + LX * j] : Dtj[kDtLds ? 0 : l]) * ss[q][b][i + LX * l];
-    sw[q][k * P + t] = acc;
+      acc += sD[l + LX * j] * ss[q][b][i + LX * l];
+    // plane k of su is dead once every thread passed this plane's barrier
+    // (its r/s reads come before it): the point's r/s part of w goes there
+    su[q][k * P + t] = acc;
   }
 
   double dsum = 0.0;
   if (active) {
 #pragma unroll
     for (int m = 0; m < LX; m++) {
-      const T r = wc[m] + sw[q][m * P + t];
+      const T r = wc[m] + su[q][m * P + t];
       w[base + m * P] = r;
       if (DOT) dsum += (double)r * (double)uc[m];
     }
