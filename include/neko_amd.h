@@ -166,11 +166,46 @@ int64_t nm_box_mask_size(int nelx, int nely, int nelz, int lx, int perx,
 int nm_box_mask(int nelx, int nely, int nelz, int lx, int perx, int pery,
                 int perz, int32_t *msk, int64_t *nmsk);
 
+/* --- general conforming hex meshes ------------------------------------- */
+/* Neko's binary mesh file (src/io/nmsh_file.f90:70-207, records of
+ * src/io/format/nmsh.f90): 3-D hex meshes with straight edges.  info gives
+ * the element, zone, periodic-zone and curve counts; read returns the
+ * element vertices in Neko's symmetric order (the p1,p2,p4,p3,p5,p6,p8,p7
+ * swap of nmsh_file.f90:159) and their point ids after the periodic zones'
+ * id replacement (mesh_apply_periodic_facet, src/mesh/mesh.f90:1837-1866).
+ * Curved meshes are rejected (NM_ERR_UNSUPPORTED). */
+int nm_nmsh_info(const char *path, int64_t *nel, int64_t *nzones,
+                 int64_t *nperiodic, int64_t *ncurves);
+int nm_nmsh_read(const char *path, int64_t nel, double *vxyz, int64_t *pid,
+                 int32_t *el_idx);
+/* facet neighbours, slot e*6+f, 1-based element or 0 on the boundary
+ * (msh%facet_neigh, src/mesh/mesh.f90:590-628) */
+int nm_mesh_facet_neigh(int64_t nel, const int64_t *pid, int32_t *fneigh);
+/* Neko's local gs maps of a general mesh (gather_scatter.f90:513-1375),
+ * same layout as nm_box_gs_map; shared points identified from the point
+ * ids as Neko's dofmap numbers them (src/sem/dofmap.f90:264-797) */
+int nm_mesh_gs_sizes(int64_t nel, int lx, const int32_t *fneigh,
+                     int64_t *nlocal, int64_t *facet_offset);
+int nm_mesh_gs_map(int64_t nel, int lx, const int64_t *pid,
+                   const int32_t *fneigh, int32_t *dg, int32_t *gd,
+                   int32_t *blk_len, int32_t *blk_off, int64_t *nlocal,
+                   int64_t *facet_offset, int64_t *nblks);
+/* zero-Dirichlet mask of every facet without a neighbour (1-based; the
+ * Poisson example's set_bc, examples/poisson/setup.f90) */
+int64_t nm_mesh_mask_size(int64_t nel, int lx, const int32_t *fneigh);
+int nm_mesh_mask(int64_t nel, int lx, const int32_t *fneigh, int32_t *msk,
+                 int64_t *nmsk);
+
 /* --- problem (operator) handles ----------------------------------------- */
 /* Whole box problem built on the device (coef on the GPU). */
 int nm_op_create_box(nm_op_t **op, int nelx, int nely, int nelz, int lx,
                      int perx, int pery, int perz, double amp, uint64_t seed,
                      double h1, double h2, hipStream_t s);
+/* Whole problem on a general straight-edged hex mesh (e.g. from
+ * nm_nmsh_read): the set-up of examples/poisson/driver.f90 (dofmap, gs,
+ * coef, Dirichlet on every facet without a neighbour), constant h1/h2. */
+int nm_op_create_mesh(nm_op_t **op, int64_t nel, int lx, const double *vxyz,
+                      const int64_t *pid, double h1, double h2, hipStream_t s);
 /* Problem from Neko's own host arrays (coef_t, gs_t, bc mask) -- what the
  * Fortran ksp_t/pc_t plug-ins pass at init (src/krylov/krylov.f90:73-201,
  * src/krylov/precon.f90:40-65). */
@@ -194,6 +229,10 @@ int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
  * (lane-parallel, blocks <= 8 entries). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
 int nm_op_set_gs_variant(nm_op_t *op, int variant);
+/* preconditioner of the op's CG/GMRES/batch/serve solvers: 0 Jacobi
+ * (src/krylov/bcknd/cpu/pc_jacobi.f90, default), 1 identity (the ksp
+ * default without a preconditioner, src/krylov/pc_identity.f90) */
+int nm_op_set_pc(nm_op_t *op, int kind, hipStream_t s);
 /* autotuner: times all Ax (FP64, FP32) and gs variants on the problem's own
  * data and keeps the fastest; returns the chosen codes */
 int nm_op_autotune(nm_op_t *op, int reps, int *best64, int *best32,

@@ -832,7 +832,9 @@ int nm_cg_solve(nm_cg_t *h, double *x, const double *f, double abs_tol,
     (void)hipEventDestroy(ev);
     return st;
   }
-  mon->iter = cg->total;
+  // an exhausted run reports max_iter + 1, the value cg.f90's
+  // `do iter = 1, max_iter` loop variable has after the loop (ksp_results%iter)
+  mon->iter = (!cg->converged && cg->total >= max_iter) ? max_iter + 1 : cg->total;
   mon->outer = cg->sweeps;
   mon->res_start = cg->res_start;
   mon->res_final = cg->rnorm;

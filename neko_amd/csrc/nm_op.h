@@ -56,6 +56,11 @@ struct Op {
   int64_t bytes = 0;            // their total size
   int box[3] = {0, 0, 0}, per[3] = {0, 0, 0};
   bool trilinear = false;  // straight-edged elements: geometry in-kernel
+  // diagonal preconditioner of the CG/GMRES solvers: 0 Jacobi (f*.dinv
+  // holds 1/diag), 1 identity (f*.dinv holds ones); alt* keep the other
+  int pc = 0;
+  double *alt64 = nullptr;
+  float *alt32 = nullptr;
   double h2c = 0.0;        // constant h2 (with trilinear geometry)
   double amp = 0.0;     // box perturbation amplitude (box operators)
   uint64_t seed = 0;    // box perturbation seed

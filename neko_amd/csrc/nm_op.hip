@@ -365,33 +365,22 @@ using namespace nm;
 extern "C" {
 
 
-// Build a whole box problem: host mesh/gs maps/mask, device coordinates,
-// geometric factors, mass, multiplicity and Jacobi diagonal.
-// h1 constant; h2 constant (h2 != 0 turns the mass term on, ifh2).
-int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
-                     int perx, int pery, int perz, double amp, uint64_t seed,
-                     double h1, double h2, hipStream_t s) {
-  *out = nullptr;
-  if (lx < 2 || lx > 12) {
-    set_error("nm_op_create_box: lx=%d outside 2..12", lx);
-    return NM_ERR_ARG;
-  }
-  if ((perx && nelx < 3) || (pery && nely < 3) || (perz && nelz < 3)) {
-    set_error("nm_op_create_box: periodic direction needs >= 3 elements");
-    return NM_ERR_ARG;
-  }
-  nm_op_t *h = new nm_op_t;
+// Build a whole problem from a straight-edged mesh (element vertices in
+// Neko's symmetric order) and its host gs maps (Neko's 1-based layout) and
+// mask (1-based): device coordinates, geometric factors, mass,
+// multiplicity, Jacobi diagonal, and the element edges of the in-kernel
+// (trilinear) geometry.  h1 constant; h2 constant (h2 != 0 turns the mass
+// term on, ifh2).  On failure the handle is deleted.
+static int op_build_linear(nm_op_t *h, int64_t nel, int lx, const std::vector<double> &vxyz,
+                           double h1, double h2, int64_t nlocal, int64_t foff, int64_t nb,
+                           std::vector<int32_t> &msk, int64_t m, hipStream_t s) {
   Op *op = &h->op;
-  op->nel = (int64_t)nelx * nely * nelz;
+  op->nel = nel;
   op->lx = lx;
-  op->n = op->nel * lx * lx * lx;
+  op->n = nel * lx * lx * lx;
   op->h1c = h1;
   op->ifh2 = h2 != 0.0;
-  op->box[0] = nelx; op->box[1] = nely; op->box[2] = nelz;
-  op->per[0] = perx; op->per[1] = pery; op->per[2] = perz;
-  op->amp = amp;
-  op->seed = seed;
-  const int64_t n = op->n, nel = op->nel;
+  const int64_t n = op->n;
   int st = NM_OK;
   auto fail = [&](int code) {
     delete h;
@@ -403,10 +392,7 @@ int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
   for (int k = 0; k < lx; k++)
     for (int j = 0; j < lx; j++)
       for (int i = 0; i < lx; i++) w3[i + lx * (j + lx * k)] = w[i] * w[j] * w[k];
-  // mesh
-  std::vector<double> vxyz(nel * 24);
   std::vector<int> dfrmd(nel);
-  nm_box_mesh(nelx, nely, nelz, amp, seed, vxyz.data());
   nm_dfrmd(nel, vxyz.data(), dfrmd.data());
   // device arrays
   double *d_z, *d_v, *d_w3;
@@ -467,36 +453,105 @@ int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
     op->trilinear = true;
     op->h2c = h2;
   }
-  // gs maps + mask on the host (Neko's layout), then device layout
-  int64_t nlocal, foff, nb;
-  if ((st = nm_box_gs_sizes(nelx, nely, nelz, lx, perx, pery, perz, &nlocal, &foff)))
+  // gs maps (already in op->h_*), then the device layout
+  op->h_blk_len.resize(nb);
+  op->h_blk_off.resize(nb);
+  if ((st = op_set_gs(op, nlocal, foff, nb, op->h_dg.data(), op->h_gd.data(),
+                      op->h_blk_len.data(), op->h_blk_off.data(), s)))
     return fail(st);
+  for (int64_t i = 0; i < m; i++) msk[i] -= 1;
+  op->nmsk = m;
+  if (!(op->msk = op->alloc<int32_t>(m))) return fail(NM_ERR_NOMEM);
+  hipMemcpyAsync(op->msk, msk.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("operator set-up: device copies failed");
+    return fail(NM_ERR_HIP);
+  }
+  if ((st = op_finish(op, s))) return fail(st);
+  return NM_OK;
+}
+
+// Build a whole box problem: host mesh/gs maps/mask, then op_build_linear.
+int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
+                     int perx, int pery, int perz, double amp, uint64_t seed,
+                     double h1, double h2, hipStream_t s) {
+  *out = nullptr;
+  if (lx < 2 || lx > 12) {
+    set_error("nm_op_create_box: lx=%d outside 2..12", lx);
+    return NM_ERR_ARG;
+  }
+  if ((perx && nelx < 3) || (pery && nely < 3) || (perz && nelz < 3)) {
+    set_error("nm_op_create_box: periodic direction needs >= 3 elements");
+    return NM_ERR_ARG;
+  }
+  nm_op_t *h = new nm_op_t;
+  Op *op = &h->op;
+  op->box[0] = nelx; op->box[1] = nely; op->box[2] = nelz;
+  op->per[0] = perx; op->per[1] = pery; op->per[2] = perz;
+  op->amp = amp;
+  op->seed = seed;
+  const int64_t nel = (int64_t)nelx * nely * nelz;
+  std::vector<double> vxyz(nel * 24);
+  nm_box_mesh(nelx, nely, nelz, amp, seed, vxyz.data());
+  int64_t nlocal, foff, nb;
+  int st;
+  if ((st = nm_box_gs_sizes(nelx, nely, nelz, lx, perx, pery, perz, &nlocal, &foff))) {
+    delete h;
+    return st;
+  }
   op->h_dg.resize(nlocal);
   op->h_gd.resize(nlocal);
   op->h_blk_len.resize(foff);
   op->h_blk_off.resize(foff);
   if ((st = nm_box_gs_map(nelx, nely, nelz, lx, perx, pery, perz, op->h_dg.data(),
                           op->h_gd.data(), op->h_blk_len.data(), op->h_blk_off.data(),
-                          &nlocal, &foff, &nb)))
-    return fail(st);
-  op->h_blk_len.resize(nb);
-  op->h_blk_off.resize(nb);
-  if ((st = op_set_gs(op, nlocal, foff, nb, op->h_dg.data(), op->h_gd.data(),
-                      op->h_blk_len.data(), op->h_blk_off.data(), s)))
-    return fail(st);
+                          &nlocal, &foff, &nb))) {
+    delete h;
+    return st;
+  }
   const int64_t mcap = nm_box_mask_size(nelx, nely, nelz, lx, perx, pery, perz);
   std::vector<int32_t> msk(mcap > 0 ? mcap : 1);
   int64_t m = 0;
   nm_box_mask(nelx, nely, nelz, lx, perx, pery, perz, msk.data(), &m);
-  for (int64_t i = 0; i < m; i++) msk[i] -= 1;
-  op->nmsk = m;
-  if (!(op->msk = op->alloc<int32_t>(m))) return fail(NM_ERR_NOMEM);
-  hipMemcpyAsync(op->msk, msk.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s);
-  if (hipStreamSynchronize(s) != hipSuccess) {
-    set_error("nm_op_create_box: device setup failed");
-    return fail(NM_ERR_HIP);
+  if ((st = op_build_linear(h, nel, lx, vxyz, h1, h2, nlocal, foff, nb, msk, m, s))) return st;
+  *out = h;
+  return NM_OK;
+}
+
+// Build a problem on a general conforming straight-edged hex mesh (e.g.
+// one read by nm_nmsh_read): facet connectivity, gs maps and mask from the
+// point ids (nm_mesh_*), Dirichlet on every facet without a neighbour,
+// then op_build_linear.  vxyz / pid: host arrays, 24 doubles / 8 ids per
+// element.
+int nm_op_create_mesh(nm_op_t **out, int64_t nel, int lx, const double *vxyz,
+                      const int64_t *pid, double h1, double h2, hipStream_t s) {
+  *out = nullptr;
+  if (nel <= 0 || lx < 2 || lx > 12) {
+    set_error("nm_op_create_mesh: nel=%lld lx=%d (lx must be 2..12)", (long long)nel, lx);
+    return NM_ERR_ARG;
   }
-  if ((st = op_finish(op, s))) return fail(st);
+  std::vector<int32_t> fneigh(nel * 6);
+  int st;
+  if ((st = nm_mesh_facet_neigh(nel, pid, fneigh.data()))) return st;
+  int64_t nlocal, foff, nb;
+  if ((st = nm_mesh_gs_sizes(nel, lx, fneigh.data(), &nlocal, &foff))) return st;
+  nm_op_t *h = new nm_op_t;
+  Op *op = &h->op;
+  op->h_dg.resize(nlocal);
+  op->h_gd.resize(nlocal);
+  op->h_blk_len.resize(foff);
+  op->h_blk_off.resize(foff);
+  if ((st = nm_mesh_gs_map(nel, lx, pid, fneigh.data(), op->h_dg.data(), op->h_gd.data(),
+                           op->h_blk_len.data(), op->h_blk_off.data(), &nlocal, &foff, &nb))) {
+    delete h;
+    return st;
+  }
+  const int64_t mcap = nm_mesh_mask_size(nel, lx, fneigh.data());
+  std::vector<int32_t> msk(mcap > 0 ? mcap : 1);
+  int64_t m = 0;
+  nm_mesh_mask(nel, lx, fneigh.data(), msk.data(), &m);
+  std::vector<double> v(vxyz, vxyz + nel * 24);
+  if ((st = op_build_linear(h, nel, lx, v, h1, h2, nlocal, foff, nb, msk, m, s))) return st;
   *out = h;
   return NM_OK;
 }
@@ -652,6 +707,34 @@ int nm_op_set_all_deformed(nm_op_t *h, hipStream_t s) {
 
 int nm_op_set_gs_variant(nm_op_t *h, int variant) {
   h->op.gs_variant = variant;
+  return NM_OK;
+}
+
+// Preconditioner of the op's solvers: 0 Jacobi (pc_jacobi, the default),
+// 1 identity (the ksp default when no preconditioner is given, krylov.f90
+// M_ident: z = r).  The arrays are swapped, not recomputed, so switching
+// back and forth is cheap and exact.
+int nm_op_set_pc(nm_op_t *h, int kind, hipStream_t s) {
+  Op *op = &h->op;
+  if (kind != 0 && kind != 1) {
+    set_error("nm_op_set_pc: kind %d (0 Jacobi, 1 identity)", kind);
+    return NM_ERR_ARG;
+  }
+  if (kind == op->pc) return NM_OK;
+  const int64_t n = op->n;
+  if (!op->alt64) {  // first switch: the other array holds ones
+    NM_ALLOC(op->alt64, double, n);
+    int st = launch_vec<double>(V_CFILL, op->alt64, nullptr, nullptr, nullptr, 1.0, n, s);
+    if (st) return st;
+  }
+  if (op->has32 && !op->alt32) {  // FP32 copy of what becomes the FP64 array
+    NM_ALLOC(op->alt32, float, n);
+    if (launch_d2f(op->alt64, op->alt32, n, s)) return NM_ERR_HIP;
+  }
+  std::swap(op->f64.dinv, op->alt64);
+  if (op->has32) std::swap(op->f32.dinv, op->alt32);
+  op->pc = kind;
+  NM_HIP_TRY(hipStreamSynchronize(s));
   return NM_OK;
 }
 

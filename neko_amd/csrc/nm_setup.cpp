@@ -329,6 +329,10 @@ static void hoare_sort(int32_t *dg, int32_t *gd, int64_t lo, int64_t hi,
   }
 }
 
+void hoare_sort_dofs(int32_t *dg, int32_t *gd, int64_t lo, int64_t hi) {
+  hoare_sort(dg, gd, lo, hi, 0);
+}
+
 }  // namespace nm
 
 extern "C" {

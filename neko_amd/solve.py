@@ -12,6 +12,7 @@ call goes to libneko_amd.so.  There is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -96,6 +97,22 @@ def box_mask(nelx, nely, nelz, lx, periodic=(0, 0, 0)):
     return msk[: mm.value].copy()
 
 
+def read_nmsh(path):
+    """Neko .nmsh file -> (vxyz (nel, 8, 3), pid (nel, 8), el_idx (nel,)),
+    vertices in Neko's symmetric order, periodic zones applied
+    (nm_nmsh_info / nm_nmsh_read)."""
+    b = os.fsencode(str(path))
+    nel, nz, nper, ncur = (C.c_int64() for _ in range(4))
+    check(lib().nm_nmsh_info(b, C.byref(nel), C.byref(nz), C.byref(nper),
+                             C.byref(ncur)), "nm_nmsh_info")
+    vxyz = np.empty((nel.value, 8, 3), dtype=np.float64)
+    pid = np.empty((nel.value, 8), dtype=np.int64)
+    el = np.empty(nel.value, dtype=np.int32)
+    check(lib().nm_nmsh_read(b, nel.value, vxyz.ctypes.data, pid.ctypes.data,
+                             el.ctypes.data), "nm_nmsh_read")
+    return vxyz, pid, el
+
+
 class Problem:
     """One discretised problem resident on the device (geometry, gs maps,
     mask, mass, multiplicity, Jacobi diagonal)."""
@@ -124,6 +141,33 @@ class Problem:
         pr.box_dims = (nelx, nely, nelz)
         pr.periodic = tuple(p)
         return pr
+
+    @classmethod
+    def mesh(cls, vxyz, pid, lx, h1=1.0, h2=0.0):
+        """General straight-edged hex mesh (vxyz: (nel, 8, 3) vertices in
+        Neko's symmetric order, pid: (nel, 8) point ids), Dirichlet on every
+        facet without a neighbour (nm_op_create_mesh)."""
+        v = np.ascontiguousarray(vxyz, dtype=np.float64).reshape(-1, 24)
+        p = np.ascontiguousarray(pid, dtype=np.int64).reshape(-1, 8)
+        if len(v) != len(p):
+            raise ValueError("vxyz and pid disagree on the element count")
+        h = C.c_void_p()
+        check(lib().nm_op_create_mesh(C.byref(h), len(v), int(lx), v.ctypes.data,
+                                      p.ctypes.data, float(h1), float(h2),
+                                      _stream()), "nm_op_create_mesh")
+        return cls(h.value)
+
+    @classmethod
+    def from_nmsh(cls, path, lx, h1=1.0, h2=0.0):
+        """Problem on a Neko .nmsh mesh (read_nmsh + mesh)."""
+        vxyz, pid, _ = read_nmsh(path)
+        return cls.mesh(vxyz, pid, lx, h1=h1, h2=h2)
+
+    def set_pc(self, kind: str):
+        """Preconditioner of this problem's solvers: "jacobi" (default) or
+        "identity" (plain CG, the ksp default without a preconditioner)."""
+        k = {"jacobi": 0, "identity": 1}[kind]
+        check(lib().nm_op_set_pc(self.h, k, _stream()), "nm_op_set_pc")
 
     @classmethod
     def from_arrays(cls, nel, lx, dx, g, B, mult_unused, gs, msk, h1=None,

@@ -113,22 +113,82 @@ def dgll(z):
     return d, dt
 
 
+def read_nmsh(path):
+    """Neko .nmsh reader restated for the tests (nmsh_file.f90:70-207 with
+    the record layouts of format/nmsh.f90): int32 nelv, int32 gdim; nelv
+    packed hex records (int32 el_idx, 8 x (int32 v_idx, 3 x f64)); int32
+    nzones and 9-int32 zone records (e, f, p_e, p_f, glb_pt_ids(4), type);
+    int32 ncurves.  Vertices are returned in Neko's symmetric order (file
+    p1,p2,p4,p3,p5,p6,p8,p7 -> Neko 1..8, nmsh_file.f90:159); periodic zones
+    (type 5) replace the ids of their facet points by glb_pt_ids, on the
+    point object, i.e. in every element holding that point
+    (mesh_apply_periodic_facet, mesh.f90:1837-1866).  Returns (vxyz (nel,8,
+    3), pid (nel,8), zones (nz,9), ncurves)."""
+    b = open(path, "rb").read()
+    nel, gdim = np.frombuffer(b, dtype="<i4", count=2)
+    if gdim != 3:
+        raise ValueError("only 3-D meshes")
+    rec = np.dtype([("el", "<i4"), ("v", [("idx", "<i4"), ("xyz", "<f8", 3)], 8)])
+    h = np.frombuffer(b, dtype=rec, count=int(nel), offset=8)
+    off = 8 + int(nel) * rec.itemsize
+    nz = int(np.frombuffer(b, dtype="<i4", count=1, offset=off)[0])
+    zones = np.frombuffer(b, dtype="<i4", count=nz * 9, offset=off + 4).reshape(nz, 9)
+    ncur = int(np.frombuffer(b, dtype="<i4", count=1, offset=off + 4 + 36 * nz)[0])
+    order = [0, 1, 3, 2, 4, 5, 7, 6]
+    vxyz = np.ascontiguousarray(h["v"]["xyz"][:, order, :], dtype=np.float64)
+    pid = np.ascontiguousarray(h["v"]["idx"][:, order], dtype=np.int64)
+    face_nodes = [[1, 5, 7, 3], [2, 6, 8, 4], [1, 2, 6, 5], [3, 4, 8, 7],
+                  [1, 2, 4, 3], [5, 6, 8, 7]]
+    loc = {int(g): e for e, g in enumerate(h["el"])}
+    new_id = {}
+    for z in zones:
+        if z[8] != 5:
+            continue
+        e = loc[int(z[0])]
+        for c in range(4):
+            new_id[int(pid[e, face_nodes[z[1] - 1][c] - 1])] = int(z[4 + c])
+    if new_id:
+        pid = np.array([[new_id.get(int(q), int(q)) for q in row] for row in pid],
+                       dtype=np.int64)
+    return vxyz, pid, zones.copy(), ncur
+
+
 class OracleCase:
     """A whole Neko-style problem on a (possibly perturbed/periodic) box."""
 
     def __init__(self, nelx, nely, nelz, lx, amp=0.0, seed=0,
                  periodic=(False, False, False), h1=1.0, h2=0.0):
         L = lib()
-        self.nelx, self.nely, self.nelz, self.lx = nelx, nely, nelz, lx
+        self.nelx, self.nely, self.nelz = nelx, nely, nelz
         self.amp, self.seed, self.periodic = amp, seed, tuple(periodic)
         nel = nelx * nely * nelz
+        vxyz = np.zeros(nel * 24)
+        pid = np.zeros(nel * 8, dtype=np.int64)
+        L.or_box_mesh(nelx, nely, nelz, int(periodic[0]), int(periodic[1]),
+                      int(periodic[2]), float(amp), int(seed), vxyz, pid)
+        self._setup(vxyz, pid, lx, h1, h2)
+
+    @classmethod
+    def from_mesh(cls, vxyz, pid, lx, h1=1.0, h2=0.0):
+        """The same problem on a general conforming straight-edged hex mesh
+        (vertices in Neko's symmetric order, point ids), Dirichlet on every
+        facet without a neighbour."""
+        c = cls.__new__(cls)
+        c.nelx = c.nely = c.nelz = None
+        c.amp, c.seed, c.periodic = None, None, None
+        c._setup(np.ascontiguousarray(vxyz, dtype=np.float64).reshape(-1),
+                 np.ascontiguousarray(pid, dtype=np.int64).reshape(-1), lx,
+                 h1, h2)
+        return c
+
+    def _setup(self, vxyz, pid, lx, h1, h2):
+        L = lib()
+        self.lx = lx
+        nel = len(pid) // 8
         self.nel = nel
         self.n = n = nel * lx ** 3
-        self.vxyz = np.zeros(nel * 24)
-        self.pid = np.zeros(nel * 8, dtype=np.int64)
-        L.or_box_mesh(nelx, nely, nelz, int(periodic[0]), int(periodic[1]),
-                      int(periodic[2]), float(amp), int(seed), self.vxyz,
-                      self.pid)
+        self.vxyz = vxyz
+        self.pid = pid
         self.fneigh = np.zeros(nel * 6, dtype=np.int32)
         L.or_facet_neigh(nel, self.pid, self.fneigh)
         self.dfrmd = np.zeros(nel, dtype=np.int32)
@@ -257,6 +317,17 @@ class OracleCase:
             np.sin(pi * self.zc)
         f = self.B * f
         self.gs(f)
+        self.mask_apply(f)
+        return f
+
+    def rhs_poisson_example(self):
+        """set_f of examples/poisson/setup.f90: f = 500 exp(-|X - (4,4,4)|^2
+        / 2) per local point, gs sum, times mult; then the Dirichlet mask
+        (driver.f90: bclst%apply(f))."""
+        r2 = (self.x - 4.0) ** 2 + (self.y - 4.0) ** 2 + (self.zc - 4.0) ** 2
+        f = 500.0 * np.exp(-r2 / 2.0)
+        self.gs(f)
+        f = f * self.mult
         self.mask_apply(f)
         return f
 
