@@ -111,6 +111,7 @@ typedef struct nm_gmres_s nm_gmres_t;
 typedef struct nm_server_s nm_server_t;
 typedef struct nm_batch_s nm_batch_t;
 typedef struct nm_phmg_s nm_phmg_t;
+typedef struct nm_proj_s nm_proj_t;
 
 typedef struct {
   int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
@@ -253,6 +254,26 @@ int nm_op_rhs_normal(const nm_op_t *op, uint64_t seed, double *f,
 /* Jacobi diagonal with every element treated as deformed (the coarse grid
  * of pc_hsmg.f90:312-318 sets msh%dfrmd_el = .true.) */
 int nm_op_set_all_deformed(nm_op_t *op, hipStream_t s);
+
+/* --- solution-space projection (projection_t, src/common/projection.f90) */
+/* The last L (<= 32) solutions and their images A x kept A-orthonormal on
+ * the device.  pre: project the RHS b in place before a solve
+ * (pre_solving/project_on, :236-280,:476-551); post: add the projected part
+ * back to the solution x and take x into the basis (post_solving/
+ * project_back/proj_ortho, :282-358,:717-834).  tstep and activ_step as in
+ * Neko (projection from step activ_step + 1); dt_last_change < 0 for a
+ * constant time step (time_step_controller_t%dt_last_change otherwise). */
+int nm_proj_create(nm_proj_t **pj, nm_op_t *op, int L, int activ_step);
+void nm_proj_free(nm_proj_t *pj);
+int nm_proj_clear(nm_proj_t *pj, hipStream_t s);
+int nm_proj_pre(nm_proj_t *pj, double *b, int tstep, int dt_last_change,
+                hipStream_t s);
+int nm_proj_post(nm_proj_t *pj, double *x, int tstep, int dt_last_change,
+                 hipStream_t s);
+/* basis size m; proj_m and proj_res (original residual) of the last
+ * project_on, as print_proj_info logs them */
+int nm_proj_info(nm_proj_t *pj, int *m, int *proj_m, double *proj_res,
+                 hipStream_t s);
 
 /* --- Jacobi-PCG (ksp_t 'cg' + pc_t 'jacobi') ----------------------------- */
 /* precision 0: FP64; 1: FP32 inner CG inside FP64 iterative refinement */

@@ -134,6 +134,12 @@ SIGNATURES = {
     "nm_server_result": (_i, [_vp, _i, C.POINTER(ServeResult)]),
     "nm_server_solution": (_i, [_vp, _i, _vp, _vp]),
     "nm_server_memory": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "nm_proj_create": (_i, [C.POINTER(_vp), _vp, _i, _i]),
+    "nm_proj_free": (None, [_vp]),
+    "nm_proj_clear": (_i, [_vp, _vp]),
+    "nm_proj_pre": (_i, [_vp, _vp, _i, _i, _vp]),
+    "nm_proj_post": (_i, [_vp, _vp, _i, _i, _vp]),
+    "nm_proj_info": (_i, [_vp, _pi, _pi, C.POINTER(_d), _vp]),
     "nm_cg_create": (_i, [C.POINTER(_vp), _vp, _i]),
     "nm_cg_free": (None, [_vp]),
     "nm_cg_solve": (_i, [_vp, _vp, _vp, _d, _i, _d, C.POINTER(Monitor), _vp]),

@@ -337,6 +337,50 @@ class CG:
             pass
 
 
+class Projection:
+    """Solution-space projection around repeated solves (Neko's
+    projection_t, src/common/projection.f90): pre() projects the RHS in
+    place before a solve, post() adds the projected part back to the
+    solution and takes it into the basis (up to L vectors, A-orthonormal).
+    dt_last_change < 0: constant time step."""
+
+    def __init__(self, problem: Problem, L: int, activ_step: int = 5):
+        self.h = C.c_void_p()
+        self.problem = problem
+        check(lib().nm_proj_create(C.byref(self.h), problem.h, int(L),
+                                   int(activ_step)), "nm_proj_create")
+
+    def pre(self, b: torch.Tensor, tstep: int, dt_last_change: int = -1):
+        check(lib().nm_proj_pre(self.h, _ptr(b), int(tstep), int(dt_last_change),
+                                _stream()), "nm_proj_pre")
+
+    def post(self, x: torch.Tensor, tstep: int, dt_last_change: int = -1):
+        check(lib().nm_proj_post(self.h, _ptr(x), int(tstep), int(dt_last_change),
+                                 _stream()), "nm_proj_post")
+
+    def clear(self):
+        check(lib().nm_proj_clear(self.h, _stream()), "nm_proj_clear")
+
+    def info(self):
+        """(basis size m, vectors used by the last projection, its original
+        residual sqrt(b.mult.b / volume))"""
+        m, pm, res = C.c_int(), C.c_int(), C.c_double()
+        check(lib().nm_proj_info(self.h, C.byref(m), C.byref(pm), C.byref(res),
+                                 _stream()), "nm_proj_info")
+        return m.value, pm.value, res.value
+
+    def free(self):
+        if self.h:
+            lib().nm_proj_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class HSMG:
     """hsmg preconditioner (pc_t 'hsmg', src/krylov/pc_hsmg.f90) on a box
     problem: Schwarz/FDM smoothing on lx and lx_mid, Jacobi-CG on lx=2.

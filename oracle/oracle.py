@@ -695,3 +695,102 @@ class OraclePhmg:
                                         x, np.ascontiguousarray(f), max_iter,
                                         float(abs_tol), lgmres)
         return x, mon
+
+
+# --------------------------------------------------------------------------
+# Solution-space projection (projection_t, src/common/projection.f90, CPU
+# path), numpy over an OracleCase.  Parity unpinned by reference vectors
+# (the reference ships none for it); pinned by the invariants in
+# tests/test_oracle_proj.py (A-orthonormal basis, B = A X, W-orthogonal
+# projected RHS, A xbar = b - b_proj, dependent vectors dropped, restart).
+# --------------------------------------------------------------------------
+class OracleProjection:
+    def __init__(self, case, L, activ_step=5, tol=1e-7):
+        self.case, self.L, self.activ, self.tol = case, L, activ_step, tol
+        n = case.n
+        self.m = 0
+        self.xx = np.zeros((max(L, 1), n))
+        self.bb = np.zeros((max(L, 1), n))
+        self.xbar = np.zeros(n)
+        self.proj_res = 0.0
+        self.proj_m = 0
+
+    def clear(self):
+        # bcknd_clear (:855-875)
+        self.m = 0
+        self.proj_m = 0
+        self.xx[:] = 0.0
+        self.bb[:] = 0.0
+
+    def pre(self, b, tstep, dt_last_change=-1):
+        # pre_solving (:236-280), b projected in place
+        if not (tstep > self.activ and self.L > 0):
+            return
+        if dt_last_change >= 0:
+            if dt_last_change == 0:
+                self.clear()
+                return
+            if not (dt_last_change > self.activ - 1):
+                return
+        self.project_on(b)
+
+    def post(self, x, tstep, dt_last_change=-1):
+        # post_solving (:282-301)
+        if not (tstep > self.activ and self.L > 0):
+            return
+        if dt_last_change >= 0 and not (dt_last_change > self.activ - 1):
+            return
+        self.project_back(x)
+
+    def project_on(self, b):
+        # cpu_project_on (:476-551): two rounds of classical Gram-Schmidt
+        m, w = self.m, self.case.mult
+        if m <= 0:
+            return
+        self.proj_res = float(np.sqrt(np.dot(b * w, b) / self.case.volume))
+        self.proj_m = m
+        alpha = np.array([np.dot(self.xx[k] * w, b) for k in range(m)])
+        xbar = alpha[0] * self.xx[0]
+        b -= alpha[0] * self.bb[0]
+        for k in range(1, m):
+            xbar = xbar + alpha[k] * self.xx[k]
+            b -= alpha[k] * self.bb[k]
+        work = np.array([np.dot(self.xx[k] * w, b) for k in range(m)])
+        for k in range(m):
+            xbar = xbar + work[k] * self.xx[k]
+            b -= work[k] * self.bb[k]
+        self.xbar = xbar
+
+    def project_back(self, x):
+        # bcknd_project_back (:317-358), CPU branch
+        if self.m > 0:
+            x += self.xbar
+        self.m = 1 if self.m == self.L else min(self.m + 1, self.L)
+        m = self.m
+        self.xx[m - 1] = x
+        self.bb[m - 1] = self.case.ax_full(x)
+        self.ortho()
+
+    def ortho(self):
+        # cpu_proj_ortho (:717-834)
+        m, w = self.m, self.case.mult
+        if m <= 0:
+            return
+        xm, bm = self.xx[m - 1], self.bb[m - 1]
+        alpha = np.array([0.5 * (np.dot(self.xx[k] * w, bm) + np.dot(self.bb[k] * w, xm))
+                          for k in range(m)])
+        nrm = np.sqrt(alpha[m - 1])
+        for k in range(m - 1):
+            xm -= alpha[k] * self.xx[k]
+            bm -= alpha[k] * self.bb[k]
+        beta = np.array([0.5 * (np.dot(self.xx[k] * w, bm) + np.dot(self.bb[k] * w, xm))
+                         for k in range(m - 1)])
+        for k in range(m - 1):
+            xm -= beta[k] * self.xx[k]
+            bm -= beta[k] * self.bb[k]
+        am = np.sqrt(np.dot(xm * w, bm))
+        if am > self.tol * nrm:
+            xm *= 1.0 / am
+            bm *= 1.0 / am
+        else:
+            self.m = m - 1
