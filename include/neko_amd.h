@@ -230,6 +230,25 @@ int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
  * (lane-parallel, blocks <= 8 entries). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
 int nm_op_set_gs_variant(nm_op_t *op, int variant);
+/* new constant coefficients h1, h2 (h2 != 0: mass term on) of an operator
+ * with constant h1, as the pnpn schemes set them before each solve
+ * (pressure h1 = 1/rho, h2 = 0: src/fluid/bcknd/cpu/pnpn_res_cpu.f90:74-78;
+ * velocity h1 = mu, h2 = rho bd/dt: :212-216); h2 B and the Jacobi diagonal
+ * are rebuilt.  Solvers created before re-capture their graphs on their
+ * next solve; batches keep the coefficients they were created with. */
+int nm_op_set_coeffs(nm_op_t *op, double h1, double h2, hipStream_t s);
+/* weak gradient opgrad (src/math/operators.f90:307-343, CPU kernel
+ * src/math/bcknd/cpu/cpu_opgrad.f90:263-329), device pointers, local;
+ * straight-edged operators (geometry from the element edges) */
+int nm_opgrad(nm_op_t *op, double *ux, double *uy, double *uz, const double *u,
+              hipStream_t s);
+/* velocity residual pnpn_vel_res_cpu_compute (pnpn_res_cpu.f90:182-241):
+ * sets h1 = mu, h2 = rho bd/dt, then res_c = -A u_c - opgrad_c(p) + f_c
+ * (local operator; gs and boundary conditions stay with the caller) */
+int nm_vel_res(nm_op_t *op, double mu, double rho, double bd, double dt,
+               const double *u, const double *v, const double *w, const double *p,
+               const double *fx, const double *fy, const double *fz,
+               double *ures, double *vres, double *wres, hipStream_t s);
 /* preconditioner of the op's CG/GMRES/batch/serve solvers: 0 Jacobi
  * (src/krylov/bcknd/cpu/pc_jacobi.f90, default), 1 identity (the ksp
  * default without a preconditioner, src/krylov/pc_identity.f90) */

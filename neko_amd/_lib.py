@@ -88,6 +88,9 @@ SIGNATURES = {
     "nm_op_create_mesh": (_i, [C.POINTER(_vp), _i64, _i, _vp, _vp, _d, _d,
                                _vp]),
     "nm_op_set_pc": (_i, [_vp, _i, _vp]),
+    "nm_op_set_coeffs": (_i, [_vp, _d, _d, _vp]),
+    "nm_opgrad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "nm_vel_res": (_i, [_vp, _d, _d, _d, _d] + [_vp] * 10 + [_vp]),
     "nm_op_create": (_i, [C.POINTER(_vp), _i, _i] + [_vp] * 9 +
                      [_d, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i,
                       _vp, _vp]),

@@ -39,6 +39,8 @@ struct CgSolver {
   CgScal *sc = nullptr;      // device: [0] main/inner, [1] outer
   CgScal *h_sc = nullptr;    // pinned host mirror
   hipGraphExec_t g64 = nullptr, g32 = nullptr;
+  uint64_t epoch = 0;        // op->epoch the graphs were captured at
+  int64_t npart_cap = 0;     // capacity of dpart
   int graph_iters = 4;
   int64_t bytes = 0;         // device memory held by this solver
   std::vector<void *> owned;

@@ -535,6 +535,40 @@ static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
   return NM_OK;
 }
 
+// (Re)capture the iteration graph for the op's current variant codes,
+// coefficients and preconditioner arrays (kernel arguments are baked into
+// a captured graph); cg_begin calls it again whenever op->epoch moved.
+static int cg_capture(CgSolver *cg) {
+  Op *op = cg->op;
+  // per-workgroup partials of the fused p.Ap (trilinear Ax variants)
+  const int code = cg->mixed ? op->ax_code<float>() : op->ax_code<double>();
+  cg->npart = 0;
+  if (code >= 200) {
+    cg->npart = ax_grid(op->nel, op->lx, code);
+    if (cg->npart > cg->npart_cap) {
+      if (!(cg->dpart = cg->alloc<double>(cg->npart))) {
+        set_error("cg: out of device memory");
+        return NM_ERR_NOMEM;
+      }
+      cg->npart_cap = cg->npart;
+    }
+  }
+  hipGraphExec_t *g = cg->mixed ? &cg->g32 : &cg->g64;
+  if (*g) {
+    NM_HIP_TRY(hipStreamSynchronize(cg->s));
+    NM_HIP_TRY(hipGraphExecDestroy(*g));
+    *g = nullptr;
+  }
+  int st;
+  if (!cg->mixed)
+    st = capture_iters<double, double>(cg, cg->x, cg->r, cg->p, cg->w, cg->sc, g);
+  else
+    st = capture_iters<float, float>(cg, cg->e32, cg->r32, cg->p32, cg->w32, cg->sc, g);
+  if (st) return st;
+  cg->epoch = op->epoch;
+  return NM_OK;
+}
+
 int cg_solver_create(CgSolver **out, Op *op, int precision) {
   *out = nullptr;
   CgSolver *cg = new CgSolver;
@@ -574,20 +608,7 @@ int cg_solver_create(CgSolver **out, Op *op, int precision) {
     set_error("cg: pinned host allocation failed");
     return fail(NM_ERR_NOMEM);
   }
-  // per-workgroup partials of the fused p.Ap (trilinear Ax variants)
-  const int code = precision ? op->ax_code<float>() : op->ax_code<double>();
-  if (code >= 200) {
-    cg->npart = ax_grid(op->nel, op->lx, code);
-    if (!(cg->dpart = cg->alloc<double>(cg->npart))) {
-      set_error("cg: out of device memory");
-      return fail(NM_ERR_NOMEM);
-    }
-  }
-  int st;
-  if (!precision)
-    st = capture_iters<double, double>(cg, cg->x, cg->r, cg->p, cg->w, cg->sc, &cg->g64);
-  else
-    st = capture_iters<float, float>(cg, cg->e32, cg->r32, cg->p32, cg->w32, cg->sc, &cg->g32);
+  int st = cg_capture(cg);
   if (st) return fail(st);
   if (hipStreamSynchronize(cg->s) != hipSuccess) return fail(NM_ERR_HIP);
   *out = cg;
@@ -659,6 +680,14 @@ int cg_begin(CgSolver *cg, const double *f, double abs_tol, double rel_tol,
   if (cg->pending) {
     set_error("cg: a solve is still in flight on this solver");
     return NM_ERR_STATE;
+  }
+  if (cg->epoch != op.epoch) {
+    if (cg->mixed) {
+      int st0 = cg->op->ensure32(cg->s);
+      if (st0) return st0;
+    }
+    int st1 = cg_capture(cg);
+    if (st1) return st1;
   }
   cg->abs_tol = abs_tol;
   cg->rel_tol = rel_tol;

@@ -59,6 +59,9 @@ struct Op {
   // diagonal preconditioner of the CG/GMRES solvers: 0 Jacobi (f*.dinv
   // holds 1/diag), 1 identity (f*.dinv holds ones); alt* keep the other
   int pc = 0;
+  // bumped whenever kernel arguments a captured solver graph bakes in
+  // change (variant codes, coefficients, preconditioner arrays)
+  uint64_t epoch = 0;
   double *alt64 = nullptr;
   float *alt32 = nullptr;
   double h2c = 0.0;        // constant h2 (with trilinear geometry)
@@ -101,6 +104,8 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
               const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
               const int32_t *blk_off, hipStream_t s);
 int op_finish(Op *op, hipStream_t s);
+// dd = 1/gs(h1 diag(K) + h2 B) with the op's current coefficients
+int op_jacobi(Op *op, double *dd, hipStream_t s);
 // Regroup a device gs layout (blk_ptr/gidx, pairs from pair0) by the element
 // of each block's / pair's lowest dof; the order inside blocks is kept.
 int gs_regroup(int64_t nel, int64_t n3, int64_t nblks, int64_t pair0, int64_t npairs,

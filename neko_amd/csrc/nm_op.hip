@@ -311,6 +311,20 @@ int op_finish(Op *op, hipStream_t s) {
   if ((st = launch_combine(op->red_part, 1, op->red_out, s))) return st;
   NM_HIP_TRY(hipMemcpyAsync(&op->volume, op->red_out, sizeof(double), hipMemcpyDeviceToHost, s));
   NM_HIP_TRY(hipStreamSynchronize(s));
+  double *jd = op->pc == 0 ? dd : op->alt64;
+  if (op->pc != 0 && (st = launch_vec<double>(V_CFILL, dd, nullptr, nullptr, nullptr, 1.0, n, s)))
+    return st;
+  if ((st = op_jacobi(op, jd, s))) return st;
+  float *jd32 = op->pc == 0 ? op->f32.dinv : op->alt32;
+  if (op->has32 && jd32 && launch_d2f(jd, jd32, n, s)) return NM_ERR_HIP;
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  return NM_OK;
+}
+
+// dd = 1/gs(h1 diag(K) + h2 B) with the op's current h1/h2
+int op_jacobi(Op *op, double *dd, hipStream_t s) {
+  const int64_t n = op->n;
+  int st;
   const double *g[6];
   for (int c = 0; c < 6; c++) g[c] = op->f64.g[c];
   if ((st = launch_jacobi_diag(op->nel, op->lx, op->dxt, g, op->dfrmd, dd, s))) return st;
@@ -691,6 +705,7 @@ int nm_op_set_epb(nm_op_t *h, int epb64, int epb32) {
   }
   h->op.epb64 = epb64;
   h->op.epb32 = epb32;
+  h->op.epoch++;
   return NM_OK;
 }
 
@@ -707,6 +722,41 @@ int nm_op_set_all_deformed(nm_op_t *h, hipStream_t s) {
 
 int nm_op_set_gs_variant(nm_op_t *h, int variant) {
   h->op.gs_variant = variant;
+  h->op.epoch++;
+  return NM_OK;
+}
+
+// New constant Helmholtz coefficients h1, h2 of an operator with constant
+// h1 (what the pnpn schemes set before each solve: pressure h1 = 1/rho,
+// h2 = 0, pnpn_res_cpu.f90:74-78; velocity h1 = mu, h2 = rho bd / dt,
+// :212-216): h2 B and the Jacobi diagonal are rebuilt, FP32 copies too.
+int nm_op_set_coeffs(nm_op_t *h, double h1, double h2, hipStream_t s) {
+  Op *op = &h->op;
+  if (op->f64.h1) {
+    set_error("nm_op_set_coeffs: the operator has a pointwise h1 field");
+    return NM_ERR_UNSUPPORTED;
+  }
+  const int64_t n = op->n;
+  int st;
+  op->h1c = h1;
+  op->h2c = h2;
+  op->ifh2 = h2 != 0.0;
+  if (op->ifh2) {
+    if (!op->f64.h2b) NM_ALLOC(op->f64.h2b, double, n);
+    if ((st = launch_vec<double>(V_CMULT2, op->f64.h2b, op->B, nullptr, nullptr, h2, n, s)))
+      return st;
+    if (op->has32) {
+      if (!op->f32.h2b) NM_ALLOC(op->f32.h2b, float, n);
+      if (launch_d2f(op->f64.h2b, op->f32.h2b, n, s)) return NM_ERR_HIP;
+    }
+  }
+  // the Jacobi array is f64.dinv, or alt64 while the identity is selected
+  double *jac64 = op->pc == 0 ? op->f64.dinv : op->alt64;
+  float *jac32 = op->pc == 0 ? op->f32.dinv : op->alt32;
+  if ((st = op_jacobi(op, jac64, s))) return st;
+  if (op->has32 && jac32 && launch_d2f(jac64, jac32, n, s)) return NM_ERR_HIP;
+  op->epoch++;
+  NM_HIP_TRY(hipStreamSynchronize(s));
   return NM_OK;
 }
 
@@ -734,6 +784,7 @@ int nm_op_set_pc(nm_op_t *h, int kind, hipStream_t s) {
   std::swap(op->f64.dinv, op->alt64);
   if (op->has32) std::swap(op->f32.dinv, op->alt32);
   op->pc = kind;
+  op->epoch++;
   NM_HIP_TRY(hipStreamSynchronize(s));
   return NM_OK;
 }
@@ -792,6 +843,7 @@ int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
     const float tg = timeit([&] { op.gs<double>(w, nullptr, s); });
     if (tg < bg) { bg = tg; cg = v; }
   }
+  op.epoch++;
   op.gs_variant = cg;
   hipEventDestroy(e0);
   hipEventDestroy(e1);

@@ -163,6 +163,28 @@ class Problem:
         vxyz, pid, _ = read_nmsh(path)
         return cls.mesh(vxyz, pid, lx, h1=h1, h2=h2)
 
+    def set_coeffs(self, h1: float, h2: float):
+        """New constant Helmholtz coefficients (nm_op_set_coeffs)."""
+        check(lib().nm_op_set_coeffs(self.h, float(h1), float(h2), _stream()),
+              "nm_op_set_coeffs")
+
+    def opgrad(self, u: torch.Tensor):
+        """Weak gradient (ux, uy, uz) of u (Neko's opgrad), local."""
+        out = [torch.empty_like(u) for _ in range(3)]
+        check(lib().nm_opgrad(self.h, *[_ptr(o) for o in out], _ptr(u), _stream()),
+              "nm_opgrad")
+        return out
+
+    def vel_res(self, u, v, w, p, f, mu, rho, bd, dt):
+        """pnpn velocity residual: sets h1 = mu, h2 = rho bd / dt and returns
+        (-A u - opgrad_x p + f_x, ...) (nm_vel_res), local."""
+        out = [torch.empty_like(u) for _ in range(3)]
+        check(lib().nm_vel_res(self.h, float(mu), float(rho), float(bd), float(dt),
+                               _ptr(u), _ptr(v), _ptr(w), _ptr(p),
+                               *[_ptr(q) for q in f], *[_ptr(o) for o in out],
+                               _stream()), "nm_vel_res")
+        return out
+
     def set_pc(self, kind: str):
         """Preconditioner of this problem's solvers: "jacobi" (default) or
         "identity" (plain CG, the ksp default without a preconditioner)."""

@@ -69,6 +69,12 @@ void or_mask_apply(double *x, const int *msk, int m);
 double or_glsc3(const double *a, const double *b, const double *c, int64_t n);
 
 /* ---- a12: Jacobi (pc_jacobi.f90 CPU formula) */
+/* ---- pnpn residual pieces (neko_oracle_pnpn.c) */
+void or_cofactors(int nel, int lx, const double *x, const double *y, const double *z,
+                  const double *d, double *cof);
+void or_opgrad(int nel, int lx, const double *u, const double *d, const double *w3,
+               const double *cof, double *ux, double *uy, double *uz);
+
 void or_jacobi_diag(double *d, const double *dt, const double *g11,
                     const double *g22, const double *g33, const double *g12,
                     const double *g13, const double *g23, const int *dfrmd,

@@ -94,6 +94,8 @@ def lib():
     L.or_gmres_solve.argtypes = [C.POINTER(OrProblem), C.c_void_p, _dp, _dp,
                                  C.c_int, C.c_double, C.c_int]
     L.or_ax_full.argtypes = [C.POINTER(OrProblem), _dp, _dp]
+    L.or_cofactors.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.or_opgrad.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -330,6 +332,40 @@ class OracleCase:
         f = f * self.mult
         self.mask_apply(f)
         return f
+
+    def cofactors(self):
+        """coef's drdx, drdy, drdz, dsdx, .., dtdz as a (9, n) array."""
+        if getattr(self, "_cof", None) is None:
+            cof = np.zeros(9 * self.n)
+            lib().or_cofactors(self.nel, self.lx, self.x, self.y, self.zc, self.d, cof)
+            self._cof = cof
+        return self._cof.reshape(9, self.n)
+
+    def opgrad(self, u):
+        """opgrad (cpu_opgrad_lx): the weak gradient w3 * C^T grad_r u."""
+        ux, uy, uz = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        lib().or_opgrad(self.nel, self.lx, np.ascontiguousarray(u, dtype=np.float64),
+                        self.d, self.w3, np.ascontiguousarray(self.cofactors()).reshape(-1),
+                        ux, uy, uz)
+        return ux, uy, uz
+
+    def ax_local_coeffs(self, u, h1, h2):
+        """ax_helm with constant h1, h2 (h2 != 0 adds h2 B u), no gs."""
+        w = np.zeros(self.n)
+        h1a = np.full(self.n, float(h1))
+        h2a = np.full(self.n, float(h2))
+        lib().or_ax_helm(w, np.ascontiguousarray(u, dtype=np.float64), self.d, self.dt, h1a,
+                         *self.g[:3], *self.g[3:], self.nel, self.lx, h2a, self.B,
+                         int(h2 != 0.0))
+        return w
+
+    def vel_res(self, u, v, w, p, f, mu, rho, bd, dt):
+        """pnpn_vel_res_cpu_compute (pnpn_res_cpu.f90:182-241): h1 = mu,
+        h2 = rho bd / dt; res_c = -Ax(u_c) - opgrad_c(p) + f_c (local)."""
+        h2 = rho * bd / dt
+        g = self.opgrad(p)
+        return [(-self.ax_local_coeffs(q, mu, h2)) - g[c] + f[c]
+                for c, q in enumerate((u, v, w))]
 
     def res_norm(self, r):
         return float(np.sqrt(self.glsc3(r, self.mult, r)) /

@@ -1,0 +1,94 @@
+"""GPU parity of the pnpn operator glue against the oracle: the weak
+gradient (nm_opgrad) on box and reference meshes, the fused velocity
+residual (nm_vel_res), and nm_op_set_coeffs -- new h1/h2 reach Ax, the
+Jacobi diagonal and an already created (graph-captured) CG solver."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def _box(lx=6):
+    from neko_amd.solve import Problem
+    args = (3, 4, 3, lx)
+    return (O.OracleCase(*args, amp=0.1, seed=6), Problem.box(*args, amp=0.1, seed=6))
+
+
+@pytest.mark.parametrize("lx", [4, 8])
+def test_opgrad_matches_oracle(lx):
+    print("opgrad", lx, flush=True)
+    c, p = _box(lx)
+    u = np.random.default_rng(1).standard_normal(c.n)
+    ref = c.opgrad(u)
+    got = p.opgrad(torch.from_numpy(u).cuda())
+    for g, r in zip(got, ref):
+        assert np.abs(_np(g) - r).max() <= 1e-12 * np.abs(r).max()
+
+
+def test_opgrad_on_reference_mesh():
+    from neko_amd.solve import Problem, read_nmsh
+    print("opgrad hemi", flush=True)
+    v, pid, _ = read_nmsh(os.path.join(ROOT, "tests", "golden", "nmsh", "hemi.nmsh"))
+    c = O.OracleCase.from_mesh(v, pid, 5)
+    p = Problem.mesh(v, pid, 5)
+    u = np.random.default_rng(2).standard_normal(c.n)
+    ref = c.opgrad(u)
+    got = p.opgrad(torch.from_numpy(u).cuda())
+    for g, r in zip(got, ref):
+        assert np.abs(_np(g) - r).max() <= 1e-12 * np.abs(r).max()
+
+
+def test_vel_res_matches_oracle():
+    print("vel res", flush=True)
+    c, p = _box(8)
+    rng = np.random.default_rng(3)
+    fields = [rng.standard_normal(c.n) for _ in range(7)]
+    u, v, w, pr, *f = fields
+    mu, rho, bd, dt = 1.0 / 1600, 1.0, 1.5, 1e-3
+    ref = c.vel_res(u, v, w, pr, f, mu, rho, bd, dt)
+    d = [torch.from_numpy(a).cuda() for a in fields]
+    for code in (0, 201):
+        p.set_epb(code, code)
+        got = p.vel_res(d[0], d[1], d[2], d[3], d[4:], mu, rho, bd, dt)
+        for g, r in zip(got, ref):
+            assert np.abs(_np(g) - r).max() <= 1e-12 * np.abs(r).max(), code
+
+
+def test_set_coeffs_reaches_solver():
+    """A CG solver created for h1 = 1, h2 = 0 solves the Helmholtz problem
+    after set_coeffs(h1, h2): same iterations (+-1) and solution as the
+    oracle built with those coefficients."""
+    from neko_amd.solve import CG
+    print("set coeffs", flush=True)
+    p = _box(6)[1]
+    h1, h2 = 0.05, 40.0
+    c = O.OracleCase(3, 4, 3, 6, amp=0.1, seed=6, h1=h1, h2=h2)
+    for prec in ("fp64", "mixed"):
+        p.set_coeffs(1.0, 0.0)
+        p.set_epb(201, 201)
+        cg = CG(p, prec)
+        p.set_coeffs(h1, h2)
+        f = c.rhs_normal(7)
+        tol = 1e-10 * c.res_norm(f)
+        x_or, mon_or = c.cg(f, tol, max_iter=5000)
+        x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+        mon = cg.solve(x, torch.from_numpy(f).cuda(), tol, max_iter=5000)
+        assert mon.converged
+        if prec == "fp64":
+            assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
+        rel = np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or)
+        assert rel < 1e-9, (prec, rel)
+    jac = _np(p.field(8))
+    assert np.abs(jac - c.jacobi()).max() <= 1e-12 * np.abs(c.jacobi()).max()
