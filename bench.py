@@ -47,10 +47,15 @@ def ax_bytes_per_dof(lx, code, h1_field=False, h2=False):
 def ax_flops_per_dof(lx, code):
     """FP64 flops per local DOF of one Ax launch (FMA = 2), counted from
     the kernel source: derivatives 6 lx, geometric flux 15, transposes
-    6 lx, final sum 2 (113 at lx = 8; Neko counts 15 + 12 lx); the
-    trilinear variant adds its Jacobian, cofactors, jac, G and B (91)."""
+    6 lx, final sum 2 (113 at lx = 8; Neko counts 15 + 12 lx).  The
+    trilinear variant evaluates the geometry per point instead of reading
+    it: cofactor rows as three cross products (27), jac (5), 1/jac by the
+    reciprocal estimate and two Newton steps (7), the weight w3 h1 / jac
+    (2), the flux as sc C^T grad u then C g (33 instead of 15), and the
+    per-element edge tables amortised over the element's points (~8):
+    67 more."""
     base = 2 * 3 * lx + 15 + 2 * lx + 2 * 2 * lx + 2
-    return base + (91 if code >= 200 else 0)
+    return base + (67 if code >= 200 else 0)
 
 
 def gs_bytes(info):
@@ -321,7 +326,7 @@ def roofline_entry(code, lx, gbs, tfs, nbytes, ms, n):
     """Roofline object of the dominant kernel (the FP64 Ax launch): the
     bound is the one the kernel comes closest to (HBM bytes, or the FP64
     peak for the in-kernel-geometry variant, which reads 16.6 B/DOF and
-    executes ~204 flop/DOF); both fractions are reported."""
+    executes ~180 flop/DOF at lx = 8); both fractions are reported."""
     name = (f"ax_tri_kernel<double,{lx},{code % 100 or 1}>" if code >= 200 else
             f"ax_kstep_kernel<double,{lx},{code % 100 or 1}>")
     hbm_frac = gbs / HBM_PEAK_GBS
