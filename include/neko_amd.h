@@ -249,6 +249,21 @@ int nm_vel_res(nm_op_t *op, double mu, double rho, double bd, double dt,
                const double *u, const double *v, const double *w, const double *p,
                const double *fx, const double *fy, const double *fz,
                double *ures, double *vres, double *wres, hipStream_t s);
+/* curl (src/math/operators.f90 curl, CPU kernel opr_cpu_curl,
+ * src/math/bcknd/cpu/opr_cpu.f90:126-180): w = Binv gs(B curl u), no cyclic
+ * rotation; straight-edged operators */
+int nm_curl(nm_op_t *op, double *w1, double *w2, double *w3, const double *u1,
+            const double *u2, const double *u3, hipStream_t s);
+/* pressure residual pnpn_prs_res_cpu_compute (pnpn_res_cpu.f90:31-180):
+ * sets h1 = 1/rho, h2 = 0; p_res = -A p + sum_c cdtp(ta_c) with
+ * ta = Binv gs(f/rho - (mu/rho) B curl(curl(u_e))).  The facet-normal
+ * surface terms (bc_prs_surface, bc_sym_surface) are not included: exact
+ * when the velocity vanishes on the velocity-Dirichlet facets (walls) and
+ * there are no symmetry facets. */
+int nm_prs_res(nm_op_t *op, double mu, double rho, const double *p,
+               const double *ue, const double *ve, const double *we,
+               const double *fx, const double *fy, const double *fz,
+               double *p_res, hipStream_t s);
 /* preconditioner of the op's CG/GMRES/batch/serve solvers: 0 Jacobi
  * (src/krylov/bcknd/cpu/pc_jacobi.f90, default), 1 identity (the ksp
  * default without a preconditioner, src/krylov/pc_identity.f90) */

@@ -91,6 +91,8 @@ SIGNATURES = {
     "nm_op_set_coeffs": (_i, [_vp, _d, _d, _vp]),
     "nm_opgrad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "nm_vel_res": (_i, [_vp, _d, _d, _d, _d] + [_vp] * 10 + [_vp]),
+    "nm_curl": (_i, [_vp] * 8),
+    "nm_prs_res": (_i, [_vp, _d, _d] + [_vp] * 9),
     "nm_op_create": (_i, [C.POINTER(_vp), _i, _i] + [_vp] * 9 +
                      [_d, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i,
                       _vp, _vp]),

@@ -43,6 +43,8 @@ struct Op {
   int *dfrmd = nullptr;
   double *dxt = nullptr;
   double *B = nullptr, *jac = nullptr;
+  double *binv = nullptr;      // 1/gs(B) (coef%Binv), built on first use
+  double *pnpn_work = nullptr; // 6 n-vectors of pressure-residual scratch
   double *xyz[3] = {nullptr, nullptr, nullptr};
   OpFields<double> f64;
   OpFields<float> f32;

@@ -180,3 +180,312 @@ int nm_vel_res(nm_op_t *h, double mu, double rho, double bd, double dt, const do
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pressure residual (pnpn_prs_res_cpu_compute, pnpn_res_cpu.f90:31-180),
+// without the facet-normal surface terms (bc_prs_surface / bc_sym_surface),
+// which vanish when the velocity is zero on the marked facets (walls) and
+// there are no symmetry facets; curl (opr_cpu_curl, opr_cpu.f90:126-180),
+// dudxyz (cpu_dudxyz.f90:97-159) and cdtp (cpu_cdtp.f90:191-256) on
+// trilinear operators with the geometry evaluated in the kernels.
+// ---------------------------------------------------------------------------
+namespace nm {
+namespace {
+
+// Geometry of one element for a thread (i,j): the dX/dr and dX/ds tables in
+// LDS and dX/dt in registers, as in the trilinear Ax.
+template <int LX>
+struct TriGeo {
+  double (*sa)[LX][3];
+  double (*sb)[LX][3];
+  double ct[3];
+  __device__ void cof(int k, int i, int j, double r[3], double s[3], double t[3],
+                      double &jac) const {
+    const double a0 = sa[k][j][0], a1 = sa[k][j][1], a2 = sa[k][j][2];
+    const double b0 = sb[k][i][0], b1 = sb[k][i][1], b2 = sb[k][i][2];
+    r[0] = b1 * ct[2] - ct[1] * b2; r[1] = ct[0] * b2 - b0 * ct[2]; r[2] = b0 * ct[1] - ct[0] * b1;
+    s[0] = ct[1] * a2 - a1 * ct[2]; s[1] = a0 * ct[2] - ct[0] * a2; s[2] = ct[0] * a1 - a0 * ct[1];
+    t[0] = a1 * b2 - b1 * a2; t[1] = b0 * a2 - a0 * b2; t[2] = a0 * b1 - b0 * a1;
+    jac = ct[0] * t[0] + ct[1] * t[1] + ct[2] * t[2];
+  }
+};
+
+template <int LX>
+__device__ void tri_tables(const double *__restrict__ zg, const double *se, double (*sa)[LX][3], double (*sb)[LX][3], int i, int j,
+                           double ct[3]) {
+  const double p1 = (1.0 + zg[j]) * 0.5, p0 = 1.0 - p1;
+  const double t1 = (1.0 + zg[i]) * 0.5, t0 = 1.0 - t1;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    sa[i][j][c] = (se[c * 4 + 0] * p0 + se[c * 4 + 1] * p1) * t0 +
+                  (se[c * 4 + 2] * p0 + se[c * 4 + 3] * p1) * t1;
+    sb[i][j][c] = (se[12 + c * 4 + 0] * p0 + se[12 + c * 4 + 1] * p1) * t0 +
+                  (se[12 + c * 4 + 2] * p0 + se[12 + c * 4 + 3] * p1) * t1;
+  }
+  const double ri1 = (1.0 + zg[i]) * 0.5, ri0 = 1.0 - ri1;
+  const double sj1 = (1.0 + zg[j]) * 0.5, sj0 = 1.0 - sj1;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    ct[c] = (se[24 + c * 4 + 0] * ri0 + se[24 + c * 4 + 1] * ri1) * sj0 +
+            (se[24 + c * 4 + 2] * ri0 + se[24 + c * 4 + 3] * ri1) * sj1;
+}
+
+// w = B curl(u) per point (local): the curl of opr_cpu_curl before its gs
+// (w1 = dw/dy - dv/dz, w2 = du/dz - dw/dx, w3 = dv/dx - du/dy, each
+// derivative ((f_r dr + f_s ds) + f_t dt) / jac)
+template <int LX>
+__global__ void __launch_bounds__(LX *LX)
+    curl_tri_kernel(const double *__restrict__ u1, const double *__restrict__ u2,
+                    const double *__restrict__ u3, const double *__restrict__ dx,
+                    const double *__restrict__ zg, const double *__restrict__ wg,
+                    const double *__restrict__ edges, int64_t nel, double *__restrict__ w1,
+                    double *__restrict__ w2, double *__restrict__ w3o) {
+  constexpr int P = LX * LX;
+  __shared__ double su[3][LX * P];
+  __shared__ double sa[LX][LX][3], sb[LX][LX][3], se[36];
+  const int64_t e = blockIdx.x;
+  if (e >= nel) return;
+  const int t = threadIdx.x, i = t % LX, j = t / LX;
+  const int64_t base = e * (int64_t)(LX * P) + t;
+  const double *uf[3] = {u1, u2, u3};
+  double uc[3][LX];
+#pragma unroll
+  for (int f = 0; f < 3; f++)
+#pragma unroll
+    for (int k = 0; k < LX; k++) {
+      uc[f][k] = uf[f][base + k * P];
+      su[f][k * P + t] = uc[f][k];
+    }
+  for (int x = t; x < 36; x += P) se[x] = edges[e * 36 + x];
+  __syncthreads();
+  TriGeo<LX> G{sa, sb, {0, 0, 0}};
+  tri_tables<LX>(zg, se, sa, sb, i, j, G.ct);
+  const double wij = wg[i] * wg[j];
+  __syncthreads();
+#pragma unroll 1
+  for (int k = 0; k < LX; k++) {
+    double d[3][3];  // d[field][r,s,t]
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+      const double *pl = &su[f][k * P];
+      double fr = 0.0, fs = 0.0, ft = 0.0;
+#pragma unroll
+      for (int l = 0; l < LX; l++) {
+        fr += dx[i + LX * l] * pl[l + LX * j];
+        fs += dx[j + LX * l] * pl[i + LX * l];
+        ft += dx[k + LX * l] * uc[f][l];
+      }
+      d[f][0] = fr; d[f][1] = fs; d[f][2] = ft;
+    }
+    double r[3], s[3], tv[3], jac;
+    G.cof(k, i, j, r, s, tv, jac);
+    const double jinv = 1.0 / jac;
+    const double B = jac * (wij * wg[k]);
+    // dudxyz(f, c): ((f_r r_c + f_s s_c) + f_t t_c) / jac
+    auto dd = [&](int f, int c) { return ((d[f][0] * r[c] + d[f][1] * s[c]) + d[f][2] * tv[c]) * jinv; };
+    const int64_t q = base + k * P;
+    w1[q] = (dd(2, 1) - dd(1, 2)) * B;
+    w2[q] = (dd(0, 2) - dd(2, 0)) * B;
+    w3o[q] = (dd(1, 0) - dd(0, 1)) * B;
+  }
+}
+
+// out = -out + sum_c cdtp(t_c, r_c, s_c, t_c): with X_r = w3 (r . tv),
+// X_s = w3 (s . tv), X_t = w3 (t . tv) per point, out(i,j,k) gains
+// sum_l D(l,i) X_r(l,j,k) + D(l,j) X_s(i,l,k) + D(l,k) X_t(i,j,l)
+template <int LX>
+__global__ void __launch_bounds__(LX *LX)
+    cdtp_sum_tri_kernel(const double *__restrict__ t1, const double *__restrict__ t2,
+                        const double *__restrict__ t3, const double *__restrict__ dx,
+                        const double *__restrict__ zg, const double *__restrict__ wg,
+                        const double *__restrict__ edges, int64_t nel, double *__restrict__ out) {
+  constexpr int P = LX * LX;
+  __shared__ double sr[LX * P], ss[LX * P];
+  __shared__ double sa[LX][LX][3], sb[LX][LX][3], se[36];
+  const int64_t e = blockIdx.x;
+  if (e >= nel) return;
+  const int t = threadIdx.x, i = t % LX, j = t / LX;
+  const int64_t base = e * (int64_t)(LX * P) + t;
+  for (int x = t; x < 36; x += P) se[x] = edges[e * 36 + x];
+  __syncthreads();
+  TriGeo<LX> G{sa, sb, {0, 0, 0}};
+  tri_tables<LX>(zg, se, sa, sb, i, j, G.ct);
+  const double wij = wg[i] * wg[j];
+  __syncthreads();
+  double xt[LX];
+#pragma unroll
+  for (int k = 0; k < LX; k++) {
+    double r[3], s[3], tv[3], jac;
+    G.cof(k, i, j, r, s, tv, jac);
+    const int64_t q = base + k * P;
+    const double a0 = t1[q], a1 = t2[q], a2 = t3[q];
+    const double w3 = wij * wg[k];
+    sr[k * P + t] = w3 * (a0 * r[0] + a1 * r[1] + a2 * r[2]);
+    ss[k * P + t] = w3 * (a0 * s[0] + a1 * s[1] + a2 * s[2]);
+    xt[k] = w3 * (a0 * tv[0] + a1 * tv[1] + a2 * tv[2]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < LX; k++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < LX; l++) {
+      acc += dx[l + LX * i] * sr[k * P + l + LX * j];
+      acc += dx[l + LX * j] * ss[k * P + i + LX * l];
+      acc += dx[l + LX * k] * xt[l];
+    }
+    const int64_t q = base + k * P;
+    out[q] = (-out[q]) + acc;
+  }
+}
+
+// y_c *= binv (c = 1..3)
+__global__ void scale3_kernel(double *__restrict__ a, double *__restrict__ b,
+                              double *__restrict__ c, const double *__restrict__ binv,
+                              int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double v = binv[i];
+    a[i] = a[i] * v;
+    b[i] = b[i] * v;
+    c[i] = c[i] * v;
+  }
+}
+
+// wa <- wa binv (the curl's Binv), then ta_c = f_c / rho - ((wa_c mu/rho) B)
+__global__ void prs_src_kernel(double *__restrict__ ta1, double *__restrict__ ta2,
+                               double *__restrict__ ta3, const double *__restrict__ wa1,
+                               const double *__restrict__ wa2, const double *__restrict__ wa3,
+                               const double *__restrict__ fx, const double *__restrict__ fy,
+                               const double *__restrict__ fz, const double *__restrict__ binv,
+                               const double *__restrict__ B, double rho, double mu, int64_t n) {
+  const double mr = mu / rho;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double bi = binv[i], b = B[i];
+    ta1[i] = fx[i] / rho - ((wa1[i] * bi) * mr) * b;
+    ta2[i] = fy[i] / rho - ((wa2[i] * bi) * mr) * b;
+    ta3[i] = fz[i] / rho - ((wa3[i] * bi) * mr) * b;
+  }
+}
+
+int ensure_binv(Op *op, hipStream_t s) {
+  if (op->binv) return NM_OK;
+  const int64_t n = op->n;
+  if (!(op->binv = op->alloc<double>(n))) {
+    set_error("pressure residual: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  int st;
+  if ((st = launch_vec<double>(V_COPY, op->binv, op->B, nullptr, nullptr, 0.0, n, s))) return st;
+  if ((st = op->gs<double>(op->binv, nullptr, s))) return st;
+  return launch_vec<double>(V_INVCOL1, op->binv, nullptr, nullptr, nullptr, 0.0, n, s);
+}
+
+int launch_curl(const Op &op, const double *u1, const double *u2, const double *u3,
+                double *w1, double *w2, double *w3, hipStream_t s) {
+  const OpFields<double> &F = op.f64;
+  const dim3 g((unsigned)op.nel);
+#define NM_CURL(L)                                                                        \
+  case L:                                                                                 \
+    hipLaunchKernelGGL(curl_tri_kernel<L>, g, dim3(L * L), 0, s, u1, u2, u3, F.dx, F.zg,  \
+                       F.wg, F.edges, op.nel, w1, w2, w3);                                \
+    break;
+  switch (op.lx) {
+    NM_CURL(2) NM_CURL(3) NM_CURL(4) NM_CURL(5) NM_CURL(6) NM_CURL(7) NM_CURL(8) NM_CURL(9)
+    NM_CURL(10)
+    default:
+      set_error("curl: lx=%d outside 2..10", op.lx);
+      return NM_ERR_UNSUPPORTED;
+  }
+#undef NM_CURL
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+int launch_cdtp_sum(const Op &op, const double *t1, const double *t2, const double *t3,
+                    double *out, hipStream_t s) {
+  const OpFields<double> &F = op.f64;
+  const dim3 g((unsigned)op.nel);
+#define NM_CDTP(L)                                                                        \
+  case L:                                                                                 \
+    hipLaunchKernelGGL(cdtp_sum_tri_kernel<L>, g, dim3(L * L), 0, s, t1, t2, t3, F.dx,    \
+                       F.zg, F.wg, F.edges, op.nel, out);                                 \
+    break;
+  switch (op.lx) {
+    NM_CDTP(2) NM_CDTP(3) NM_CDTP(4) NM_CDTP(5) NM_CDTP(6) NM_CDTP(7) NM_CDTP(8) NM_CDTP(9)
+    NM_CDTP(10)
+    default:
+      set_error("cdtp: lx=%d outside 2..10", op.lx);
+      return NM_ERR_UNSUPPORTED;
+  }
+#undef NM_CDTP
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// curl including its gs and Binv (opr_cpu_curl): w = Binv gs(B curl u)
+int curl_full(Op *op, const double *u1, const double *u2, const double *u3, double *w1,
+              double *w2, double *w3, hipStream_t s) {
+  int st;
+  if ((st = launch_curl(*op, u1, u2, u3, w1, w2, w3, s))) return st;
+  for (double *w : {w1, w2, w3})
+    if ((st = op->gs<double>(w, nullptr, s))) return st;
+  hipLaunchKernelGGL(scale3_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w1, w2, w3,
+                     (const double *)op->binv, op->n);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+}  // namespace
+}  // namespace nm
+
+extern "C" {
+
+// curl (src/math/operators.f90 curl; opr_cpu_curl): w = Binv gs(B curl u)
+int nm_curl(nm_op_t *h, double *w1, double *w2, double *w3, const double *u1,
+            const double *u2, const double *u3, hipStream_t s) {
+  Op *op = &h->op;
+  int st = check_tri(*op, "nm_curl");
+  if (st || (st = ensure_binv(op, s))) return st;
+  return curl_full(op, u1, u2, u3, w1, w2, w3, s);
+}
+
+// p_res = -A p + sum_c cdtp(ta_c) with h1 = 1/rho, h2 = 0, where ta =
+// Binv gs(f / rho - (mu/rho) B curl(curl(u_e))) (pnpn_prs_res_cpu_compute
+// without the surface terms); local like the reference (gs of p_res is the
+// solver's).
+int nm_prs_res(nm_op_t *h, double mu, double rho, const double *p, const double *ue,
+               const double *ve, const double *we, const double *fx, const double *fy,
+               const double *fz, double *p_res, hipStream_t s) {
+  Op *op = &h->op;
+  int st = check_tri(*op, "nm_prs_res");
+  if (st || (st = ensure_binv(op, s))) return st;
+  const int64_t n = op->n;
+  if (!op->pnpn_work && !(op->pnpn_work = op->alloc<double>(6 * n))) {
+    set_error("nm_prs_res: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  double *ta[3] = {op->pnpn_work, op->pnpn_work + n, op->pnpn_work + 2 * n};
+  double *wa[3] = {op->pnpn_work + 3 * n, op->pnpn_work + 4 * n, op->pnpn_work + 5 * n};
+  const double h1 = 1.0 / rho;
+  if (!(op->h1c == h1 && !op->ifh2) && (st = nm_op_set_coeffs(h, h1, 0.0, s))) return st;
+  if ((st = curl_full(op, ue, ve, we, ta[0], ta[1], ta[2], s))) return st;
+  // second curl up to its gs; its Binv is applied inside prs_src_kernel
+  if ((st = launch_curl(*op, ta[0], ta[1], ta[2], wa[0], wa[1], wa[2], s))) return st;
+  for (double *w : wa)
+    if ((st = op->gs<double>(w, nullptr, s))) return st;
+  hipLaunchKernelGGL(prs_src_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, ta[0], ta[1], ta[2],
+                     (const double *)wa[0], (const double *)wa[1], (const double *)wa[2], fx,
+                     fy, fz, (const double *)op->binv, (const double *)op->B, rho, mu, n);
+  NM_HIP_TRY(hipGetLastError());
+  for (double *t : ta)
+    if ((st = op->gs<double>(t, nullptr, s))) return st;
+  hipLaunchKernelGGL(scale3_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, ta[0], ta[1], ta[2],
+                     (const double *)op->binv, n);
+  NM_HIP_TRY(hipGetLastError());
+  if ((st = op->apply<double>(p_res, p, nullptr, s, false, false))) return st;
+  return launch_cdtp_sum(*op, ta[0], ta[1], ta[2], p_res, s);
+}
+
+}  // extern "C"

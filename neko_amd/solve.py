@@ -185,6 +185,22 @@ class Problem:
                                _stream()), "nm_vel_res")
         return out
 
+    def curl(self, u1, u2, u3):
+        """Neko's curl: Binv gs(B curl u) (nm_curl)."""
+        out = [torch.empty_like(u1) for _ in range(3)]
+        check(lib().nm_curl(self.h, *[_ptr(o) for o in out], _ptr(u1), _ptr(u2), _ptr(u3),
+                            _stream()), "nm_curl")
+        return out
+
+    def prs_res(self, p, ue, f, mu, rho):
+        """pnpn pressure residual without surface terms (nm_prs_res):
+        sets h1 = 1/rho, h2 = 0 and returns -A p + sum_c cdtp(ta_c)."""
+        out = torch.empty_like(p)
+        check(lib().nm_prs_res(self.h, float(mu), float(rho), _ptr(p),
+                               *[_ptr(q) for q in ue], *[_ptr(q) for q in f], _ptr(out),
+                               _stream()), "nm_prs_res")
+        return out
+
     def set_pc(self, kind: str):
         """Preconditioner of this problem's solvers: "jacobi" (default) or
         "identity" (plain CG, the ksp default without a preconditioner)."""

@@ -74,6 +74,10 @@ void or_cofactors(int nel, int lx, const double *x, const double *y, const doubl
                   const double *d, double *cof);
 void or_opgrad(int nel, int lx, const double *u, const double *d, const double *w3,
                const double *cof, double *ux, double *uy, double *uz);
+void or_dudxyz(int nel, int lx, const double *u, const double *dr, const double *ds,
+               const double *dt, const double *jac, const double *d, double *du);
+void or_cdtp(int nel, int lx, const double *x, const double *dr, const double *ds,
+             const double *dt, const double *d, const double *w3, double *dtx);
 
 void or_jacobi_diag(double *d, const double *dt, const double *g11,
                     const double *g22, const double *g33, const double *g12,

@@ -96,6 +96,8 @@ def lib():
     L.or_ax_full.argtypes = [C.POINTER(OrProblem), _dp, _dp]
     L.or_cofactors.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
     L.or_opgrad.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.or_dudxyz.argtypes = [C.c_int, C.c_int] + [_dp] * 7
+    L.or_cdtp.argtypes = [C.c_int, C.c_int] + [_dp] * 7
     _lib = L
     return L
 
@@ -348,6 +350,58 @@ class OracleCase:
                         self.d, self.w3, np.ascontiguousarray(self.cofactors()).reshape(-1),
                         ux, uy, uz)
         return ux, uy, uz
+
+    def dudxyz(self, u, c):
+        """du/dx_c (cpu_dudxyz_lx with coef's dr_c, ds_c, dt_c, jacinv)."""
+        cof = self.cofactors()
+        du = np.zeros(self.n)
+        lib().or_dudxyz(self.nel, self.lx, np.ascontiguousarray(u, dtype=np.float64),
+                        np.ascontiguousarray(cof[c]), np.ascontiguousarray(cof[3 + c]),
+                        np.ascontiguousarray(cof[6 + c]), self.jac, self.d, du)
+        return du
+
+    def cdtp(self, x, c):
+        """cdtp(x, dr_c, ds_c, dt_c) (cpu_cdtp_lx)."""
+        cof = self.cofactors()
+        out = np.zeros(self.n)
+        lib().or_cdtp(self.nel, self.lx, np.ascontiguousarray(x, dtype=np.float64),
+                      np.ascontiguousarray(cof[c]), np.ascontiguousarray(cof[3 + c]),
+                      np.ascontiguousarray(cof[6 + c]), self.d, self.w3, out)
+        return out
+
+    def binv(self):
+        b = self.B.copy()
+        self.gs(b)
+        return 1.0 / b
+
+    def curl(self, u1, u2, u3):
+        """opr_cpu_curl (opr_cpu.f90:126-180): w1 = dw/dy - dv/dz, ..., then
+        w *= B, gs, w *= Binv (no cyclic rotation)."""
+        w1 = self.dudxyz(u3, 1) - self.dudxyz(u2, 2)
+        w2 = self.dudxyz(u1, 2) - self.dudxyz(u3, 0)
+        w3 = self.dudxyz(u2, 0) - self.dudxyz(u1, 1)
+        bi = self.binv()
+        out = []
+        for w in (w1, w2, w3):
+            w = w * self.B
+            self.gs(w)
+            out.append(w * bi)
+        return out
+
+    def prs_res(self, p, ue, f, mu, rho):
+        """pnpn_prs_res_cpu_compute (pnpn_res_cpu.f90:31-180) with h1 = 1/rho,
+        h2 = 0 and no surface terms (no velocity-prescribed or symmetry
+        facets carrying a normal flux)."""
+        ta = self.curl(*ue)
+        wa = self.curl(*ta)
+        bi = self.binv()
+        ta = [f[c] / rho - ((wa[c] * (mu / rho)) * self.B) for c in range(3)]
+        for t in ta:
+            self.gs(t)
+        ta = [t * bi for t in ta]
+        wa = [self.cdtp(ta[c], c) for c in range(3)]
+        pr = self.ax_local_coeffs(p, 1.0 / rho, 0.0)
+        return (-pr) + wa[0] + wa[1] + wa[2]
 
     def ax_local_coeffs(self, u, h1, h2):
         """ax_helm with constant h1, h2 (h2 != 0 adds h2 B u), no gs."""

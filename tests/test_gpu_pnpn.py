@@ -92,3 +92,20 @@ def test_set_coeffs_reaches_solver():
         assert rel < 1e-9, (prec, rel)
     jac = _np(p.field(8))
     assert np.abs(jac - c.jacobi()).max() <= 1e-12 * np.abs(c.jacobi()).max()
+
+
+def test_curl_and_prs_res_match_oracle():
+    print("curl / prs res", flush=True)
+    c, p = _box(6)
+    rng = np.random.default_rng(9)
+    f = [rng.standard_normal(c.n) for _ in range(7)]
+    ue, fv, pr = f[:3], f[3:6], f[6]
+    d = [torch.from_numpy(a).cuda() for a in f]
+    ref = c.curl(*ue)
+    got = p.curl(d[0], d[1], d[2])
+    for g, r in zip(got, ref):
+        assert np.abs(_np(g) - r).max() <= 1e-11 * np.abs(r).max()
+    mu, rho = 1.0 / 1600, 1.2
+    ref = c.prs_res(pr, ue, fv, mu, rho)
+    got = p.prs_res(d[6], d[0:3], d[3:6], mu, rho)
+    assert np.abs(_np(got) - ref).max() <= 1e-10 * np.abs(ref).max()

@@ -4,7 +4,10 @@ B a exactly (the cofactors are jac times the inverse Jacobian, so w3 C^T
 grad_r u = w3 jac grad_x u = B grad_x u); the cofactors equal the closed
 form cross products of the trilinear map's derivatives; the velocity
 residual with u = v = w = 0 and p = 0 is f; its h2 term is h2 B u for a
-constant u (the stiffness part annihilates constants)."""
+constant u (the stiffness part annihilates constants); dudxyz of a linear
+field is its slope; cdtp is the adjoint of opgrad; the curl of a gradient
+vanishes and curl(-y, x, 0) = (0, 0, 2); the pressure residual of a
+curl-free velocity with zero forcing is -A p."""
 import numpy as np
 import pytest
 
@@ -67,3 +70,38 @@ def test_vel_res_special_cases(case):
     h2 = 2.0 * 1.5 / 1e-3
     for rc in res:
         assert np.abs(rc + h2 * c.B * 2.0).max() <= 1e-10 * h2 * np.abs(c.B).max()
+
+
+def test_dudxyz_linear_and_cdtp_adjoint(case):
+    c = case
+    a = (0.3, -1.2, 0.7)
+    lin = a[0] * c.x + a[1] * c.y + a[2] * c.zc
+    for k in range(3):
+        assert np.abs(c.dudxyz(lin, k) - a[k]).max() < 1e-12
+    rng = np.random.default_rng(5)
+    x, y = rng.standard_normal(c.n), rng.standard_normal(c.n)
+    for k in range(3):
+        lhs = np.dot(c.cdtp(x, k), y)
+        rhs = np.dot(x, c.opgrad(y)[k])
+        assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + 1.0)
+
+
+def test_curl_closed_forms(case):
+    c = case
+    zero = np.zeros(c.n)
+    w = c.curl(-c.y, c.x, zero)
+    for q, v in zip(w, (0.0, 0.0, 2.0)):
+        assert np.abs(q - v).max() < 1e-12
+    g = (2 * c.x + c.y, c.x + 3 * c.zc, 3 * c.y)   # grad(x^2 + x y + 3 y z)
+    for q in c.curl(*g):
+        assert np.abs(q).max() < 1e-12
+
+
+def test_prs_res_of_curl_free_field(case):
+    c = case
+    zero = np.zeros(c.n)
+    g = (2 * c.x + c.y, c.x + 3 * c.zc, 3 * c.y)
+    p = np.random.default_rng(6).standard_normal(c.n)
+    r = c.prs_res(p, g, (zero, zero, zero), mu=0.01, rho=2.0)
+    ref = -c.ax_local_coeffs(p, 0.5, 0.0)
+    assert np.abs(r - ref).max() < 1e-10 * np.abs(ref).max()
