@@ -119,6 +119,9 @@ typedef struct {
   double volume;
   double h1c;
   int ifh2;
+  /* facet pairs the face-descriptor gs covers (whole shared faces) and the
+   * pairs it leaves to the pair kernel */
+  int64_t nfaces, nrest;
 } nm_op_info_t;
 
 typedef struct {

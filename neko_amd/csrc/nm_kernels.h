@@ -85,7 +85,17 @@ struct GsArgs {
   int64_t pair0;           // offset of the first facet pair in gidx
   int64_t npairs;
   const int *skip;
-  int variant = 0;         // 0 thread per block/pair, 1 lane-parallel (blocks <= 8)
+  int variant = 0;         // 0 thread per block/pair, 1 lane-parallel (blocks <= 8),
+                           // 2 as 1 with facet pairs from face descriptors
+  // variant 2: whole element faces shared by two elements, one descriptor
+  // each ({element a, element b, code}, code = face a | face b << 3 |
+  // swap << 6 | flip u << 7 | flip v << 8), and the facet pairs no
+  // descriptor covers (pairs of 0-based indices)
+  const int32_t *faces = nullptr;
+  int64_t nfaces = 0;
+  int lx = 0;
+  const int32_t *rest = nullptr;
+  int64_t nrest = 0;
 };
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s);

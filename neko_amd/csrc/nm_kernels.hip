@@ -673,12 +673,68 @@ __global__ void __launch_bounds__(256)
   if (act) u[idx] = v + o;  // a + b == b + a exactly
 }
 
+// Facet pairs of one element face shared by two elements, from a
+// descriptor: thread (d, q) handles interior face point q = (u-1) + L (v-1)
+// of descriptor d; its partner in element b is the point the descriptor's
+// square symmetry maps (u, v) to.  Both get u_a + u_b (== u_b + u_a
+// exactly, so the sum is the same bits as the pair kernel's).  The indices
+// are computed, not loaded: no index traffic and no dependent load.
+__device__ inline int64_t face_point(int f, int u, int v, int lx) {
+  const int fixed = (f & 1) ? lx - 1 : 0;
+  const int dir = f >> 1;
+  const int i = dir == 0 ? fixed : u;
+  const int j = dir == 0 ? u : (dir == 1 ? fixed : v);
+  const int k = dir == 2 ? fixed : v;
+  return i + (int64_t)lx * (j + (int64_t)lx * k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gs_face_kernel(T *__restrict__ u, const int32_t *__restrict__ faces, int64_t nfaces,
+                   int lx, const int *__restrict__ skip, int64_t per) {
+  if (skip != nullptr && *skip) return;
+  const int L = lx - 2, LL = L * L;
+  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
+  const int64_t d = t / LL;
+  if (d >= nfaces) return;
+  const int q = (int)(t - d * LL);
+  const int uu = q % L + 1, vv = q / L + 1;
+  const int32_t ea = faces[3 * d], eb = faces[3 * d + 1], code = faces[3 * d + 2];
+  const int fa = code & 7, fb = (code >> 3) & 7;
+  int ub = (code & 64) ? vv : uu, vb = (code & 64) ? uu : vv;
+  if (code & 128) ub = L + 1 - ub;
+  if (code & 256) vb = L + 1 - vb;
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  const int64_t ia = ea * n3 + face_point(fa, uu, vv, lx);
+  const int64_t ib = eb * n3 + face_point(fb, ub, vb, lx);
+  const T acc = u[ia] + u[ib];
+  u[ia] = acc;
+  u[ib] = acc;
+}
+
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s) {
   const int64_t items = a.nblks + a.npairs;
   if (items <= 0) return NM_OK;
   const int bs = 256;
-  if (a.variant == 1) {
+  if (a.variant == 2) {
+    if (a.nblks > 0) {
+      const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
+      hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per);
+    }
+    if (a.nfaces > 0) {
+      const int64_t LL = (int64_t)(a.lx - 2) * (a.lx - 2);
+      const int64_t per = xcd_per((a.nfaces * LL + bs - 1) / bs);
+      hipLaunchKernelGGL(gs_face_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                         a.u, a.faces, a.nfaces, a.lx, a.skip, per);
+    }
+    if (a.nrest > 0) {
+      const int64_t per = xcd_per((2 * a.nrest + bs - 1) / bs);
+      hipLaunchKernelGGL(gs_pair_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                         a.u, a.rest, (int64_t)0, a.nrest, a.skip, per);
+    }
+  } else if (a.variant == 1) {
     if (a.nblks > 0) {
       const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,

@@ -34,6 +34,9 @@ struct Op {
   // gs (device layout)
   int64_t nlocal = 0, nblks = 0, npairs = 0, pair0 = 0;
   int32_t *blk_ptr = nullptr, *gidx = nullptr;
+  // facet pairs as whole-face descriptors (3 int32 each) + the rest
+  int32_t *gs_faces = nullptr, *gs_rest = nullptr;
+  int64_t nfaces = 0, nrest = 0;
   // Neko-layout copies (1-based) kept for the drop-in gather/scatter tests
   std::vector<int32_t> h_dg, h_gd, h_blk_len, h_blk_off;
   int64_t facet_offset = 0;
@@ -110,6 +113,10 @@ int op_finish(Op *op, hipStream_t s);
 int op_jacobi(Op *op, double *dd, hipStream_t s);
 // Regroup a device gs layout (blk_ptr/gidx, pairs from pair0) by the element
 // of each block's / pair's lowest dof; the order inside blocks is kept.
+// Facet pairs (0-based index pairs) -> whole-face descriptors (3 int32:
+// element a, element b, code) + the pairs no descriptor covers.
+void gs_face_pairs(int64_t nel, int lx, const int32_t *pairs, int64_t npairs,
+                   std::vector<int32_t> &faces, std::vector<int32_t> &rest);
 int gs_regroup(int64_t nel, int64_t n3, int64_t nblks, int64_t pair0, int64_t npairs,
                std::vector<int32_t> &ptr, std::vector<int32_t> &gi);
 

@@ -5,6 +5,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "nm_common.h"
@@ -74,7 +75,14 @@ template int32_t *Op::alloc<int32_t>(int64_t);
 template <typename T>
 int Op::gs(T *u, const int *skip, hipStream_t s) const {
   GsArgs<T> a{u, blk_ptr, gidx, nblks, pair0, npairs, skip};
-  a.variant = (gs_variant == 1 && max_blk <= 8) ? 1 : 0;
+  a.variant = (gs_variant >= 1 && max_blk <= 8) ? gs_variant : 0;
+  if (a.variant == 2) {
+    a.faces = gs_faces;
+    a.nfaces = nfaces;
+    a.lx = lx;
+    a.rest = gs_rest;
+    a.nrest = nrest;
+  }
   return launch_gs<T>(a, s);
 }
 template int Op::gs<double>(double *, const int *, hipStream_t) const;
@@ -249,6 +257,94 @@ int gs_regroup(int64_t nel, int64_t n3, int64_t nblks, int64_t pair0, int64_t np
   return NM_OK;
 }
 
+// Whole element faces among the facet pairs (0-based local index pairs):
+// the (lx-2)^2 pairs between face fa of element ea and face fb of element
+// eb form one descriptor when one of the 8 symmetries of the square maps
+// every point (u, v) of face a (1..lx-2 in the face's in-plane element
+// coordinates) to its partner on face b.  Pairs that form no such face
+// (none on conforming meshes) stay explicit in `rest`.  Descriptors keep
+// the order of their first pair (the locality-regrouped order).
+void gs_face_pairs(int64_t nel, int lx, const int32_t *pairs, int64_t npairs,
+                   std::vector<int32_t> &faces, std::vector<int32_t> &rest) {
+  (void)nel;
+  const int L = lx - 2, l1 = lx - 1;
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  faces.clear();
+  rest.clear();
+  if (L < 1) {
+    rest.assign(pairs, pairs + 2 * npairs);
+    return;
+  }
+  // face and in-face coordinates of a local point (interior face points)
+  auto locate = [&](int64_t p, int &f, int &u, int &v) {
+    const int i = (int)(p % lx), j = (int)((p / lx) % lx), k = (int)(p / ((int64_t)lx * lx));
+    const int bi = (i == 0 || i == l1), bj = (j == 0 || j == l1), bk = (k == 0 || k == l1);
+    if (bi + bj + bk != 1) return false;
+    if (bi) { f = (i == l1); u = j; v = k; }
+    else if (bj) { f = 2 + (j == l1); u = i; v = k; }
+    else { f = 4 + (k == l1); u = i; v = j; }
+    return true;
+  };
+  struct Group { int64_t ea, eb; int fa, fb; std::vector<int64_t> idx; };
+  std::vector<Group> groups;
+  std::unordered_map<uint64_t, size_t> key2g;
+  for (int64_t q = 0; q < npairs; q++) {
+    int64_t a = pairs[2 * q], b = pairs[2 * q + 1];
+    if (a > b) std::swap(a, b);
+    int fa, ua, va, fb, ub, vb;
+    const int64_t ea = a / n3, eb = b / n3;
+    if (!locate(a % n3, fa, ua, va) || !locate(b % n3, fb, ub, vb)) {
+      rest.push_back(pairs[2 * q]);
+      rest.push_back(pairs[2 * q + 1]);
+      continue;
+    }
+    const uint64_t key = ((uint64_t)ea * 6 + fa) * 6 * (uint64_t)nel + (uint64_t)eb * 6 + fb;
+    auto it = key2g.find(key);
+    size_t g;
+    if (it == key2g.end()) {
+      g = groups.size();
+      key2g.emplace(key, g);
+      groups.push_back(Group{ea, eb, fa, fb, {}});
+    } else {
+      g = it->second;
+    }
+    groups[g].idx.push_back(q);
+  }
+  for (const Group &G : groups) {
+    int code = -1;
+    if ((int64_t)G.idx.size() == (int64_t)L * L) {
+      for (int sym = 0; sym < 8 && code < 0; sym++) {
+        bool ok = true;
+        std::vector<char> seen((size_t)L * L, 0);
+        for (int64_t q : G.idx) {
+          int64_t a = pairs[2 * q], b = pairs[2 * q + 1];
+          if (a > b) std::swap(a, b);
+          int fa, ua, va, fb, ub, vb;
+          locate(a % n3, fa, ua, va);
+          locate(b % n3, fb, ub, vb);
+          int tu = (sym & 1) ? va : ua, tv = (sym & 1) ? ua : va;
+          if (sym & 2) tu = L + 1 - tu;
+          if (sym & 4) tv = L + 1 - tv;
+          const size_t slot = (size_t)(ua - 1) + (size_t)L * (va - 1);
+          if (tu != ub || tv != vb || seen[slot]) { ok = false; break; }
+          seen[slot] = 1;
+        }
+        if (ok) code = G.fa | (G.fb << 3) | ((sym & 1) << 6) | ((sym & 2) << 6) | ((sym & 4) << 6);
+      }
+    }
+    if (code >= 0) {
+      faces.push_back((int32_t)G.ea);
+      faces.push_back((int32_t)G.eb);
+      faces.push_back(code);
+    } else {
+      for (int64_t q : G.idx) {
+        rest.push_back(pairs[2 * q]);
+        rest.push_back(pairs[2 * q + 1]);
+      }
+    }
+  }
+}
+
 int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
               const int32_t *dg, const int32_t *gd, const int32_t *blk_len,
               const int32_t *blk_off, hipStream_t s) {
@@ -280,6 +376,21 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
     const int st = gs_regroup(op->nel, (int64_t)op->lx * op->lx * op->lx, nblks, op->pair0,
                               op->npairs, ptr, gi);
     if (st) return st;
+  }
+  {
+    std::vector<int32_t> faces, rest;
+    gs_face_pairs(op->nel, op->lx, gi.data() + op->pair0, op->npairs, faces, rest);
+    op->nfaces = (int64_t)faces.size() / 3;
+    op->nrest = (int64_t)rest.size() / 2;
+    NM_ALLOC(op->gs_faces, int32_t, faces.size());
+    NM_ALLOC(op->gs_rest, int32_t, rest.size());
+    if (!faces.empty())
+      NM_HIP_TRY(hipMemcpyAsync(op->gs_faces, faces.data(), sizeof(int32_t) * faces.size(),
+                                hipMemcpyHostToDevice, s));
+    if (!rest.empty())
+      NM_HIP_TRY(hipMemcpyAsync(op->gs_rest, rest.data(), sizeof(int32_t) * rest.size(),
+                                hipMemcpyHostToDevice, s));
+    NM_HIP_TRY(hipStreamSynchronize(s));
   }
   NM_ALLOC(op->blk_ptr, int32_t, nblks + 1);
   NM_ALLOC(op->gidx, int32_t, nlocal);
@@ -664,6 +775,8 @@ int nm_op_info(const nm_op_t *h, nm_op_info_t *info) {
   info->volume = op.volume;
   info->h1c = op.h1c;
   info->ifh2 = op.ifh2;
+  info->nfaces = op.nfaces;
+  info->nrest = op.nrest;
   return NM_OK;
 }
 
@@ -837,8 +950,8 @@ int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
   op.epb32 = c32;
   float bg = 1e30f;
   int cg = 0;
-  for (int v = 0; v < 2; v++) {
-    if (v == 1 && op.max_blk > 8) continue;
+  for (int v = 0; v < 3; v++) {
+    if (v >= 1 && op.max_blk > 8) continue;
     op.gs_variant = v;
     const float tg = timeit([&] { op.gs<double>(w, nullptr, s); });
     if (tg < bg) { bg = tg; cg = v; }

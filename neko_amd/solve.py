@@ -27,7 +27,18 @@ FIELD_B, FIELD_MULT, FIELD_DINV, FIELD_X, FIELD_Y, FIELD_Z = range(6, 12)
 FIELD_H2B, FIELD_JAC = 12, 13
 
 
-def _ptr(t: torch.Tensor) -> int:
+def _ptr(t: torch.Tensor, n: int | None = None, dtype=torch.float64) -> int:
+    """Device address of a vector handed to the C ABI, which cannot check
+    what it receives: a contiguous device tensor of the expected dtype and
+    length (a wrong dtype or size would read and write outside it)."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("expected a device (cuda) tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"expected {n} entries, got {t.numel()}")
     return t.data_ptr()
 
 
@@ -171,7 +182,7 @@ class Problem:
     def opgrad(self, u: torch.Tensor):
         """Weak gradient (ux, uy, uz) of u (Neko's opgrad), local."""
         out = [torch.empty_like(u) for _ in range(3)]
-        check(lib().nm_opgrad(self.h, *[_ptr(o) for o in out], _ptr(u), _stream()),
+        check(lib().nm_opgrad(self.h, *[_ptr(o, self.n) for o in out], _ptr(u, self.n), _stream()),
               "nm_opgrad")
         return out
 
@@ -180,15 +191,17 @@ class Problem:
         (-A u - opgrad_x p + f_x, ...) (nm_vel_res), local."""
         out = [torch.empty_like(u) for _ in range(3)]
         check(lib().nm_vel_res(self.h, float(mu), float(rho), float(bd), float(dt),
-                               _ptr(u), _ptr(v), _ptr(w), _ptr(p),
-                               *[_ptr(q) for q in f], *[_ptr(o) for o in out],
+                               _ptr(u, self.n), _ptr(v, self.n), _ptr(w, self.n),
+                               _ptr(p, self.n), *[_ptr(q, self.n) for q in f],
+                               *[_ptr(o, self.n) for o in out],
                                _stream()), "nm_vel_res")
         return out
 
     def curl(self, u1, u2, u3):
         """Neko's curl: Binv gs(B curl u) (nm_curl)."""
         out = [torch.empty_like(u1) for _ in range(3)]
-        check(lib().nm_curl(self.h, *[_ptr(o) for o in out], _ptr(u1), _ptr(u2), _ptr(u3),
+        check(lib().nm_curl(self.h, *[_ptr(o, self.n) for o in out], _ptr(u1, self.n),
+                            _ptr(u2, self.n), _ptr(u3, self.n),
                             _stream()), "nm_curl")
         return out
 
@@ -196,8 +209,9 @@ class Problem:
         """pnpn pressure residual without surface terms (nm_prs_res):
         sets h1 = 1/rho, h2 = 0 and returns -A p + sum_c cdtp(ta_c)."""
         out = torch.empty_like(p)
-        check(lib().nm_prs_res(self.h, float(mu), float(rho), _ptr(p),
-                               *[_ptr(q) for q in ue], *[_ptr(q) for q in f], _ptr(out),
+        check(lib().nm_prs_res(self.h, float(mu), float(rho), _ptr(p, self.n),
+                               *[_ptr(q, self.n) for q in ue], *[_ptr(q, self.n) for q in f],
+                               _ptr(out, self.n),
                                _stream()), "nm_prs_res")
         return out
 
@@ -245,7 +259,7 @@ class Problem:
         (nm_op_rhs_normal; equal to it up to last-ulp differences of the
         device log/cos)."""
         f = torch.empty(self.n, dtype=torch.float64, device="cuda")
-        check(lib().nm_op_rhs_normal(self.h, int(seed), _ptr(f), _stream()),
+        check(lib().nm_op_rhs_normal(self.h, int(seed), _ptr(f, self.n), _stream()),
               "nm_op_rhs_normal")
         return f
 
@@ -289,35 +303,37 @@ class Problem:
     def ax_local(self, w: torch.Tensor, u: torch.Tensor):
         """ax_t%compute: w = A_local u (no assembly)."""
         if u.dtype == torch.float32:
-            check(lib().nm_op_ax_local32(self.h, _ptr(w), _ptr(u), _stream()),
+            check(lib().nm_op_ax_local32(self.h, _ptr(w, self.n, torch.float32),
+                                          _ptr(u, self.n, torch.float32), _stream()),
                   "ax_local32")
         else:
-            check(lib().nm_op_ax_local(self.h, _ptr(w), _ptr(u), _stream()),
+            check(lib().nm_op_ax_local(self.h, _ptr(w, self.n), _ptr(u, self.n), _stream()),
                   "ax_local")
         return w
 
     def ax_gs(self, w, u):
         """w = gs(A_local u): the Ax+gs unit of work."""
         if u.dtype == torch.float32:
-            check(lib().nm_op_ax_gs32(self.h, _ptr(w), _ptr(u), _stream()), "ax_gs32")
+            check(lib().nm_op_ax_gs32(self.h, _ptr(w, self.n, torch.float32),
+                                       _ptr(u, self.n, torch.float32), _stream()), "ax_gs32")
         else:
-            check(lib().nm_op_ax_gs(self.h, _ptr(w), _ptr(u), _stream()), "ax_gs")
+            check(lib().nm_op_ax_gs(self.h, _ptr(w, self.n), _ptr(u, self.n), _stream()), "ax_gs")
         return w
 
     def ax_full(self, w, u):
         """Ax%compute + gs_h%op(ADD) + blst%apply (cg.f90:207-209)."""
-        check(lib().nm_op_ax(self.h, _ptr(w), _ptr(u), _stream()), "ax")
+        check(lib().nm_op_ax(self.h, _ptr(w, self.n), _ptr(u, self.n), _stream()), "ax")
         return w
 
     def gs_op(self, u, op=GS_OP_ADD):
         if op != GS_OP_ADD:
             raise NotImplementedError("handle gs is ADD only; use the drop-in "
                                       "hip_gather_kernel for MUL/MIN/MAX")
-        check(lib().nm_op_gs(self.h, _ptr(u), _stream()), "gs")
+        check(lib().nm_op_gs(self.h, _ptr(u, self.n), _stream()), "gs")
         return u
 
     def bc_apply(self, u):
-        check(lib().nm_op_mask(self.h, _ptr(u), _stream()), "mask")
+        check(lib().nm_op_mask(self.h, _ptr(u, self.n), _stream()), "mask")
         return u
 
     def rhs_normal(self, seed: int) -> torch.Tensor:
@@ -357,7 +373,8 @@ class CG:
     def solve(self, x: torch.Tensor, f: torch.Tensor, abs_tol: float,
               max_iter: int = 100000, inner_rel: float = 0.0) -> KspMonitor:
         mon = Monitor()
-        check(lib().nm_cg_solve(self.h, _ptr(x), _ptr(f), float(abs_tol),
+        n = self.problem.n
+        check(lib().nm_cg_solve(self.h, _ptr(x, n), _ptr(f, n), float(abs_tol),
                                 int(max_iter), float(inner_rel), C.byref(mon),
                                 _stream()), "nm_cg_solve")
         return KspMonitor(mon.iter, mon.res_start, mon.res_final,
@@ -389,11 +406,11 @@ class Projection:
                                    int(activ_step)), "nm_proj_create")
 
     def pre(self, b: torch.Tensor, tstep: int, dt_last_change: int = -1):
-        check(lib().nm_proj_pre(self.h, _ptr(b), int(tstep), int(dt_last_change),
+        check(lib().nm_proj_pre(self.h, _ptr(b, self.problem.n), int(tstep), int(dt_last_change),
                                 _stream()), "nm_proj_pre")
 
     def post(self, x: torch.Tensor, tstep: int, dt_last_change: int = -1):
-        check(lib().nm_proj_post(self.h, _ptr(x), int(tstep), int(dt_last_change),
+        check(lib().nm_proj_post(self.h, _ptr(x, self.problem.n), int(tstep), int(dt_last_change),
                                  _stream()), "nm_proj_post")
 
     def clear(self):
@@ -433,12 +450,14 @@ class HSMG:
 
     def apply(self, z: torch.Tensor, r: torch.Tensor):
         """z = M^{-1} r (pc_hsmg.f90:481-627); r is left untouched."""
-        check(lib().nm_hsmg_apply(self.h, _ptr(z), _ptr(r), _stream()),
+        check(lib().nm_hsmg_apply(self.h, _ptr(z, self.problem.n), _ptr(r, self.problem.n),
+                                   _stream()),
               "nm_hsmg_apply")
 
     def schwarz_fine(self, e: torch.Tensor, r: torch.Tensor):
         """e = schwarz(r) on the fine level; r is masked in place."""
-        check(lib().nm_hsmg_schwarz_fine(self.h, _ptr(e), _ptr(r), _stream()),
+        check(lib().nm_hsmg_schwarz_fine(self.h, _ptr(e, self.problem.n),
+                                          _ptr(r, self.problem.n), _stream()),
               "nm_hsmg_schwarz_fine")
 
     def free(self):
@@ -471,7 +490,8 @@ class PHMG:
         self.h = h
 
     def apply(self, z: torch.Tensor, r: torch.Tensor):
-        check(lib().nm_phmg_apply(self.h, _ptr(z), _ptr(r), _stream()),
+        check(lib().nm_phmg_apply(self.h, _ptr(z, self.problem.n), _ptr(r, self.problem.n),
+                                   _stream()),
               "nm_phmg_apply")
 
     def eigs(self):
@@ -517,7 +537,8 @@ class GMRES:
     def solve(self, x: torch.Tensor, f: torch.Tensor, abs_tol: float,
               max_iter: int = 10000) -> KspMonitor:
         mon = Monitor()
-        check(lib().nm_gmres_solve(self.h, _ptr(x), _ptr(f), float(abs_tol),
+        n = self.problem.n
+        check(lib().nm_gmres_solve(self.h, _ptr(x, n), _ptr(f, n), float(abs_tol),
                                    int(max_iter), C.byref(mon), _stream()),
               "nm_gmres_solve")
         return KspMonitor(mon.iter, mon.res_start, mon.res_final,
@@ -557,8 +578,8 @@ class Batch:
 
     def solve(self, xs, fs, rel_tol=1e-10, max_iter=20000, delta=0.0):
         k = len(self.problems)
-        xa = (C.c_void_p * k)(*[_ptr(x) for x in xs])
-        fa = (C.c_void_p * k)(*[_ptr(f) for f in fs])
+        xa = (C.c_void_p * k)(*[_ptr(x, p.n) for x, p in zip(xs, self.problems)])
+        fa = (C.c_void_p * k)(*[_ptr(f, p.n) for f, p in zip(fs, self.problems)])
         mons = (Monitor * k)()
         check(lib().nm_batch_solve(self.h, xa, fa, float(rel_tol), int(max_iter),
                                    float(delta), mons, _stream()), "nm_batch_solve")

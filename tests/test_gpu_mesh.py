@@ -117,3 +117,20 @@ def test_pc_switch_is_exact():
     xj2, mj2 = run()
     assert mj2.iter == mj.iter and torch.equal(xj2, xj)
     assert mi.iter != mj.iter
+
+
+@pytest.mark.parametrize("name", ["poisson_128.nmsh", "hemi.nmsh", "lid.nmsh",
+                                  "tgv_512.nmsh"])
+def test_mesh_gs_variants_bit_exact(name):
+    """All gs variants, including whole-face descriptors with the square
+    symmetries an unstructured mesh needs, give the oracle's bits."""
+    print("mesh gs", name, flush=True)
+    c, p = _pair(name, 5)
+    assert p.info.nrest == 0 and p.info.nfaces * 9 == p.info.npairs
+    u = np.random.default_rng(4).standard_normal(c.n)
+    ref = c.gs(u.copy())
+    for v in (0, 1, 2):
+        p.set_gs_variant(v)
+        ud = torch.from_numpy(u.copy()).cuda()
+        p.gs_op(ud)
+        assert np.array_equal(_np(ud), ref), v

@@ -80,13 +80,16 @@ def test_ax_h1_field_and_h2(lx):
     assert _relmax(_np(w), c.ax_local(u)) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("case", CASES[:7])
 def test_gs_bit_exact(case, variant):
     nx, ny, nz, lx, amp, per = case
     c = O.OracleCase(nx, ny, nz, lx, amp=amp, seed=3, periodic=per)
     p = _problem_from_oracle(c)
     p.set_gs_variant(variant)
+    if variant == 2 and lx > 2:
+        # every facet pair of a conforming box lies on a whole shared face
+        assert p.info.nrest == 0 and p.info.nfaces * (lx - 2) ** 2 == p.info.npairs
     u = np.random.default_rng(1).standard_normal(c.n)
     ud = _t(u)
     p.gs_op(ud)
@@ -315,3 +318,18 @@ def test_cg_fused_pap_helmholtz_periodic():
     assert mon.converged
     assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
     assert err < 1e-9
+
+
+def test_wrong_dtype_or_size_is_rejected_on_the_host():
+    """The C ABI takes raw pointers; the Python layer refuses a vector of
+    the wrong dtype or length before any kernel sees it (an FP32 buffer
+    given to the FP64 gs would be read and written past its end)."""
+    from neko_amd.solve import Problem
+    p = Problem.box(2, 2, 2, 4)
+    with pytest.raises(TypeError):
+        p.gs_op(torch.zeros(p.n, dtype=torch.float32, device="cuda"))
+    with pytest.raises(ValueError):
+        p.gs_op(torch.zeros(p.n - 1, dtype=torch.float64, device="cuda"))
+    with pytest.raises(TypeError):
+        p.ax_local(torch.zeros(p.n, dtype=torch.float64, device="cuda"),
+                   torch.zeros(p.n, dtype=torch.float32, device="cuda"))
