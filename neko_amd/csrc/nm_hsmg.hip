@@ -513,15 +513,59 @@ static void host_gs(void *ctx, double *u) {
   (void)dg;
 }
 
+// Level operators and the Schwarz extended gs on the fine operator's mesh:
+// a box (nm_op_create_box) or a general mesh (nm_op_create_mesh, whose
+// topology the operator keeps).
+static bool has_topology(const Op *op) { return op->box[0] != 0 || !op->h_pid.empty(); }
+
+static int level_op(const Op *fine, int lx, double h1, double h2, nm_op_t **out,
+                    hipStream_t s) {
+  if (fine->box[0] != 0) {
+    const int *b = fine->box, *p = fine->per;
+    return nm_op_create_box(out, b[0], b[1], b[2], lx, p[0], p[1], p[2], fine->amp,
+                            fine->seed, h1, h2, s);
+  }
+  return nm_op_create_mesh(out, fine->nel, lx, fine->h_vxyz.data(), fine->h_pid.data(), h1,
+                           h2, s);
+}
+
+// 1 where element facet e*6+f has a neighbour (box lattice or mesh)
+static std::vector<char> facet_has_neighbour(const Op *op) {
+  const int64_t nel = op->nel;
+  std::vector<char> nb(nel * 6, 0);
+  if (op->box[0] == 0) {
+    for (int64_t q = 0; q < nel * 6; q++) nb[q] = op->h_fneigh[q] != 0;
+    return nb;
+  }
+  const int64_t nx_ = op->box[0], ny_ = op->box[1], nz_ = op->box[2];
+  for (int64_t e = 0; e < nel; e++) {
+    const int64_t ex = e % nx_, ey = (e / nx_) % ny_, ez = e / (nx_ * ny_);
+    const bool f[6] = {op->per[0] || ex > 0, op->per[0] || ex < nx_ - 1,
+                       op->per[1] || ey > 0, op->per[1] || ey < ny_ - 1,
+                       op->per[2] || ez > 0, op->per[2] || ez < nz_ - 1};
+    for (int q = 0; q < 6; q++) nb[e * 6 + q] = f[q];
+  }
+  return nb;
+}
+
 template <typename H>
-static int gsdev_box(H *own, GsDev *g, const int box[3], const int per[3], int lx,
-                     hipStream_t s) {
+static int gsdev_level(H *own, GsDev *g, const Op *op, int lx, hipStream_t s) {
   int64_t nl, fo, nb;
-  int st = nm_box_gs_sizes(box[0], box[1], box[2], lx, per[0], per[1], per[2], &nl, &fo);
-  if (st) return st;
-  std::vector<int32_t> dg(nl), gd(nl), bl(fo), bo(fo);
-  st = nm_box_gs_map(box[0], box[1], box[2], lx, per[0], per[1], per[2], dg.data(),
-                     gd.data(), bl.data(), bo.data(), &nl, &fo, &nb);
+  const int *box = op->box, *per = op->per;
+  int st;
+  std::vector<int32_t> dg, gd, bl, bo;
+  if (box[0] != 0) {
+    if ((st = nm_box_gs_sizes(box[0], box[1], box[2], lx, per[0], per[1], per[2], &nl, &fo)))
+      return st;
+    dg.resize(nl); gd.resize(nl); bl.resize(fo); bo.resize(fo);
+    st = nm_box_gs_map(box[0], box[1], box[2], lx, per[0], per[1], per[2], dg.data(),
+                       gd.data(), bl.data(), bo.data(), &nl, &fo, &nb);
+  } else {
+    if ((st = nm_mesh_gs_sizes(op->nel, lx, op->h_fneigh.data(), &nl, &fo))) return st;
+    dg.resize(nl); gd.resize(nl); bl.resize(fo); bo.resize(fo);
+    st = nm_mesh_gs_map(op->nel, lx, op->h_pid.data(), op->h_fneigh.data(), dg.data(),
+                        gd.data(), bl.data(), bo.data(), &nl, &fo, &nb);
+  }
   if (st) return st;
   std::vector<int32_t> ptr(nb + 1), gi(nl);
   g->max_blk = 0;
@@ -536,8 +580,7 @@ static int gsdev_box(H *own, GsDev *g, const int box[3], const int per[3], int l
   g->npairs = (nl - (fo - 1)) / 2;
   ptr.resize(nb + 1);
   {
-    const int64_t nel = (int64_t)box[0] * box[1] * box[2];
-    const int st2 = gs_regroup(nel, (int64_t)lx * lx * lx, nb, g->pair0, g->npairs, ptr, gi);
+    const int st2 = gs_regroup(op->nel, (int64_t)lx * lx * lx, nb, g->pair0, g->npairs, ptr, gi);
     if (st2) return st2;
   }
   if (!(g->blk_ptr = own->template alloc<int32_t>(nb + 1)) ||
@@ -559,7 +602,7 @@ static int schwarz_setup(H *h, Schwarz *sw, const Op *op, int facet_bc, hipStrea
   sw->nl = nl;
   sw->nel = nel;
   sw->op = op;
-  int st = gsdev_box(h, &sw->ext, op->box, op->per, nl, s);
+  int st = gsdev_level(h, &sw->ext, op, nl, s);
   if (st) return st;
   const int64_t ns = nel * nl * nl * nl;
   if (!(sw->S = h->template alloc<double>(nel * 6 * nl * nl)) || !(sw->D = h->template alloc<double>(ns)) ||
@@ -577,13 +620,9 @@ static int schwarz_setup(H *h, Schwarz *sw, const Op *op, int facet_bc, hipStrea
   nm_gll(lx, zg.data(), wg.data(), d1.data(), dt.data());
   // facet types: 0 where a neighbour exists, facet_bc on the boundary
   std::vector<int> ft(nel * 6);
-  const int64_t nx_ = op->box[0], ny_ = op->box[1], nz_ = op->box[2];
-  for (int64_t e = 0; e < nel; e++) {
-    const int64_t ex = e % nx_, ey = (e / nx_) % ny_, ez = e / (nx_ * ny_);
-    const bool nb[6] = {op->per[0] || ex > 0, op->per[0] || ex < nx_ - 1,
-                        op->per[1] || ey > 0, op->per[1] || ey < ny_ - 1,
-                        op->per[2] || ez > 0, op->per[2] || ez < nz_ - 1};
-    for (int f = 0; f < 6; f++) ft[e * 6 + f] = nb[f] ? 0 : facet_bc;
+  {
+    const std::vector<char> nb = facet_has_neighbour(op);
+    for (int64_t q = 0; q < nel * 6; q++) ft[q] = nb[q] ? 0 : facet_bc;
   }
   std::vector<double> hS(nel * 6 * nl * nl), hD(ns);
   HostGs hg{op};
@@ -852,15 +891,17 @@ struct nm_gmres_s;
 extern "C" {
 
 
-// hsmg on a box operator created by nm_op_create_box.  facet_bc: Neko's
-// facet_type of the boundary facets (1 strong Dirichlet as set for
-// pressure, 2 as set for velocity; fluid_pnpn_bc_fctry.f90:162,259).
+// hsmg on an operator built on a box (nm_op_create_box) or a general mesh
+// (nm_op_create_mesh).  facet_bc: Neko's facet_type of the boundary facets
+// (1 strong Dirichlet as set for pressure, 2 as set for velocity;
+// fluid_pnpn_bc_fctry.f90:162,259).
 int nm_hsmg_create(nm_hsmg_t **out, nm_op_t *fine_h, int facet_bc, int niter,
                    hipStream_t s) {
   *out = nullptr;
   Op *fine = &fine_h->op;
-  if (fine->box[0] == 0) {
-    set_error("nm_hsmg_create: operator was not built by nm_op_create_box");
+  if (!has_topology(fine)) {
+    set_error("nm_hsmg_create: operator has no mesh topology (build it with "
+              "nm_op_create_box or nm_op_create_mesh)");
     return NM_ERR_ARG;
   }
   const int lf = fine->lx;
@@ -877,15 +918,10 @@ int nm_hsmg_create(nm_hsmg_t **out, nm_op_t *fine_h, int facet_bc, int niter,
     delete hh;
     return c;
   };
-  // level operators on the same mesh (the amplitude/seed live in the
-  // mesh: rebuild it with the same parameters)
-  const int *b = fine->box, *p = fine->per;
-  int st = nm_op_create_box(&h->mid_h, b[0], b[1], b[2], lm, p[0], p[1], p[2], fine->amp,
-                            fine->seed, 1.0, 0.0, s);
+  // level operators on the same mesh
+  int st = level_op(fine, lm, 1.0, 0.0, &h->mid_h, s);
   if (st) return fail(st);
-  st = nm_op_create_box(&h->crs_h, b[0], b[1], b[2], lc, p[0], p[1], p[2], fine->amp,
-                        fine->seed, fine->h1c, 0.0, s);
-  if (st) return fail(st);
+  if ((st = level_op(fine, lc, fine->h1c, 0.0, &h->crs_h, s))) return fail(st);
   h->mid = &h->mid_h->op;
   h->crs = &h->crs_h->op;
   if ((st = nm_op_set_all_deformed(h->crs_h, s))) return fail(st);
@@ -956,8 +992,9 @@ int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_it
                    int level_h2_fine, int smoother_fp32, hipStream_t s) {
   *out = nullptr;
   Op *fine = &fine_h->op;
-  if (fine->box[0] == 0) {
-    set_error("nm_phmg_create: operator was not built by nm_op_create_box");
+  if (!has_topology(fine)) {
+    set_error("nm_phmg_create: operator has no mesh topology (build it with "
+              "nm_op_create_box or nm_op_create_mesh)");
     return NM_ERR_ARG;
   }
   const int lf = fine->lx;
@@ -980,13 +1017,9 @@ int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_it
   // level operators: h1 and ifh2 of the fine level; h2 = 1 (Neko's coef
   // default, phmg.f90:252-254 copies h1 and ifh2 only) or the fine h2
   const double h2l = fine->ifh2 ? (level_h2_fine ? fine->h2c : 1.0) : 0.0;
-  const int *b = fine->box, *p = fine->per;
-  int st = nm_op_create_box(&h->mid_h, b[0], b[1], b[2], lm, p[0], p[1], p[2], fine->amp,
-                            fine->seed, fine->h1c, h2l, s);
+  int st = level_op(fine, lm, fine->h1c, h2l, &h->mid_h, s);
   if (st) return fail(st);
-  if ((st = nm_op_create_box(&h->crs_h, b[0], b[1], b[2], lc, p[0], p[1], p[2], fine->amp,
-                             fine->seed, fine->h1c, h2l, s)))
-    return fail(st);
+  if ((st = level_op(fine, lc, fine->h1c, h2l, &h->crs_h, s))) return fail(st);
   h->lev[0] = fine;
   h->lev[1] = &h->mid_h->op;
   h->lev[2] = &h->crs_h->op;

@@ -60,6 +60,11 @@ struct Op {
   std::vector<void *> owned;    // every device allocation of this op
   int64_t bytes = 0;            // their total size
   int box[3] = {0, 0, 0}, per[3] = {0, 0, 0};
+  // general-mesh operators (nm_op_create_mesh): the topology the
+  // multigrid levels and the Schwarz extended gs are rebuilt from
+  std::vector<double> h_vxyz;
+  std::vector<int64_t> h_pid;
+  std::vector<int32_t> h_fneigh;
   bool trilinear = false;  // straight-edged elements: geometry in-kernel
   // diagonal preconditioner of the CG/GMRES solvers: 0 Jacobi (f*.dinv
   // holds 1/diag), 1 identity (f*.dinv holds ones); alt* keep the other

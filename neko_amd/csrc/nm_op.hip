@@ -676,6 +676,9 @@ int nm_op_create_mesh(nm_op_t **out, int64_t nel, int lx, const double *vxyz,
   int64_t m = 0;
   nm_mesh_mask(nel, lx, fneigh.data(), msk.data(), &m);
   std::vector<double> v(vxyz, vxyz + nel * 24);
+  op->h_vxyz = v;
+  op->h_pid.assign(pid, pid + nel * 8);
+  op->h_fneigh = fneigh;
   if ((st = op_build_linear(h, nel, lx, v, h1, h2, nlocal, foff, nb, msk, m, s))) return st;
   *out = h;
   return NM_OK;

@@ -185,6 +185,14 @@ class OracleCase:
                  h1, h2)
         return c
 
+    def same_mesh(self, lx, h1=1.0, h2=0.0):
+        """The problem on this case's mesh at another order (multigrid
+        levels, the Schwarz extended space)."""
+        if self.nelx is not None:
+            return OracleCase(self.nelx, self.nely, self.nelz, lx, amp=self.amp,
+                              seed=self.seed, periodic=self.periodic, h1=h1, h2=h2)
+        return OracleCase.from_mesh(self.vxyz, self.pid, lx, h1=h1, h2=h2)
+
     def _setup(self, vxyz, pid, lx, h1, h2):
         L = lib()
         self.lx = lx
@@ -579,13 +587,11 @@ class OracleHsmg:
         else:
             lm = 4
         lc = 2
-        kw = dict(amp=case.amp, seed=case.seed, periodic=case.periodic)
-        dims = (case.nelx, case.nely, case.nelz)
-        self.mid = OracleCase(*dims, lm, **kw)
-        self.crs = OracleCase(*dims, lc, h1=float(case.h1[0]), h2=0.0, **kw)
+        self.mid = case.same_mesh(lm)
+        self.crs = case.same_mesh(lc, h1=float(case.h1[0]), h2=0.0)
         self.crs.dfrmd[:] = 1  # hsmg_init: msh%all_deformed()
-        self.ext_f = OracleCase(*dims, lf + 2, **kw)
-        self.ext_m = OracleCase(*dims, lm + 2, **kw)
+        self.ext_f = case.same_mesh(lf + 2)
+        self.ext_m = case.same_mesh(lm + 2)
         nel = case.nel
         self.lf, self.lm, self.lc = lf, lm, lc
         ft = np.where(case.fneigh != 0, 0, facet_bc).astype(np.int32)
@@ -749,10 +755,8 @@ class OraclePhmg:
             h2l = float(case.h2[0]) if case.ifh2 else 0.0
         else:
             h2l = 1.0 if case.ifh2 else 0.0
-        kw = dict(amp=case.amp, seed=case.seed, periodic=case.periodic, h1=h1c, h2=h2l)
-        dims = (case.nelx, case.nely, case.nelz)
-        self.mid = OracleCase(*dims, hs.lm, **kw)
-        self.crs = OracleCase(*dims, hs.lc, **kw)
+        self.mid = case.same_mesh(hs.lm, h1=h1c, h2=h2l)
+        self.crs = case.same_mesh(hs.lc, h1=h1c, h2=h2l)
         self.pc_crs = self.crs.jacobi()
         self.ch = [OracleCheby(case, hs.sw_f, seed, power_its),
                    OracleCheby(self.mid, hs.sw_m, seed, power_its)]

@@ -246,3 +246,56 @@ def test_phmg_fp32_smoothers(case):
     assert mon.converged
     assert abs(mon.iter - m_or.iter) <= 2, (mon.iter, m_or.iter)
     assert np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or) < 1e-9
+
+
+# --- the same preconditioners on the reference's own meshes -----------------
+def _mesh_pair(name, lx, h1=1.0, h2=0.0):
+    import os
+    from neko_amd.solve import Problem, read_nmsh
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    v, pid, _ = read_nmsh(os.path.join(root, "tests", "golden", "nmsh", name))
+    return (O.OracleCase.from_mesh(v, pid, lx, h1=h1, h2=h2),
+            Problem.mesh(v, pid, lx, h1=h1, h2=h2))
+
+
+@pytest.mark.parametrize("name,lx", [("hemi.nmsh", 5), ("lid.nmsh", 6)])
+def test_hsmg_on_reference_mesh(name, lx):
+    """Schwarz on the unstructured hemisphere mesh (and the periodic lid
+    mesh): levels and the extended (lx+2) gs come from the mesh topology;
+    one application within 1e-11, GMRES + hsmg iterations +-1 and the
+    solution within 1e-9 of the oracle's."""
+    from neko_amd.solve import GMRES, HSMG
+    print("hsmg mesh", name, lx, flush=True)
+    c, p = _mesh_pair(name, lx)
+    H = O.OracleHsmg(c, facet_bc=1)
+    hs = HSMG(p, facet_bc=1)
+    r = c.rhs_normal(13)
+    z_or = H.apply(r)
+    z = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    hs.apply(z, _t(r))
+    assert _relmax(_np(z), z_or) < 1e-11
+    f = c.rhs_normal(5)
+    tol = 1e-10 * c.res_norm(f)
+    x_or, m_or = H.gmres(f, tol, lgmres=30)
+    g = GMRES(p, pc=hs, lgmres=30)
+    x = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    mon = g.solve(x, _t(f), tol)
+    assert mon.converged and m_or.converged
+    assert abs(mon.iter - m_or.iter) <= 1, (mon.iter, m_or.iter)
+    assert np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or) < 1e-9
+
+
+def test_phmg_on_reference_mesh():
+    from neko_amd.solve import PHMG
+    print("phmg mesh", flush=True)
+    c, p = _mesh_pair("hemi.nmsh", 5, h1=1.0 / 1600, h2=1500.0)
+    P = O.OraclePhmg(c, facet_bc=2, level_h2="fine")
+    ph = PHMG(p, facet_bc=2, smooth_its=4, level_h2="fine")
+    eg = ph.eigs()
+    for l in range(2):
+        assert abs(eg[l][0] - P.ch[l].tha) < 1e-9 * P.ch[l].tha
+    r = c.rhs_normal(21)
+    z_or = P.apply(r)
+    z = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+    ph.apply(z, _t(r))
+    assert _relmax(_np(z), z_or) < 1e-9
