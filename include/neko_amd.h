@@ -229,8 +229,10 @@ const void *nm_op_field(const nm_op_t *op, int which);
 int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
                   int32_t *blk_len, int32_t *blk_off);
 /* Ax variant code: elements per workgroup (1,2,4,8,16; 0 = default) plus
- * 100 for the G-prefetching variant; gs variant 0 (thread per block) or 1
- * (lane-parallel, blocks <= 8 entries). */
+ * 100 for the G-prefetching variant, 200 for the geometry evaluated in the
+ * kernel (straight-edged operators); gs variant 0 (thread per block), 1
+ * (lane-parallel, blocks <= 8 entries) or 2 (as 1, facet pairs as whole
+ * shared faces with computed indices). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
 int nm_op_set_gs_variant(nm_op_t *op, int variant);
 /* new constant coefficients h1, h2 (h2 != 0: mass term on) of an operator
@@ -322,7 +324,7 @@ int nm_cg_solve(nm_cg_t *cg, double *x, const double *f, double abs_tol,
 
 /* --- hsmg preconditioner (pc_t 'hsmg', src/krylov/pc_hsmg.f90) ---------- */
 /* Three-level V-cycle lx -> lx_mid (4, or max(lx-1,3) below lx 5) -> 2 on
- * the mesh of a box operator: overlapping Schwarz with FDM local solves
+ * the mesh of a box or mesh operator: overlapping Schwarz with FDM local solves
  * (src/math/schwarz.f90:112-634, src/math/fdm.f90:110-665) on the fine and
  * middle levels, `niter` Jacobi-CG iterations (abs. tol 1e-9) on the
  * coarse level (pc_hsmg.f90:147-365 init, :481-627 solve, CPU branch).
@@ -369,7 +371,7 @@ int nm_gmres_solve(nm_gmres_t *g, double *x, const double *f, double abs_tol,
                    int max_iter, nm_gmres_monitor_t *mon, hipStream_t s);
 
 /* --- batched solves (no counterpart in Neko; same CG semantics) -------- */
-/* Many box operators (nm_op_create_box, constant h1) of uneven size and
+/* Many straight-edged operators (box or mesh constructors, constant h1) of uneven size and
  * order solved together: one FP32 CG iteration of every problem per
  * captured graph step (one batched Ax launch per lx, one merged gs),
  * FP64 residual replacement per problem. */

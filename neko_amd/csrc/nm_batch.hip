@@ -313,7 +313,7 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
   for (int i = 0; i < ninst; i++) {
     Op *op = &ops[i]->op;
     if (!op->trilinear || op->f64.h1) {
-      set_error("nm_batch_create: problem %d is not a box operator with constant h1", i);
+      set_error("nm_batch_create: problem %d is not a straight-edged operator with constant h1", i);
       return fail(NM_ERR_ARG);
     }
     int st = op->ensure32(s);
