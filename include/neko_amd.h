@@ -410,6 +410,11 @@ void nm_server_free(nm_server_t *sv);
 int nm_server_add_box_mesh(nm_server_t *sv, int nelx, int nely, int nelz,
                            int lx, double amp, uint64_t seed, int tune_reps,
                            int prewarm, int *mesh_id, hipStream_t s);
+/* the same for a general straight-edged mesh (vertices in Neko's symmetric
+ * order and point ids, e.g. from nm_nmsh_read) */
+int nm_server_add_mesh(nm_server_t *sv, int64_t nel, int lx, const double *vxyz,
+                       const int64_t *pid, int tune_reps, int prewarm,
+                       int *mesh_id, hipStream_t s);
 int nm_server_mesh_op(nm_server_t *sv, int mesh_id, nm_op_t **op);
 /* a request: mesh, RHS seed (nm_op_rhs_normal), relative tolerance,
  * arrival time in seconds after the start of nm_server_run */

@@ -134,6 +134,7 @@ SIGNATURES = {
     "nm_server_free": (None, [_vp]),
     "nm_server_add_box_mesh": (_i, [_vp, _i, _i, _i, _i, _d, C.c_uint64, _i, _i,
                                     C.POINTER(_i), _vp]),
+    "nm_server_add_mesh": (_i, [_vp, _i64, _i, _vp, _vp, _i, _i, C.POINTER(_i), _vp]),
     "nm_server_mesh_op": (_i, [_vp, _i, C.POINTER(_vp)]),
     "nm_server_submit": (_i, [_vp, _i, C.c_uint64, _d, _i, _d, C.POINTER(_i)]),
     "nm_server_run": (_i, [_vp, C.POINTER(_d)]),

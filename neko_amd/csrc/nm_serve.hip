@@ -193,21 +193,22 @@ int nm_server_create(nm_server_t **out, double hbm_budget_bytes, int max_infligh
 
 void nm_server_free(nm_server_t *h) { delete h; }
 
-int nm_server_add_box_mesh(nm_server_t *h, int nelx, int nely, int nelz, int lx,
-                           double amp, uint64_t seed, int tune_reps, int prewarm,
-                           int *mesh_id, hipStream_t s) {
+// Make an operator resident: autotune it (kernel variants are fixed
+// before any CG state captures its graphs), charge it to the budget and
+// pre-create `prewarm` CG states.  On failure the operator is freed.
+static int register_mesh(nm_server_t *h, nm_op_t *op, int tune_reps, int prewarm,
+                         int *mesh_id, hipStream_t s) {
   Server &sv = h->sv;
   ServeMesh m;
-  int st = nm_op_create_box(&m.op, nelx, nely, nelz, lx, 0, 0, 0, amp, seed, 1.0, 0.0, s);
-  if (st) return st;
-  // kernel variants are fixed before any CG state captures its graphs
+  m.op = op;
+  int st;
   if (tune_reps > 0 && (st = nm_op_autotune(m.op, tune_reps, nullptr, nullptr, nullptr, s))) {
     nm_op_free(m.op);
     return st;
   }
   if (sv.used + m.op->op.bytes > sv.budget) {
     nm_op_free(m.op);
-    set_error("nm_server_add_box_mesh: mesh does not fit the HBM budget");
+    set_error("server: mesh does not fit the HBM budget");
     return NM_ERR_NOMEM;
   }
   sv.used += m.op->op.bytes;
@@ -222,6 +223,26 @@ int nm_server_add_box_mesh(nm_server_t *h, int nelx, int nely, int nelz, int lx,
     sv.meshes[*mesh_id].idle.push_back(cg);
   }
   return NM_OK;
+}
+
+int nm_server_add_box_mesh(nm_server_t *h, int nelx, int nely, int nelz, int lx,
+                           double amp, uint64_t seed, int tune_reps, int prewarm,
+                           int *mesh_id, hipStream_t s) {
+  nm_op_t *op = nullptr;
+  int st = nm_op_create_box(&op, nelx, nely, nelz, lx, 0, 0, 0, amp, seed, 1.0, 0.0, s);
+  if (st) return st;
+  return register_mesh(h, op, tune_reps, prewarm, mesh_id, s);
+}
+
+// A general straight-edged mesh (e.g. from nm_nmsh_read), Poisson operator
+// with Dirichlet on every facet without a neighbour.
+int nm_server_add_mesh(nm_server_t *h, int64_t nel, int lx, const double *vxyz,
+                       const int64_t *pid, int tune_reps, int prewarm, int *mesh_id,
+                       hipStream_t s) {
+  nm_op_t *op = nullptr;
+  int st = nm_op_create_mesh(&op, nel, lx, vxyz, pid, 1.0, 0.0, s);
+  if (st) return st;
+  return register_mesh(h, op, tune_reps, prewarm, mesh_id, s);
 }
 
 int nm_server_mesh_op(nm_server_t *h, int mesh_id, nm_op_t **op) {

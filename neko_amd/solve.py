@@ -630,6 +630,17 @@ class Server:
         self.meshes.append((nelx, nely, nelz, lx, amp, seed))
         return mid.value
 
+    def add_mesh(self, vxyz, pid, lx=8, prewarm=1, tune_reps=0):
+        """Register a general straight-edged mesh (e.g. read_nmsh output)."""
+        v = np.ascontiguousarray(vxyz, dtype=np.float64).reshape(-1, 24)
+        p = np.ascontiguousarray(pid, dtype=np.int64).reshape(-1, 8)
+        mid = C.c_int()
+        check(lib().nm_server_add_mesh(self.h, len(v), int(lx), v.ctypes.data, p.ctypes.data,
+                                       int(tune_reps), int(prewarm), C.byref(mid),
+                                       _stream()), "nm_server_add_mesh")
+        self.meshes.append(("mesh", len(v), lx))
+        return mid.value
+
     def mesh_problem(self, mesh_id: int) -> "Problem":
         """The resident operator of a mesh (owned by the server)."""
         oph = C.c_void_p()

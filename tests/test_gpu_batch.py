@@ -95,3 +95,31 @@ def test_batch_rejects_operator_without_geometry():
                             h1c=1.0, dfrmd=c.dfrmd)
     with pytest.raises(NekoAmdError):
         Batch([p])
+
+
+def test_batch_with_reference_meshes():
+    """Problems on the reference's own meshes (the unstructured hemisphere
+    and the periodic lid-driven cavity mesh) batched with box problems:
+    each reaches the FP64 oracle's solution within 1e-9."""
+    import os
+    from neko_amd.solve import Batch, Problem, read_nmsh
+    print("batch meshes", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    probs, cases, fs = [], [], []
+    for name, lx in (("hemi.nmsh", 5), ("lid.nmsh", 6)):
+        v, pid, _ = read_nmsh(os.path.join(root, "tests", "golden", "nmsh", name))
+        cases.append(O.OracleCase.from_mesh(v, pid, lx))
+        probs.append(Problem.mesh(v, pid, lx))
+    cases.append(O.OracleCase(3, 2, 2, 6, amp=0.1, seed=1))
+    probs.append(Problem.box(3, 2, 2, 6, amp=0.1, seed=1))
+    for k, c in enumerate(cases):
+        fs.append(torch.from_numpy(c.rhs_normal(70 + k)).to("cuda"))
+    b = Batch(probs)
+    xs = [torch.zeros_like(f) for f in fs]
+    mons = b.solve(xs, fs, rel_tol=1e-10)
+    for c, f, x, m in zip(cases, fs, xs, mons):
+        assert m.converged
+        fn = _np(f)
+        x_or, _ = c.cg(fn, 1e-10 * c.res_norm(fn), max_iter=5000)
+        err = np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or)
+        assert err < 1e-9, err

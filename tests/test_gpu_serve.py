@@ -88,3 +88,28 @@ def test_serve_tight_budget_serialises():
         assert b.t_admit >= a.t_done - 1e-9
     _, peak = sv.memory()
     assert peak <= used_one * 1.2
+
+
+def test_serve_reference_meshes():
+    """Requests on the reference's hemisphere and lid meshes, interleaved
+    with a box mesh: every result equals the same solve run alone."""
+    import os
+    from neko_amd.solve import Server, read_nmsh
+    print("serve meshes", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sv = Server(8e9, max_inflight=3, precision="mixed", chunk_iters=8,
+                keep_solutions=True)
+    ids = []
+    for name, lx in (("hemi.nmsh", 5), ("lid.nmsh", 6)):
+        v, pid, _ = read_nmsh(os.path.join(root, "tests", "golden", "nmsh", name))
+        ids.append(sv.add_mesh(v, pid, lx=lx, prewarm=1))
+    ids.append(sv.add_box_mesh(3, 3, 3, lx=6, prewarm=1))
+    reqs = [(sv.submit(ids[k % 3], 300 + k, rel_tol=1e-8), ids[k % 3], 300 + k)
+            for k in range(6)]
+    sv.run()
+    for rid, mid, seed in reqs:
+        r = sv.result(rid)
+        assert r.state == 2 and r.converged
+        x_alone, mon = _alone(sv, mid, seed, 1e-8, "mixed")
+        assert mon.iter == r.iter
+        assert torch.equal(sv.solution(rid), x_alone)
