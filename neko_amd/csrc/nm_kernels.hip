@@ -220,22 +220,39 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
   const int64_t base = ebase + t;
 
   T uc[LX];
+  if (CGP) {
+    // all loads of the column first: p, r, dinv and x are distinct arrays,
+    // but the compiler cannot know it, and interleaved stores would make
+    // every plane wait for the previous plane's stores (one HBM round trip
+    // per plane instead of one per column)
+    T po[LX], pr[LX], pd[LX], px[LX];
 #pragma unroll
-  for (int k = 0; k < LX; k++) {
-    if (CGP) {
-      T v = T(0);
-      if (active) {
-        const int64_t ix = base + k * P;
-        const T po = cp.p[ix];
-        v = cp.dinv[ix] * cp.r[ix] + cp.beta * po;
-        cp.p[ix] = v;
-        cp.x[ix] = cp.x[ix] + cp.alpha * po;
-      }
-      uc[k] = v;
-    } else {
-      uc[k] = active ? u[base + k * P] : T(0);
+    for (int k = 0; k < LX; k++) {
+      const int64_t ix = base + k * P;
+      po[k] = active ? cp.p[ix] : T(0);
+      pr[k] = active ? cp.r[ix] : T(0);
+      pd[k] = active ? cp.dinv[ix] : T(0);
+      px[k] = active ? cp.x[ix] : T(0);
     }
-    su[q][k * P + t] = uc[k];
+#pragma unroll
+    for (int k = 0; k < LX; k++) {
+      uc[k] = pd[k] * pr[k] + cp.beta * po[k];
+      su[q][k * P + t] = uc[k];
+    }
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < LX; k++) {
+        const int64_t ix = base + k * P;
+        cp.p[ix] = uc[k];
+        cp.x[ix] = px[k] + cp.alpha * po[k];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < LX; k++) {
+      uc[k] = active ? u[base + k * P] : T(0);
+      su[q][k * P + t] = uc[k];
+    }
   }
   for (int x = t; x < 36; x += P) sedge[q][x] = active ? ed[x] : T(0);
 
