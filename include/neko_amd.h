@@ -122,6 +122,9 @@ typedef struct {
   /* facet pairs the face-descriptor gs covers (whole shared faces) and the
    * pairs it leaves to the pair kernel */
   int64_t nfaces, nrest;
+  /* 1 when the fused CG update takes the descriptor face sums of gs (no
+   * masked node on a descriptor face interior, and not switched off) */
+  int cg_face_sums;
 } nm_op_info_t;
 
 typedef struct {
@@ -235,6 +238,11 @@ int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
  * shared faces with computed indices). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
 int nm_op_set_gs_variant(nm_op_t *op, int variant);
+/* Fused CG schedule: let (1, the default) or forbid (0) its update kernel
+ * to add the partner values of descriptor-face points itself instead of a
+ * face gs pass (same sums; no counterpart in the reference, whose CG runs
+ * gs as one pass). */
+int nm_op_set_cg_face_sums(nm_op_t *op, int on);
 /* new constant coefficients h1, h2 (h2 != 0: mass term on) of an operator
  * with constant h1, as the pnpn schemes set them before each solve
  * (pressure h1 = 1/rho, h2 = 0: src/fluid/bcknd/cpu/pnpn_res_cpu.f90:74-78;

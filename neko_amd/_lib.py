@@ -23,7 +23,7 @@ class OpInfo(C.Structure):
     _fields_ = [("nel", _i64), ("n", _i64), ("nlocal", _i64), ("nblks", _i64),
                 ("npairs", _i64), ("nmsk", _i64), ("facet_offset", _i64),
                 ("lx", _i), ("volume", _d), ("h1c", _d), ("ifh2", _i),
-                ("nfaces", _i64), ("nrest", _i64)]
+                ("nfaces", _i64), ("nrest", _i64), ("cg_face_sums", _i)]
 
 
 class ServeResult(C.Structure):
@@ -103,6 +103,7 @@ SIGNATURES = {
     "nm_op_gs_maps": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "nm_op_set_epb": (_i, [_vp, _i, _i]),
     "nm_op_set_gs_variant": (_i, [_vp, _i]),
+    "nm_op_set_cg_face_sums": (_i, [_vp, _i]),
     "nm_op_autotune": (_i, [_vp, _i, _pi, _pi, _pi, _vp]),
     "nm_op_ax_local": (_i, [_vp, _vp, _vp, _vp]),
     "nm_op_ax_gs": (_i, [_vp, _vp, _vp, _vp]),

@@ -174,13 +174,15 @@ __global__ void __launch_bounds__(RED_THREADS)
   }
 }
 
-__global__ void cg_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ sc) {
+// part holds npart partial rtr sums, then npart partial rtz sums
+__global__ void cg_fin_kernel(const double *__restrict__ part, int npart,
+                              CgScal *__restrict__ sc) {
   if (sc->done) return;
   __shared__ double sh[16];
   double a = 0.0, b = 0.0;
-  for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) {
+  for (int q = threadIdx.x; q < npart; q += blockDim.x) {
     a += part[q];
-    b += part[RED_BLOCKS + q];
+    b += part[npart + q];
   }
   bsum2(a, b, sh);
   if (threadIdx.x == 0) {
@@ -223,6 +225,100 @@ __global__ void __launch_bounds__(RED_THREADS)
     part[blockIdx.x] = a;
     part[RED_BLOCKS + blockIdx.x] = b;
   }
+}
+
+// As cg_update2_kernel, with the facet-pair sums of gs taken here: w holds
+// the local Ax result after the block sums, the facet pairs no descriptor
+// covers and the mask, while the interior points of descriptor faces still
+// hold their own element's value.  Such a point adds its partner's value
+// (w_a + w_b, the same bits the face kernel would have stored, since the
+// sum commutes exactly) where w is read anyway, which saves the face
+// kernel's read and write of w.  One element per loop step, so the point's
+// (i, j, k) come from the thread index.  UPF_BLOCKS workgroups (more than
+// the plain update's RED_BLOCKS: the partner load waits on the face-table
+// load, and the extra waves hide that chain; 2048 timed best against 1024,
+// 4096 and 8192, and against keeping 2 or 4 elements in flight per thread),
+// each writing its partial rtr to part[b] and rtz to part[UPF_BLOCKS + b].
+// (Walking chunks of consecutive elements, dealt out so that a point's
+// face partners run on the same XCD at the same time, timed slower: the
+// partner lines cost less than the lost spread of the element walk.)
+constexpr int UPF_BLOCKS = 2048;
+template <typename T, int LX>
+__global__ void __launch_bounds__(RED_THREADS)
+    cg_update2f_kernel(T *__restrict__ r, const T *__restrict__ w, const T *__restrict__ mult,
+                       const T *__restrict__ dinv, const int2 *__restrict__ fnb, int64_t nel,
+                       const CgScal *__restrict__ sc, double *__restrict__ part) {
+  if (sc->done) return;
+  constexpr int N3 = LX * LX * LX;
+  constexpr int NQ = (N3 + RED_THREADS - 1) / RED_THREADS;  // points per thread and element
+  __shared__ double sh[16];
+  const T alpha_t = (T)sc->alpha;
+  // where this thread's points sit in an element (the same for every
+  // element): face f of a face-interior point and its (u, v), else f = -1
+  int fq[NQ], uq[NQ], vq[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; t++) {
+    const int q = threadIdx.x + t * RED_THREADS;
+    const int ii = q % LX, jj = (q / LX) % LX, kk = q / (LX * LX);
+    const bool bi = ii == 0 || ii == LX - 1, bj = jj == 0 || jj == LX - 1,
+               bk = kk == 0 || kk == LX - 1;
+    fq[t] = (q < N3 && (int)bi + (int)bj + (int)bk == 1)
+                ? (bi ? (ii != 0) : (bj ? 2 + (jj != 0) : 4 + (kk != 0)))
+                : -1;
+    uq[t] = bi ? jj : ii;
+    vq[t] = bk ? jj : kk;
+  }
+  double a = 0.0, b = 0.0;
+  for (int64_t e = blockIdx.x; e < nel; e += UPF_BLOCKS) {
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      const int q = threadIdx.x + t * RED_THREADS;
+      if (q >= N3) break;
+      const int64_t i = e * N3 + q;
+      T wi = w[i];
+      if (fq[t] >= 0) {
+        const int2 d = fnb[e * 6 + fq[t]];
+        if (d.x >= 0) {
+          int ub = (d.y & 64) ? vq[t] : uq[t], vb = (d.y & 64) ? uq[t] : vq[t];
+          if (d.y & 128) ub = LX - 1 - ub;
+          if (d.y & 256) vb = LX - 1 - vb;
+          wi = wi + w[(int64_t)d.x * N3 + face_point((d.y >> 3) & 7, ub, vb, LX)];
+        }
+      }
+      const T ri = r[i] - alpha_t * wi;
+      r[i] = ri;
+      const double rm = (double)ri * (double)ri * (double)mult[i];
+      a += rm;
+      b += rm * (double)dinv[i];
+    }
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[UPF_BLOCKS + blockIdx.x] = b;
+  }
+}
+
+template <typename T>
+static int launch_update2f(const Op &op, T *r, const T *w, const CgScal *sc, double *part,
+                           hipStream_t s) {
+  const OpFields<T> &F = op.fields<T>();
+  const int2 *fnb = (const int2 *)op.face_nbr;
+#define NM_U2F(L)                                                                          \
+  case L:                                                                                  \
+    hipLaunchKernelGGL((cg_update2f_kernel<T, L>), dim3(UPF_BLOCKS), dim3(RED_THREADS), 0, \
+                       s, r, w, F.mult, F.dinv, fnb, op.nel, sc, part);                    \
+    break;
+  switch (op.lx) {
+    NM_U2F(3) NM_U2F(4) NM_U2F(5) NM_U2F(6) NM_U2F(7) NM_U2F(8) NM_U2F(9) NM_U2F(10)
+    NM_U2F(11) NM_U2F(12) NM_U2F(13) NM_U2F(14)
+    default:
+      set_error("cg: face sums in the update need 3 <= lx <= %d", NM_FACE_FUSE_MAX_LX);
+      return NM_ERR_UNSUPPORTED;
+  }
+#undef NM_U2F
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
 }
 
 // the x update the fused schedule still owes: x += alpha p
@@ -406,15 +502,25 @@ static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
   const int64_t n = op.n;
   if constexpr (std::is_same<T, TX>::value) {
     if (cgp) {
-      // p update and the previous x update ride in the Ax kernel
-      int st = op.apply_cgp<T>(w, p, r, x, &sc->beta, &sc->alpha, &sc->done, s, dpart);
+      // p update and the previous x update ride in the Ax kernel, the
+      // descriptor face sums of gs in the update kernel
+      const bool ff = op.face_fuse && op.face_fuse_on;
+      int st = op.apply_cgp<T>(w, p, r, x, &sc->beta, &sc->alpha, &sc->done, s, dpart, ff);
       if (st) return st;
       hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, dpart,
                          npart, part, sc);
       hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+      if (ff) {
+        // partials into dpart (its p.Ap partials are folded by now)
+        if ((st = launch_update2f<T>(op, r, w, sc, dpart, s))) return st;
+        hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, dpart, UPF_BLOCKS,
+                           sc);
+        NM_HIP_TRY(hipGetLastError());
+        return NM_OK;
+      }
       hipLaunchKernelGGL(cg_update2_kernel<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, r, w,
                          F.mult, F.dinv, n, sc, part);
-      hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+      hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, RED_BLOCKS, sc);
       NM_HIP_TRY(hipGetLastError());
       return NM_OK;
     }
@@ -432,7 +538,7 @@ static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
   hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
   hipLaunchKernelGGL((cg_update_kernel<T, TX>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, x,
                      r, p, w, F.mult, F.dinv, n, sc, part);
-  hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+  hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, RED_BLOCKS, sc);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -545,12 +651,14 @@ static int cg_capture(CgSolver *cg) {
   cg->npart = 0;
   if (code >= 200) {
     cg->npart = ax_grid(op->nel, op->lx, code);
-    if (cg->npart > cg->npart_cap) {
-      if (!(cg->dpart = cg->alloc<double>(cg->npart))) {
+    // the buffer also takes the face-sum update's 2 UPF_BLOCKS partials
+    const int64_t need = std::max<int64_t>(cg->npart, 2 * UPF_BLOCKS);
+    if (need > cg->npart_cap) {
+      if (!(cg->dpart = cg->alloc<double>(need))) {
         set_error("cg: out of device memory");
         return NM_ERR_NOMEM;
       }
-      cg->npart_cap = cg->npart;
+      cg->npart_cap = need;
     }
   }
   hipGraphExec_t *g = cg->mixed ? &cg->g32 : &cg->g64;

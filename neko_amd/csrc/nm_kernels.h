@@ -12,6 +12,25 @@ namespace nm {
 constexpr int RED_BLOCKS = 1024;
 constexpr int RED_THREADS = 256;
 
+// XCDs of the chip; workgroup b of a launch runs on XCD b % NM_NXCD.
+constexpr int NM_NXCD = 8;
+
+// Largest lx for which the CG update kernel can take the descriptor face
+// sums of gs itself.
+constexpr int NM_FACE_FUSE_MAX_LX = 14;
+
+// Local offset of point (u, v) of element face f (f = 2 dir + side, dir 0:
+// i fixed, u = j, v = k; dir 1: j fixed, u = i, v = k; dir 2: k fixed, u = i,
+// v = j; side 1 is the index lx - 1).
+__device__ inline int64_t face_point(int f, int u, int v, int lx) {
+  const int fixed = (f & 1) ? lx - 1 : 0;
+  const int dir = f >> 1;
+  const int i = dir == 0 ? fixed : u;
+  const int j = dir == 0 ? u : (dir == 1 ? fixed : v);
+  const int k = dir == 2 ? fixed : v;
+  return i + (int64_t)lx * (j + (int64_t)lx * k);
+}
+
 // Arguments of the tensor-product Helmholtz operator
 //   w = sum_{rst} D^T (h1 G) D u  (+ h2B * u)
 template <typename T>

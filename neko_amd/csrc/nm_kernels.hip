@@ -600,7 +600,6 @@ template int launch_ax<float>(const AxArgs<float> &, int, hipStream_t);
 // logical work, so the element cache lines a range of gs entries shares stay
 // in one L2 instead of being fetched by all eight.  The launch must use a
 // grid of 8 * per blocks; logical blocks >= the real count exit.
-constexpr int NM_NXCD = 8;
 __device__ inline int64_t xcd_block(int64_t per) {
   const int64_t b = blockIdx.x;
   return (b % NM_NXCD) * per + b / NM_NXCD;
@@ -696,15 +695,6 @@ __global__ void __launch_bounds__(256)
 // square symmetry maps (u, v) to.  Both get u_a + u_b (== u_b + u_a
 // exactly, so the sum is the same bits as the pair kernel's).  The indices
 // are computed, not loaded: no index traffic and no dependent load.
-__device__ inline int64_t face_point(int f, int u, int v, int lx) {
-  const int fixed = (f & 1) ? lx - 1 : 0;
-  const int dir = f >> 1;
-  const int i = dir == 0 ? fixed : u;
-  const int j = dir == 0 ? u : (dir == 1 ? fixed : v);
-  const int k = dir == 2 ? fixed : v;
-  return i + (int64_t)lx * (j + (int64_t)lx * k);
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256)
     gs_face_kernel(T *__restrict__ u, const int32_t *__restrict__ faces, int64_t nfaces,

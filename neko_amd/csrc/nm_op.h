@@ -37,6 +37,16 @@ struct Op {
   // facet pairs as whole-face descriptors (3 int32 each) + the rest
   int32_t *gs_faces = nullptr, *gs_rest = nullptr;
   int64_t nfaces = 0, nrest = 0;
+  // per element face (6 per element): {neighbour element, code} of the
+  // descriptor covering it, seen from this element (code as in GsArgs::faces:
+  // own face in bits 0-2, the neighbour's face in bits 3-5), neighbour -1
+  // where no descriptor covers the face
+  int32_t *face_nbr = nullptr;
+  std::vector<int32_t> h_face_nbr;
+  // may the CG update take the descriptor face sums (no masked node on a
+  // descriptor face interior)?
+  bool face_fuse = false;
+  bool face_fuse_on = true;  // caller's switch (nm_op_set_cg_face_sums)
   // Neko-layout copies (1-based) kept for the drop-in gather/scatter tests
   std::vector<int32_t> h_dg, h_gd, h_blk_len, h_blk_off;
   int64_t facet_offset = 0;
@@ -90,9 +100,12 @@ struct Op {
             bool mask = true, double *dot_part = nullptr) const;
   // w = mask(gs(A p)) with the CG prologue fused into the trilinear Ax:
   // p <- dinv r + (*beta) p and x += (*alpha) p_old while loading p
+  // With defer_faces the interior points of descriptor faces are left
+  // unsummed (the caller's update kernel adds the partner values).
   template <typename T>
   int apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double *alpha,
-                const int *skip, hipStream_t s, double *dot_part) const;
+                const int *skip, hipStream_t s, double *dot_part,
+                bool defer_faces = false) const;
   // variant code of the Ax kernel used for precision T
   template <typename T>
   int ax_code() const;

@@ -132,7 +132,7 @@ int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
 }
 template <typename T>
 int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double *alpha,
-                  const int *skip, hipStream_t s, double *dot_part) const {
+                  const int *skip, hipStream_t s, double *dot_part, bool defer_faces) const {
   const OpFields<T> &F = fields<T>();
   const int code = ax_code<T>();
   if (code < 200 || !dot_part) {
@@ -162,14 +162,29 @@ int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double
   a.cg_alpha = alpha;
   int st = launch_ax<T>(a, code, s);
   if (st) return st;
-  if ((st = gs<T>(w, skip, s))) return st;
+  if (defer_faces) {
+    if (!face_fuse || !face_fuse_on) {
+      set_error("apply_cgp: face sums cannot be deferred on this operator");
+      return NM_ERR_STATE;
+    }
+    // the blocks, then the facet pairs no descriptor covers
+    GsArgs<T> b{w, blk_ptr, gidx, nblks, pair0, 0, skip};
+    b.variant = max_blk <= 8 ? 1 : 0;
+    if (nblks > 0 && (st = launch_gs<T>(b, s))) return st;
+    GsArgs<T> c{w, nullptr, gs_rest, 0, 0, nrest, skip};
+    c.variant = 1;
+    if (nrest > 0 && (st = launch_gs<T>(c, s))) return st;
+  } else if ((st = gs<T>(w, skip, s))) {
+    return st;
+  }
   return launch_mask<T>(w, msk, nmsk, skip, s);
 }
 template int Op::apply_cgp<double>(double *, double *, const double *, double *,
                                    const double *, const double *, const int *, hipStream_t,
-                                   double *) const;
+                                   double *, bool) const;
 template int Op::apply_cgp<float>(float *, float *, const float *, float *, const double *,
-                                  const double *, const int *, hipStream_t, double *) const;
+                                  const double *, const int *, hipStream_t, double *,
+                                  bool) const;
 
 template int Op::apply<double>(double *, const double *, const int *,
                                hipStream_t, bool, bool, double *) const;
@@ -382,6 +397,24 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
     gs_face_pairs(op->nel, op->lx, gi.data() + op->pair0, op->npairs, faces, rest);
     op->nfaces = (int64_t)faces.size() / 3;
     op->nrest = (int64_t)rest.size() / 2;
+    // the same descriptors per element face; the partner's view of a
+    // descriptor swaps the faces and inverts the square symmetry (with the
+    // swap the two flips trade places)
+    op->h_face_nbr.assign((size_t)op->nel * 12, -1);
+    for (int64_t d = 0; d < op->nfaces; d++) {
+      const int32_t ea = faces[3 * d], eb = faces[3 * d + 1], code = faces[3 * d + 2];
+      const int fa = code & 7, fb = (code >> 3) & 7;
+      const int sw = (code >> 6) & 1, fu = (code >> 7) & 1, fv = (code >> 8) & 1;
+      const int iu = sw ? fv : fu, iv = sw ? fu : fv;
+      const size_t sa = ((size_t)ea * 6 + fa) * 2, sb = ((size_t)eb * 6 + fb) * 2;
+      op->h_face_nbr[sa] = eb;
+      op->h_face_nbr[sa + 1] = code;
+      op->h_face_nbr[sb] = ea;
+      op->h_face_nbr[sb + 1] = fb | (fa << 3) | (sw << 6) | (iu << 7) | (iv << 8);
+    }
+    NM_ALLOC(op->face_nbr, int32_t, op->h_face_nbr.size());
+    NM_HIP_TRY(hipMemcpyAsync(op->face_nbr, op->h_face_nbr.data(),
+                              sizeof(int32_t) * op->h_face_nbr.size(), hipMemcpyHostToDevice, s));
     NM_ALLOC(op->gs_faces, int32_t, faces.size());
     NM_ALLOC(op->gs_rest, int32_t, rest.size());
     if (!faces.empty())
@@ -406,6 +439,29 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
 // (coef.f90:403-419, :1142-1155; pc_jacobi.f90:93-165).
 int op_finish(Op *op, hipStream_t s) {
   const int64_t n = op->n;
+  // The CG update may take the descriptor face sums unless a masked node
+  // lies on a descriptor face interior (the mask acts after the sum).
+  op->face_fuse = false;
+  if (op->nfaces > 0 && op->lx >= 3 && op->lx <= NM_FACE_FUSE_MAX_LX &&
+      op->h_face_nbr.size() == (size_t)op->nel * 12) {
+    std::vector<int32_t> mk(op->nmsk > 0 ? op->nmsk : 1);
+    if (op->nmsk > 0)
+      NM_HIP_TRY(hipMemcpy(mk.data(), op->msk, sizeof(int32_t) * op->nmsk,
+                           hipMemcpyDeviceToHost));
+    const int lx = op->lx;
+    const int64_t n3 = (int64_t)lx * lx * lx;
+    bool ok = true;
+    for (int64_t q = 0; q < op->nmsk && ok; q++) {
+      const int64_t e = mk[q] / n3, l = mk[q] % n3;
+      const int i = (int)(l % lx), j = (int)((l / lx) % lx), k = (int)(l / ((int64_t)lx * lx));
+      const bool bi = i == 0 || i == lx - 1, bj = j == 0 || j == lx - 1,
+                 bk = k == 0 || k == lx - 1;
+      if ((int)bi + (int)bj + (int)bk != 1) continue;
+      const int f = bi ? (i != 0) : (bj ? 2 + (j != 0) : 4 + (k != 0));
+      if (op->h_face_nbr[((size_t)e * 6 + f) * 2] >= 0) ok = false;
+    }
+    op->face_fuse = ok;
+  }
   if (!op->f64.mult) NM_ALLOC(op->f64.mult, double, n);
   int st = launch_vec<double>(V_CFILL, op->f64.mult, nullptr, nullptr, nullptr, 1.0, n, s);
   if (st) return st;
@@ -780,6 +836,7 @@ int nm_op_info(const nm_op_t *h, nm_op_info_t *info) {
   info->ifh2 = op.ifh2;
   info->nfaces = op.nfaces;
   info->nrest = op.nrest;
+  info->cg_face_sums = op.face_fuse && op.face_fuse_on ? 1 : 0;
   return NM_OK;
 }
 
@@ -834,6 +891,12 @@ int nm_op_set_all_deformed(nm_op_t *h, hipStream_t s) {
                             hipMemcpyHostToDevice, s));
   NM_HIP_TRY(hipStreamSynchronize(s));
   return op_finish(op, s);
+}
+
+int nm_op_set_cg_face_sums(nm_op_t *h, int on) {
+  h->op.face_fuse_on = on != 0;
+  h->op.epoch++;
+  return NM_OK;
 }
 
 int nm_op_set_gs_variant(nm_op_t *h, int variant) {

@@ -292,6 +292,12 @@ class Problem:
     def set_gs_variant(self, v):
         check(lib().nm_op_set_gs_variant(self.h, int(v)), "set_gs_variant")
 
+    def set_cg_face_sums(self, on):
+        """Let the fused CG update add the descriptor-face partner values
+        (default) or run the face gs pass instead."""
+        check(lib().nm_op_set_cg_face_sums(self.h, int(bool(on))), "set_cg_face_sums")
+        check(lib().nm_op_info(self.h, C.byref(self.info)), "nm_op_info")
+
     def autotune(self, reps=5):
         """Pick the fastest Ax / gs kernel variants on this problem."""
         a, b, g = C.c_int(), C.c_int(), C.c_int()

@@ -87,14 +87,20 @@ def test_gs_bit_exact(case, variant):
     c = O.OracleCase(nx, ny, nz, lx, amp=amp, seed=3, periodic=per)
     p = _problem_from_oracle(c)
     p.set_gs_variant(variant)
-    if variant == 2 and lx > 2:
+    if variant >= 2 and lx > 2:
         # every facet pair of a conforming box lies on a whole shared face
         assert p.info.nrest == 0 and p.info.nfaces * (lx - 2) ** 2 == p.info.npairs
     u = np.random.default_rng(1).standard_normal(c.n)
+    # negative zeros: a shared node whose entries are all -0 sums to -0
+    u[np.random.default_rng(2).random(c.n) < 0.3] = -0.0
     ud = _t(u)
     p.gs_op(ud)
     ref = c.gs(u.copy())
-    assert np.array_equal(_np(ud), ref)
+    assert np.array_equal(_np(ud).view(np.int64), ref.view(np.int64))
+    z = np.full(c.n, -0.0)
+    zd = _t(z)
+    p.gs_op(zd)
+    assert np.array_equal(_np(zd).view(np.int64), c.gs(z.copy()).view(np.int64))
 
 
 @pytest.mark.parametrize("code", [0, 2, 4, 100, 102, 104])
@@ -187,12 +193,15 @@ def test_box_setup_matches_oracle(case):
 
 
 def _solve_pair(nx, ny, nz, lx, amp, per, rhs, precision="fp64", h1=1.0, h2=0.0,
-                code=None):
+                code=None, face_sums=None):
     from neko_amd import solve as S
     c = O.OracleCase(nx, ny, nz, lx, amp=amp, seed=21, periodic=per, h1=h1, h2=h2)
     p = S.Problem.box(nx, ny, nz, lx, periodic=per, amp=amp, seed=21, h1=h1, h2=h2)
     if code is not None:
         p.set_epb(code, code)
+    if face_sums is not None:
+        p.set_cg_face_sums(face_sums)
+        assert p.info.cg_face_sums == int(face_sums)
     if rhs == "sine":
         f_or = c.rhs_sine()
     else:
@@ -304,6 +313,24 @@ def test_cg_fused_pap_trilinear(precision):
     print("cg fused", precision, flush=True)
     mon, mon_or, err = _solve_pair(4, 3, 3, 8, 0.1, (0, 0, 0), "normal",
                                    precision=precision, code=201)
+    assert mon.converged
+    if precision == "fp64":
+        assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
+    else:
+        assert mon.res_final < 1e-10 * mon.res_start
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("lx,per", [(5, (1, 0, 1)), (8, (0, 1, 0))])
+@pytest.mark.parametrize("face_sums", [False, True])
+def test_cg_update_face_sums(precision, lx, per, face_sums):
+    """The fused CG update adding the descriptor-face partner values itself
+    (and with it switched off, the face gs pass): oracle iterations +-1
+    (FP64), 1e-10 reached (mixed), solution within 1e-9 either way."""
+    print("cg face sums", precision, lx, per, face_sums, flush=True)
+    mon, mon_or, err = _solve_pair(3, 4, 3, lx, 0.1, per, "normal",
+                                   precision=precision, code=201, face_sums=face_sums)
     assert mon.converged
     if precision == "fp64":
         assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
