@@ -125,6 +125,9 @@ typedef struct {
   /* 1 when the fused CG update takes the descriptor face sums of gs (no
    * masked node on a descriptor face interior, and not switched off) */
   int cg_face_sums;
+  /* 1 when every multiplicity weight is 1/c for a count c <= 255 (the fused
+   * CG update then reads one byte per point instead of mult) */
+  int mult_counts;
 } nm_op_info_t;
 
 typedef struct {

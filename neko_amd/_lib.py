@@ -23,7 +23,8 @@ class OpInfo(C.Structure):
     _fields_ = [("nel", _i64), ("n", _i64), ("nlocal", _i64), ("nblks", _i64),
                 ("npairs", _i64), ("nmsk", _i64), ("facet_offset", _i64),
                 ("lx", _i), ("volume", _d), ("h1c", _d), ("ifh2", _i),
-                ("nfaces", _i64), ("nrest", _i64), ("cg_face_sums", _i)]
+                ("nfaces", _i64), ("nrest", _i64), ("cg_face_sums", _i),
+                ("mult_counts", _i)]
 
 
 class ServeResult(C.Structure):

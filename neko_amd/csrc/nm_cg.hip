@@ -243,15 +243,23 @@ __global__ void __launch_bounds__(RED_THREADS)
 // face partners run on the same XCD at the same time, timed slower: the
 // partner lines cost less than the lost spread of the element walk.)
 constexpr int UPF_BLOCKS = 2048;
-template <typename T, int LX>
+// With CNT the multiplicity weight comes from the one-byte count (mult is
+// exactly 1/count): tab[count] = (T)(1.0 / count), the bits of mult itself.
+template <typename T, int LX, bool CNT>
 __global__ void __launch_bounds__(RED_THREADS)
     cg_update2f_kernel(T *__restrict__ r, const T *__restrict__ w, const T *__restrict__ mult,
-                       const T *__restrict__ dinv, const int2 *__restrict__ fnb, int64_t nel,
-                       const CgScal *__restrict__ sc, double *__restrict__ part) {
+                       const uint8_t *__restrict__ mcnt, const T *__restrict__ dinv,
+                       const int2 *__restrict__ fnb, int64_t nel, const CgScal *__restrict__ sc,
+                       double *__restrict__ part) {
   if (sc->done) return;
   constexpr int N3 = LX * LX * LX;
   constexpr int NQ = (N3 + RED_THREADS - 1) / RED_THREADS;  // points per thread and element
   __shared__ double sh[16];
+  __shared__ T tab[CNT ? 256 : 1];
+  if (CNT) {
+    for (int c = threadIdx.x; c < 256; c += RED_THREADS) tab[c] = c ? (T)(1.0 / (double)c) : T(0);
+    __syncthreads();
+  }
   const T alpha_t = (T)sc->alpha;
   // where this thread's points sit in an element (the same for every
   // element): face f of a face-interior point and its (u, v), else f = -1
@@ -287,7 +295,8 @@ __global__ void __launch_bounds__(RED_THREADS)
       }
       const T ri = r[i] - alpha_t * wi;
       r[i] = ri;
-      const double rm = (double)ri * (double)ri * (double)mult[i];
+      const T mi = CNT ? tab[mcnt[i]] : mult[i];
+      const double rm = (double)ri * (double)ri * (double)mi;
       a += rm;
       b += rm * (double)dinv[i];
     }
@@ -306,8 +315,14 @@ static int launch_update2f(const Op &op, T *r, const T *w, const CgScal *sc, dou
   const int2 *fnb = (const int2 *)op.face_nbr;
 #define NM_U2F(L)                                                                          \
   case L:                                                                                  \
-    hipLaunchKernelGGL((cg_update2f_kernel<T, L>), dim3(UPF_BLOCKS), dim3(RED_THREADS), 0, \
-                       s, r, w, F.mult, F.dinv, fnb, op.nel, sc, part);                    \
+    if (op.mcnt_ok)                                                                        \
+      hipLaunchKernelGGL((cg_update2f_kernel<T, L, true>), dim3(UPF_BLOCKS),               \
+                         dim3(RED_THREADS), 0, s, r, w, F.mult, op.mcnt, F.dinv, fnb,      \
+                         op.nel, sc, part);                                                \
+    else                                                                                   \
+      hipLaunchKernelGGL((cg_update2f_kernel<T, L, false>), dim3(UPF_BLOCKS),              \
+                         dim3(RED_THREADS), 0, s, r, w, F.mult, op.mcnt, F.dinv, fnb,      \
+                         op.nel, sc, part);                                                \
     break;
   switch (op.lx) {
     NM_U2F(3) NM_U2F(4) NM_U2F(5) NM_U2F(6) NM_U2F(7) NM_U2F(8) NM_U2F(9) NM_U2F(10)

@@ -19,6 +19,10 @@ constexpr int NM_NXCD = 8;
 // sums of gs itself.
 constexpr int NM_FACE_FUSE_MAX_LX = 14;
 
+// Multiplicity counts: cnt[i] = c with mult[i] == 1.0 / c exactly and
+// 1 <= c <= 255, else *bad = 1 (cnt[i] then 0).
+int launch_mult_count(const double *mult, uint8_t *cnt, int *bad, int64_t n, hipStream_t s);
+
 // Local offset of point (u, v) of element face f (f = 2 dir + side, dir 0:
 // i fixed, u = j, v = k; dir 1: j fixed, u = i, v = k; dir 2: k fixed, u = i,
 // v = j; side 1 is the index lx - 1).

@@ -719,6 +719,26 @@ __global__ void __launch_bounds__(256)
   u[ib] = acc;
 }
 
+__global__ void mult_count_kernel(const double *__restrict__ mult, uint8_t *__restrict__ cnt,
+                                  int *__restrict__ bad, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double m = mult[i];
+    const double c = m > 0.0 ? rint(1.0 / m) : 0.0;
+    const bool ok = c >= 1.0 && c <= 255.0 && 1.0 / c == m;
+    cnt[i] = ok ? (uint8_t)c : (uint8_t)0;
+    if (!ok) *bad = 1;
+  }
+}
+
+int launch_mult_count(const double *mult, uint8_t *cnt, int *bad, int64_t n, hipStream_t s) {
+  if (n <= 0) return NM_OK;
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(mult_count_kernel, dim3((unsigned)nb), dim3(256), 0, s, mult, cnt, bad, n);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s) {
   const int64_t items = a.nblks + a.npairs;

@@ -47,6 +47,10 @@ struct Op {
   // descriptor face interior)?
   bool face_fuse = false;
   bool face_fuse_on = true;  // caller's switch (nm_op_set_cg_face_sums)
+  // multiplicity counts (mult == 1/count, count <= 255 everywhere: mcnt_ok)
+  uint8_t *mcnt = nullptr;
+  int32_t *mcnt_bad = nullptr;
+  bool mcnt_ok = false;
   // Neko-layout copies (1-based) kept for the drop-in gather/scatter tests
   std::vector<int32_t> h_dg, h_gd, h_blk_len, h_blk_off;
   int64_t facet_offset = 0;

@@ -61,6 +61,7 @@ T *Op::alloc(int64_t count) {
 template double *Op::alloc<double>(int64_t);
 template float *Op::alloc<float>(int64_t);
 template int32_t *Op::alloc<int32_t>(int64_t);
+template uint8_t *Op::alloc<uint8_t>(int64_t);
 
 #define NM_ALLOC(var, T, cnt)                                        \
   do {                                                               \
@@ -467,6 +468,20 @@ int op_finish(Op *op, hipStream_t s) {
   if (st) return st;
   if ((st = op->gs<double>(op->f64.mult, nullptr, s))) return st;
   if ((st = launch_vec<double>(V_INVCOL1, op->f64.mult, nullptr, nullptr, nullptr, 0.0, n, s))) return st;
+  // the multiplicities as one byte each (mult == 1.0 / count exactly): the
+  // fused CG update reads these and looks 1/count up instead of mult
+  if (!op->mcnt) {
+    NM_ALLOC(op->mcnt, uint8_t, n);
+    NM_ALLOC(op->mcnt_bad, int32_t, 1);
+  }
+  NM_HIP_TRY(hipMemsetAsync(op->mcnt_bad, 0, sizeof(int32_t), s));
+  if ((st = launch_mult_count(op->f64.mult, op->mcnt, op->mcnt_bad, n, s))) return st;
+  {
+    int32_t bad = 1;
+    NM_HIP_TRY(hipMemcpyAsync(&bad, op->mcnt_bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NM_HIP_TRY(hipStreamSynchronize(s));
+    op->mcnt_ok = bad == 0;
+  }
   if (!op->red_part) NM_ALLOC(op->red_part, double, RED_BLOCKS * 4);
   if (!op->red_out) NM_ALLOC(op->red_out, double, 4);
   // volume = sum(B) via glsc3(B, 1, 1); dinv serves as the ones vector
@@ -837,6 +852,7 @@ int nm_op_info(const nm_op_t *h, nm_op_info_t *info) {
   info->nfaces = op.nfaces;
   info->nrest = op.nrest;
   info->cg_face_sums = op.face_fuse && op.face_fuse_on ? 1 : 0;
+  info->mult_counts = op.mcnt_ok ? 1 : 0;
   return NM_OK;
 }
 

@@ -85,7 +85,7 @@ def test_hemi_jacobi_pcg_matches_oracle(precision):
     assert mon_or.converged
     p.set_epb(201, 201)
     # the fused update takes the face sums, square symmetries included
-    assert p.info.cg_face_sums == 1
+    assert p.info.cg_face_sums == 1 and p.info.mult_counts == 1
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     mon = CG(p, precision).solve(x, torch.from_numpy(f).cuda(), tol, max_iter=5000)
     assert mon.converged

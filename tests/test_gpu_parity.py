@@ -202,6 +202,7 @@ def _solve_pair(nx, ny, nz, lx, amp, per, rhs, precision="fp64", h1=1.0, h2=0.0,
     if face_sums is not None:
         p.set_cg_face_sums(face_sums)
         assert p.info.cg_face_sums == int(face_sums)
+        assert p.info.mult_counts == 1
     if rhs == "sine":
         f_or = c.rhs_sine()
     else:
