@@ -58,10 +58,65 @@ def ax_flops_per_dof(lx, code):
     return base + (67 if code >= 200 else 0)
 
 
-def gs_bytes(info):
+def gs_bytes(info, word=8):
     # per gs entry: read u, write u, read its 0-based local index
     # (+ one 4-byte block offset per block)
-    return info.nlocal * (8 + 8 + 4) + info.nblks * 4
+    return info.nlocal * (word + word + 4) + info.nblks * 4
+
+
+def pcg_bytes_per_dof(info, n, word=8):
+    """HBM bytes per local DOF of one Jacobi-PCG iteration in the minimal
+    fused schedule the roofline is graded against: p update (read r, dinv,
+    mult, p; write p) 40, stored-G Ax 64, gs, p.Ap (read w, mult, p) 24,
+    x/r update with rtr (read x, p, r, w, mult; write x, r) 56, all in FP64
+    words (195.6 B/DOF at 64^3, lx 8); FP32 halves the words, not the gs
+    indices."""
+    return (40 + 64 + 24 + 56) * word / 8 + gs_bytes(info, word) / n
+
+
+def replacement_bytes_per_dof(info, n):
+    """One FP64 residual replacement of the mixed solve: Ax+gs in FP64 plus
+    r = f - A x and x += x32, r32 = (float) r (read f, w, x, x32; write x,
+    r32) 40 B/DOF."""
+    return 64 + gs_bytes(info) / n + 40
+
+
+def fused_pcg_bytes_per_dof(info, n, lx, word=8):
+    """Algorithmic HBM bytes per local DOF of one iteration of the schedule
+    this library runs (trilinear Ax with the CG prologue: read p, r, dinv, x,
+    write p, x, w, plus 36 edge-vector words per element; gs block sums on
+    the edge and vertex entries; the update: read r, w, dinv, a one-byte
+    multiplicity count, write r, plus the partner value of every face-pair
+    entry)."""
+    face = 2 * info.npairs
+    ax = 7 * word + 36 * word / lx ** 3
+    blocks = ((info.nlocal - face) * (2 * word + 4) + 4 * info.nblks) / n
+    update = 4 * word + 1 + word * face / n
+    return ax + blocks + update
+
+
+def pcg_roofline(prec, mon, dt, info, n, lx):
+    """PCG iterations/s and solves/s of a finished solve as fractions of
+    the HBM roofline, for the minimal fused schedule of the reference's
+    kernels (the grading basis) and for the bytes of the schedule run
+    here (the fused face sums need cg_face_sums)."""
+    bw = HBM_PEAK_GBS * 1e9
+    word = 8 if prec == "fp64" else 4
+    rep = 0.0 if prec == "fp64" else mon.outer * replacement_bytes_per_dof(info, n)
+    out = {}
+    for key, b_it in (("minimal_schedule", pcg_bytes_per_dof(info, n, word)),
+                      ("schedule_run", fused_pcg_bytes_per_dof(info, n, lx, word)
+                       if info.cg_face_sums else None)):
+        if b_it is None:
+            continue
+        t_min = (mon.iter * b_it + rep) * n / bw
+        it_roof = bw / (b_it * n)
+        out[key] = {"bytes_per_dof_iteration": round(b_it, 2),
+                    "it_per_s_roofline": round(it_roof, 1),
+                    "it_frac": round(mon.iter / dt / it_roof, 4),
+                    "solves_per_s_roofline": round(1.0 / t_min, 4),
+                    "solves_frac": round(t_min / dt, 4)}
+    return out
 
 
 def main():
@@ -206,7 +261,8 @@ def main():
                             "outer": mon.outer, "res_start": mon.res_start,
                             "res_final": mon.res_final, "converged": mon.converged,
                             "solves_per_s": round(1.0 / dt, 4),
-                            "it_per_s": round(mon.iter / dt, 2)}
+                            "it_per_s": round(mon.iter / dt, 2),
+                            "roofline": pcg_roofline(prec, mon, dt, info, n, lx)}
             solves[prec]["_x"] = x
             log(f"[bench] {prec}: {mon.iter} it ({mon.outer} outer) {dt:.2f}s "
                 f"res {mon.res_start:.3e}->{mon.res_final:.3e}")
