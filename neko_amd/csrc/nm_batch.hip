@@ -167,6 +167,11 @@ struct Batch {
   int32_t *gs_ptr = nullptr, *gs_idx = nullptr, *msk = nullptr;
   int64_t gs_nblks = 0, gs_pair0 = 0, gs_npairs = 0, nmsk = 0;
   int gs_maxblk = 0;
+  // lane-parallel gs (blocks <= 8 entries): each problem's blocks padded to
+  // whole workgroups of 32 blocks and its pairs to 128 pairs, with the
+  // problem of every logical workgroup, so a converged or replacing
+  // problem's gs workgroups exit at once
+  int32_t *wg_blk = nullptr, *wg_pair = nullptr;
   double *x = nullptr, *f = nullptr, *w64 = nullptr, *part = nullptr, *dot_part = nullptr;
   double *rpart = nullptr;  // scratch of the per-problem replacement
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
@@ -222,6 +227,13 @@ static int batch_iteration(Batch *B, hipStream_t s) {
   GsArgs<float> ga{B->w32, B->gs_ptr, B->gs_idx, B->gs_nblks, B->gs_pair0, B->gs_npairs,
                    nullptr};
   ga.variant = B->gs_maxblk <= 8 ? 1 : 0;
+  if (ga.variant == 1) {
+    ga.wg_blk = B->wg_blk;
+    ga.wg_pair = B->wg_pair;
+    ga.scal = (const char *)B->sc;
+    ga.scal_size = (int)sizeof(CgScal);
+    ga.done_off = (int)offsetof(CgScal, done);
+  }
   int st = launch_gs<float>(ga, s);
   if (st) return st;
   if ((st = launch_mask<float>(B->w32, B->msk, B->nmsk, nullptr, s))) return st;
@@ -310,6 +322,7 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
   if (hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking) != hipSuccess) return fail(NM_ERR_HIP);
   int64_t off = 0, ms = 0, nblk_max = 0;
   int64_t gs_entries = 0, gs_blocks = 0, gs_pairs = 0;
+  int maxblk_all = 0;
   for (int i = 0; i < ninst; i++) {
     Op *op = &ops[i]->op;
     if (!op->trilinear || op->f64.h1) {
@@ -328,6 +341,7 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
     gs_entries += op->nlocal;
     gs_blocks += op->nblks;
     gs_pairs += op->npairs;
+    maxblk_all = std::max(maxblk_all, op->max_blk);
   }
   if (off >= ((int64_t)1 << 31)) {
     set_error("nm_batch_create: %lld dofs exceed the 32-bit gs index range", (long long)off);
@@ -364,8 +378,9 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       !(B->rpart = B->alloc<double>(2 * RED_BLOCKS)) || !(B->sc = B->alloc<CgScal>(ninst)) ||
       !(B->inst = B->alloc<BatchInst>(ninst)) || !(B->blk_inst = B->alloc<int32_t>(nblk_max)) ||
       !(B->chunk_inst = B->alloc<int32_t>(B->nchunk)) ||
-      !(B->gs_ptr = B->alloc<int32_t>(gs_blocks + 1)) ||
-      !(B->gs_idx = B->alloc<int32_t>(gs_entries)) || !(B->msk = B->alloc<int32_t>(ms))) {
+      !(B->gs_ptr = B->alloc<int32_t>(gs_blocks + 32 * (int64_t)ninst + 1)) ||
+      !(B->gs_idx = B->alloc<int32_t>(gs_entries + 256 * (int64_t)ninst)) ||
+      !(B->msk = B->alloc<int32_t>(ms))) {
     set_error("nm_batch_create: out of device memory");
     return fail(NM_ERR_NOMEM);
   }
@@ -381,12 +396,13 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       return fail(NM_ERR_HIP);
   // instance table, chunk table, merged gs and mask (offsets added)
   B->h_inst.resize(ninst);
-  std::vector<int32_t> cinst(B->nchunk), gptr, gidx, gpairs, msk;
+  std::vector<int32_t> cinst(B->nchunk), gptr, gidx, gpairs, msk, wgb, wgp;
   gptr.reserve(gs_blocks + 1);
   gidx.reserve(gs_entries);
   gpairs.reserve(2 * gs_pairs);
   msk.reserve(ms);
   int maxblk = 0;
+  const bool pad = maxblk_all <= 8;  // the lane-parallel kernels run
   for (int i = 0; i < ninst; i++) {
     Op *op = B->ops[i];
     BatchInst &I = B->h_inst[i];
@@ -410,6 +426,13 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
     for (int64_t b = 0; b < op->nblks; b++) gptr.push_back(base + ptr[b]);
     for (int64_t q = 0; q < ptr[op->nblks]; q++) gidx.push_back(gi[q] + (int32_t)I.voff);
     for (int64_t q = op->pair0; q < op->nlocal; q++) gpairs.push_back(gi[q] + (int32_t)I.voff);
+    if (pad) {
+      // empty blocks and (-1, -1) pairs up to whole workgroups
+      while (gptr.size() % 32 != 0) gptr.push_back((int32_t)gidx.size());
+      while (gpairs.size() % 256 != 0) gpairs.push_back(-1);
+      while ((int64_t)wgb.size() * 32 < (int64_t)gptr.size()) wgb.push_back(i);
+      while ((int64_t)wgp.size() * 256 < (int64_t)gpairs.size()) wgp.push_back(i);
+    }
     for (int64_t q = 0; q < op->nmsk; q++) msk.push_back(mk[q] + (int32_t)I.voff);
     maxblk = std::max(maxblk, op->max_blk);
     // FP32 Jacobi and multiplicity into the slices
@@ -419,13 +442,30 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
     hipLaunchKernelGGL(d2f_slice_kernel, dim3(g), dim3(256), 0, B->s, B->mult + I.voff,
                        op->f64.mult, op->n);
   }
+  B->gs_nblks = (int64_t)gptr.size();
   gptr.push_back((int32_t)gidx.size());
-  B->gs_nblks = gs_blocks;
   B->gs_pair0 = (int64_t)gidx.size();
   B->gs_npairs = (int64_t)gpairs.size() / 2;
   B->gs_maxblk = maxblk;
   gidx.insert(gidx.end(), gpairs.begin(), gpairs.end());
   B->nmsk = (int64_t)msk.size();
+  if (pad) {
+    // tables over every logical workgroup of the two launches (the XCD
+    // order rounds the count up to a multiple of 8; those extra workgroups
+    // have no entries)
+    const int64_t nb = NM_NXCD * ((B->gs_nblks / 32 + NM_NXCD - 1) / NM_NXCD);
+    const int64_t np = NM_NXCD * ((B->gs_npairs / 128 + NM_NXCD - 1) / NM_NXCD);
+    wgb.resize(std::max<int64_t>(nb, 1), 0);
+    wgp.resize(std::max<int64_t>(np, 1), 0);
+    if (!(B->wg_blk = B->alloc<int32_t>((int64_t)wgb.size())) ||
+        !(B->wg_pair = B->alloc<int32_t>((int64_t)wgp.size())))
+      return fail(NM_ERR_NOMEM);
+    if (hipMemcpyAsync(B->wg_blk, wgb.data(), sizeof(int32_t) * wgb.size(),
+                       hipMemcpyHostToDevice, B->s) != hipSuccess ||
+        hipMemcpyAsync(B->wg_pair, wgp.data(), sizeof(int32_t) * wgp.size(),
+                       hipMemcpyHostToDevice, B->s) != hipSuccess)
+      return fail(NM_ERR_HIP);
+  }
   if (hipMemcpyAsync(B->chunk_inst, cinst.data(), sizeof(int32_t) * cinst.size(),
                      hipMemcpyHostToDevice, B->s) != hipSuccess ||
       hipMemcpyAsync(B->gs_ptr, gptr.data(), sizeof(int32_t) * gptr.size(),

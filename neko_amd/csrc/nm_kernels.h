@@ -119,6 +119,13 @@ struct GsArgs {
   int lx = 0;
   const int32_t *rest = nullptr;
   int64_t nrest = 0;
+  // batched problems (variant 1): every workgroup's blocks / pairs belong
+  // to one problem, wg_blk[b] / wg_pair[b] (logical workgroup b); the
+  // workgroup is a no-op while that problem's int flag at
+  // scal + problem * scal_size + done_off is set.  Pairs of -1 are padding.
+  const int32_t *wg_blk = nullptr, *wg_pair = nullptr;
+  const char *scal = nullptr;
+  int scal_size = 0, done_off = 0;
 };
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s);

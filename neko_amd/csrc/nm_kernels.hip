@@ -642,13 +642,22 @@ __global__ void __launch_bounds__(256)
 // coalesced and every entry's value load is issued by its own lane.  The
 // group sums its values in map order through lane shuffles (same order as
 // the thread-per-block kernel, hence the same bits).
+// done flag of the problem that owns logical workgroup lb (batched gs)
+__device__ inline bool wg_done(const int32_t *__restrict__ wg, const char *__restrict__ scal,
+                               int scal_size, int done_off, int64_t lb) {
+  return wg != nullptr && *(const int *)(scal + (int64_t)wg[lb] * scal_size + done_off) != 0;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     gs_blk8_kernel(T *__restrict__ u, const int32_t *__restrict__ blk_ptr,
                    const int32_t *__restrict__ gidx, int64_t nblks,
-                   const int *__restrict__ skip, int64_t per) {
+                   const int *__restrict__ skip, int64_t per, const int32_t *__restrict__ wg,
+                   const char *__restrict__ scal, int scal_size, int done_off) {
   if (skip != nullptr && *skip) return;
-  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
+  const int64_t lb = xcd_block(per);
+  if (wg_done(wg, scal, scal_size, done_off, lb)) return;
+  const int64_t t = lb * blockDim.x + threadIdx.x;
   const int64_t g = t >> 3;
   const int m = (int)(t & 7);
   const int lane0 = (int)(threadIdx.x & 63) & ~7;
@@ -675,15 +684,19 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     gs_pair_kernel(T *__restrict__ u, const int32_t *__restrict__ gidx,
                    int64_t pair0, int64_t npairs, const int *__restrict__ skip,
-                   int64_t per) {
+                   int64_t per, const int32_t *__restrict__ wg, const char *__restrict__ scal,
+                   int scal_size, int done_off) {
   if (skip != nullptr && *skip) return;
-  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
-  const bool act = t < 2 * npairs;
+  const int64_t lb = xcd_block(per);
+  if (wg_done(wg, scal, scal_size, done_off, lb)) return;
+  const int64_t t = lb * blockDim.x + threadIdx.x;
+  bool act = t < 2 * npairs;
   int32_t idx = 0;
   T v = T(0);
   if (act) {
     idx = gidx[pair0 + t];
-    v = u[idx];
+    act = idx >= 0;  // padding pairs (batched gs) are (-1, -1)
+    if (act) v = u[idx];
   }
   const T o = __shfl_xor(v, 1, 64);
   if (act) u[idx] = v + o;  // a + b == b + a exactly
@@ -748,7 +761,7 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
     if (a.nblks > 0) {
       const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
-                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per);
+                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, nullptr, nullptr, 0, 0);
     }
     if (a.nfaces > 0) {
       const int64_t LL = (int64_t)(a.lx - 2) * (a.lx - 2);
@@ -759,18 +772,20 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
     if (a.nrest > 0) {
       const int64_t per = xcd_per((2 * a.nrest + bs - 1) / bs);
       hipLaunchKernelGGL(gs_pair_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
-                         a.u, a.rest, (int64_t)0, a.nrest, a.skip, per);
+                         a.u, a.rest, (int64_t)0, a.nrest, a.skip, per, nullptr, nullptr, 0, 0);
     }
   } else if (a.variant == 1) {
     if (a.nblks > 0) {
       const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
-                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per);
+                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, a.wg_blk, a.scal,
+                         a.scal_size, a.done_off);
     }
     if (a.npairs > 0) {
       const int64_t per = xcd_per((2 * a.npairs + bs - 1) / bs);
       hipLaunchKernelGGL(gs_pair_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
-                         a.u, a.gidx, a.pair0, a.npairs, a.skip, per);
+                         a.u, a.gidx, a.pair0, a.npairs, a.skip, per, a.wg_pair, a.scal,
+                         a.scal_size, a.done_off);
     }
   } else {
     const int64_t per = xcd_per((items + bs - 1) / bs);
