@@ -359,6 +359,11 @@ struct GsDev {
   int32_t *blk_ptr = nullptr, *gidx = nullptr;
   int64_t nblks = 0, pair0 = 0, npairs = 0;
   int max_blk = 0;
+  // the facet pairs as whole-face descriptors (gs variant 2) when every
+  // pair lies on one, with the element size lx of this layout
+  int32_t *faces = nullptr;
+  int64_t nfaces = 0;
+  int lx = 0;
 };
 
 struct Schwarz {
@@ -410,7 +415,10 @@ struct Hsmg {
 template <typename T>
 static int gs_dev(const GsDev &g, T *u, hipStream_t s) {
   GsArgs<T> a{u, g.blk_ptr, g.gidx, g.nblks, g.pair0, g.npairs, nullptr};
-  a.variant = g.max_blk <= 8 ? 1 : 0;
+  a.variant = g.max_blk <= 8 ? (g.faces ? 2 : 1) : 0;
+  a.faces = g.faces;
+  a.nfaces = g.nfaces;
+  a.lx = g.lx;
   return launch_gs<T>(a, s);
 }
 
@@ -587,6 +595,21 @@ static int gsdev_level(H *own, GsDev *g, const Op *op, int lx, hipStream_t s) {
       !(g->gidx = own->template alloc<int32_t>(nl))) {
     set_error("hsmg: out of device memory");
     return NM_ERR_NOMEM;
+  }
+  if (lx > 2 && g->max_blk <= 8) {
+    std::vector<int32_t> faces, rest;
+    gs_face_pairs(op->nel, lx, gi.data() + g->pair0, g->npairs, faces, rest);
+    if (rest.empty() && !faces.empty()) {
+      if (!(g->faces = own->template alloc<int32_t>((int64_t)faces.size()))) {
+        set_error("hsmg: out of device memory");
+        return NM_ERR_NOMEM;
+      }
+      g->nfaces = (int64_t)faces.size() / 3;
+      g->lx = lx;
+      NM_HIP_TRY(hipMemcpyAsync(g->faces, faces.data(), sizeof(int32_t) * faces.size(),
+                                hipMemcpyHostToDevice, s));
+      NM_HIP_TRY(hipStreamSynchronize(s));
+    }
   }
   NM_HIP_TRY(hipMemcpyAsync(g->blk_ptr, ptr.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice, s));
   NM_HIP_TRY(hipMemcpyAsync(g->gidx, gi.data(), sizeof(int32_t) * nl, hipMemcpyHostToDevice, s));
