@@ -112,6 +112,7 @@ typedef struct nm_server_s nm_server_t;
 typedef struct nm_batch_s nm_batch_t;
 typedef struct nm_phmg_s nm_phmg_t;
 typedef struct nm_proj_s nm_proj_t;
+typedef struct nm_surface_s nm_surface_t;
 
 typedef struct {
   int64_t nel, n, nlocal, nblks, npairs, nmsk, facet_offset;
@@ -272,14 +273,38 @@ int nm_curl(nm_op_t *op, double *w1, double *w2, double *w3, const double *u1,
             const double *u2, const double *u3, hipStream_t s);
 /* pressure residual pnpn_prs_res_cpu_compute (pnpn_res_cpu.f90:31-180):
  * sets h1 = 1/rho, h2 = 0; p_res = -A p + sum_c cdtp(ta_c) with
- * ta = Binv gs(f/rho - (mu/rho) B curl(curl(u_e))).  The facet-normal
- * surface terms (bc_prs_surface, bc_sym_surface) are not included: exact
- * when the velocity vanishes on the velocity-Dirichlet facets (walls) and
- * there are no symmetry facets. */
+ * ta = Binv gs(f/rho - (mu/rho) B curl(curl(u_e))), without the
+ * facet-normal surface terms (exact when the velocity vanishes on the
+ * velocity-Dirichlet facets and there are no symmetry facets; with them:
+ * nm_prs_res_surf). */
 int nm_prs_res(nm_op_t *op, double mu, double rho, const double *p,
                const double *ue, const double *ve, const double *we,
                const double *fx, const double *fy, const double *fz,
                double *p_res, hipStream_t s);
+/* facet-normal surface of a boundary zone (facet_normal_t,
+ * src/bc/facet_normal.f90:233-326; normals and areas as
+ * coef_generate_area_and_normal, src/sem/coef.f90:1195-1370): facets =
+ * nfacets (element 0-based, facet 1..6 in Neko's numbering) pairs, in the
+ * zone's order; box and mesh operators (they carry coordinates) */
+int nm_surface_create(nm_surface_t **out, nm_op_t *op, const int32_t *facets,
+                      int64_t nfacets, hipStream_t s);
+void nm_surface_free(nm_surface_t *sf);
+int nm_surface_info(const nm_surface_t *sf, int64_t *npoints);
+/* host copies of the unique points (0-based local) and their area-weighted
+ * normals, npoints each */
+int nm_surface_normals(const nm_surface_t *sf, int32_t *pts, double *nx,
+                       double *ny, double *nz);
+/* the whole pressure residual pnpn_prs_res_cpu_compute (pnpn_res_cpu.f90:
+ * 31-180): nm_prs_res, then p_res -= (bd/dt) (u . n A) on the points of
+ * prs (bc_prs_surface: the velocity-prescribed facets, u the current
+ * velocity) and -= (ta . n A) on those of sym (bc_sym_surface); either
+ * surface may be NULL */
+int nm_prs_res_surf(nm_op_t *op, double mu, double rho, double bd, double dt,
+                    const double *p, const double *ue, const double *ve,
+                    const double *we, const double *u, const double *v,
+                    const double *w, const double *fx, const double *fy,
+                    const double *fz, const nm_surface_t *prs,
+                    const nm_surface_t *sym, double *p_res, hipStream_t s);
 /* preconditioner of the op's CG/GMRES/batch/serve solvers: 0 Jacobi
  * (src/krylov/bcknd/cpu/pc_jacobi.f90, default), 1 identity (the ksp
  * default without a preconditioner, src/krylov/pc_identity.f90) */

@@ -95,6 +95,11 @@ SIGNATURES = {
     "nm_vel_res": (_i, [_vp, _d, _d, _d, _d] + [_vp] * 10 + [_vp]),
     "nm_curl": (_i, [_vp] * 8),
     "nm_prs_res": (_i, [_vp, _d, _d] + [_vp] * 9),
+    "nm_prs_res_surf": (_i, [_vp, _d, _d, _d, _d] + [_vp] * 14),
+    "nm_surface_create": (_i, [C.POINTER(_vp), _vp, _vp, _i64, _vp]),
+    "nm_surface_free": (None, [_vp]),
+    "nm_surface_info": (_i, [_vp, _pi64]),
+    "nm_surface_normals": (_i, [_vp] * 5),
     "nm_op_create": (_i, [C.POINTER(_vp), _i, _i] + [_vp] * 9 +
                      [_d, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i,
                       _vp, _vp]),

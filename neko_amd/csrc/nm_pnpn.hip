@@ -12,7 +12,10 @@
 // res_c = -res_c - opgrad_c(p) + f_c, where res_c holds A u_c on entry.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <unordered_map>
+#include <vector>
 
 #include "neko_amd.h"
 #include "nm_common.h"
@@ -440,6 +443,17 @@ int curl_full(Op *op, const double *u1, const double *u2, const double *u3, doub
 }  // namespace
 }  // namespace nm
 
+// p_res = -A p + sum_c cdtp(ta_c) with h1 = 1/rho, h2 = 0, where ta =
+// Binv gs(f / rho - (mu/rho) B curl(curl(u_e))) (pnpn_prs_res_cpu_compute);
+// with surfaces also the facet-normal terms.  Local like the reference (gs
+// of p_res is the solver's).
+static int prs_res_impl(nm_op_t *h, double mu, double rho, const double *p, const double *ue,
+                        const double *ve, const double *we, const double *fx,
+                        const double *fy, const double *fz, double *p_res, hipStream_t s,
+                        const char *who, double dtbd = 0.0, const double *u = nullptr,
+                        const double *v = nullptr, const double *w = nullptr,
+                        const nm_surface_t *prs = nullptr, const nm_surface_t *sym = nullptr);
+
 extern "C" {
 
 // curl (src/math/operators.f90 curl; opr_cpu_curl): w = Binv gs(B curl u)
@@ -451,19 +465,236 @@ int nm_curl(nm_op_t *h, double *w1, double *w2, double *w3, const double *u1,
   return curl_full(op, u1, u2, u3, w1, w2, w3, s);
 }
 
-// p_res = -A p + sum_c cdtp(ta_c) with h1 = 1/rho, h2 = 0, where ta =
-// Binv gs(f / rho - (mu/rho) B curl(curl(u_e))) (pnpn_prs_res_cpu_compute
-// without the surface terms); local like the reference (gs of p_res is the
-// solver's).
 int nm_prs_res(nm_op_t *h, double mu, double rho, const double *p, const double *ue,
                const double *ve, const double *we, const double *fx, const double *fy,
                const double *fz, double *p_res, hipStream_t s) {
+  return prs_res_impl(h, mu, rho, p, ue, ve, we, fx, fy, fz, p_res, s, "nm_prs_res");
+}
+
+int nm_prs_res_surf(nm_op_t *h, double mu, double rho, double bd, double dt, const double *p,
+                    const double *ue, const double *ve, const double *we, const double *u,
+                    const double *v, const double *w, const double *fx, const double *fy,
+                    const double *fz, const nm_surface_t *prs, const nm_surface_t *sym,
+                    double *p_res, hipStream_t s) {
+  if (!(dt != 0.0)) {
+    set_error("nm_prs_res_surf: dt must be nonzero");
+    return NM_ERR_ARG;
+  }
+  if (prs && (!u || !v || !w)) {
+    set_error("nm_prs_res_surf: the velocity-prescribed surface needs u, v, w");
+    return NM_ERR_ARG;
+  }
+  return prs_res_impl(h, mu, rho, p, ue, ve, we, fx, fy, fz, p_res, s, "nm_prs_res_surf",
+                      bd / dt, u, v, w, prs, sym);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Facet-normal surfaces (facet_normal_t, src/bc/facet_normal.f90:233-326)
+// ---------------------------------------------------------------------------
+struct nm_surface_s {
+  nm::Op *op = nullptr;
+  int64_t np = 0;
+  int32_t *pts = nullptr;      // unique local points (0-based), first-seen order
+  double *nrm[3] = {nullptr, nullptr, nullptr};  // area-weighted normals
+  std::vector<int32_t> h_pts;
+  std::vector<double> h_nrm[3];
+  ~nm_surface_s() {
+    if (pts) (void)hipFree(pts);
+    for (double *q : nrm)
+      if (q) (void)hipFree(q);
+  }
+};
+
+namespace nm {
+namespace {
+
+// p_res[k] -= c (a0 n0 + a1 n1 + a2 n2)[k] over a surface's points, the sum
+// in Neko's order ((a0 n0 + a1 n1) + a2 n2); c == 1 skips the product
+__global__ void surf_sub_kernel(double *__restrict__ p_res, const int32_t *__restrict__ pts,
+                                const double *__restrict__ n0, const double *__restrict__ n1,
+                                const double *__restrict__ n2, const double *__restrict__ a0,
+                                const double *__restrict__ a1, const double *__restrict__ a2,
+                                double c, int scale, int64_t np) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int32_t k = pts[q];
+  const double t = a0[k] * n0[q] + a1[k] * n1[q] + a2[k] * n2[q];
+  p_res[k] = p_res[k] - (scale ? c * t : t);
+}
+
+int surf_sub(double *p_res, const nm_surface_t *sf, const double *a0, const double *a1,
+             const double *a2, double c, bool scale, hipStream_t s) {
+  if (sf->np <= 0) return NM_OK;
+  hipLaunchKernelGGL(surf_sub_kernel, dim3((unsigned)((sf->np + 255) / 256)), dim3(256), 0, s,
+                     p_res, sf->pts, sf->nrm[0], sf->nrm[1], sf->nrm[2], a0, a1, a2, c,
+                     scale ? 1 : 0, sf->np);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+}  // namespace
+}  // namespace nm
+
+extern "C" {
+
+// Area-weighted normals of the given facets' points as
+// coef_generate_area_and_normal (src/sem/coef.f90:1195-1370) forms them:
+// tangent derivatives (D along the face's two directions), their cross
+// product with sign -,+ / +,- / -,+ on facets 1,2 / 3,4 / 5,6, area =
+// |cross| w_a w_b, unit normal above machine epsilon; summed per unique
+// local point over the facets in list order (facet_normal_finalize).
+int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int64_t nfacets,
+                      hipStream_t s) {
+  using namespace nm;
+  *out = nullptr;
   Op *op = &h->op;
-  int st = check_tri(*op, "nm_prs_res");
+  if (!op->xyz[0]) {
+    set_error("nm_surface_create: the operator carries no coordinates (box or mesh operators do)");
+    return NM_ERR_STATE;
+  }
+  const int lx = op->lx;
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  std::vector<double> D(lx * lx), z(lx), wq(lx), dd(lx * lx), ddt(lx * lx);
+  NM_HIP_TRY(hipMemcpyAsync(D.data(), op->f64.dx, sizeof(double) * lx * lx,
+                            hipMemcpyDeviceToHost, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  if (nm_gll(lx, z.data(), wq.data(), dd.data(), ddt.data())) return NM_ERR_ARG;
+  auto Dm = [&](int i, int m) { return D[i + lx * m]; };
+  nm_surface_t *sf = new nm_surface_t;
+  sf->op = op;
+  std::unordered_map<int64_t, int64_t> slot;
+  std::vector<double> X[3];
+  for (int c = 0; c < 3; c++) X[c].resize(n3);
+  int64_t cur = -1;
+  const double eps = 2.220446049250313e-16;
+  for (int64_t q = 0; q < nfacets; q++) {
+    const int64_t e = facets[2 * q];
+    const int f = facets[2 * q + 1];
+    if (e < 0 || e >= op->nel || f < 1 || f > 6) {
+      set_error("nm_surface_create: facet %lld is (%lld, %d); elements 0..%lld, facets 1..6",
+                (long long)q, (long long)e, f, (long long)op->nel - 1);
+      delete sf;
+      return NM_ERR_ARG;
+    }
+    if (e != cur) {
+      for (int c = 0; c < 3; c++)
+        NM_HIP_TRY(hipMemcpyAsync(X[c].data(), op->xyz[c] + e * n3, sizeof(double) * n3,
+                                  hipMemcpyDeviceToHost, s));
+      NM_HIP_TRY(hipStreamSynchronize(s));
+      cur = e;
+    }
+    auto at = [&](int c, int i, int j, int k) { return X[c][i + lx * (j + lx * k)]; };
+    const int fixed = (f % 2 == 1) ? 0 : lx - 1;
+    const int dir = (f - 1) / 2;  // 0: i fixed, 1: j fixed, 2: k fixed
+    const double sgn = (f == 1 || f == 4 || f == 5) ? -1.0 : 1.0;
+    for (int b = 0; b < lx; b++)
+      for (int a = 0; a < lx; a++) {
+        // face point: (a, b) = (j, k) / (i, k) / (i, j)
+        const int i = dir == 0 ? fixed : a;
+        const int j = dir == 0 ? a : (dir == 1 ? fixed : b);
+        const int k = dir == 2 ? fixed : b;
+        double t1[3], t2[3];
+        for (int c = 0; c < 3; c++) {
+          double dr = 0.0, ds = 0.0, dt = 0.0;
+          for (int m = 0; m < lx; m++) {
+            dr += Dm(i, m) * at(c, m, j, k);
+            ds += Dm(j, m) * at(c, i, m, k);
+            dt += Dm(k, m) * at(c, i, j, m);
+          }
+          t1[c] = dir == 0 ? ds : dr;
+          t2[c] = dir == 2 ? ds : dt;
+        }
+        const double A = t1[1] * t2[2] - t1[2] * t2[1];
+        const double B = t1[2] * t2[0] - t1[0] * t2[2];
+        const double C = t1[0] * t2[1] - t1[1] * t2[0];
+        const double area = std::sqrt(A * A + B * B + C * C) * (wq[a] * wq[b]);
+        double nx = sgn * A, ny = sgn * B, nz = sgn * C;
+        const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+        if (len > eps) {
+          nx = nx / len;
+          ny = ny / len;
+          nz = nz / len;
+        }
+        const int64_t loc = e * n3 + i + lx * (j + (int64_t)lx * k);
+        auto it = slot.find(loc);
+        int64_t sl;
+        if (it == slot.end()) {
+          sl = (int64_t)sf->h_pts.size();
+          slot.emplace(loc, sl);
+          sf->h_pts.push_back((int32_t)loc);
+          for (int c = 0; c < 3; c++) sf->h_nrm[c].push_back(0.0);
+        } else {
+          sl = it->second;
+        }
+        sf->h_nrm[0][sl] = sf->h_nrm[0][sl] + nx * area;
+        sf->h_nrm[1][sl] = sf->h_nrm[1][sl] + ny * area;
+        sf->h_nrm[2][sl] = sf->h_nrm[2][sl] + nz * area;
+      }
+  }
+  sf->np = (int64_t)sf->h_pts.size();
+  const size_t np1 = (size_t)std::max<int64_t>(sf->np, 1);
+  if (hipMalloc(&sf->pts, sizeof(int32_t) * np1) != hipSuccess) {
+    delete sf;
+    set_error("nm_surface_create: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  for (int c = 0; c < 3; c++)
+    if (hipMalloc(&sf->nrm[c], sizeof(double) * np1) != hipSuccess) {
+      delete sf;
+      set_error("nm_surface_create: out of device memory");
+      return NM_ERR_NOMEM;
+    }
+  if (sf->np > 0) {
+    NM_HIP_TRY(hipMemcpyAsync(sf->pts, sf->h_pts.data(), sizeof(int32_t) * sf->np,
+                              hipMemcpyHostToDevice, s));
+    for (int c = 0; c < 3; c++)
+      NM_HIP_TRY(hipMemcpyAsync(sf->nrm[c], sf->h_nrm[c].data(), sizeof(double) * sf->np,
+                                hipMemcpyHostToDevice, s));
+  }
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  *out = sf;
+  return NM_OK;
+}
+
+void nm_surface_free(nm_surface_t *sf) { delete sf; }
+
+int nm_surface_info(const nm_surface_t *sf, int64_t *npoints) {
+  *npoints = sf->np;
+  return NM_OK;
+}
+
+// host copies: points (0-based local), area-weighted normals (npoints each)
+int nm_surface_normals(const nm_surface_t *sf, int32_t *pts, double *nx, double *ny,
+                       double *nz) {
+  for (int64_t q = 0; q < sf->np; q++) {
+    pts[q] = sf->h_pts[q];
+    nx[q] = sf->h_nrm[0][q];
+    ny[q] = sf->h_nrm[1][q];
+    nz[q] = sf->h_nrm[2][q];
+  }
+  return NM_OK;
+}
+
+}  // extern "C"
+
+static int prs_res_impl(nm_op_t *h, double mu, double rho, const double *p, const double *ue,
+                        const double *ve, const double *we, const double *fx,
+                        const double *fy, const double *fz, double *p_res, hipStream_t s,
+                        const char *who, double dtbd, const double *u, const double *v,
+                        const double *w, const nm_surface_t *prs, const nm_surface_t *sym) {
+  using namespace nm;
+  Op *op = &h->op;
+  int st = check_tri(*op, who);
   if (st || (st = ensure_binv(op, s))) return st;
+  if ((prs && prs->op != op) || (sym && sym->op != op)) {
+    set_error("%s: a surface belongs to another operator", who);
+    return NM_ERR_ARG;
+  }
   const int64_t n = op->n;
   if (!op->pnpn_work && !(op->pnpn_work = op->alloc<double>(6 * n))) {
-    set_error("nm_prs_res: out of device memory");
+    set_error("%s: out of device memory", who);
     return NM_ERR_NOMEM;
   }
   double *ta[3] = {op->pnpn_work, op->pnpn_work + n, op->pnpn_work + 2 * n};
@@ -473,8 +704,8 @@ int nm_prs_res(nm_op_t *h, double mu, double rho, const double *p, const double 
   if ((st = curl_full(op, ue, ve, we, ta[0], ta[1], ta[2], s))) return st;
   // second curl up to its gs; its Binv is applied inside prs_src_kernel
   if ((st = launch_curl(*op, ta[0], ta[1], ta[2], wa[0], wa[1], wa[2], s))) return st;
-  for (double *w : wa)
-    if ((st = op->gs<double>(w, nullptr, s))) return st;
+  for (double *x : wa)
+    if ((st = op->gs<double>(x, nullptr, s))) return st;
   hipLaunchKernelGGL(prs_src_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, ta[0], ta[1], ta[2],
                      (const double *)wa[0], (const double *)wa[1], (const double *)wa[2], fx,
                      fy, fz, (const double *)op->binv, (const double *)op->B, rho, mu, n);
@@ -485,7 +716,10 @@ int nm_prs_res(nm_op_t *h, double mu, double rho, const double *p, const double 
                      (const double *)op->binv, n);
   NM_HIP_TRY(hipGetLastError());
   if ((st = op->apply<double>(p_res, p, nullptr, s, false, false))) return st;
-  return launch_cdtp_sum(*op, ta[0], ta[1], ta[2], p_res, s);
+  if ((st = launch_cdtp_sum(*op, ta[0], ta[1], ta[2], p_res, s))) return st;
+  // surface terms (pnpn_res_cpu.f90:134-174): p_res - (bd/dt)(u . nA) on the
+  // velocity-prescribed surface, then - (ta . nA) on the symmetry surface
+  if (prs && (st = surf_sub(p_res, prs, u, v, w, dtbd, true, s))) return st;
+  if (sym && (st = surf_sub(p_res, sym, ta[0], ta[1], ta[2], 1.0, false, s))) return st;
+  return NM_OK;
 }
-
-}  // extern "C"

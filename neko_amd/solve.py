@@ -215,6 +215,24 @@ class Problem:
                                _stream()), "nm_prs_res")
         return out
 
+    def prs_res_surf(self, p, ue, u, f, mu, rho, bd, dt, prs=None, sym=None):
+        """The whole pnpn pressure residual (nm_prs_res_surf): prs_res, then
+        the facet-normal terms of the velocity-prescribed surface `prs` (with
+        the current velocity u) and of the symmetry surface `sym` (Surface
+        objects of this problem, or None)."""
+        for sf in (prs, sym):
+            if sf is not None and sf.problem is not self:
+                raise ValueError("surface of another problem")
+        out = torch.empty_like(p)
+        uu = [_ptr(q, self.n) for q in u] if u is not None else [None, None, None]
+        check(lib().nm_prs_res_surf(self.h, float(mu), float(rho), float(bd), float(dt),
+                                    _ptr(p, self.n), *[_ptr(q, self.n) for q in ue], *uu,
+                                    *[_ptr(q, self.n) for q in f],
+                                    prs.h if prs is not None else None,
+                                    sym.h if sym is not None else None,
+                                    _ptr(out, self.n), _stream()), "nm_prs_res_surf")
+        return out
+
     def set_pc(self, kind: str):
         """Preconditioner of this problem's solvers: "jacobi" (default) or
         "identity" (plain CG, the ksp default without a preconditioner)."""
@@ -389,6 +407,46 @@ class CG:
     def free(self):
         if self.h:
             lib().nm_cg_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Surface:
+    """Facet-normal surface of a boundary zone (Neko's facet_normal_t): the
+    unique points of the given (element 0-based, facet 1..6) pairs and their
+    area-weighted normals (nm_surface_create)."""
+
+    def __init__(self, problem: Problem, facets):
+        import numpy as np
+        f = np.ascontiguousarray(np.asarray(facets, dtype=np.int32).reshape(-1, 2))
+        self.problem = problem
+        self.h = C.c_void_p()
+        check(lib().nm_surface_create(C.byref(self.h), problem.h,
+                                      f.ctypes.data_as(C.c_void_p), int(f.shape[0]),
+                                      _stream()), "nm_surface_create")
+
+    def normals(self):
+        """(points, nx, ny, nz) as numpy arrays (host copies)."""
+        import numpy as np
+        npt = C.c_int64()
+        check(lib().nm_surface_info(self.h, C.byref(npt)), "nm_surface_info")
+        m = npt.value
+        pts = np.zeros(max(m, 1), dtype=np.int32)
+        nx, ny, nz = (np.zeros(max(m, 1)) for _ in range(3))
+        check(lib().nm_surface_normals(self.h, pts.ctypes.data_as(C.c_void_p),
+                                       nx.ctypes.data_as(C.c_void_p),
+                                       ny.ctypes.data_as(C.c_void_p),
+                                       nz.ctypes.data_as(C.c_void_p)), "nm_surface_normals")
+        return pts[:m], nx[:m], ny[:m], nz[:m]
+
+    def free(self):
+        if self.h:
+            lib().nm_surface_free(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -411,6 +411,115 @@ class OracleCase:
         pr = self.ax_local_coeffs(p, 1.0 / rho, 0.0)
         return (-pr) + wa[0] + wa[1] + wa[2]
 
+    def facet_normals(self, facets):
+        """facet_normal_t over a list of (element, facet 1..6) pairs
+        (src/bc/facet_normal.f90:233-326): the unique local points of the
+        facets in first-seen order and their area-weighted normals summed over
+        the facets in list order, the normal and area of each facet point as
+        coef_generate_area_and_normal (src/sem/coef.f90:1195-1370) makes them:
+        cross products of the tangent derivatives (D applied along the face's
+        two directions), signs -,+ / +,- / -,+ for facets 1,2 / 3,4 / 5,6,
+        area = |cross| w_a w_b, normal = cross / |cross| above machine
+        epsilon."""
+        lx = self.lx
+        D = self.d.reshape(lx, lx, order="F")  # D[i, j] = d[i + lx j]
+        X = [c.reshape(self.nel, lx, lx, lx) for c in (self.x, self.y, self.zc)]  # [e, k, j, i]
+        w = self.w
+        eps = np.finfo(np.float64).eps
+        slot = {}
+        pts, nrm = [], []
+        for e, f in facets:
+            e = int(e)
+            f = int(f)
+            xe = [c[e] for c in X]
+            # derivatives along r (i), s (j), t (k) of x, y, z: arrays [k, j, i]
+            dr = [np.einsum("im,kjm->kji", D, c) for c in xe]
+            ds = [np.einsum("jm,kmi->kji", D, c) for c in xe]
+            dt = [np.einsum("km,mji->kji", D, c) for c in xe]
+            if f in (1, 2):
+                t1, t2, sgn = ds, dt, (-1.0 if f == 1 else 1.0)
+                sel = (slice(None), slice(None), 0 if f == 1 else lx - 1)
+            elif f in (3, 4):
+                t1, t2, sgn = dr, dt, (1.0 if f == 3 else -1.0)
+                sel = (slice(None), 0 if f == 3 else lx - 1, slice(None))
+            else:
+                t1, t2, sgn = dr, ds, (-1.0 if f == 5 else 1.0)
+                sel = (0 if f == 5 else lx - 1, slice(None), slice(None))
+            a = t1[1] * t2[2] - t1[2] * t2[1]
+            b = t1[2] * t2[0] - t1[0] * t2[2]
+            c = t1[0] * t2[1] - t1[1] * t2[0]
+            a, b, c = a[sel], b[sel], c[sel]  # 2-D over the face's (slow, fast) axes
+            dot = a * a + b * b + c * c
+            wgt = np.outer(w, w)  # weight of (slow, fast) face coordinates
+            area = np.sqrt(dot) * wgt
+            nx, ny, nz = sgn * a, sgn * b, sgn * c
+            ln = np.sqrt(nx ** 2 + ny ** 2 + nz ** 2)
+            big = ln > eps
+            nx = np.where(big, nx / np.where(big, ln, 1.0), nx)
+            ny = np.where(big, ny / np.where(big, ln, 1.0), ny)
+            nz = np.where(big, nz / np.where(big, ln, 1.0), nz)
+            kk, jj, ii = np.meshgrid(np.arange(lx), np.arange(lx), np.arange(lx), indexing="ij")
+            loc = (e * lx ** 3 + ii + lx * (jj + lx * kk))[sel]
+            for q in range(lx * lx):
+                s_, f_ = divmod(q, lx)
+                k = int(loc[s_, f_])
+                if k not in slot:
+                    slot[k] = len(pts)
+                    pts.append(k)
+                    nrm.append([0.0, 0.0, 0.0])
+                v = nrm[slot[k]]
+                v[0] = v[0] + nx[s_, f_] * area[s_, f_]
+                v[1] = v[1] + ny[s_, f_] * area[s_, f_]
+                v[2] = v[2] + nz[s_, f_] * area[s_, f_]
+        nrm = np.array(nrm, dtype=np.float64).reshape(-1, 3)
+        return np.array(pts, dtype=np.int64), nrm[:, 0].copy(), nrm[:, 1].copy(), nrm[:, 2].copy()
+
+    def prs_res_surf(self, p, ue, u, f, mu, rho, bd, dt, prs=None, sym=None):
+        """pnpn_prs_res_cpu_compute (pnpn_res_cpu.f90:31-180) with the surface
+        terms: p_res -= (bd/dt) (u . n A) on the velocity-prescribed facets'
+        points and -= (ta . n A) on the symmetry facets' points, n A from
+        facet_normals, ta = Binv gs(f/rho - (mu/rho) B curl curl u_e)
+        (facet_normal_apply_surfvec, src/bc/facet_normal.f90:146-171)."""
+        ta = self.curl(*ue)
+        wa = self.curl(*ta)
+        bi = self.binv()
+        ta = [f[c] / rho - ((wa[c] * (mu / rho)) * self.B) for c in range(3)]
+        for t in ta:
+            self.gs(t)
+        ta = [t * bi for t in ta]
+        wa = [self.cdtp(ta[c], c) for c in range(3)]
+        pr = self.ax_local_coeffs(p, 1.0 / rho, 0.0)
+        out = (-pr) + wa[0] + wa[1] + wa[2]
+        dtbd = bd / dt
+        tp = [np.zeros(self.n) for _ in range(3)]
+        if prs is not None:
+            pts, nx, ny, nz = prs
+            tp[0][pts] = u[0][pts] * nx
+            tp[1][pts] = u[1][pts] * ny
+            tp[2][pts] = u[2][pts] * nz
+        ts = [np.zeros(self.n) for _ in range(3)]
+        if sym is not None:
+            pts, nx, ny, nz = sym
+            ts[0][pts] = ta[0][pts] * nx
+            ts[1][pts] = ta[1][pts] * ny
+            ts[2][pts] = ta[2][pts] * nz
+        return out - (dtbd * (tp[0] + tp[1] + tp[2])) - (ts[0] + ts[1] + ts[2])
+
+    def box_facets(self, dims, side):
+        """(element, facet) pairs of one side of a box mesh built by
+        OracleCase (element e = ex + nx (ey + ny ez)); side = facet number
+        1..6 (x-, x+, y-, y+, z-, z+)."""
+        nx, ny, nz = dims
+        out = []
+        for ez in range(nz):
+            for ey in range(ny):
+                for ex in range(nx):
+                    on = {1: ex == 0, 2: ex == nx - 1, 3: ey == 0, 4: ey == ny - 1,
+                          5: ez == 0, 6: ez == nz - 1}[side]
+                    if on:
+                        out.append((ex + nx * (ey + ny * ez), side))
+        return out
+
     def ax_local_coeffs(self, u, h1, h2):
         """ax_helm with constant h1, h2 (h2 != 0 adds h2 B u), no gs."""
         w = np.zeros(self.n)

@@ -109,3 +109,36 @@ def test_curl_and_prs_res_match_oracle():
     ref = c.prs_res(pr, ue, fv, mu, rho)
     got = p.prs_res(d[6], d[0:3], d[3:6], mu, rho)
     assert np.abs(_np(got) - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_surface_normals_and_prs_res_surf_match_oracle():
+    """Facet-normal surfaces (two sides sharing an edge, so edge points get
+    two facets' contributions) and the whole pressure residual with a
+    velocity-prescribed and a symmetry surface: points bit-equal, normals
+    1e-12, residual 1e-10 against the oracle."""
+    from neko_amd.solve import Surface
+    print("surfaces / prs res surf", flush=True)
+    c, p = _box(6)
+    dims = (3, 4, 3)
+    prs_f = c.box_facets(dims, 2) + c.box_facets(dims, 4)
+    sym_f = c.box_facets(dims, 5)
+    sp, ss = Surface(p, prs_f), Surface(p, sym_f)
+    for sf, fl in ((sp, prs_f), (ss, sym_f)):
+        pts, nx, ny, nz = sf.normals()
+        rp, rx, ry, rz = c.facet_normals(fl)
+        assert np.array_equal(pts.astype(np.int64), rp)
+        for g, r in ((nx, rx), (ny, ry), (nz, rz)):
+            assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max()
+    rng = np.random.default_rng(12)
+    f = [rng.standard_normal(c.n) for _ in range(10)]
+    ue, uc, fv, pr = f[:3], f[3:6], f[6:9], f[9]
+    d = [torch.from_numpy(a).cuda() for a in f]
+    mu, rho, bd, dt = 1.0 / 1600, 1.2, 1.5, 2e-3
+    ref = c.prs_res_surf(pr, ue, uc, fv, mu, rho, bd, dt,
+                         prs=c.facet_normals(prs_f), sym=c.facet_normals(sym_f))
+    got = p.prs_res_surf(d[9], d[0:3], d[3:6], d[6:9], mu, rho, bd, dt, prs=sp, sym=ss)
+    assert np.abs(_np(got) - ref).max() <= 1e-10 * np.abs(ref).max()
+    # without surfaces it is nm_prs_res
+    got0 = p.prs_res_surf(d[9], d[0:3], None, d[6:9], mu, rho, bd, dt)
+    ref0 = p.prs_res(d[9], d[0:3], d[6:9], mu, rho)
+    assert torch.equal(got0, ref0)

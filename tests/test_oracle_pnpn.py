@@ -105,3 +105,85 @@ def test_prs_res_of_curl_free_field(case):
     r = c.prs_res(p, g, (zero, zero, zero), mu=0.01, rho=2.0)
     ref = -c.ax_local_coeffs(p, 0.5, 0.0)
     assert np.abs(r - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+# --- facet normals and the pressure residual's surface terms ----------------
+ALL_SIDES = (1, 2, 3, 4, 5, 6)
+
+
+def _sides(c, dims, sides):
+    out = []
+    for s in sides:
+        out += c.box_facets(dims, s)
+    return out
+
+
+def test_facet_normals_of_undeformed_box_are_axis_aligned():
+    """Undeformed unit box: on side s every point's area-weighted normal is
+    along the side's axis with the side's sign, the weight is the GLL
+    quadrature weight of the face (h_a h_b / 4 w_a w_b), and the weights of
+    a side sum to its area 1."""
+    dims = (3, 2, 4)
+    c = O.OracleCase(*dims, 5, amp=0.0, seed=1)
+    h = [1.0 / d for d in dims]
+    for s in ALL_SIDES:
+        pts, nx, ny, nz = c.facet_normals(c.box_facets(dims, s))
+        n = np.stack([nx, ny, nz])
+        ax, sign = (s - 1) // 2, (-1.0 if s % 2 == 1 else 1.0)
+        others = [a for a in range(3) if a != ax]
+        assert np.abs(n[others]).max() < 1e-15
+        assert np.all(sign * n[ax] > 0)
+        assert abs(sign * n[ax].sum() - 1.0) < 1e-13
+        # one facet's corner point: w_0 w_0 h_a h_b / 4
+        w0 = c.w[0]
+        corner = w0 * w0 * h[others[0]] * h[others[1]] / 4.0
+        assert abs(sign * n[ax][0] - corner) < 1e-15
+
+
+def test_closed_surface_normals_sum_to_zero():
+    """Deformed box: the area-weighted normals over all six sides integrate
+    n over a closed (bilinear-faced) surface, which vanishes; GLL is exact
+    for the faces' quadratic integrands."""
+    dims = (3, 2, 3)
+    c = O.OracleCase(*dims, 6, amp=0.1, seed=5)
+    pts, nx, ny, nz = c.facet_normals(_sides(c, dims, ALL_SIDES))
+    for comp in (nx, ny, nz):
+        assert abs(comp.sum()) < 1e-13
+
+
+def test_divergence_theorem_on_deformed_box():
+    """Sum over the boundary of (x nx + y ny + z nz) A = integral of div(x)
+    = 3 volume, and of y nx A = integral of d(y)/dx = 0 (trilinear
+    elements, exact quadrature)."""
+    dims = (3, 2, 3)
+    c = O.OracleCase(*dims, 6, amp=0.1, seed=5)
+    pts, nx, ny, nz = c.facet_normals(_sides(c, dims, ALL_SIDES))
+    flux = (c.x[pts] * nx + c.y[pts] * ny + c.zc[pts] * nz).sum()
+    assert abs(flux - 3.0 * c.B.sum()) < 1e-12
+    assert abs((c.y[pts] * nx).sum()) < 1e-13
+
+
+def test_prs_res_surface_terms_closed_forms():
+    """With no surfaces the whole residual is prs_res; a unit x-velocity
+    through the x+ side (area 1) of the velocity-prescribed surface lowers
+    the residual's sum by bd/dt; with u_e = 0 and f = rho B e_x the
+    symmetry term's ta is e_x, so a symmetry surface on the x+ side lowers
+    the sum by 1."""
+    dims = (3, 2, 3)
+    c = O.OracleCase(*dims, 6, amp=0.1, seed=5)
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal(c.n)
+    ue = [rng.standard_normal(c.n) for _ in range(3)]
+    f = [rng.standard_normal(c.n) for _ in range(3)]
+    mu, rho, bd, dt = 1e-3, 1.3, 1.5, 1e-2
+    base = c.prs_res(p, ue, f, mu, rho)
+    assert np.array_equal(c.prs_res_surf(p, ue, None, f, mu, rho, bd, dt), base)
+    surf = c.facet_normals(c.box_facets(dims, 2))
+    one = [np.ones(c.n), np.zeros(c.n), np.zeros(c.n)]
+    d = c.prs_res_surf(p, ue, one, f, mu, rho, bd, dt, prs=surf) - base
+    assert abs(d.sum() + bd / dt) < 1e-11 * (bd / dt)
+    z = [np.zeros(c.n)] * 3
+    fb = [rho * c.B, np.zeros(c.n), np.zeros(c.n)]
+    base0 = c.prs_res(p, z, fb, mu, rho)
+    d = c.prs_res_surf(p, z, None, fb, mu, rho, bd, dt, sym=surf) - base0
+    assert abs(d.sum() + 1.0) < 1e-12
