@@ -171,7 +171,7 @@ struct Batch {
   // whole workgroups of 32 blocks and its pairs to 128 pairs, with the
   // problem of every logical workgroup, so a converged or replacing
   // problem's gs workgroups exit at once
-  int32_t *wg_blk = nullptr, *wg_pair = nullptr;
+  int32_t *wg_blk = nullptr, *wg_pair = nullptr, *wg_msk = nullptr;
   double *x = nullptr, *f = nullptr, *w64 = nullptr, *part = nullptr, *dot_part = nullptr;
   double *rpart = nullptr;  // scratch of the per-problem replacement
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
@@ -236,7 +236,12 @@ static int batch_iteration(Batch *B, hipStream_t s) {
   }
   int st = launch_gs<float>(ga, s);
   if (st) return st;
-  if ((st = launch_mask<float>(B->w32, B->msk, B->nmsk, nullptr, s))) return st;
+  if (B->wg_msk)
+    st = launch_mask_wg(B->w32, B->msk, B->nmsk, B->wg_msk, B->sc, (int)sizeof(CgScal),
+                        (int)offsetof(CgScal, done), s);
+  else
+    st = launch_mask<float>(B->w32, B->msk, B->nmsk, nullptr, s);
+  if (st) return st;
   const unsigned ni = (unsigned)B->ops.size();
   hipLaunchKernelGGL(balpha_kernel, dim3(ni), dim3(CHUNK), 0, s, B->dot_part, B->inst, B->sc);
   hipLaunchKernelGGL(bupd_kernel, dim3(vb), dim3(CHUNK), 0, s, B->x32, B->r32, B->p32, B->w32,
@@ -380,7 +385,7 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       !(B->chunk_inst = B->alloc<int32_t>(B->nchunk)) ||
       !(B->gs_ptr = B->alloc<int32_t>(gs_blocks + 32 * (int64_t)ninst + 1)) ||
       !(B->gs_idx = B->alloc<int32_t>(gs_entries + 256 * (int64_t)ninst)) ||
-      !(B->msk = B->alloc<int32_t>(ms))) {
+      !(B->msk = B->alloc<int32_t>(ms + 256 * (int64_t)ninst))) {
     set_error("nm_batch_create: out of device memory");
     return fail(NM_ERR_NOMEM);
   }
@@ -396,7 +401,7 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       return fail(NM_ERR_HIP);
   // instance table, chunk table, merged gs and mask (offsets added)
   B->h_inst.resize(ninst);
-  std::vector<int32_t> cinst(B->nchunk), gptr, gidx, gpairs, msk, wgb, wgp;
+  std::vector<int32_t> cinst(B->nchunk), gptr, gidx, gpairs, msk, wgb, wgp, wgm;
   gptr.reserve(gs_blocks + 1);
   gidx.reserve(gs_entries);
   gpairs.reserve(2 * gs_pairs);
@@ -434,6 +439,9 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       while ((int64_t)wgp.size() * 256 < (int64_t)gpairs.size()) wgp.push_back(i);
     }
     for (int64_t q = 0; q < op->nmsk; q++) msk.push_back(mk[q] + (int32_t)I.voff);
+    // mask entries padded to whole workgroups of 256, one problem each
+    while (msk.size() % 256 != 0) msk.push_back(-1);
+    while ((int64_t)wgm.size() * 256 < (int64_t)msk.size()) wgm.push_back(i);
     maxblk = std::max(maxblk, op->max_blk);
     // FP32 Jacobi and multiplicity into the slices
     const unsigned g = (unsigned)std::min<int64_t>((op->n + 255) / 256, 65536);
@@ -449,6 +457,12 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
   B->gs_maxblk = maxblk;
   gidx.insert(gidx.end(), gpairs.begin(), gpairs.end());
   B->nmsk = (int64_t)msk.size();
+  if (!wgm.empty()) {
+    if (!(B->wg_msk = B->alloc<int32_t>((int64_t)wgm.size()))) return fail(NM_ERR_NOMEM);
+    if (hipMemcpyAsync(B->wg_msk, wgm.data(), sizeof(int32_t) * wgm.size(),
+                       hipMemcpyHostToDevice, B->s) != hipSuccess)
+      return fail(NM_ERR_HIP);
+  }
   if (pad) {
     // tables over every logical workgroup of the two launches (the XCD
     // order rounds the count up to a multiple of 8; those extra workgroups

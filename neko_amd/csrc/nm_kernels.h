@@ -142,6 +142,10 @@ int launch_neko_scatter(const T *v, int m, const int *dg, T *u, const int *gd,
 template <typename T>
 int launch_mask(T *x, const int32_t *msk0, int64_t m, const int *skip,
                 hipStream_t s);
+// batched FP32 mask with per-workgroup problem skip (entries padded to 256
+// per problem with -1; wg[b] = problem of workgroup b)
+int launch_mask_wg(float *x, const int32_t *msk, int64_t m, const int32_t *wg,
+                   const void *scal, int scal_size, int done_off, hipStream_t s);
 
 // Deterministic reductions: partial sums into part[RED_BLOCKS * nv] then a
 // one-block combine into out[nv] (double).

@@ -911,6 +911,29 @@ __global__ void mask_kernel(T *__restrict__ x, const int32_t *__restrict__ msk,
   if (t < m) x[msk[t]] = T(0);
 }
 
+// Batched mask: workgroup b's 256 entries belong to problem wg[b] (entries
+// padded with -1), a no-op while that problem's int flag at
+// scal + problem * scal_size + done_off is set.
+__global__ void mask_wg_kernel(float *__restrict__ x, const int32_t *__restrict__ msk,
+                               int64_t m, const int32_t *__restrict__ wg,
+                               const char *__restrict__ scal, int scal_size, int done_off) {
+  if (*(const int *)(scal + (int64_t)wg[blockIdx.x] * scal_size + done_off)) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) {
+    const int32_t k = msk[t];
+    if (k >= 0) x[k] = 0.0f;
+  }
+}
+
+int launch_mask_wg(float *x, const int32_t *msk, int64_t m, const int32_t *wg,
+                   const void *scal, int scal_size, int done_off, hipStream_t s) {
+  if (m <= 0) return NM_OK;
+  hipLaunchKernelGGL(mask_wg_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, msk,
+                     m, wg, (const char *)scal, scal_size, done_off);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
 template <typename T>
 int launch_mask(T *x, const int32_t *msk0, int64_t m, const int *skip,
                 hipStream_t s) {
