@@ -61,20 +61,6 @@ __device__ inline void bsum_pair(double &a, double &b) {
   __syncthreads();
 }
 
-// p = dinv r + beta p
-__global__ void __launch_bounds__(CHUNK)
-    bp_kernel(float *__restrict__ p, const float *__restrict__ r,
-              const float *__restrict__ dinv, const int32_t *__restrict__ chunk_inst,
-              int64_t nchunk, const CgScal *__restrict__ sc) {
-  for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
-    const CgScal &S = sc[chunk_inst[c]];
-    if (S.done) continue;
-    const float beta = (float)S.beta;
-    const int64_t i = c * CHUNK + threadIdx.x;
-    p[i] = dinv[i] * r[i] + beta * p[i];
-  }
-}
-
 // alpha_i from the fused p.Ap partials of the problem's workgroups
 __global__ void __launch_bounds__(CHUNK)
     balpha_kernel(const double *__restrict__ dot_part, const BatchInst *__restrict__ inst,
@@ -91,18 +77,18 @@ __global__ void __launch_bounds__(CHUNK)
   }
 }
 
-// x32 += alpha p; r -= alpha w; chunk partials of rtr and rtz
+// fused schedule (p and x updates in the Ax prologue): r -= alpha w;
+// chunk partials of rtr and rtz
 __global__ void __launch_bounds__(CHUNK)
-    bupd_kernel(float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
-                const float *__restrict__ w, const float *__restrict__ mult,
-                const float *__restrict__ dinv, const int32_t *__restrict__ chunk_inst,
-                int64_t nchunk, const CgScal *__restrict__ sc, double *__restrict__ part) {
+    bupd2_kernel(float *__restrict__ r, const float *__restrict__ w,
+                 const float *__restrict__ mult, const float *__restrict__ dinv,
+                 const int32_t *__restrict__ chunk_inst, int64_t nchunk,
+                 const CgScal *__restrict__ sc, double *__restrict__ part) {
   for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
     const CgScal &S = sc[chunk_inst[c]];
     if (S.done) continue;  // uniform over the block
     const float alpha = (float)S.alpha;
     const int64_t i = c * CHUNK + threadIdx.x;
-    x[i] = x[i] + alpha * p[i];
     const float ri = r[i] - alpha * w[i];
     r[i] = ri;
     double a = (double)ri * (double)ri * (double)mult[i];
@@ -200,10 +186,12 @@ struct Batch {
   }
 };
 
+// One FP32 CG step of every unfinished problem.  Fused schedule: the p
+// update (p = dinv r + beta p) and the previous step's x update ride in the
+// batched Ax kernel (x is one step behind; a replacement flushes it), the
+// update kernel does r -= alpha w and the rtr/rtz partials.
 static int batch_iteration(Batch *B, hipStream_t s) {
   const unsigned vb = (unsigned)std::min<int64_t>(B->nchunk, 65536);
-  hipLaunchKernelGGL(bp_kernel, dim3(vb), dim3(CHUNK), 0, s, B->p32, B->r32, B->dinv,
-                     B->chunk_inst, B->nchunk, B->sc);
   for (const LxGroup &g : B->groups) {
     AxBatchArgs a;
     a.w = B->w32;
@@ -221,6 +209,11 @@ static int batch_iteration(Batch *B, hipStream_t s) {
     a.scal_size = (int)sizeof(CgScal);
     a.done_off = (int)offsetof(CgScal, done);
     a.dot_part = B->dot_part;
+    a.cg_r = B->r32;
+    a.cg_dinv = B->dinv;
+    a.cg_x = B->x32;
+    a.beta_off = (int)offsetof(CgScal, beta);
+    a.alpha_off = (int)offsetof(CgScal, alpha);
     int st = launch_ax_batch(a, s);
     if (st) return st;
   }
@@ -244,8 +237,8 @@ static int batch_iteration(Batch *B, hipStream_t s) {
   if (st) return st;
   const unsigned ni = (unsigned)B->ops.size();
   hipLaunchKernelGGL(balpha_kernel, dim3(ni), dim3(CHUNK), 0, s, B->dot_part, B->inst, B->sc);
-  hipLaunchKernelGGL(bupd_kernel, dim3(vb), dim3(CHUNK), 0, s, B->x32, B->r32, B->p32, B->w32,
-                     B->mult, B->dinv, B->chunk_inst, B->nchunk, B->sc, B->part);
+  hipLaunchKernelGGL(bupd2_kernel, dim3(vb), dim3(CHUNK), 0, s, B->r32, B->w32, B->mult,
+                     B->dinv, B->chunk_inst, B->nchunk, B->sc, B->part);
   hipLaunchKernelGGL(bfin_kernel, dim3(ni), dim3(CHUNK), 0, s, B->part, B->nchunk, B->inst,
                      B->sc);
   NM_HIP_TRY(hipGetLastError());
@@ -605,7 +598,8 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
         left--;
       } else if (S.done) {
         const int64_t o = B->voff[i];
-        if ((st = mx_slice_replace(*B->ops[i], B->x + o, B->x32 + o, B->r32 + o, B->f + o,
+        if ((st = mx_slice_flush(B->x32 + o, B->p32 + o, B->ops[i]->n, B->sc + i, B->s)) ||
+            (st = mx_slice_replace(*B->ops[i], B->x + o, B->x32 + o, B->r32 + o, B->f + o,
                                    B->w64 + o, B->sc + i, B->rpart, B->s)))
           return st;
       }

@@ -76,6 +76,8 @@ int cg_wait(CgSolver *cg);
 int mx_slice_begin(const Op &op, double *x, float *x32, float *r32, float *p32,
                    const double *f, CgScal *sc, double *part, double abs_tol, double rel,
                    double delta, int max_iter, hipStream_t s);
+// x32 += alpha p32, alpha = 0 (the fused schedule's owed x update)
+int mx_slice_flush(float *x32, const float *p32, int64_t n, CgScal *sc, hipStream_t s);
 int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
                      double *w, CgScal *sc, double *part, hipStream_t s);
 

@@ -872,6 +872,17 @@ int mx_slice_begin(const Op &op, double *x, float *x32, float *r32, float *p32,
   return NM_OK;
 }
 
+// the x update a fused (CG prologue) schedule still owes before a
+// replacement: x32 += alpha p32, then alpha = 0 (the next prologue adds
+// nothing for the kept p)
+int mx_slice_flush(float *x32, const float *p32, int64_t n, CgScal *sc, hipStream_t s) {
+  hipLaunchKernelGGL(cg_flush_kernel<float>, dim3(vec_blocks(n)), dim3(256), 0, s, x32, p32, n,
+                     sc);
+  hipLaunchKernelGGL(cg_clear_alpha_kernel, dim3(1), dim3(1), 0, s, sc);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
 int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
                      double *w, CgScal *sc, double *part, hipStream_t s) {
   const int64_t n = op.n;

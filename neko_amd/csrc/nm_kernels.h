@@ -90,6 +90,12 @@ struct AxBatchArgs {
   const void *scal;           // per-instance scalar records
   int scal_size, done_off;    // record size, offset of the int `done`
   double *dot_part;           // per global workgroup: sum u.w
+  // CG prologue (cg_r != nullptr): u is p, updated in place to
+  // dinv r + beta p while x += alpha p_old, with the instance's double
+  // beta and alpha at these record offsets
+  const float *cg_r = nullptr, *cg_dinv = nullptr;
+  float *cg_x = nullptr;
+  int beta_off = 0, alpha_off = 0;
 };
 int launch_ax_batch(const AxBatchArgs &a, hipStream_t s);
 

@@ -411,23 +411,32 @@ __global__ void __launch_bounds__(LX *LX *EPB)
 // belongs to instance blk_inst[.] (instances never share a workgroup); the
 // instance's flag `done` (CgScal-compatible layout: int at done_off bytes)
 // turns its workgroups into no-ops.
-template <int LX, int EPB>
+template <int LX, int EPB, bool CGP>
 __global__ void __launch_bounds__(LX *LX *EPB)
     ax_tri_batch_kernel(float *__restrict__ w, const float *__restrict__ u,
                         const float *__restrict__ dx, const float *__restrict__ zg,
                         const float *__restrict__ wg, const BatchInst *__restrict__ inst,
                         const int32_t *__restrict__ blk_inst, int64_t blk_base,
                         const char *__restrict__ scal, int scal_size, int done_off,
-                        double *__restrict__ dot_part) {
+                        double *__restrict__ dot_part, const float *__restrict__ cg_r,
+                        const float *__restrict__ cg_dinv, float *__restrict__ cg_x,
+                        int beta_off, int alpha_off) {
   const int64_t gb = blk_base + blockIdx.x;
   const int ii = blk_inst[gb];
-  if (*(const int *)(scal + (int64_t)ii * scal_size + done_off)) return;
+  const char *rec = scal + (int64_t)ii * scal_size;
+  if (*(const int *)(rec + done_off)) return;
   const BatchInst I = inst[ii];
   const int64_t e = (gb - I.blk0) * EPB + threadIdx.y;
   const bool active = e < I.nel;
-  ax_tri_body<float, LX, EPB, true>(w, u, dx, zg, wg, I.edges + (active ? e : 0) * 36,
-                                    I.h1c, I.h2c, active,
-                                    I.voff + e * (int64_t)(LX * LX * LX), dot_part + gb);
+  CgPro<float> cp{};
+  if (CGP)
+    cp = CgPro<float>{const_cast<float *>(u), cg_r, cg_dinv, cg_x,
+                      (float)*(const double *)(rec + beta_off),
+                      (float)*(const double *)(rec + alpha_off)};
+  ax_tri_body<float, LX, EPB, true, CGP>(w, CGP ? nullptr : u, dx, zg, wg,
+                                         I.edges + (active ? e : 0) * 36, I.h1c, I.h2c, active,
+                                         I.voff + e * (int64_t)(LX * LX * LX), dot_part + gb,
+                                         cp);
 }
 
 template <int LX>
@@ -436,10 +445,17 @@ static int ax_batch_lx(const AxBatchArgs &a, hipStream_t s) {
 #define NM_AXB_CASE(E)                                                                   \
   case E:                                                                                \
     if constexpr (P * E <= 256) {                                                        \
-      hipLaunchKernelGGL((ax_tri_batch_kernel<LX, E>), dim3((unsigned)a.nblocks),        \
-                         dim3(P, E), 0, s, a.w, a.u, a.dx, a.zg, a.wg, a.inst,           \
-                         a.blk_inst, a.blk_base, (const char *)a.scal, a.scal_size,      \
-                         a.done_off, a.dot_part);                                        \
+      if (a.cg_r)                                                                        \
+        hipLaunchKernelGGL((ax_tri_batch_kernel<LX, E, true>), dim3((unsigned)a.nblocks),  \
+                           dim3(P, E), 0, s, a.w, a.u, a.dx, a.zg, a.wg, a.inst,         \
+                           a.blk_inst, a.blk_base, (const char *)a.scal, a.scal_size,    \
+                           a.done_off, a.dot_part, a.cg_r, a.cg_dinv, a.cg_x,            \
+                           a.beta_off, a.alpha_off);                                     \
+      else                                                                               \
+        hipLaunchKernelGGL((ax_tri_batch_kernel<LX, E, false>), dim3((unsigned)a.nblocks), \
+                           dim3(P, E), 0, s, a.w, a.u, a.dx, a.zg, a.wg, a.inst,         \
+                           a.blk_inst, a.blk_base, (const char *)a.scal, a.scal_size,    \
+                           a.done_off, a.dot_part, nullptr, nullptr, nullptr, 0, 0);     \
       NM_HIP_TRY(hipGetLastError());                                                     \
       return NM_OK;                                                                      \
     }                                                                                    \
