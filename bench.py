@@ -488,7 +488,8 @@ def batched(S, ninst, seed=47, rel_tol=1e-10):
     N(0,1); FP32 CG with FP64 residual replacement to 1e-10 relative, all
     instances in one batch (per-lx autotuned batched Ax, captured graph
     steps) vs the same instances solved one after another by the
-    single-problem mixed solver."""
+    single-problem mixed solver on the same trilinear operator form (fused
+    CG schedule)."""
     import numpy as np
     import torch
     rng = np.random.default_rng(seed)
@@ -520,6 +521,7 @@ def batched(S, ninst, seed=47, rel_tol=1e-10):
     t_seq = 0.0
     seq_iters, max_dev = [], 0.0
     for p, f, xb in zip(probs, fs, xs):
+        p.set_epb(201, 201)  # in-kernel geometry and the fused CG, as the batch
         cg = S.CG(p, "mixed")
         x = torch.zeros_like(f)
         mult = p.field(S.FIELD_MULT)
