@@ -524,6 +524,19 @@ __global__ void surf_sub_kernel(double *__restrict__ p_res, const int32_t *__res
   p_res[k] = p_res[k] - (scale ? c * t : t);
 }
 
+// coordinates of the listed elements, packed: out[(q * 3 + c) * n3 + l]
+__global__ void gather_elems_kernel(double *__restrict__ out, const double *__restrict__ x,
+                                    const double *__restrict__ y, const double *__restrict__ z,
+                                    const int32_t *__restrict__ el, int64_t ne, int64_t n3) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne * n3) return;
+  const int64_t q = t / n3, l = t - q * n3;
+  const int64_t src = (int64_t)el[q] * n3 + l;
+  out[(q * 3 + 0) * n3 + l] = x[src];
+  out[(q * 3 + 1) * n3 + l] = y[src];
+  out[(q * 3 + 2) * n3 + l] = z[src];
+}
+
 int surf_sub(double *p_res, const nm_surface_t *sf, const double *a0, const double *a1,
              const double *a2, double c, bool scale, hipStream_t s) {
   if (sf->np <= 0) return NM_OK;
@@ -562,30 +575,55 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
   NM_HIP_TRY(hipStreamSynchronize(s));
   if (nm_gll(lx, z.data(), wq.data(), dd.data(), ddt.data())) return NM_ERR_ARG;
   auto Dm = [&](int i, int m) { return D[i + lx * m]; };
-  nm_surface_t *sf = new nm_surface_t;
-  sf->op = op;
-  std::unordered_map<int64_t, int64_t> slot;
-  std::vector<double> X[3];
-  for (int c = 0; c < 3; c++) X[c].resize(n3);
-  int64_t cur = -1;
-  const double eps = 2.220446049250313e-16;
+  // the facets' elements, their coordinates gathered on the device and
+  // copied back in one transfer
+  std::unordered_map<int64_t, int64_t> epos;
+  std::vector<int32_t> elist;
   for (int64_t q = 0; q < nfacets; q++) {
     const int64_t e = facets[2 * q];
     const int f = facets[2 * q + 1];
     if (e < 0 || e >= op->nel || f < 1 || f > 6) {
       set_error("nm_surface_create: facet %lld is (%lld, %d); elements 0..%lld, facets 1..6",
                 (long long)q, (long long)e, f, (long long)op->nel - 1);
-      delete sf;
       return NM_ERR_ARG;
     }
-    if (e != cur) {
-      for (int c = 0; c < 3; c++)
-        NM_HIP_TRY(hipMemcpyAsync(X[c].data(), op->xyz[c] + e * n3, sizeof(double) * n3,
-                                  hipMemcpyDeviceToHost, s));
-      NM_HIP_TRY(hipStreamSynchronize(s));
-      cur = e;
+    if (epos.emplace(e, (int64_t)elist.size()).second) elist.push_back((int32_t)e);
+  }
+  const int64_t ne = (int64_t)elist.size();
+  std::vector<double> XE((size_t)std::max<int64_t>(ne, 1) * 3 * n3);
+  if (ne > 0) {
+    int32_t *d_el = nullptr;
+    double *d_x = nullptr;
+    if (hipMalloc(&d_el, sizeof(int32_t) * ne) != hipSuccess ||
+        hipMalloc(&d_x, sizeof(double) * ne * 3 * n3) != hipSuccess) {
+      if (d_el) (void)hipFree(d_el);
+      set_error("nm_surface_create: out of device memory");
+      return NM_ERR_NOMEM;
     }
-    auto at = [&](int c, int i, int j, int k) { return X[c][i + lx * (j + lx * k)]; };
+    hipError_t e1 = hipMemcpyAsync(d_el, elist.data(), sizeof(int32_t) * ne,
+                                   hipMemcpyHostToDevice, s);
+    if (e1 == hipSuccess) {
+      hipLaunchKernelGGL(gather_elems_kernel, dim3((unsigned)((ne * n3 + 255) / 256)), dim3(256),
+                         0, s, d_x, (const double *)op->xyz[0], (const double *)op->xyz[1],
+                         (const double *)op->xyz[2], (const int32_t *)d_el, ne, n3);
+      e1 = hipGetLastError();
+    }
+    if (e1 == hipSuccess)
+      e1 = hipMemcpyAsync(XE.data(), d_x, sizeof(double) * ne * 3 * n3, hipMemcpyDeviceToHost, s);
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(s);
+    (void)hipFree(d_el);
+    (void)hipFree(d_x);
+    NM_HIP_TRY(e1);
+  }
+  nm_surface_t *sf = new nm_surface_t;
+  sf->op = op;
+  std::unordered_map<int64_t, int64_t> slot;
+  const double eps = 2.220446049250313e-16;
+  for (int64_t q = 0; q < nfacets; q++) {
+    const int64_t e = facets[2 * q];
+    const int f = facets[2 * q + 1];
+    const double *xe = XE.data() + epos[e] * 3 * n3;
+    auto at = [&](int c, int i, int j, int k) { return xe[c * n3 + i + lx * (j + lx * k)]; };
     const int fixed = (f % 2 == 1) ? 0 : lx - 1;
     const int dir = (f - 1) / 2;  // 0: i fixed, 1: j fixed, 2: k fixed
     const double sgn = (f == 1 || f == 4 || f == 5) ? -1.0 : 1.0;
