@@ -212,6 +212,11 @@ int nm_mesh_mask(int64_t nel, int lx, const int32_t *fneigh, int32_t *msk,
 int nm_op_create_box(nm_op_t **op, int nelx, int nely, int nelz, int lx,
                      int perx, int pery, int perz, double amp, uint64_t seed,
                      double h1, double h2, hipStream_t s);
+/* zone records of a .nmsh file (nmsh_file.f90:326-380), 4 int32 each:
+ * element (0-based, file order), facet 1..6, type (5 periodic, 7 labelled
+ * boundary), label (p_f of the record: the boundary label of a type-7
+ * zone); nzones as nm_nmsh_info reports */
+int nm_nmsh_zones(const char *path, int64_t nzones, int32_t *out);
 /* Whole problem on a general straight-edged hex mesh (e.g. from
  * nm_nmsh_read): the set-up of examples/poisson/driver.f90 (dofmap, gs,
  * coef, Dirichlet on every facet without a neighbour), constant h1/h2. */

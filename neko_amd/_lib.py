@@ -81,6 +81,7 @@ SIGNATURES = {
                                                           _d, _d, _vp]),
     "nm_nmsh_info": (_i, [C.c_char_p, _pi64, _pi64, _pi64, _pi64]),
     "nm_nmsh_read": (_i, [C.c_char_p, _i64, _vp, _vp, _vp]),
+    "nm_nmsh_zones": (_i, [C.c_char_p, _i64, _vp]),
     "nm_mesh_facet_neigh": (_i, [_i64, _vp, _vp]),
     "nm_mesh_gs_sizes": (_i, [_i64, _i, _vp, _pi64, _pi64]),
     "nm_mesh_gs_map": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _pi64,

@@ -237,6 +237,58 @@ int nm_nmsh_info(const char *path, int64_t *nel, int64_t *nzones, int64_t *nperi
 // later zone overrides an earlier one).  el_idx (may be NULL): the file's
 // global element numbers.  Curved elements (ncurves > 0) are rejected: the
 // operator here takes straight-edged (trilinear) elements only.
+// The zone records of a .nmsh file (nmsh_file.f90:326-380): per zone
+// {element (0-based, in file order), facet 1..6, type (5 periodic, 7
+// labelled), label (the zone's p_f: the boundary label of a type-7 zone)},
+// 4 int32 each; nzones from nm_nmsh_info.
+int nm_nmsh_zones(const char *path, int64_t nzones, int32_t *out) {
+  int64_t nel = 0, nz = 0, np = 0, nc = 0;
+  int st = nm_nmsh_info(path, &nel, &nz, &np, &nc);
+  if (st) return st;
+  if (nz != nzones) {
+    set_error("nmsh %s: holds %lld zones, caller expects %lld", path, (long long)nz,
+              (long long)nzones);
+    return NM_ERR_ARG;
+  }
+  std::vector<double> v((size_t)std::max<int64_t>(nel, 1) * 24);
+  std::vector<int64_t> pid((size_t)std::max<int64_t>(nel, 1) * 8);
+  std::vector<int32_t> el((size_t)std::max<int64_t>(nel, 1));
+  if ((st = nm_nmsh_read(path, nel, v.data(), pid.data(), el.data()))) return st;
+  std::unordered_map<int32_t, int64_t> glb2loc;
+  for (int64_t e = 0; e < nel; e++) glb2loc[el[e]] = e;
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) {
+    set_error("nmsh %s: cannot open", path);
+    return NM_ERR_ARG;
+  }
+  NmshHead h;
+  if ((st = nmsh_head(fp, path, h))) {
+    std::fclose(fp);
+    return st;
+  }
+  std::vector<int32_t> z((size_t)std::max<int64_t>(nz, 1) * 9);
+  const bool ok = nz == 0 || (std::fseek(fp, h.zone_off + 4, SEEK_SET) == 0 &&
+                              std::fread(z.data(), 36, (size_t)nz, fp) == (size_t)nz);
+  std::fclose(fp);
+  if (!ok) {
+    set_error("nmsh %s: truncated zones", path);
+    return NM_ERR_ARG;
+  }
+  for (int64_t q = 0; q < nz; q++) {
+    const int32_t *zq = z.data() + q * 9;
+    auto it = glb2loc.find(zq[0]);
+    if (it == glb2loc.end()) {
+      set_error("nmsh %s: zone %lld names element %d", path, (long long)q, zq[0]);
+      return NM_ERR_ARG;
+    }
+    out[4 * q] = (int32_t)it->second;
+    out[4 * q + 1] = zq[1];
+    out[4 * q + 2] = zq[8];
+    out[4 * q + 3] = zq[3];
+  }
+  return NM_OK;
+}
+
 int nm_nmsh_read(const char *path, int64_t nel, double *vxyz, int64_t *pid,
                  int32_t *el_idx) {
   FILE *fp = std::fopen(path, "rb");

@@ -124,6 +124,29 @@ def read_nmsh(path):
     return vxyz, pid, el
 
 
+def read_nmsh_zones(path):
+    """Zone records of a .nmsh file as an (nzones, 4) int32 array: element
+    (0-based), facet 1..6, type (5 periodic, 7 labelled boundary), label
+    (nm_nmsh_zones)."""
+    b = os.fsencode(str(path))
+    nel, nz, nper, ncur = (C.c_int64() for _ in range(4))
+    check(lib().nm_nmsh_info(b, C.byref(nel), C.byref(nz), C.byref(nper),
+                             C.byref(ncur)), "nm_nmsh_info")
+    out = np.zeros((max(nz.value, 1), 4), dtype=np.int32)
+    check(lib().nm_nmsh_zones(b, nz.value, out.ctypes.data), "nm_nmsh_zones")
+    return out[: nz.value]
+
+
+def zone_facets(zones, label=None):
+    """(element, facet) pairs of the labelled boundary zones (type 7), in
+    file order; label None: all of them (the whole non-periodic boundary)."""
+    z = np.asarray(zones)
+    sel = z[:, 2] == 7
+    if label is not None:
+        sel &= z[:, 3] == int(label)
+    return [(int(e), int(f)) for e, f in z[sel][:, :2]]
+
+
 class Problem:
     """One discretised problem resident on the device (geometry, gs maps,
     mask, mass, multiplicity, Jacobi diagonal)."""

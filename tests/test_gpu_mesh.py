@@ -136,3 +136,28 @@ def test_mesh_gs_variants_bit_exact(name):
         ud = torch.from_numpy(u.copy()).cuda()
         p.gs_op(ud)
         assert np.array_equal(_np(ud), ref), v
+
+
+def test_surfaces_from_reference_mesh_zones():
+    """Facet-normal surfaces built from the hemisphere mesh's own labelled
+    zones: each label's normals equal the oracle's; all labels together
+    close the boundary, so the area-weighted normals sum to zero and the
+    flux of x is 3 x the volume (straight-edged elements, exact GLL)."""
+    from neko_amd.solve import Surface, read_nmsh_zones, zone_facets
+    print("surfaces hemi", flush=True)
+    c, p = _pair("hemi.nmsh", 5)
+    zones = read_nmsh_zones(os.path.join(GOLD, "hemi.nmsh"))
+    for label in sorted(set(int(l) for l in zones[zones[:, 2] == 7][:, 3])):
+        fl = zone_facets(zones, label)
+        pts, nx, ny, nz = Surface(p, fl).normals()
+        rp, rx, ry, rz = c.facet_normals(fl)
+        assert np.array_equal(pts.astype(np.int64), rp)
+        scale = max(np.abs(rx).max(), np.abs(ry).max(), np.abs(rz).max())
+        for g, r in ((nx, rx), (ny, ry), (nz, rz)):
+            assert np.abs(g - r).max() <= 1e-12 * scale
+    pts, nx, ny, nz = Surface(p, zone_facets(zones)).normals()
+    area = np.sqrt(nx ** 2 + ny ** 2 + nz ** 2).sum()
+    for comp in (nx, ny, nz):
+        assert abs(comp.sum()) <= 1e-12 * area
+    flux = (c.x[pts] * nx + c.y[pts] * ny + c.zc[pts] * nz).sum()
+    assert abs(flux - 3.0 * c.B.sum()) <= 1e-11 * abs(flux)

@@ -185,3 +185,22 @@ def test_product_rejects_facet_held_three_times():
     fn = np.zeros(18, dtype=np.int32)
     assert L.nm_mesh_facet_neigh(3, pid.ctypes.data, fn.ctypes.data) != 0
     assert b"held by 3" in L.nm_last_error()
+
+
+@pytest.mark.parametrize("name", ["hemi.nmsh", "lid.nmsh", "poisson_128.nmsh", "tgv_512.nmsh"])
+def test_zone_records_match_oracle_reader(name):
+    """nm_nmsh_zones: every zone record's element (mapped to file order),
+    facet, type and label equal the oracle reader's raw records."""
+    from neko_amd.solve import read_nmsh, read_nmsh_zones, zone_facets
+    path = os.path.join(GOLD, name)
+    z = read_nmsh_zones(path)
+    _, _, zo, _ = O.read_nmsh(path)
+    _, _, el = read_nmsh(path)
+    assert z.shape[0] == zo.shape[0]
+    if z.shape[0]:
+        assert np.array_equal(el[z[:, 0]], zo[:, 0])
+        assert np.array_equal(z[:, 1], zo[:, 1])
+        assert np.array_equal(z[:, 2], zo[:, 8])
+        assert np.array_equal(z[:, 3], zo[:, 3])
+    # labelled facets are the ones the reference reader marks (type 7 only)
+    assert len(zone_facets(z)) == int((zo[:, 8] == 7).sum()) if z.shape[0] else True
