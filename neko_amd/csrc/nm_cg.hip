@@ -405,11 +405,115 @@ __global__ void __launch_bounds__(RED_THREADS)
   }
 }
 
+// lx divisible by 4, FP32: four neighbouring points per thread (16-byte
+// loads and stores); with fewer quads per element than threads, a
+// workgroup takes 256 / (N3 / 4) elements per step.
+template <int LX, bool CNT>
+__global__ void __launch_bounds__(RED_THREADS)
+    cg_update2f_quad_kernel(float *__restrict__ r, const float *__restrict__ w,
+                            const float *__restrict__ mult, const uint8_t *__restrict__ mcnt,
+                            const float *__restrict__ dinv, const int2 *__restrict__ fnb,
+                            int64_t nel, const CgScal *__restrict__ sc,
+                            double *__restrict__ part) {
+  static_assert(LX % 4 == 0, "point quads need lx divisible by 4");
+  constexpr int N3 = LX * LX * LX, NH = N3 / 4;
+  constexpr int EPI = NH >= RED_THREADS ? 1 : RED_THREADS / NH;  // elements per step
+  constexpr int NQ = NH >= RED_THREADS ? (NH + RED_THREADS - 1) / RED_THREADS : 1;
+  if (sc->done) return;
+  __shared__ double sh[16];
+  __shared__ float tab[CNT ? 256 : 1];
+  if (CNT) {
+    for (int c = threadIdx.x; c < 256; c += RED_THREADS) tab[c] = c ? (float)(1.0 / (double)c) : 0.0f;
+    __syncthreads();
+  }
+  const float alpha_t = (float)sc->alpha;
+  const int eo = EPI > 1 ? (int)threadIdx.x / NH : 0;  // element offset in the step
+  const int h0 = EPI > 1 ? (int)threadIdx.x % NH : (int)threadIdx.x;
+  int fq[NQ][4], uq[NQ][4], vq[NQ][4];
+#pragma unroll
+  for (int t = 0; t < NQ; t++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int q = 4 * (h0 + t * RED_THREADS) + c;
+      const int ii = q % LX, jj = (q / LX) % LX, kk = q / (LX * LX);
+      const bool bi = ii == 0 || ii == LX - 1, bj = jj == 0 || jj == LX - 1,
+                 bk = kk == 0 || kk == LX - 1;
+      fq[t][c] = (q < N3 && (int)bi + (int)bj + (int)bk == 1)
+                     ? (bi ? (ii != 0) : (bj ? 2 + (jj != 0) : 4 + (kk != 0)))
+                     : -1;
+      uq[t][c] = bi ? jj : ii;
+      vq[t][c] = bk ? jj : kk;
+    }
+  double a = 0.0, b = 0.0;
+  if (eo < EPI) {
+    for (int64_t e = (int64_t)blockIdx.x * EPI + eo; e < nel; e += (int64_t)UPF_BLOCKS * EPI) {
+#pragma unroll
+      for (int t = 0; t < NQ; t++) {
+        const int h = h0 + t * RED_THREADS;
+        if (h >= NH) break;
+        const int64_t i = e * N3 + 4 * h;
+        const float4 wv = *reinterpret_cast<const float4 *>(w + i);
+        const float4 rv = *reinterpret_cast<const float4 *>(r + i);
+        const float4 dv = *reinterpret_cast<const float4 *>(dinv + i);
+        float m[4];
+        if (CNT) {
+          const uchar4 cv = *reinterpret_cast<const uchar4 *>(mcnt + i);
+          m[0] = tab[cv.x];
+          m[1] = tab[cv.y];
+          m[2] = tab[cv.z];
+          m[3] = tab[cv.w];
+        } else {
+          const float4 mv = *reinterpret_cast<const float4 *>(mult + i);
+          m[0] = mv.x;
+          m[1] = mv.y;
+          m[2] = mv.z;
+          m[3] = mv.w;
+        }
+        float wi[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          if (fq[t][c] >= 0) {
+            const int2 d = fnb[e * 6 + fq[t][c]];
+            if (d.x >= 0) {
+              int ub = (d.y & 64) ? vq[t][c] : uq[t][c], vb = (d.y & 64) ? uq[t][c] : vq[t][c];
+              if (d.y & 128) ub = LX - 1 - ub;
+              if (d.y & 256) vb = LX - 1 - vb;
+              wi[c] = wi[c] + w[(int64_t)d.x * N3 + face_point((d.y >> 3) & 7, ub, vb, LX)];
+            }
+          }
+        }
+        float4 ro;
+        ro.x = rv.x - alpha_t * wi[0];
+        ro.y = rv.y - alpha_t * wi[1];
+        ro.z = rv.z - alpha_t * wi[2];
+        ro.w = rv.w - alpha_t * wi[3];
+        *reinterpret_cast<float4 *>(r + i) = ro;
+        const float ro_[4] = {ro.x, ro.y, ro.z, ro.w};
+        const float dv_[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const double rr = (double)ro_[c] * (double)ro_[c] * (double)m[c];
+          a += rr;
+          b += rr * (double)dv_[c];
+        }
+      }
+    }
+  }
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[UPF_BLOCKS + blockIdx.x] = b;
+  }
+}
+
 template <typename T, int LX, bool CNT>
 static void launch_update2f_lx(const Op &op, T *r, const T *w, const int2 *fnb,
                                const CgScal *sc, double *part, hipStream_t s) {
   const OpFields<T> &F = op.fields<T>();
-  if constexpr (LX % 2 == 0 && std::is_same<T, float>::value)
+  if constexpr (LX % 4 == 0 && std::is_same<T, float>::value)
+    hipLaunchKernelGGL((cg_update2f_quad_kernel<LX, CNT>), dim3(UPF_BLOCKS), dim3(RED_THREADS),
+                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part);
+  else if constexpr (LX % 2 == 0 && std::is_same<T, float>::value)
     hipLaunchKernelGGL((cg_update2f_pair_kernel<T, LX, CNT>), dim3(UPF_BLOCKS),
                        dim3(RED_THREADS), 0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc,
                        part);
