@@ -88,9 +88,9 @@ void hip_addcol3(void *a, void *b, void *c, int *n, hipStream_t strm);
 void hip_addcol4(void *a, void *b, void *c, void *d, int *n,
                  hipStream_t strm);
 
-/* replaces hip_jacobi_update, src/krylov/bcknd/device/hip/pc_jacobi.hip:108.
- * Computes the CPU formula (src/krylov/bcknd/cpu/pc_jacobi.f90:168-264)
- * with every element treated as deformed; glb_cmd_queue. */
+/* replaces hip_jacobi_update, src/krylov/bcknd/device/hip/pc_jacobi.hip:108,
+ * with that kernel's formula (:42-104: G12/G13/G23 cross terms at every face
+ * point; they vanish except on element edges and corners); glb_cmd_queue. */
 void hip_jacobi_update(void *d, void *dxt, void *dyt, void *dzt, void *G11,
                        void *G22, void *G33, void *G12, void *G13, void *G23,
                        int *nel, int *lxp);

@@ -201,26 +201,18 @@ void hip_addcol4(void *a, void *b, void *c, void *d, int *n, hipStream_t strm) {
   check(launch_vec<double>(V_ADDCOL4, (double *)a, (const double *)b, (const double *)c, (const double *)d, 0.0, *n, strm));
 }
 
-// pc_jacobi.hip:108 signature.  Computes the CPU formula of
-// pc_jacobi.f90:168-264 with every element treated as deformed (the device
-// entry point has no deformation flags; hsmg sets all elements deformed,
-// pc_hsmg.f90:265).  Use nm_jacobi_update for per-element flags.
+// pc_jacobi.hip:108 signature and formula: the diagonal of the reference's
+// device kernel (cross terms G12/G13/G23 at every face point, which makes
+// it the exact diagonal of the element stiffness), on glb_cmd_queue.  The
+// CPU formula (pc_jacobi.f90:168-264: cross terms only at the corners of
+// deformed elements) is nm_jacobi_update.
 void hip_jacobi_update(void *d, void *dxt, void *dyt, void *dzt, void *G11,
                        void *G22, void *G33, void *G12, void *G13, void *G23,
                        int *nel, int *lxp) {
   (void)dyt; (void)dzt;
-  static int *ones = nullptr;
-  static int ones_n = 0;
-  if (*nel > ones_n) {
-    if (ones) NM_HIP_CHECK(hipFree(ones));
-    NM_HIP_CHECK(hipMalloc(&ones, sizeof(int) * (*nel)));
-    std::vector<int> h(*nel, 1);
-    NM_HIP_CHECK(hipMemcpy(ones, h.data(), sizeof(int) * (*nel), hipMemcpyHostToDevice));
-    ones_n = *nel;
-  }
   const double *g[6] = {(const double *)G11, (const double *)G22, (const double *)G33,
                         (const double *)G12, (const double *)G13, (const double *)G23};
-  check(launch_jacobi_diag(*nel, *lxp, (const double *)dxt, g, ones, (double *)d, glb()));
+  check(launch_jacobi_diag(*nel, *lxp, (const double *)dxt, g, nullptr, (double *)d, glb()));
 }
 
 // Same with Neko's per-element deformation flags (dfrmd_el, device int32).

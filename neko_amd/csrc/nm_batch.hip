@@ -557,8 +557,15 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
   Batch *B = &h->b;
   const int ni = (int)B->ops.size();
   if (delta <= 0.0) delta = 0.1;
-  hipEvent_t ev;
-  NM_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  // the ordering event is destroyed on every exit path, errors included
+  struct EventGuard {
+    hipEvent_t ev = nullptr;
+    ~EventGuard() {
+      if (ev) (void)hipEventDestroy(ev);
+    }
+  } guard;
+  NM_HIP_TRY(hipEventCreateWithFlags(&guard.ev, hipEventDisableTiming));
+  hipEvent_t ev = guard.ev;
   NM_HIP_TRY(hipEventRecord(ev, stream));
   NM_HIP_TRY(hipStreamWaitEvent(B->s, ev, 0));
   for (int i = 0; i < ni; i++) {
@@ -627,7 +634,6 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
   }
   NM_HIP_TRY(hipEventRecord(ev, B->s));
   NM_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
-  NM_HIP_TRY(hipEventDestroy(ev));
   return NM_OK;
 }
 

@@ -176,7 +176,9 @@ int launch_coef(int64_t nel, int lx, const double *x, const double *y,
                 const double *z, const double *dx, const double *w3,
                 double *g[6], double *jac, double *B, hipStream_t s);
 
-// Jacobi diagonal (CPU formula of pc_jacobi.f90), before h1/h2/gs/invert.
+// Jacobi diagonal before h1/h2/gs/invert: CPU formula of pc_jacobi.f90
+// with per-element flags dfrmd, or (dfrmd == nullptr) the formula of the
+// reference's device kernel (pc_jacobi.hip), cross terms at all face points.
 int launch_jacobi_diag(int64_t nel, int lx, const double *dxt,
                        const double *const g[6], const int *dfrmd,
                        double *d, hipStream_t s);

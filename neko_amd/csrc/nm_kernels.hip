@@ -1202,7 +1202,8 @@ int launch_coef(int64_t nel, int lx, const double *x, const double *y,
 }
 
 // ---------------------------------------------------------------------------
-// Jacobi diagonal, CPU formula (pc_jacobi.f90:168-264): sum_l of G11(l,j,k)
+// Jacobi diagonal, CPU formula (pc_jacobi.f90:168-264), or with dfrmd ==
+// nullptr the device formula (pc_jacobi.hip:42-104): sum_l of G11(l,j,k)
 // dxt(i,l)^2 + G22(i,l,k) dyt(j,l)^2 + G33(i,j,l) dzt(k,l)^2, plus the
 // G12/G13/G23 corner corrections on deformed elements only.
 // ---------------------------------------------------------------------------
@@ -1226,12 +1227,36 @@ __global__ void jacobi_diag_kernel(int64_t nel, int lx,
 #define DT(a, b) dxt[(a) + lx * (b)]
 #define A(arr, a, b, c) arr[o + (a) + lx * ((b) + lx * (c))]
   double acc = 0.0;
-  for (int l = 0; l < lx; l++) acc = acc + A(G11, l, j, k) * (DT(i, l) * DT(i, l));
-  for (int l = 0; l < lx; l++) acc = acc + A(G22, i, l, k) * (DT(j, l) * DT(j, l));
-  for (int l = 0; l < lx; l++) acc = acc + A(G33, i, j, l) * (DT(k, l) * DT(k, l));
+  if (dfrmd == nullptr) {
+    // the device kernel's products (g t) t
+    for (int l = 0; l < lx; l++) acc += A(G11, l, j, k) * DT(i, l) * DT(i, l);
+    for (int l = 0; l < lx; l++) acc += A(G22, i, l, k) * DT(j, l) * DT(j, l);
+    for (int l = 0; l < lx; l++) acc += A(G33, i, j, l) * DT(k, l) * DT(k, l);
+  } else {
+    // the CPU kernel's G dxt**2
+    for (int l = 0; l < lx; l++) acc = acc + A(G11, l, j, k) * (DT(i, l) * DT(i, l));
+    for (int l = 0; l < lx; l++) acc = acc + A(G22, i, l, k) * (DT(j, l) * DT(j, l));
+    for (int l = 0; l < lx; l++) acc = acc + A(G33, i, j, l) * (DT(k, l) * DT(k, l));
+  }
   const int L = lx - 1;
   const bool ci = (i == 0 || i == L), cj = (j == 0 || j == L), ck = (k == 0 || k == L);
-  if (dfrmd[e] && ci && cj && ck) {
+  if (dfrmd == nullptr) {
+    // device formula (pc_jacobi.hip:86-99): the cross terms of every face
+    // point, each branch in the reference's order; D(i,i) vanishes inside
+    // the element, so only edge and corner points receive nonzero terms
+    if (ci) {
+      acc += A(G12, i, j, k) * DT(i, i) * DT(j, j);
+      acc += A(G13, i, j, k) * DT(i, i) * DT(k, k);
+    }
+    if (cj) {
+      acc += A(G12, i, j, k) * DT(j, j) * DT(i, i);
+      acc += A(G23, i, j, k) * DT(j, j) * DT(k, k);
+    }
+    if (ck) {
+      acc += A(G13, i, j, k) * DT(k, k) * DT(i, i);
+      acc += A(G23, i, j, k) * DT(k, k) * DT(j, j);
+    }
+  } else if (dfrmd[e] && ci && cj && ck) {
     // the three correction loops of the reference visit each corner once
     acc = acc + A(G12, i, j, k) * DT(i, i) * DT(j, j) + A(G13, i, j, k) * DT(i, i) * DT(k, k);
     acc = acc + A(G12, i, j, k) * DT(j, j) * DT(i, i) + A(G23, i, j, k) * DT(j, j) * DT(k, k);

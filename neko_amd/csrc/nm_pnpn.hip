@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdint>
 #include <unordered_map>
+#include <memory>
 #include <vector>
 
 #include "neko_amd.h"
@@ -573,7 +574,10 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
   NM_HIP_TRY(hipMemcpyAsync(D.data(), op->f64.dx, sizeof(double) * lx * lx,
                             hipMemcpyDeviceToHost, s));
   NM_HIP_TRY(hipStreamSynchronize(s));
-  if (nm_gll(lx, z.data(), wq.data(), dd.data(), ddt.data())) return NM_ERR_ARG;
+  if (nm_gll(lx, z.data(), wq.data(), dd.data(), ddt.data())) {
+    set_error("nm_surface_create: no GLL rule for lx = %d", lx);
+    return NM_ERR_ARG;
+  }
   auto Dm = [&](int i, int m) { return D[i + lx * m]; };
   // the facets' elements, their coordinates gathered on the device and
   // copied back in one transfer
@@ -615,7 +619,8 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
     (void)hipFree(d_x);
     NM_HIP_TRY(e1);
   }
-  nm_surface_t *sf = new nm_surface_t;
+  // owned until the end: every early return below frees it and its buffers
+  std::unique_ptr<nm_surface_t> sf(new nm_surface_t);
   sf->op = op;
   std::unordered_map<int64_t, int64_t> slot;
   const double eps = 2.220446049250313e-16;
@@ -674,13 +679,11 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
   sf->np = (int64_t)sf->h_pts.size();
   const size_t np1 = (size_t)std::max<int64_t>(sf->np, 1);
   if (hipMalloc(&sf->pts, sizeof(int32_t) * np1) != hipSuccess) {
-    delete sf;
     set_error("nm_surface_create: out of device memory");
     return NM_ERR_NOMEM;
   }
   for (int c = 0; c < 3; c++)
     if (hipMalloc(&sf->nrm[c], sizeof(double) * np1) != hipSuccess) {
-      delete sf;
       set_error("nm_surface_create: out of device memory");
       return NM_ERR_NOMEM;
     }
@@ -692,7 +695,7 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
                                 hipMemcpyHostToDevice, s));
   }
   NM_HIP_TRY(hipStreamSynchronize(s));
-  *out = sf;
+  *out = sf.release();
   return NM_OK;
 }
 

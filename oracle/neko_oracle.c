@@ -1067,6 +1067,47 @@ void or_jacobi_diag(double *d, const double *dt, const double *g11,
 #undef DT
 }
 
+/* The diagonal computed by the reference's HIP kernel jacobi_kernel
+ * (src/krylov/bcknd/device/hip/pc_jacobi.hip:42-104), which the drop-in
+ * hip_jacobi_update reproduces: the same three sums as the CPU formula,
+ * each term g*t*t, then for every point with i (j, k) in {0, lx-1} the
+ * G12/G13 (G12/G23, G13/G23) cross terms, in the kernel's order, on every
+ * element (no deformation flags). */
+void or_jacobi_diag_dev(double *d, const double *dt, const double *g11,
+                        const double *g22, const double *g33, const double *g12,
+                        const double *g13, const double *g23, int nel, int lx) {
+#define DT(a, b) dt[(a) + lx * (b)]
+  const int L = lx - 1;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < nel; e++) {
+    const int64_t o = (int64_t)e * lx * lx * lx;
+#define A(arr, i, j, k) arr[o + (i) + lx * ((j) + lx * (k))]
+    for (int k = 0; k < lx; k++)
+      for (int j = 0; j < lx; j++)
+        for (int i = 0; i < lx; i++) {
+          double v = 0.0;
+          for (int l = 0; l < lx; l++) v = v + A(g11, l, j, k) * DT(i, l) * DT(i, l);
+          for (int l = 0; l < lx; l++) v = v + A(g22, i, l, k) * DT(j, l) * DT(j, l);
+          for (int l = 0; l < lx; l++) v = v + A(g33, i, j, l) * DT(k, l) * DT(k, l);
+          if (i == 0 || i == L) {
+            v = v + A(g12, i, j, k) * DT(i, i) * DT(j, j);
+            v = v + A(g13, i, j, k) * DT(i, i) * DT(k, k);
+          }
+          if (j == 0 || j == L) {
+            v = v + A(g12, i, j, k) * DT(j, j) * DT(i, i);
+            v = v + A(g23, i, j, k) * DT(j, j) * DT(k, k);
+          }
+          if (k == 0 || k == L) {
+            v = v + A(g13, i, j, k) * DT(k, k) * DT(i, i);
+            v = v + A(g23, i, j, k) * DT(k, k) * DT(j, j);
+          }
+          A(d, i, j, k) = v;
+        }
+#undef A
+  }
+#undef DT
+}
+
 /* ===================================================================== */
 /* a10: Jacobi-PCG (cg.f90:140-279)                                       */
 /* ===================================================================== */

@@ -83,6 +83,10 @@ void or_jacobi_diag(double *d, const double *dt, const double *g11,
                     const double *g22, const double *g33, const double *g12,
                     const double *g13, const double *g23, const int *dfrmd,
                     int nel, int lx);
+/* diagonal of the reference's device kernel (pc_jacobi.hip:42-104) */
+void or_jacobi_diag_dev(double *d, const double *dt, const double *g11,
+                        const double *g22, const double *g33, const double *g12,
+                        const double *g13, const double *g23, int nel, int lx);
 
 /* ---- a10: Jacobi-PCG (cg.f90) */
 typedef struct {

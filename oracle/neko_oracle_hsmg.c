@@ -33,10 +33,11 @@
  * the 1-D FDM pencil by generic finite-element assembly of the three
  * elements, the eigen-decomposition by scipy.linalg.eigh, plane_space by
  * undeformed element lengths, the whole Schwarz/FDM path by the exact
- * inverse on one undeformed Dirichlet element, and the V-cycle by its
- * effect on GMRES.  Not pinned beyond the reference's own semantics: the
- * Schwarz overlap weights and the coarse-grid composition (no closed form;
- * restated from schwarz.f90 / pc_hsmg.f90 line by line).
+ * inverse on one undeformed Dirichlet element, the overlap weights by a
+ * brute-force count of the extended subdomains holding each point, and the
+ * V-cycle by the additive two-level operator S_f + P (S_m + P_c A_c^-1
+ * P_c^T) P^T with an exact coarse solve (dense matrices, numpy
+ * interpolation), and by its effect on GMRES.
  */
 #include <math.h>
 #include <stdint.h>

@@ -87,6 +87,7 @@ def lib():
     L.or_glsc3.restype = C.c_double
     L.or_glsc3.argtypes = [_dp, _dp, _dp, C.c_int64]
     L.or_jacobi_diag.argtypes = [_dp, _dp] + [_dp] * 6 + [_ip, C.c_int, C.c_int]
+    L.or_jacobi_diag_dev.argtypes = [_dp, _dp] + [_dp] * 6 + [C.c_int, C.c_int]
     L.or_cg_solve.restype = OrMonitor
     L.or_cg_solve.argtypes = [C.POINTER(OrProblem), C.c_void_p, _dp, _dp,
                               C.c_int, C.c_double]
@@ -760,6 +761,12 @@ class OracleHsmg:
         e = np.zeros(self.case.n)
         rr = np.array(r, dtype=np.float64)
         lib().or_schwarz_compute(C.byref(self.sw_f), e, rr)
+        return e
+
+    def schwarz_mid(self, r):
+        e = np.zeros(self.mid.n)
+        rr = np.array(r, dtype=np.float64)
+        lib().or_schwarz_compute(C.byref(self.sw_m), e, rr)
         return e
 
     def gmres(self, f, abs_tol, max_iter=10000, lgmres=30):

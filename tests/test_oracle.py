@@ -20,9 +20,20 @@ Pinning of each oracle function (reference behaviour it restates):
   or_ax_helm (a4)           tensor-product 1-D GLL stiffness on undeformed
                             boxes; symmetry and A 1 = 0 on deformed meshes.
   or_glsc3 (a8)             reference known answer 4711.
+  or_coef / or_ax_helm on   independent numpy evaluation from the GLL
+  deformed elements         coordinates: Jacobian by D, its inverse and
+                            determinant by numpy.linalg, G = w3 det J J^-1
+                            J^-T; the quadratic form v.A u per element.
   or_jacobi_diag (a12)      = diag of the assembled operator on undeformed
-                            meshes (cross terms vanish); corner cross terms on
-                            deformed meshes pinned by the restatement only.
+                            meshes (cross terms vanish); on deformed meshes
+                            = the element stiffness diagonal (unit-vector
+                            probes of or_ax_helm) at corners, face and
+                            element interiors, and that diagonal minus
+                            2 G_ab D_ii D_jj on edge interiors (the one
+                            cross term pc_jacobi.f90:922-955 omits there).
+  or_jacobi_diag_dev        = the element stiffness diagonal everywhere (the
+                            device kernel pc_jacobi.hip:42-104 adds the cross
+                            terms at every face point).
   or_cg_solve (a10)         dense direct solve on a tiny mesh; spectral
                             accuracy on the sine problem.
   or_gmres_solve (a11)      agrees with CG and the dense solve.
@@ -240,6 +251,55 @@ def test_ax_symmetric_and_constant_nullspace_deformed():
     assert x @ ax > 0
 
 
+def _metric_numpy(c):
+    """Jacobian of the GLL coordinates by D, and G_ab = w3 det(J) sum_k
+    (J^-1)_ak (J^-1)_bk by numpy.linalg (no cofactor formula)."""
+    lx, n3 = c.lx, c.lx ** 3
+    D = c.d.reshape(lx, lx, order="F")
+    shape = (c.nel, lx, lx, lx)  # (e, k, j, i)
+    X = [a.reshape(shape) for a in (c.x, c.y, c.zc)]
+    J = np.zeros(shape + (3, 3))
+    for m, Xm in enumerate(X):
+        J[..., m, 0] = np.einsum("il,ekjl->ekji", D, Xm)   # d/dr
+        J[..., m, 1] = np.einsum("jl,ekli->ekji", D, Xm)   # d/ds
+        J[..., m, 2] = np.einsum("kl,elji->ekji", D, Xm)   # d/dt
+    det = np.linalg.det(J)
+    Ji = np.linalg.inv(J)            # (r,s,t) <- (x,y,z)
+    w3 = c.w3.reshape(1, lx, lx, lx)
+    G = np.einsum("...ak,...bk->...ab", Ji, Ji) * (w3 * det)[..., None, None]
+    return D, J, det, G
+
+
+def test_coef_matches_numpy_metric_deformed():
+    c = O.OracleCase(2, 3, 2, 6, amp=0.2, seed=21)
+    _, _, det, G = _metric_numpy(c)
+    scale = np.max(np.abs(G))
+    for (a, b), got in zip([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)], c.g):
+        assert np.max(np.abs(got - G[..., a, b].reshape(-1))) < 1e-13 * scale
+    w3 = np.tile(c.w3, c.nel)
+    assert np.max(np.abs(c.B - w3 * det.reshape(-1))) < 1e-14 * np.max(np.abs(c.B))
+
+
+def test_ax_quadratic_form_matches_numpy_deformed():
+    c = O.OracleCase(2, 2, 3, 5, amp=0.2, seed=22, h1=1.7)
+    lx, n3 = c.lx, c.lx ** 3
+    D, _, _, G = _metric_numpy(c)
+    rng = np.random.default_rng(3)
+    u, v = rng.standard_normal(c.n), rng.standard_normal(c.n)
+    au = c.ax_local(u)
+    shape = (c.nel, lx, lx, lx)
+
+    def grad(f):
+        f = f.reshape(shape)
+        return np.stack([np.einsum("il,ekjl->ekji", D, f),
+                         np.einsum("jl,ekli->ekji", D, f),
+                         np.einsum("kl,elji->ekji", D, f)], axis=-1)
+    gu, gv = grad(u), grad(v)
+    form = 1.7 * np.einsum("ekjia,ekjiab,ekjib->e", gv, G, gu)
+    got = np.einsum("ei,ei->e", v.reshape(c.nel, n3), au.reshape(c.nel, n3))
+    assert np.max(np.abs(got - form)) < 1e-12 * np.max(np.abs(form))
+
+
 # ---------------------------------------------------------------- a8 ------
 def test_glsc3_known_answer():
     n = 4711
@@ -266,6 +326,59 @@ def test_jacobi_is_assembled_diagonal_undeformed():
     dinv = c.jacobi()
     diag = Q @ np.diag(A)
     assert np.max(np.abs(1.0 / dinv - diag)) < 1e-12 * np.max(np.abs(diag))
+
+
+def _element_stiffness_diag(c):
+    """diag of each element's stiffness by unit-vector probes of or_ax_helm
+    (the operator is element-local: probe point q of every element at
+    once)."""
+    n3 = c.lx ** 3
+    diag = np.zeros(c.n)
+    for q in range(n3):
+        u = np.zeros(c.n)
+        u[q::n3] = 1.0
+        diag[q::n3] = c.ax_local(u)[q::n3]
+    return diag
+
+
+@pytest.mark.parametrize("lx", [3, 4, 6])
+def test_jacobi_formulas_vs_element_stiffness_deformed(lx):
+    c = O.OracleCase(2, 2, 2, lx, amp=0.2, seed=31)
+    assert np.all(c.dfrmd == 1)
+    diag = _element_stiffness_diag(c)
+    L = O.lib()
+    cpu = np.zeros(c.n)
+    L.or_jacobi_diag(cpu, c.dt, *c.g, c.dfrmd, c.nel, lx)
+    dev = np.zeros(c.n)
+    L.or_jacobi_diag_dev(dev, c.dt, *c.g, c.nel, lx)
+    scale = np.max(np.abs(diag))
+    # the device formula is the exact element diagonal at every point
+    assert np.max(np.abs(dev - diag)) < 1e-13 * scale
+    # the CPU formula: exact except on edge interiors, where it lacks the
+    # one nonzero cross term 2 G_ab D_ii D_jj of the two fixed directions
+    n3 = lx ** 3
+    Dd = np.diag(c.d.reshape(lx, lx, order="F"))
+    g12, g13, g23 = c.g[3], c.g[4], c.g[5]
+    end = lambda a: a == 0 or a == lx - 1
+    want = diag.copy()
+    kinds = set()
+    for p in range(n3):
+        i, j, k = p % lx, (p // lx) % lx, p // (lx * lx)
+        ends = end(i) + end(j) + end(k)
+        kinds.add(ends)
+        if ends != 2:
+            continue
+        idx = np.arange(c.nel) * n3 + p
+        if end(i) and end(j):
+            want[idx] -= 2 * g12[idx] * Dd[i] * Dd[j]
+        elif end(i) and end(k):
+            want[idx] -= 2 * g13[idx] * Dd[i] * Dd[k]
+        else:
+            want[idx] -= 2 * g23[idx] * Dd[j] * Dd[k]
+    assert kinds == ({0, 1, 2, 3} if lx > 2 else {3})
+    assert np.max(np.abs(cpu - want)) < 1e-13 * scale
+    # and the omitted term is not negligible (the test can see it)
+    assert np.max(np.abs(cpu - diag)) > 1e-6 * scale
 
 
 # ---------------------------------------------------------------- a10/a11 -
