@@ -672,9 +672,11 @@ static void gs_qsort(int *dg, int *gd, int lo, int hi) {
   }
 }
 
-int or_gs_map(int nel, int lx, const int64_t *dof, const int *fneigh,
-              int *dg, int *gd, int *nlocal_out, int *facet_offset_out,
-              int *blk_len, int *blk_off, int *nblks_out) {
+/* The registration half of gs_init_mapping: dg (gs ids) and gd (1-based
+ * local indices) in registration order, before the sort; returns nlocal
+ * and the facet offset. */
+int or_gs_register(int nel, int lx, const int64_t *dof, const int *fneigh,
+                   int *dg, int *gd, int *facet_offset_out) {
   /* Single rank: dofmap%shared_dof is .false. everywhere, so every point
    * takes the "local" branch of gs_init_mapping.  Registration order:
    * element loop over the 8 vertices (:555-650), table cleared (:652),
@@ -743,8 +745,21 @@ int or_gs_map(int nel, int lx, const int64_t *dof, const int *fneigh,
 #undef PUSH
 #undef LIN
   ht_free(&dm);
-  const int nlocal = n;
-  const int facet_offset = nlocal_dof + 1;
+  *facet_offset_out = nlocal_dof + 1;
+  return n;
+}
+
+/* gs_qsort_dofmap over the whole arrays (1..n), exposed for the comparison
+ * with the reference's own routine */
+void or_gs_qsort_dofmap(int *dg, int *gd, int n) {
+  if (n > 0) gs_qsort(dg, gd, 1, n);
+}
+
+int or_gs_map(int nel, int lx, const int64_t *dof, const int *fneigh,
+              int *dg, int *gd, int *nlocal_out, int *facet_offset_out,
+              int *blk_len, int *blk_off, int *nblks_out) {
+  int facet_offset = 0;
+  const int nlocal = or_gs_register(nel, lx, dof, fneigh, dg, gd, &facet_offset);
   if (nlocal > 0) gs_qsort(dg, gd, 1, nlocal);
   /* gs_find_blks (:1333-1373) */
   int nb = 0, i = 1;

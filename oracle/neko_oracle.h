@@ -39,6 +39,9 @@ void or_xyz(int nel, int lx, const double *zg, const double *vxyz,
             double *x, double *y, double *z);
 
 /* ---- a6: gs mapping (gather_scatter.f90: gs_init_mapping) */
+int or_gs_register(int nel, int lx, const int64_t *dof, const int *fneigh,
+                   int *dg, int *gd, int *facet_offset_out);
+void or_gs_qsort_dofmap(int *dg, int *gd, int n);
 int or_gs_map(int nel, int lx, const int64_t *dof, const int *fneigh,
               int *dg, int *gd, int *nlocal, int *facet_offset,
               int *blk_len, int *blk_off, int *nblks);

@@ -24,9 +24,33 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 
+_REF_SO = os.path.join(_HERE, "_ref", "libref_gs_sort.so")
+_REF_SRC = "/root/reference/src/gs/gather_scatter.f90"
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _SO
+
+
+def build_ref() -> str | None:
+    """Compile the reference's own gs_qsort_dofmap (Fortran, flang) into
+    oracle/_ref/ when the reference tree is present (oracle/build_ref.sh);
+    None when it is not."""
+    if not os.path.exists(_REF_SRC):
+        return None
+    subprocess.run([os.path.join(_HERE, "build_ref.sh")], check=True,
+                   stdout=subprocess.DEVNULL)
+    return _REF_SO
+
+
+def ref_lib():
+    """The reference-built gs sort (oracle/_ref), or None if not built."""
+    if not os.path.exists(_REF_SO):
+        return None
+    R = C.CDLL(_REF_SO)
+    R.ref_gs_qsort_dofmap.argtypes = [_ip, _ip, C.c_int]
+    return R
 
 
 _lib = None
@@ -73,6 +97,10 @@ def lib():
     L.or_dfrmd_el.argtypes = [C.c_int, _dp, _ip]
     L.or_dof_ids.argtypes = [C.c_int, C.c_int, _lp, _lp]
     L.or_xyz.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.or_gs_register.restype = C.c_int
+    L.or_gs_register.argtypes = [C.c_int, C.c_int, _lp, _ip, _ip, _ip,
+                                 C.POINTER(C.c_int)]
+    L.or_gs_qsort_dofmap.argtypes = [_ip, _ip, C.c_int]
     L.or_gs_map.argtypes = [C.c_int, C.c_int, _lp, _ip, _ip, _ip,
                             C.POINTER(C.c_int), C.POINTER(C.c_int), _ip, _ip,
                             C.POINTER(C.c_int)]

@@ -12,9 +12,13 @@ Pinning of each oracle function (reference behaviour it restates):
   (a5, a6)                  gs(1) = multiplicity; structural invariants of the
                             maps; reference sizes (nlocal of the 8^3 case).
                             The tie order inside a block (unstable Hoare sort)
-                            is pinned only by the line-by-line restatement:
-                            parity unpinned beyond it (no reference fixture
-                            holds map arrays).
+                            equals that of the reference's own
+                            gs_qsort_dofmap, compiled from the reference
+                            source (oracle/build_ref.sh), bit for bit on the
+                            registered arrays of the 8^3, 48^3 periodic and
+                            64^3 problems.  The registration order before
+                            the sort is the line-by-line restatement of
+                            gs_init_mapping's loops.
   or_coef (a3)              closed forms on undeformed boxes; sum B = exact
                             trilinear volume (independent Gauss quadrature).
   or_ax_helm (a4)           tensor-product 1-D GLL stiffness on undeformed
@@ -411,3 +415,49 @@ def test_cg_sine_spectral_accuracy():
     u = np.sin(np.pi * c.x) * np.sin(np.pi * c.y) * np.sin(np.pi * c.zc)
     assert mon.converged
     assert np.max(np.abs(x - u)) < 1e-6
+
+
+# ---------------------------------------------------------------- a6 ------
+def _registered(dims, lx, per=(0, 0, 0)):
+    L = O.lib()
+    nx, ny, nz = dims
+    nel = nx * ny * nz
+    vxyz = np.zeros(nel * 24)
+    pid = np.zeros(nel * 8, dtype=np.int64)
+    L.or_box_mesh(nx, ny, nz, *per, 0.0, 0, vxyz, pid)
+    fn = np.zeros(nel * 6, dtype=np.int32)
+    L.or_facet_neigh(nel, pid, fn)
+    dof = np.zeros(nel * lx ** 3, dtype=np.int64)
+    L.or_dof_ids(nel, lx, pid, dof)
+    del vxyz
+    dg = np.zeros(nel * lx ** 3, dtype=np.int32)
+    gd = np.zeros_like(dg)
+    fo = O.C.c_int()
+    n = L.or_gs_register(nel, lx, dof, fn, dg, gd, O.C.byref(fo))
+    return dg[:n].copy(), gd[:n].copy()
+
+
+@pytest.mark.parametrize("dims,lx,per", [
+    ((8, 8, 8), 8, (0, 0, 0)),      # the 8^3 Poisson problem
+    ((3, 4, 5), 6, (1, 0, 1)),
+    ((5, 3, 3), 10, (0, 1, 0)),
+    ((48, 48, 48), 8, (1, 0, 0)),   # the 48^3 periodic-x Helmholtz problem
+    ((64, 64, 64), 8, (0, 0, 0)),   # the 64^3 Poisson problem
+])
+def test_gs_sort_tie_order_matches_reference_routine(dims, lx, per):
+    """The order of equal gs ids after the sort (the summation order inside
+    every gs block) equals that of the reference's own gs_qsort_dofmap
+    (gather_scatter.f90:1291-1330, compiled from the reference source by
+    oracle/build_ref.sh) on the same registered arrays, bit for bit."""
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    dg, gd = _registered(dims, lx, per)
+    dg2, gd2 = dg.copy(), gd.copy()
+    O.lib().or_gs_qsort_dofmap(dg, gd, len(dg))
+    R.ref_gs_qsort_dofmap(dg2, gd2, len(dg2))
+    assert np.array_equal(dg, dg2)
+    assert np.array_equal(gd, gd2)
+    # and the sort is not trivial: equal ids are reordered against the
+    # registration order somewhere
+    assert np.all(np.diff(dg) >= 0)
