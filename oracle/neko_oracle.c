@@ -1140,6 +1140,15 @@ void or_ax_full(const or_problem_t *p, double *w, const double *u) {
 #define CG_P_SPACE 7
 #define BLK 1024
 
+/* optional per-iteration residual record of or_cg_solve: hist[it-1] =
+ * rnorm after iteration it, for it <= cap (test monitor only) */
+static double *cg_hist = NULL;
+static int cg_hist_cap = 0;
+void or_cg_history(double *hist, int cap) {
+  cg_hist = hist;
+  cg_hist_cap = hist ? cap : 0;
+}
+
 or_monitor_t or_cg_solve(const or_problem_t *pb, const double *pcd, double *x,
                          const double *f, int max_iter, double abs_tol) {
   const int64_t n = (int64_t)pb->nel * pb->lx * pb->lx * pb->lx;
@@ -1194,6 +1203,7 @@ or_monitor_t or_cg_solve(const or_problem_t *pb, const double *pcd, double *x,
       rtr = tmp;
     }
     rnorm = sqrt(rtr) * norm_fac;
+    if (cg_hist && iter <= cg_hist_cap) cg_hist[iter - 1] = rnorm;
     if (p_cur == CG_P_SPACE || rnorm < abs_tol || iter == max_iter) {
 #pragma omp parallel for schedule(static)
       for (int64_t i0 = 0; i0 < n; i0 += BLK) {

@@ -113,6 +113,8 @@ typedef struct {
   int converged;
 } or_monitor_t;
 
+/* record rnorm of every iteration of the next or_cg_solve calls (NULL: off) */
+void or_cg_history(double *hist, int cap);
 or_monitor_t or_cg_solve(const or_problem_t *p, const double *pcd,
                          double *x, const double *f, int max_iter,
                          double abs_tol);

@@ -116,6 +116,7 @@ def lib():
     L.or_glsc3.argtypes = [_dp, _dp, _dp, C.c_int64]
     L.or_jacobi_diag.argtypes = [_dp, _dp] + [_dp] * 6 + [_ip, C.c_int, C.c_int]
     L.or_jacobi_diag_dev.argtypes = [_dp, _dp] + [_dp] * 6 + [C.c_int, C.c_int]
+    L.or_cg_history.argtypes = [C.c_void_p, C.c_int]
     L.or_cg_solve.restype = OrMonitor
     L.or_cg_solve.argtypes = [C.POINTER(OrProblem), C.c_void_p, _dp, _dp,
                               C.c_int, C.c_double]
@@ -571,15 +572,26 @@ class OracleCase:
         return float(np.sqrt(self.glsc3(r, self.mult, r)) /
                      np.sqrt(self.volume))
 
-    def cg(self, f, abs_tol, max_iter=10000, pc=None):
+    def cg(self, f, abs_tol, max_iter=10000, pc=None, history=False):
+        """Jacobi-PCG (cg.f90); history=True also returns rnorm after
+        every iteration."""
         x = np.zeros(self.n)
         if pc is None:
             pc = self.jacobi()
         self._pc = pc
-        mon = lib().or_cg_solve(C.byref(self.problem()),
-                                pc.ctypes.data_as(C.c_void_p), x,
-                                np.ascontiguousarray(f), max_iter,
-                                float(abs_tol))
+        hist = np.zeros(max_iter) if history else None
+        if history:
+            lib().or_cg_history(hist.ctypes.data_as(C.c_void_p), max_iter)
+        try:
+            mon = lib().or_cg_solve(C.byref(self.problem()),
+                                    pc.ctypes.data_as(C.c_void_p), x,
+                                    np.ascontiguousarray(f), max_iter,
+                                    float(abs_tol))
+        finally:
+            if history:
+                lib().or_cg_history(None, 0)
+        if history:
+            return x, mon, hist[: mon.iter].copy()
         return x, mon
 
     def gmres(self, f, abs_tol, max_iter=10000, lgmres=30, pc=None):

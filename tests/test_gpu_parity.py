@@ -323,12 +323,16 @@ def test_cg_fused_pap_trilinear(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("lx,per", [(5, (1, 0, 1)), (8, (0, 1, 0))])
+@pytest.mark.parametrize("lx,per", [(5, (1, 0, 1)), (8, (0, 1, 0)), (4, (0, 0, 1)),
+                                    (12, (1, 0, 0))])
 @pytest.mark.parametrize("face_sums", [False, True])
 def test_cg_update_face_sums(precision, lx, per, face_sums):
     """The fused CG update adding the descriptor-face partner values itself
     (and with it switched off, the face gs pass): oracle iterations +-1
-    (FP64), 1e-10 reached (mixed), solution within 1e-9 either way."""
+    (FP64), 1e-10 reached (mixed), solution within 1e-9 either way.  lx 4,
+    8 and 12 run the three layouts of the FP32 four-point update (several
+    elements per workgroup, one, two passes per thread), lx 5 the
+    single-point form."""
     print("cg face sums", precision, lx, per, face_sums, flush=True)
     mon, mon_or, err = _solve_pair(3, 4, 3, lx, 0.1, per, "normal",
                                    precision=precision, code=201, face_sums=face_sums)
