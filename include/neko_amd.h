@@ -101,6 +101,19 @@ void nm_jacobi_update(void *d, void *dxt, void *G11, void *G22, void *G33,
                       void *G12, void *G13, void *G23, void *dfrmd, int *nel,
                       int *lxp, hipStream_t strm);
 
+/* No counterpart in the reference: hip_glsc3_many (above) for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  h_dev[l] = sum_i w_i
+ * v_l[i] mult_i for l < j, written to device memory on stream s (nothing
+ * is copied to the host); v: device array of j device pointers.  Calls no
+ * allocating routine: the partial sums live in a static device array of
+ * the library (512 KiB per GPU, held until the process ends) that bounds
+ * j to 1..64 (else NM_ERR_ARG).  Calls that overlap on one GPU share that
+ * array and race. */
+int nm_glsc3_many_noalloc(double *h_dev, const double *w,
+                          const double *const *v, const double *mult, int j,
+                          int n, hipStream_t s);
+
 /* ======================================================================
  * 2. Handle API
  * ====================================================================== */

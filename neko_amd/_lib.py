@@ -65,6 +65,7 @@ SIGNATURES = {
     "hip_addcol4": (None, [_vp, _vp, _vp, _vp, _pi, _vp]),
     "hip_jacobi_update": (None, [_vp] * 10 + [_pi, _pi]),
     "nm_jacobi_update": (None, [_vp] * 9 + [_pi, _pi, _vp]),
+    "nm_glsc3_many_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _i, _vp]),
     # handle API
     "nm_last_error": (C.c_char_p, []),
     "nm_gll": (_i, [_i, _vp, _vp, _vp, _vp]),
