@@ -443,6 +443,12 @@ int nm_batch_solve(nm_batch_t *b, double *const *x, const double *const *f,
                    nm_monitor_t *mon, hipStream_t stream);
 int nm_batch_info(const nm_batch_t *b, int64_t *total_dof,
                   int64_t *device_bytes);
+/* statistics of the last nm_batch_solve: graph steps (one FP32 iteration
+ * of every unfinished problem each) and the local dof-iterations its Ax
+ * launches covered (problems that finished stay covered until the next
+ * table compaction, after a quarter of the covered ones have finished) */
+int nm_batch_stats(const nm_batch_t *b, int64_t *steps,
+                   double *launched_dof_iter);
 
 /* --- serving loop (no counterpart in Neko; drives the CG above) --------- */
 typedef struct {

@@ -138,6 +138,7 @@ SIGNATURES = {
     "nm_batch_autotune": (_i, [_vp, _i, _pi, _pi, _pi]),
     "nm_batch_solve": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), _d, _i, _d,
                             C.POINTER(Monitor), _vp]),
+    "nm_batch_stats": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(_d)]),
     "nm_batch_info": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nm_server_create": (_i, [C.POINTER(_vp), _d, _i, _i, _i, _i]),
     "nm_server_free": (None, [_vp]),

@@ -166,6 +166,11 @@ struct Batch {
   hipGraphExec_t graph = nullptr;
   int graph_iters = 8;
   bool compacted = false;   // tables currently cover a subset
+  int64_t covered_dof = 0;  // local dofs of the problems the tables cover
+  // statistics of the last solve: graph steps, and dof-iterations the Ax
+  // launches covered (finished problems inside still-covered tables count)
+  int64_t stat_steps = 0;
+  double stat_launched = 0.0;
   int64_t bytes = 0;
   std::vector<void *> owned;
   ~Batch() {
@@ -271,10 +276,12 @@ static int batch_tables(Batch *B, const std::vector<char> *finished = nullptr) {
   const int ni = (int)B->ops.size();
   int64_t gb = 0;
   std::vector<int32_t> bi;
+  B->covered_dof = 0;
   for (LxGroup &g : B->groups) {
     g.blk_base = gb;
     for (int m : g.members) {
       if (finished && (*finished)[m]) continue;
+      B->covered_dof += B->ops[m]->n;
       BatchInst &I = B->h_inst[m];
       I.blk0 = gb;
       I.nblk = (I.nel + g.epb - 1) / g.epb;
@@ -592,9 +599,13 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
   std::vector<char> state(ni, 0);  // 0 running, 1 finished
   int left = ni, launches = 0, covered = ni;
   const int limit = max_iter / B->graph_iters + 64;
+  B->stat_steps = 0;
+  B->stat_launched = 0.0;
   while (left > 0) {
     NM_HIP_TRY(hipGraphLaunch(B->graph, B->s));
     launches++;
+    B->stat_steps += B->graph_iters;
+    B->stat_launched += (double)B->graph_iters * (double)B->covered_dof;
     NM_HIP_TRY(hipMemcpyAsync(B->h_sc, B->sc, sizeof(CgScal) * ni, hipMemcpyDeviceToHost, B->s));
     NM_HIP_TRY(hipStreamSynchronize(B->s));
     for (int i = 0; i < ni; i++) {
@@ -615,8 +626,10 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
       set_error("nm_batch_solve: problems did not stop after %d graph launches", launches);
       return NM_ERR_STATE;
     }
-    // drop finished problems from the Ax launches once half are done
-    if (left > 0 && 2 * left <= covered) {
+    // drop finished problems from the Ax launches once a quarter of the
+    // covered ones are done (a re-capture costs far less than the Ax work
+    // of the finished problems it removes)
+    if (left > 0 && 4 * left <= 3 * covered) {
       if ((st = batch_tables(B, &state)) || (st = batch_capture(B))) return st;
       B->compacted = true;
       covered = left;
@@ -634,6 +647,12 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
   }
   NM_HIP_TRY(hipEventRecord(ev, B->s));
   NM_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
+  return NM_OK;
+}
+
+int nm_batch_stats(const nm_batch_t *h, int64_t *steps, double *launched_dof_iter) {
+  *steps = h->b.stat_steps;
+  *launched_dof_iter = h->b.stat_launched;
   return NM_OK;
 }
 

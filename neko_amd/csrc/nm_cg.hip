@@ -871,10 +871,10 @@ static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
 // a captured graph); cg_begin calls it again whenever op->epoch moved.
 static int cg_capture(CgSolver *cg) {
   Op *op = cg->op;
-  // per-workgroup partials of the fused p.Ap (trilinear Ax variants)
+  // per-workgroup partials of the fused p.Ap
   const int code = cg->mixed ? op->ax_code<float>() : op->ax_code<double>();
   cg->npart = 0;
-  if (code >= 200) {
+  {
     cg->npart = ax_grid(op->nel, op->lx, code);
     // the buffer also takes the face-sum update's 2 UPF_BLOCKS partials
     const int64_t need = std::max<int64_t>(cg->npart, 2 * UPF_BLOCKS);

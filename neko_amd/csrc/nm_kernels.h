@@ -56,9 +56,10 @@ struct AxArgs {
   const T *wg = nullptr;   // GLL weights
   T h2c = T(0);            // constant h2 (mass term h2*B*u when nonzero)
   // optional fused partial dot sum_i u_i (A_local u)_i, one double per
-  // workgroup (grid size returned by ax_grid)
+  // workgroup (grid size returned by ax_grid); the stored-geometry
+  // variants take it only together with the CG prologue
   double *dot_part = nullptr;
-  // optional fused CG prologue (trilinear variants, with dot_part): u is
+  // optional fused CG prologue (with dot_part): u is
   // the search direction p, replaced by dinv r + beta p while loading, and
   // x += alpha p_old; beta/alpha read from the device
   const T *cg_r = nullptr, *cg_dinv = nullptr;

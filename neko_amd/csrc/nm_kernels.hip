@@ -26,7 +26,29 @@ namespace nm {
 // ax_helm_cpu.f90:180-290 (generic) / :1211-1341 (lx=8); device twin
 // ax_helm_kernel.h:149-355.
 // ---------------------------------------------------------------------------
-template <typename T, int LX, int EPB, bool PF>
+// Fixed-order workgroup sum of one double per thread (the tree does not
+// need whole waves); thread 0 stores it.  Every thread must call it.
+template <int NT>
+__device__ __forceinline__ void wg_sum_store(double v, double *sdot, int flat,
+                                             double *__restrict__ out) {
+  constexpr int H = NT <= 2 ? 1 : (NT <= 4 ? 2 : (NT <= 8 ? 4 : (NT <= 16 ? 8 :
+                    (NT <= 32 ? 16 : (NT <= 64 ? 32 : (NT <= 128 ? 64 : 128))))));
+  sdot[flat] = v;
+  __syncthreads();
+  for (int h = H; h > 0; h >>= 1) {
+    if (flat < h && flat + h < NT) sdot[flat] += sdot[flat + h];
+    __syncthreads();
+  }
+  if (flat == 0) *out = sdot[0];
+}
+
+// CG: the CG prologue and the p.Ap partial ride in the operator, as in the
+// in-kernel-geometry kernel below: the operand is the new search direction
+// p = dinv r + beta p_old (formed while loading, written back over p_old),
+// the previous step's x += alpha p_old uses the same loads, and each
+// workgroup stores sum_i p_i (A_local p)_i (mass term included) to
+// dot_part[blockIdx.x].
+template <typename T, int LX, int EPB, bool PF, bool CG>
 __global__ void __launch_bounds__(LX *LX *EPB)
     ax_kstep_kernel(T *__restrict__ w, const T *__restrict__ u,
                     const T *__restrict__ dx, const T *__restrict__ h1, T h1c,
@@ -34,12 +56,17 @@ __global__ void __launch_bounds__(LX *LX *EPB)
                     const T *__restrict__ g33, const T *__restrict__ g12,
                     const T *__restrict__ g13, const T *__restrict__ g23,
                     const T *__restrict__ h2b, int64_t nel,
-                    const int *__restrict__ skip) {
+                    const int *__restrict__ skip, double *__restrict__ dot_part,
+                    const T *__restrict__ cg_r, const T *__restrict__ cg_dinv,
+                    T *__restrict__ cg_x, const double *__restrict__ cg_beta,
+                    const double *__restrict__ cg_alpha) {
   constexpr int P = LX * LX;
   constexpr int N3 = LX * P;
   __shared__ T su[EPB][N3];     // element u; plane k then holds the r/s part of w
   __shared__ T sr[EPB][2][P];   // r-flux plane, double buffered
   __shared__ T ss[EPB][2][P];   // s-flux plane, double buffered
+  __shared__ T sD[P];           // D, for the columns of the transposed contraction
+  __shared__ double sdot[CG ? P * EPB : 1];
 
   if (skip != nullptr && *skip) return;
 
@@ -55,20 +82,50 @@ __global__ void __launch_bounds__(LX *LX *EPB)
   // LDS for the r/s derivatives; plane k is P consecutive values, so both
   // the loads and the LDS stores are coalesced.
   T uc[LX];
+  if (CG) {
+    // all loads of the column before any store (the arrays are distinct,
+    // which the compiler cannot know)
+    T *p = const_cast<T *>(u);
+    const T beta = (T)*cg_beta, alpha = (T)*cg_alpha;
+    T po[LX], pr[LX], pd[LX], px[LX];
 #pragma unroll
-  for (int k = 0; k < LX; k++) {
-    uc[k] = active ? u[base + k * P] : T(0);
-    su[q][k * P + t] = uc[k];
+    for (int k = 0; k < LX; k++) {
+      const int64_t ix = base + k * P;
+      po[k] = active ? p[ix] : T(0);
+      pr[k] = active ? cg_r[ix] : T(0);
+      pd[k] = active ? cg_dinv[ix] : T(0);
+      px[k] = active ? cg_x[ix] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < LX; k++) {
+      uc[k] = pd[k] * pr[k] + beta * po[k];
+      su[q][k * P + t] = uc[k];
+    }
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < LX; k++) {
+        const int64_t ix = base + k * P;
+        p[ix] = uc[k];
+        cg_x[ix] = px[k] + alpha * po[k];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < LX; k++) {
+      uc[k] = active ? u[base + k * P] : T(0);
+      su[q][k * P + t] = uc[k];
+    }
   }
 
-  // rows D(i,:), D(j,:) and columns D(:,i), D(:,j) of the thread
-  T Di[LX], Dj[LX], Dti[LX], Dtj[LX];
+  // rows D(i,:), D(j,:) in registers; the columns D(:,i), D(:,j) of the
+  // transposed contraction from LDS (broadcast-free, conflict-light reads),
+  // which leaves the registers to the streamed geometry
+  if (q == 0) sD[t] = dx[t];
+  T Di[LX], Dj[LX];
 #pragma unroll
   for (int l = 0; l < LX; l++) {
     Di[l] = dx[i + LX * l];
     Dj[l] = dx[j + LX * l];
-    Dti[l] = dx[l + LX * i];
-    Dtj[l] = dx[l + LX * j];
   }
   // t-direction transpose accumulates in registers (scatter form)
   T wc[LX];
@@ -129,20 +186,23 @@ __global__ void __launch_bounds__(LX *LX *EPB)
     __syncthreads();
     T acc = T(0);
 #pragma unroll
-    for (int l = 0; l < LX; l++) acc += Dti[l] * sr[q][b][l + LX * j];
+    for (int l = 0; l < LX; l++) acc += sD[l + LX * i] * sr[q][b][l + LX * j];
 #pragma unroll
-    for (int l = 0; l < LX; l++) acc += Dtj[l] * ss[q][b][i + LX * l];
+    for (int l = 0; l < LX; l++) acc += sD[l + LX * j] * ss[q][b][i + LX * l];
     su[q][k * P + t] = acc;  // plane k is dead: its r/s reads precede the barrier
   }
 
+  double dsum = 0.0;
   if (active) {
 #pragma unroll
     for (int m = 0; m < LX; m++) {
       T r = wc[m] + su[q][m * P + t];
       if (h2b != nullptr) r += h2b[base + m * P] * uc[m];
       w[base + m * P] = r;
+      if (CG) dsum += (double)r * (double)uc[m];
     }
   }
+  if (CG) wg_sum_store<P * EPB>(dsum, sdot, q * P + t, dot_part + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,21 +430,7 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
       if (DOT) dsum += (double)r * (double)uc[m];
     }
   }
-  if (DOT) {
-    // fixed-order tree in LDS (workgroups need not be whole waves)
-    constexpr int NT = LX * LX * EPB;
-    constexpr int H = NT <= 1 ? 1 : (NT <= 2 ? 1 : (NT <= 4 ? 2 : (NT <= 8 ? 4 :
-                      (NT <= 16 ? 8 : (NT <= 32 ? 16 : (NT <= 64 ? 32 :
-                      (NT <= 128 ? 64 : 128)))))));
-    const int flat = q * P + t;
-    sdot[flat] = dsum;
-    __syncthreads();
-    for (int h = H; h > 0; h >>= 1) {
-      if (flat < h && flat + h < NT) sdot[flat] += sdot[flat + h];
-      __syncthreads();
-    }
-    if (flat == 0) *dot_out = sdot[0];
-  }
+  if (DOT) wg_sum_store<LX * LX * EPB>(dsum, sdot, q * P + t, dot_out);
 }
 
 template <typename T, int LX, int EPB, bool DOT, bool CGP>
@@ -520,9 +566,24 @@ static int ax_go(const AxArgs<T> &a, hipStream_t s) {
   dim3 block(LX * LX, EPB);
   dim3 grid((unsigned)((a.nel + EPB - 1) / EPB));
   if (a.nel <= 0) return NM_OK;
-  hipLaunchKernelGGL((ax_kstep_kernel<T, LX, EPB, PF>), grid, block, 0, s, a.w,
-                     a.u, a.dx, a.h1, a.h1c, a.g[0], a.g[1], a.g[2], a.g[3],
-                     a.g[4], a.g[5], a.h2b, a.nel, a.skip);
+  if (a.cg_r) {
+    if (!a.dot_part || !a.cg_dinv || !a.cg_x || !a.cg_beta || !a.cg_alpha) {
+      set_error("ax: fused CG prologue needs dot partials, dinv, x and scalars");
+      return NM_ERR_ARG;
+    }
+    hipLaunchKernelGGL((ax_kstep_kernel<T, LX, EPB, PF, true>), grid, block, 0, s, a.w, a.u,
+                       a.dx, a.h1, a.h1c, a.g[0], a.g[1], a.g[2], a.g[3], a.g[4], a.g[5],
+                       a.h2b, a.nel, a.skip, a.dot_part, a.cg_r, a.cg_dinv, a.cg_x,
+                       a.cg_beta, a.cg_alpha);
+  } else if (a.dot_part) {
+    set_error("ax: the stored-geometry variants fuse p.Ap only with the CG prologue");
+    return NM_ERR_UNSUPPORTED;
+  } else {
+    hipLaunchKernelGGL((ax_kstep_kernel<T, LX, EPB, PF, false>), grid, block, 0, s, a.w, a.u,
+                       a.dx, a.h1, a.h1c, a.g[0], a.g[1], a.g[2], a.g[3], a.g[4], a.g[5],
+                       a.h2b, a.nel, a.skip, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr);
+  }
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -551,10 +612,6 @@ static int ax_epb(const AxArgs<T> &a, int code, hipStream_t s) {
   const int epb = ax_epb_of(LX, code);
   if (tri != (a.edges != nullptr)) {
     set_error("ax: variant %d needs %s geometry", code, tri ? "trilinear" : "stored");
-    return NM_ERR_UNSUPPORTED;
-  }
-  if (a.dot_part && !tri) {
-    set_error("ax: fused dot product only in the trilinear variants");
     return NM_ERR_UNSUPPORTED;
   }
   if (tri) {

@@ -113,9 +113,10 @@ struct Op {
   // variant code of the Ax kernel used for precision T
   template <typename T>
   int ax_code() const;
-  // can the fused p.Ap dot run (trilinear variant active)?
+  // can the fused p.Ap dot run?  Every Ax variant fuses it together with
+  // the CG prologue.
   template <typename T>
-  bool fused_dot() const { return ax_code<T>() >= 200; }
+  bool fused_dot() const { return true; }
   template <typename T>
   int gs(T *u, const int *skip, hipStream_t s) const;
 };

@@ -117,11 +117,8 @@ int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
     a.zg = F.zg;
     a.wg = F.wg;
     a.h2c = (T)(ifh2 ? h2c : 0.0);
-    a.dot_part = dot_part;
-  } else if (dot_part) {
-    set_error("apply: fused dot product needs the trilinear Ax variant");
-    return NM_ERR_STATE;
   }
+  a.dot_part = dot_part;
   int st = launch_ax<T>(a, code, s);
   if (st) return st;
   if (do_gs) {
@@ -136,8 +133,8 @@ int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double
                   const int *skip, hipStream_t s, double *dot_part, bool defer_faces) const {
   const OpFields<T> &F = fields<T>();
   const int code = ax_code<T>();
-  if (code < 200 || !dot_part) {
-    set_error("apply_cgp: needs the trilinear Ax variant and dot partials");
+  if (!dot_part) {
+    set_error("apply_cgp: needs dot partials");
     return NM_ERR_STATE;
   }
   AxArgs<T> a;
@@ -148,13 +145,17 @@ int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double
   a.h1c = (T)h1c;
   for (int c = 0; c < 6; c++) a.g[c] = F.g[c];
   a.h2b = nullptr;
+  if (code >= 200) {
+    a.edges = F.edges;
+    a.zg = F.zg;
+    a.wg = F.wg;
+    a.h2c = (T)(ifh2 ? h2c : 0.0);
+  } else {
+    a.h2b = ifh2 ? F.h2b : nullptr;  // stored geometry: premultiplied h2 B
+  }
   a.nel = nel;
   a.lx = lx;
   a.skip = skip;
-  a.edges = F.edges;
-  a.zg = F.zg;
-  a.wg = F.wg;
-  a.h2c = (T)(ifh2 ? h2c : 0.0);
   a.dot_part = dot_part;
   a.cg_r = r;
   a.cg_dinv = F.dinv;

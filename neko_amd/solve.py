@@ -673,6 +673,13 @@ class Batch:
         return [KspMonitor(m.iter, m.res_start, m.res_final, bool(m.converged),
                            m.outer) for m in mons]
 
+    def stats(self):
+        """(graph steps, dof-iterations covered by the Ax launches) of the
+        last solve (nm_batch_stats)."""
+        st, dof = C.c_int64(), C.c_double()
+        check(lib().nm_batch_stats(self.h, C.byref(st), C.byref(dof)), "nm_batch_stats")
+        return st.value, dof.value
+
     def info(self):
         n, b = C.c_int64(), C.c_int64()
         check(lib().nm_batch_info(self.h, C.byref(n), C.byref(b)), "nm_batch_info")

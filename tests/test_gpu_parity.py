@@ -365,3 +365,46 @@ def test_wrong_dtype_or_size_is_rejected_on_the_host():
     with pytest.raises(TypeError):
         p.ax_local(torch.zeros(p.n, dtype=torch.float64, device="cuda"),
                    torch.zeros(p.n, dtype=torch.float32, device="cuda"))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("code", [1, 102])
+@pytest.mark.parametrize("face_sums", [False, True])
+def test_cg_fused_stored_geometry(precision, code, face_sums):
+    """The CG prologue and p.Ap fused into the stored-geometry Ax kernel
+    (the schedule of operators built from Neko's arrays): oracle iterations
+    +-1 (FP64), 1e-10 reached (mixed), solution within 1e-9."""
+    print("cg fused stored", precision, code, face_sums, flush=True)
+    mon, mon_or, err = _solve_pair(3, 4, 3, 8, 0.1, (0, 1, 0), "normal",
+                                   precision=precision, code=code, face_sums=face_sums)
+    assert mon.converged
+    if precision == "fp64":
+        assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
+    else:
+        assert mon.res_final < 1e-10 * mon.res_start
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cg_neko_arrays_h1_field_helmholtz(precision):
+    """An operator from Neko's own arrays (nm_op_create: G, B, a pointwise
+    h1 field, h2 B, the gs maps and mask) runs the fused stored-geometry
+    schedule: iterations +-1 (FP64) and solution within 1e-9 of the oracle's
+    Jacobi-PCG on the same arrays."""
+    from neko_amd import solve as S
+    c = O.OracleCase(3, 3, 3, 6, amp=0.1, seed=23, periodic=(1, 0, 0),
+                     h1=1.0 / 1600, h2=1500.0)
+    c.h1 = c.h1 * (1.0 + 0.5 * np.random.default_rng(2).random(c.n))
+    c._prob = None
+    p = _problem_from_oracle(c, with_h1_field=True)
+    f_or = c.rhs_normal(7)
+    tol = 1e-10 * c.res_norm(f_or)
+    x_or, mon_or = c.cg(f_or, tol)
+    cg = S.CG(p, precision)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    mon = cg.solve(x, _t(f_or), tol)
+    err = np.linalg.norm(_np(x) - x_or) / np.linalg.norm(x_or)
+    assert mon.converged
+    if precision == "fp64":
+        assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
+    assert err < 1e-9, err
