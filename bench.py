@@ -564,7 +564,7 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (FETCH_SIZE + WRITE_SIZE passes, scaled by the reference copy), or None
     when that summary does not cover the kernel."""
-    path = os.path.join(ROOT, "profiles", "axgs_poisson_64cube_lx8.md")
+    path = os.path.join(ROOT, "profiles", "axgs_pcg_poisson_64cube_lx8.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):

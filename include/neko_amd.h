@@ -336,6 +336,11 @@ int nm_op_ax_local(const nm_op_t *op, double *w, const double *u,
                    hipStream_t s);
 int nm_op_ax_gs(const nm_op_t *op, double *w, const double *u, hipStream_t s);
 int nm_op_ax(const nm_op_t *op, double *w, const double *u, hipStream_t s);
+/* ax_t%compute_vector (src/math/ax.f90:98-115): au, av, aw = A_local of the
+ * three components u, v, w (no gs, no mask) */
+int nm_op_ax_local_vector(const nm_op_t *op, double *au, double *av,
+                          double *aw, const double *u, const double *v,
+                          const double *w, hipStream_t s);
 int nm_op_ax_local32(nm_op_t *op, float *w, const float *u, hipStream_t s);
 int nm_op_ax_gs32(nm_op_t *op, float *w, const float *u, hipStream_t s);
 int nm_op_gs(const nm_op_t *op, double *u, hipStream_t s);
@@ -375,6 +380,19 @@ void nm_cg_free(nm_cg_t *cg);
 int nm_cg_solve(nm_cg_t *cg, double *x, const double *f, double abs_tol,
                 int max_iter, double inner_rel, nm_monitor_t *mon,
                 hipStream_t stream);
+/* the solver's own zero-Dirichlet mask (1-based msk, nmsk entries) in place
+ * of its operator's, for every operator application of its solves; NULL
+ * restores the operator's.  It carries one component's bc list (blstx,
+ * blsty or blstz) of the velocity solve, src/fluid/fluid_pnpn.f90:899. */
+int nm_cg_set_mask(nm_cg_t *cg, const int32_t *msk, int64_t nmsk,
+                   hipStream_t s);
+/* ksp_t%solve_coupled of cg (src/krylov/bcknd/cpu/cg.f90:282-305): the
+ * three components solved one after the other, component c (x[c], f[c],
+ * mon[c]) by cg[c], all on the same operator size, each to abs_tol */
+int nm_cg_solve_coupled(nm_cg_t *const *cg, double *const *x,
+                        const double *const *f, double abs_tol, int max_iter,
+                        double inner_rel, nm_monitor_t *mon,
+                        hipStream_t stream);
 
 /* --- hsmg preconditioner (pc_t 'hsmg', src/krylov/pc_hsmg.f90) ---------- */
 /* Three-level V-cycle lx -> lx_mid (4, or max(lx-1,3) below lx 5) -> 2 on

@@ -160,6 +160,9 @@ SIGNATURES = {
     "nm_cg_create": (_i, [C.POINTER(_vp), _vp, _i]),
     "nm_cg_free": (None, [_vp]),
     "nm_cg_solve": (_i, [_vp, _vp, _vp, _d, _i, _d, C.POINTER(Monitor), _vp]),
+    "nm_cg_set_mask": (_i, [_vp, _vp, _i64, _vp]),
+    "nm_cg_solve_coupled": (_i, [_vp, _vp, _vp, _d, _i, _d, C.POINTER(Monitor), _vp]),
+    "nm_op_ax_local_vector": (_i, [_vp] * 7 + [_vp]),
     "nm_memcpy": (_i, [_vp, _vp, C.c_size_t, _i, _vp]),
 }
 

@@ -40,6 +40,12 @@ struct CgSolver {
   CgScal *h_sc = nullptr;    // pinned host mirror
   hipGraphExec_t g64 = nullptr, g32 = nullptr;
   uint64_t epoch = 0;        // op->epoch the graphs were captured at
+  // the solver's own zero-Dirichlet mask (nm_cg_set_mask), in place of the
+  // operator's
+  bool own_mask = false;
+  MaskRef mk;
+  int32_t *mask_dev = nullptr;
+  int64_t mask_cap = 0;
   int64_t npart_cap = 0;     // capacity of dpart
   int graph_iters = 4;
   int64_t bytes = 0;         // device memory held by this solver
@@ -79,6 +85,7 @@ int mx_slice_begin(const Op &op, double *x, float *x32, float *r32, float *p32,
 // x32 += alpha p32, alpha = 0 (the fused schedule's owed x update)
 int mx_slice_flush(float *x32, const float *p32, int64_t n, CgScal *sc, hipStream_t s);
 int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
-                     double *w, CgScal *sc, double *part, hipStream_t s);
+                     double *w, CgScal *sc, double *part, hipStream_t s,
+                     const MaskRef *mk = nullptr);
 
 }  // namespace nm

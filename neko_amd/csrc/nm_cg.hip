@@ -722,15 +722,16 @@ __global__ void __launch_bounds__(RED_THREADS)
 template <typename T, typename TX>
 static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
                         double *part, hipStream_t s, double *dpart = nullptr,
-                        int64_t npart = 0, bool cgp = false) {
+                        int64_t npart = 0, bool cgp = false, const MaskRef *mk = nullptr) {
+  const MaskRef m = mk ? *mk : op.mask_ref();
   const OpFields<T> &F = op.fields<T>();
   const int64_t n = op.n;
   if constexpr (std::is_same<T, TX>::value) {
     if (cgp) {
       // p update and the previous x update ride in the Ax kernel, the
       // descriptor face sums of gs in the update kernel
-      const bool ff = op.face_fuse && op.face_fuse_on;
-      int st = op.apply_cgp<T>(w, p, r, x, &sc->beta, &sc->alpha, &sc->done, s, dpart, ff);
+      const bool ff = m.face_fuse && op.face_fuse_on;
+      int st = op.apply_cgp<T>(w, p, r, x, &sc->beta, &sc->alpha, &sc->done, s, dpart, ff, &m);
       if (st) return st;
       hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, dpart,
                          npart, part, sc);
@@ -752,7 +753,7 @@ static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
   }
   hipLaunchKernelGGL(cg_p_kernel<T>, dim3(vec_blocks(n)), dim3(256), 0, s, p, r, F.dinv, n, sc);
   const bool fused = npart > 0 && dpart != nullptr;
-  int st = op.apply<T>(w, p, &sc->done, s, true, true, fused ? dpart : nullptr);
+  int st = op.apply<T>(w, p, &sc->done, s, true, true, fused ? dpart : nullptr, &m);
   if (st) return st;
   if (fused)
     hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, dpart,
@@ -857,7 +858,8 @@ static int capture_iters(CgSolver *cg, TX *x, T *r, T *p, T *w, CgScal *sc,
   cg->cgp = fused && std::is_same<T, TX>::value;
   for (int k = 0; k < cg->graph_iters && st == NM_OK; k++)
     st = cg_iteration<T, TX>(*cg->op, x, r, p, w, sc, cg->part, cg->s,
-                             fused ? cg->dpart : nullptr, fused ? cg->npart : 0, cg->cgp);
+                             fused ? cg->dpart : nullptr, fused ? cg->npart : 0, cg->cgp,
+                             cg->own_mask ? &cg->mk : nullptr);
   hipError_t e = hipStreamEndCapture(cg->s, &graph);
   if (st) return st;
   NM_HIP_TRY(e);
@@ -988,10 +990,10 @@ int mx_slice_flush(float *x32, const float *p32, int64_t n, CgScal *sc, hipStrea
 }
 
 int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
-                     double *w, CgScal *sc, double *part, hipStream_t s) {
+                     double *w, CgScal *sc, double *part, hipStream_t s, const MaskRef *mk) {
   const int64_t n = op.n;
   hipLaunchKernelGGL(mx_absorb_kernel, dim3(vec_blocks(n)), dim3(256), 0, s, x, x32, n);
-  int st = op.apply<double>(w, x, nullptr, s, true, true);
+  int st = op.apply<double>(w, x, nullptr, s, true, true, nullptr, mk);
   if (st) return st;
   hipLaunchKernelGGL(mx_resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, r32, f, w,
                      op.f64.mult, op.f32.dinv, n, part);
@@ -1009,7 +1011,7 @@ static int enqueue_replace(CgSolver *cg) {
     NM_HIP_TRY(hipGetLastError());
   }
   int st = mx_slice_replace(*cg->op, cg->x, cg->e32, cg->r32, cg->f, cg->w, cg->sc, cg->part,
-                            cg->s);
+                            cg->s, cg->own_mask ? &cg->mk : nullptr);
   if (st) return st;
   NM_HIP_TRY(hipMemcpyAsync(cg->h_sc, cg->sc, 2 * sizeof(CgScal), hipMemcpyDeviceToHost, cg->s));
   NM_HIP_TRY(hipEventRecord(cg->ev, cg->s));
@@ -1182,6 +1184,48 @@ int nm_cg_create(nm_cg_t **out, nm_op_t *oph, int precision) {
 
 void nm_cg_free(nm_cg_t *h) { delete h; }
 
+// The solver's own zero-Dirichlet mask (1-based msk, nmsk entries), used in
+// place of the operator's by every operator application of its solves;
+// NULL restores the operator's.  Captured graphs are re-captured.
+int nm_cg_set_mask(nm_cg_t *h, const int32_t *msk, int64_t nmsk, hipStream_t s) {
+  CgSolver *cg = h->cg;
+  const Op &op = *cg->op;
+  if (!msk) {
+    cg->own_mask = false;
+    cg->epoch = ~(uint64_t)0;
+    return NM_OK;
+  }
+  if (nmsk < 0) {
+    set_error("nm_cg_set_mask: nmsk = %lld", (long long)nmsk);
+    return NM_ERR_ARG;
+  }
+  std::vector<int32_t> m0((size_t)std::max<int64_t>(nmsk, 1));
+  for (int64_t q = 0; q < nmsk; q++) {
+    if (msk[q] < 1 || msk[q] > op.n) {
+      set_error("nm_cg_set_mask: entry %lld is %d, outside 1..%lld", (long long)q, msk[q],
+                (long long)op.n);
+      return NM_ERR_ARG;
+    }
+    m0[q] = msk[q] - 1;
+  }
+  if (nmsk > cg->mask_cap) {
+    if (!(cg->mask_dev = cg->alloc<int32_t>(nmsk))) {
+      set_error("nm_cg_set_mask: out of device memory");
+      return NM_ERR_NOMEM;
+    }
+    cg->mask_cap = nmsk;
+  }
+  NM_HIP_TRY(hipStreamSynchronize(cg->s));
+  if (nmsk > 0)
+    NM_HIP_TRY(hipMemcpyAsync(cg->mask_dev, m0.data(), sizeof(int32_t) * nmsk,
+                              hipMemcpyHostToDevice, s));
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  cg->mk = MaskRef{cg->mask_dev, nmsk, op.face_fuse_ok(m0.data(), nmsk)};
+  cg->own_mask = true;
+  cg->epoch = ~(uint64_t)0;  // re-capture on the next solve
+  return NM_OK;
+}
+
 // Solve A x = f.  x, f: device FP64 arrays of op->n entries (caller owned).
 // abs_tol: Neko absolute tolerance on sqrt(r.mult.r)/sqrt(volume).
 // Mixed mode: FP32 CG (FP32 operator and vectors, FP64 dot products) whose
@@ -1216,6 +1260,30 @@ int nm_cg_solve(nm_cg_t *h, double *x, const double *f, double abs_tol,
   NM_HIP_TRY(hipEventRecord(ev, cg->s));
   NM_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
   NM_HIP_TRY(hipEventDestroy(ev));
+  return NM_OK;
+}
+
+// cg_solve_coupled (src/krylov/bcknd/cpu/cg.f90:282-305): the three
+// velocity components solved one after the other with the same operator
+// and preconditioner, component c by cg[c] (whose mask, nm_cg_set_mask, is
+// that component's bc list blstx / blsty / blstz), each to abs_tol.
+int nm_cg_solve_coupled(nm_cg_t *const *cg, double *const *x, const double *const *f,
+                        double abs_tol, int max_iter, double inner_rel, nm_monitor_t *mon,
+                        hipStream_t stream) {
+  for (int c = 0; c < 3; c++) {
+    if (!cg[c] || !x[c] || !f[c]) {
+      set_error("nm_cg_solve_coupled: component %d has a null solver or vector", c);
+      return NM_ERR_ARG;
+    }
+    if (cg[c]->cg->op->n != cg[0]->cg->op->n) {
+      set_error("nm_cg_solve_coupled: the three solvers work on different sizes");
+      return NM_ERR_ARG;
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    const int st = nm_cg_solve(cg[c], x[c], f[c], abs_tol, max_iter, inner_rel, &mon[c], stream);
+    if (st) return st;
+  }
   return NM_OK;
 }
 

@@ -25,6 +25,15 @@ struct OpFields {
   T *zg = nullptr, *wg = nullptr;  // GLL nodes and weights
 };
 
+// A zero-Dirichlet mask (0-based device indices) and whether the fused CG
+// update may take the descriptor face sums under it (no masked node on a
+// descriptor face interior).
+struct MaskRef {
+  const int32_t *msk = nullptr;
+  int64_t nmsk = 0;
+  bool face_fuse = false;
+};
+
 struct Op {
   int64_t nel = 0, n = 0;
   int lx = 0;
@@ -98,10 +107,16 @@ struct Op {
   int ensure32(hipStream_t s);
   template <typename T>
   const OpFields<T> &fields() const;
-  // w = mask(gs(A_local u))
+  // the operator's own mask
+  MaskRef mask_ref() const { return MaskRef{msk, nmsk, face_fuse}; }
+  // may the descriptor face sums be deferred past a mask of these host
+  // (0-based) indices?
+  bool face_fuse_ok(const int32_t *h_msk, int64_t m) const;
+  // w = mask(gs(A_local u)); mk replaces the operator's mask
   template <typename T>
   int apply(T *w, const T *u, const int *skip, hipStream_t s, bool gs = true,
-            bool mask = true, double *dot_part = nullptr) const;
+            bool mask = true, double *dot_part = nullptr,
+            const MaskRef *mk = nullptr) const;
   // w = mask(gs(A p)) with the CG prologue fused into the trilinear Ax:
   // p <- dinv r + (*beta) p and x += (*alpha) p_old while loading p
   // With defer_faces the interior points of descriptor faces are left
@@ -109,7 +124,7 @@ struct Op {
   template <typename T>
   int apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double *alpha,
                 const int *skip, hipStream_t s, double *dot_part,
-                bool defer_faces = false) const;
+                bool defer_faces = false, const MaskRef *mk = nullptr) const;
   // variant code of the Ax kernel used for precision T
   template <typename T>
   int ax_code() const;

@@ -98,7 +98,7 @@ template int Op::ax_code<float>() const;
 
 template <typename T>
 int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
-              bool do_mask, double *dot_part) const {
+              bool do_mask, double *dot_part, const MaskRef *mk) const {
   const OpFields<T> &F = fields<T>();
   AxArgs<T> a;
   a.w = w;
@@ -125,12 +125,17 @@ int Op::apply(T *w, const T *u, const int *skip, hipStream_t s, bool do_gs,
     st = gs<T>(w, skip, s);
     if (st) return st;
   }
-  if (do_mask) st = launch_mask<T>(w, msk, nmsk, skip, s);
+  if (do_mask) {
+    const MaskRef m = mk ? *mk : mask_ref();
+    st = launch_mask<T>(w, m.msk, m.nmsk, skip, s);
+  }
   return st;
 }
 template <typename T>
 int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double *alpha,
-                  const int *skip, hipStream_t s, double *dot_part, bool defer_faces) const {
+                  const int *skip, hipStream_t s, double *dot_part, bool defer_faces,
+                  const MaskRef *mk) const {
+  const MaskRef m = mk ? *mk : mask_ref();
   const OpFields<T> &F = fields<T>();
   const int code = ax_code<T>();
   if (!dot_part) {
@@ -165,7 +170,7 @@ int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double
   int st = launch_ax<T>(a, code, s);
   if (st) return st;
   if (defer_faces) {
-    if (!face_fuse || !face_fuse_on) {
+    if (!m.face_fuse || !face_fuse_on) {
       set_error("apply_cgp: face sums cannot be deferred on this operator");
       return NM_ERR_STATE;
     }
@@ -179,19 +184,19 @@ int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double
   } else if ((st = gs<T>(w, skip, s))) {
     return st;
   }
-  return launch_mask<T>(w, msk, nmsk, skip, s);
+  return launch_mask<T>(w, m.msk, m.nmsk, skip, s);
 }
 template int Op::apply_cgp<double>(double *, double *, const double *, double *,
                                    const double *, const double *, const int *, hipStream_t,
-                                   double *, bool) const;
+                                   double *, bool, const MaskRef *) const;
 template int Op::apply_cgp<float>(float *, float *, const float *, float *, const double *,
                                   const double *, const int *, hipStream_t, double *,
-                                  bool) const;
+                                  bool, const MaskRef *) const;
 
 template int Op::apply<double>(double *, const double *, const int *,
-                               hipStream_t, bool, bool, double *) const;
+                               hipStream_t, bool, bool, double *, const MaskRef *) const;
 template int Op::apply<float>(float *, const float *, const int *, hipStream_t,
-                              bool, bool, double *) const;
+                              bool, bool, double *, const MaskRef *) const;
 
 int Op::ensure32(hipStream_t s) {
   Op *op = this;
@@ -437,6 +442,23 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
   return NM_OK;
 }
 
+bool Op::face_fuse_ok(const int32_t *h_msk, int64_t m) const {
+  if (!(nfaces > 0 && lx >= 3 && lx <= NM_FACE_FUSE_MAX_LX &&
+        h_face_nbr.size() == (size_t)nel * 12))
+    return false;
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  for (int64_t q = 0; q < m; q++) {
+    const int64_t e = h_msk[q] / n3, l = h_msk[q] % n3;
+    const int i = (int)(l % lx), j = (int)((l / lx) % lx), k = (int)(l / ((int64_t)lx * lx));
+    const bool bi = i == 0 || i == lx - 1, bj = j == 0 || j == lx - 1,
+               bk = k == 0 || k == lx - 1;
+    if ((int)bi + (int)bj + (int)bk != 1) continue;
+    const int f = bi ? (i != 0) : (bj ? 2 + (j != 0) : 4 + (k != 0));
+    if (h_face_nbr[((size_t)e * 6 + f) * 2] >= 0) return false;
+  }
+  return true;
+}
+
 // mult = 1/gs(1), volume = sum B, dinv = 1/gs(h1 diag(K) + h2 B)
 // (coef.f90:403-419, :1142-1155; pc_jacobi.f90:93-165).
 int op_finish(Op *op, hipStream_t s) {
@@ -444,25 +466,12 @@ int op_finish(Op *op, hipStream_t s) {
   // The CG update may take the descriptor face sums unless a masked node
   // lies on a descriptor face interior (the mask acts after the sum).
   op->face_fuse = false;
-  if (op->nfaces > 0 && op->lx >= 3 && op->lx <= NM_FACE_FUSE_MAX_LX &&
-      op->h_face_nbr.size() == (size_t)op->nel * 12) {
+  {
     std::vector<int32_t> mk(op->nmsk > 0 ? op->nmsk : 1);
     if (op->nmsk > 0)
       NM_HIP_TRY(hipMemcpy(mk.data(), op->msk, sizeof(int32_t) * op->nmsk,
                            hipMemcpyDeviceToHost));
-    const int lx = op->lx;
-    const int64_t n3 = (int64_t)lx * lx * lx;
-    bool ok = true;
-    for (int64_t q = 0; q < op->nmsk && ok; q++) {
-      const int64_t e = mk[q] / n3, l = mk[q] % n3;
-      const int i = (int)(l % lx), j = (int)((l / lx) % lx), k = (int)(l / ((int64_t)lx * lx));
-      const bool bi = i == 0 || i == lx - 1, bj = j == 0 || j == lx - 1,
-                 bk = k == 0 || k == lx - 1;
-      if ((int)bi + (int)bj + (int)bk != 1) continue;
-      const int f = bi ? (i != 0) : (bj ? 2 + (j != 0) : 4 + (k != 0));
-      if (op->h_face_nbr[((size_t)e * 6 + f) * 2] >= 0) ok = false;
-    }
-    op->face_fuse = ok;
+    op->face_fuse = op->face_fuse_ok(mk.data(), op->nmsk);
   }
   if (!op->f64.mult) NM_ALLOC(op->f64.mult, double, n);
   int st = launch_vec<double>(V_CFILL, op->f64.mult, nullptr, nullptr, nullptr, 1.0, n, s);
@@ -1055,6 +1064,15 @@ int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
 // w = A_local u (no gs, no mask), FP64, device pointers.
 int nm_op_ax_local(const nm_op_t *h, double *w, const double *u, hipStream_t s) {
   return h->op.apply<double>(w, u, nullptr, s, false, false);
+}
+// ax_t%compute_vector (src/math/ax.f90:98-115): the local operator on the
+// three components of a vector field
+int nm_op_ax_local_vector(const nm_op_t *h, double *au, double *av, double *aw,
+                          const double *u, const double *v, const double *w, hipStream_t s) {
+  int st = h->op.apply<double>(au, u, nullptr, s, false, false);
+  if (!st) st = h->op.apply<double>(av, v, nullptr, s, false, false);
+  if (!st) st = h->op.apply<double>(aw, w, nullptr, s, false, false);
+  return st;
 }
 // w = mask(gs(A_local u)) (the operator every Krylov iteration applies).
 int nm_op_ax(const nm_op_t *h, double *w, const double *u, hipStream_t s) {

@@ -358,6 +358,14 @@ class Problem:
                   "ax_local")
         return w
 
+    def ax_local_vector(self, au, av, aw, u, v, w):
+        """ax_t%compute_vector: (au, av, aw) = A_local (u, v, w)."""
+        n = self.n
+        check(lib().nm_op_ax_local_vector(self.h, _ptr(au, n), _ptr(av, n), _ptr(aw, n),
+                                          _ptr(u, n), _ptr(v, n), _ptr(w, n), _stream()),
+              "ax_local_vector")
+        return au, av, aw
+
     def ax_gs(self, w, u):
         """w = gs(A_local u): the Ax+gs unit of work."""
         if u.dtype == torch.float32:
@@ -427,6 +435,16 @@ class CG:
         return KspMonitor(mon.iter, mon.res_start, mon.res_final,
                           bool(mon.converged), mon.outer)
 
+    def set_mask(self, msk):
+        """This solver's own zero-Dirichlet mask (1-based local indices; one
+        velocity component's bc list), or None for the operator's."""
+        if msk is None:
+            check(lib().nm_cg_set_mask(self.h, None, 0, _stream()), "nm_cg_set_mask")
+            return
+        m = np.ascontiguousarray(msk, dtype=np.int32)
+        check(lib().nm_cg_set_mask(self.h, m.ctypes.data, len(m), _stream()),
+              "nm_cg_set_mask")
+
     def free(self):
         if self.h:
             lib().nm_cg_free(self.h)
@@ -437,6 +455,24 @@ class CG:
             self.free()
         except Exception:
             pass
+
+
+def solve_coupled(solvers, xs, fs, abs_tol, max_iter=100000, inner_rel=0.0):
+    """ksp_t%solve_coupled of cg (cg.f90:282-305): component c of the
+    velocity solved by solvers[c] (same operator, its own mask) -> three
+    KspMonitor (nm_cg_solve_coupled)."""
+    if len(solvers) != 3 or len(xs) != 3 or len(fs) != 3:
+        raise ValueError("solve_coupled takes three solvers, solutions and right-hand sides")
+    n = solvers[0].problem.n
+    hs = (C.c_void_p * 3)(*[sv.h.value for sv in solvers])
+    xp = (C.c_void_p * 3)(*[_ptr(x, n) for x in xs])
+    fp = (C.c_void_p * 3)(*[_ptr(f, n) for f in fs])
+    mons = (Monitor * 3)()
+    check(lib().nm_cg_solve_coupled(hs, xp, fp, float(abs_tol), int(max_iter),
+                                    float(inner_rel), mons, _stream()),
+          "nm_cg_solve_coupled")
+    return [KspMonitor(m.iter, m.res_start, m.res_final, bool(m.converged), m.outer)
+            for m in mons]
 
 
 class Surface:
