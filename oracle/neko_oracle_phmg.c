@@ -25,6 +25,7 @@
  * polynomial on a dense problem, the V-cycle by its effect on GMRES.
  */
 #include "neko_oracle_phmg.h"
+#include "neko_oracle_tamg.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -132,7 +133,10 @@ void or_phmg_solve(or_phmg_t *h, double *zout, const double *rin) {
     or_mask_apply(h->r[l + 1], h->pb[l + 1]->msk, h->pb[l + 1]->nmsk);
     memset(h->z[l + 1], 0, sizeof(double) * n[l + 1]);
   }
-  or_cg_solve(h->pb[2], h->pc_crs, h->z[2], h->r[2], h->crs_niter, h->crs_tol);
+  if (h->amg)  /* phmg's coarse solve: the tree-AMG (phmg.f90:693-700) */
+    or_tamg_solve((or_tamg_t *)h->amg, h->z[2], h->r[2]);
+  else
+    or_cg_solve(h->pb[2], h->pc_crs, h->z[2], h->r[2], h->crs_niter, h->crs_tol);
   for (int l = 1; l >= 0; l--) {
     const or_problem_t *p = h->pb[l];
     double *z = h->z[l], *r = h->r[l], *w = h->w[l];

@@ -38,6 +38,7 @@ typedef struct {
   int crs_niter;
   double crs_tol;
   double *r[3], *z[3], *w[3];
+  void *amg;  /* or_tamg_t on the coarse level: the coarse solve (else CG) */
 } or_phmg_t;
 
 /* z = M^{-1} r: phmg_mg_cycle (phmg.f90:605-757) with levels lf -> lm ->

@@ -834,7 +834,8 @@ class OrPhmg(C.Structure):
                 ("jmc", C.c_void_p), ("jmc_t", C.c_void_p),
                 ("pc_crs", C.c_void_p), ("crs_niter", C.c_int),
                 ("crs_tol", C.c_double),
-                ("r", C.c_void_p * 3), ("z", C.c_void_p * 3), ("w", C.c_void_p * 3)]
+                ("r", C.c_void_p * 3), ("z", C.c_void_p * 3), ("w", C.c_void_p * 3),
+                ("amg", C.c_void_p)]
 
 
 def _bind_phmg(L):
@@ -897,7 +898,7 @@ class OraclePhmg:
     the fine level's h2 (level_h2="fine")."""
 
     def __init__(self, case, facet_bc=2, smooth_its=4, seed=3901, power_its=150,
-                 crs_niter=10, crs_tol=1e-9, level_h2="default"):
+                 crs_niter=10, crs_tol=1e-9, level_h2="default", coarse_amg=None):
         L = lib()
         _bind_hsmg(L)
         _bind_phmg(L)
@@ -932,6 +933,12 @@ class OraclePhmg:
             h.r[l] = self.work[0][l].ctypes.data
             h.z[l] = self.work[1][l].ctypes.data
             h.w[l] = self.work[2][l].ctypes.data
+        # coarse solve: phmg's tree-AMG (levels, iterations, Chebyshev
+        # degree; phmg.f90:150-157 defaults (3, 1, 4)) or hsmg's CG
+        self.amg = None
+        if coarse_amg is not None:
+            self.amg = OracleTamg(self.crs, *coarse_amg)
+            h.amg = self.amg.h
         self.h = h
 
     def apply(self, r):
@@ -945,6 +952,77 @@ class OraclePhmg:
                                         x, np.ascontiguousarray(f), max_iter,
                                         float(abs_tol), lgmres)
         return x, mon
+
+
+# --------------------------------------------------------------------------
+# tree-AMG (tamg_solver_t, src/multigrid/tree_amg*.f90; neko_oracle_tamg.c)
+# --------------------------------------------------------------------------
+def _bind_tamg(L):
+    if getattr(L, "_tamg_bound", False):
+        return
+    L.or_tamg_new.restype = C.c_void_p
+    L.or_tamg_new.argtypes = [C.POINTER(OrProblem), _ip, C.c_int, C.c_int, C.c_int,
+                              C.c_int]
+    L.or_tamg_delete.argtypes = [C.c_void_p]
+    L.or_tamg_levels.restype = C.c_int
+    L.or_tamg_levels.argtypes = [C.c_void_p, _lp]
+    L.or_tamg_map.argtypes = [C.c_void_p, C.c_int, _ip]
+    L.or_tamg_eigs.argtypes = [C.c_void_p, _dp, _dp]
+    L.or_tamg_solve.argtypes = [C.c_void_p, _dp, _dp]
+    L.or_tamg_matvec.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+    L.or_tamg_cheby.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int]
+    L._tamg_bound = True
+
+
+class OracleTamg:
+    """tree-AMG on the problem of `case` (its mesh's facet graph): levels,
+    V-cycles per solve, Chebyshev degree, power iterations."""
+
+    def __init__(self, case, nlvls=3, max_iter=1, cheby_degree=4, power_its=250):
+        L = lib()
+        _bind_tamg(L)
+        self.case = case
+        self._fn = np.ascontiguousarray(case.fneigh, dtype=np.int32)
+        self.h = L.or_tamg_new(C.byref(case.problem()), self._fn, int(nlvls),
+                               int(max_iter), int(cheby_degree), int(power_its))
+        if not self.h:
+            raise ValueError("or_tamg_new failed")
+        nn = np.zeros(16, dtype=np.int64)
+        self.nlvls = L.or_tamg_levels(self.h, nn)
+        self.nn = nn[: self.nlvls + 1].copy()
+
+    def __del__(self):
+        try:
+            lib().or_tamg_delete(self.h)
+        except Exception:
+            pass
+
+    def map(self, lvl):
+        out = np.zeros(self.case.n, dtype=np.int32)
+        lib().or_tamg_map(self.h, int(lvl), out)
+        return out
+
+    def eigs(self):
+        t = np.zeros(self.nlvls)
+        d = np.zeros(self.nlvls)
+        lib().or_tamg_eigs(self.h, t, d)
+        return t, d
+
+    def matvec(self, v, lvl):
+        out = np.zeros(int(self.nn[lvl]))
+        lib().or_tamg_matvec(self.h, out, np.ascontiguousarray(v, dtype=np.float64), int(lvl))
+        return out
+
+    def cheby(self, f, lvl, x0=None):
+        x = np.zeros(int(self.nn[lvl])) if x0 is None else np.array(x0, dtype=np.float64)
+        lib().or_tamg_cheby(self.h, x, np.ascontiguousarray(f, dtype=np.float64), int(lvl),
+                            1 if x0 is None else 0)
+        return x
+
+    def solve(self, r):
+        z = np.zeros(self.case.n)
+        lib().or_tamg_solve(self.h, z, np.ascontiguousarray(r, dtype=np.float64))
+        return z
 
 
 # --------------------------------------------------------------------------
