@@ -430,6 +430,24 @@ int nm_phmg_create(nm_phmg_t **h, nm_op_t *op, int facet_bc, int smooth_its,
                    int level_h2_fine, int smoother_fp32, hipStream_t s);
 void nm_phmg_free(nm_phmg_t *h);
 int nm_phmg_apply(nm_phmg_t *h, double *z, const double *r, hipStream_t s);
+/* phmg's own coarse solve, the tree-AMG tamg_solver_t
+ * (src/multigrid/tree_amg_multigrid.f90:111-227 set-up, tamg_mg_cycle;
+ * tree_amg_aggregate.f90; tree_amg.f90; tree_amg_smoother.f90) on the
+ * coarse level in place of the Jacobi-CG: `levels` hierarchy levels
+ * (elements as aggregates, then greedy aggregation of the element graph,
+ * one aggregate last), `iterations` V-cycles per coarse solve, Chebyshev
+ * smoothing of degree `cheby_degree` on every level (phmg.f90:150-157
+ * defaults 3, 1, 4).  The aggregation order and the power method's start
+ * vector come from this project's generator (seed 3901).  levels 0
+ * restores the Jacobi-CG. */
+int nm_phmg_set_coarse_amg(nm_phmg_t *h, int levels, int iterations,
+                           int cheby_degree, hipStream_t s);
+/* level sizes of the coarse AMG (nn[0] coarse-level dofs, nn[l] level-l
+ * aggregates, up to cap entries); returns the level count, 0 without AMG */
+int nm_phmg_coarse_amg_levels(const nm_phmg_t *h, int64_t *nn, int cap);
+/* the AMG's map coarse-level dof -> level-lvl aggregate (0-based), nn[0]
+ * entries */
+int nm_phmg_coarse_amg_map(const nm_phmg_t *h, int lvl, int32_t *out);
 /* (tha, dlt) of the fine and the middle level smoothers */
 int nm_phmg_eigs(const nm_phmg_t *h, double *tha_dlt);
 

@@ -157,6 +157,9 @@ SIGNATURES = {
     "nm_proj_pre": (_i, [_vp, _vp, _i, _i, _vp]),
     "nm_proj_post": (_i, [_vp, _vp, _i, _i, _vp]),
     "nm_proj_info": (_i, [_vp, _pi, _pi, C.POINTER(_d), _vp]),
+    "nm_phmg_set_coarse_amg": (_i, [_vp, _i, _i, _i, _vp]),
+    "nm_phmg_coarse_amg_levels": (_i, [_vp, C.POINTER(C.c_int64), _i]),
+    "nm_phmg_coarse_amg_map": (_i, [_vp, _i, _vp]),
     "nm_cg_create": (_i, [C.POINTER(_vp), _vp, _i]),
     "nm_cg_free": (None, [_vp]),
     "nm_cg_solve": (_i, [_vp, _vp, _vp, _d, _i, _d, C.POINTER(Monitor), _vp]),

@@ -22,6 +22,7 @@
 #include "nm_kernels.h"
 #include "nm_op.h"
 #include "nm_rng.h"
+#include "nm_tamg.h"
 
 // host set-up helpers of nm_fdm_setup.cpp (internal)
 extern "C" {
@@ -686,6 +687,29 @@ static std::vector<char> facet_has_neighbour(const Op *op) {
   return nb;
 }
 
+// element facet neighbours (msh%facet_neigh: 6 per element, 1-based, 0 on
+// the boundary) of a box lattice or a mesh operator
+static std::vector<int32_t> facet_neighbours(const Op *op) {
+  const int64_t nel = op->nel;
+  if (op->box[0] == 0) return op->h_fneigh;
+  std::vector<int32_t> nb(nel * 6, 0);
+  const int64_t nx_ = op->box[0], ny_ = op->box[1], nz_ = op->box[2];
+  for (int64_t e = 0; e < nel; e++) {
+    const int64_t c[3] = {e % nx_, (e / nx_) % ny_, e / (nx_ * ny_)};
+    const int64_t dim[3] = {nx_, ny_, nz_}, stride[3] = {1, nx_, nx_ * ny_};
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        int64_t cc = c[d] + (side ? 1 : -1);
+        if (cc < 0 || cc >= dim[d]) {
+          if (!op->per[d]) continue;
+          cc = (cc + dim[d]) % dim[d];
+        }
+        nb[e * 6 + 2 * d + side] = (int32_t)(e + (cc - c[d]) * stride[d] + 1);
+      }
+  }
+  return nb;
+}
+
 template <typename H>
 static int gsdev_level(H *own, GsDev *g, const Op *op, int lx, hipStream_t s) {
   int64_t nl, fo, nb;
@@ -954,12 +978,14 @@ struct Phmg {
   double *r[3] = {}, *z[3] = {}, *w[3] = {};
   double *part = nullptr, *out = nullptr;
   CgFixed *cg = nullptr;
+  Tamg *amg = nullptr;  // the tree-AMG coarse solve, when set (else cg)
   int smooth_its = 4, crs_niter = 10;
   double crs_tol = 1e-9;
   bool fp32 = false;                      // FP32 smoothers
   float *z32[2] = {}, *f32[2] = {};       // level solution / rhs in FP32
   std::vector<void *> owned;
   ~Phmg() {
+    delete amg;
     if (cg) cg_fixed_free(cg);
     for (void *q : owned) (void)hipFree(q);
     if (mid_h) nm_op_free(mid_h);
@@ -1017,7 +1043,11 @@ static int phmg_cycle(Phmg *h, double *zout, const double *rin, hipStream_t s) {
                                  L[l + 1]->n, s)))
       return st;
   }
-  if ((st = cg_fixed_solve(h->cg, h->z[2], h->r[2], h->crs_niter, h->crs_tol, s))) return st;
+  if (h->amg)  // phmg's coarse solve: the tree-AMG (phmg.f90:693-700)
+    st = tamg_solve(h->amg, h->z[2], h->r[2], s);
+  else
+    st = cg_fixed_solve(h->cg, h->z[2], h->r[2], h->crs_niter, h->crs_tol, s);
+  if (st) return st;
   for (int l = 1; l >= 0; l--) {
     const Op &op = *L[l];
     if ((st = tnsr3d(h->w[l], op.lx, h->z[l + 1], L[l + 1]->lx, jj[l], jt[l], jt[l], nel, s)))
@@ -1235,6 +1265,37 @@ int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_it
 }
 
 void nm_phmg_free(nm_phmg_t *h) { delete h; }
+
+// phmg's own coarse solve, the tree-AMG (phmg.f90:150-157, coarse_grid
+// levels / iterations / cheby_degree), on the coarse level; levels 0
+// restores hsmg's coarse Jacobi-CG.
+int nm_phmg_set_coarse_amg(nm_phmg_t *hh, int levels, int iterations, int cheby_degree,
+                           hipStream_t s) {
+  Phmg *h = &hh->h;
+  delete h->amg;
+  h->amg = nullptr;
+  if (levels == 0) return NM_OK;
+  const Op *crs = h->lev[2];
+  return tamg_create(&h->amg, crs, facet_neighbours(crs), levels, iterations, cheby_degree,
+                     250, s);
+}
+
+int nm_phmg_coarse_amg_levels(const nm_phmg_t *hh, int64_t *nn, int cap) {
+  const Tamg *a = hh->h.amg;
+  if (!a) return 0;
+  for (int l = 0; l <= a->nlvls && l < cap; l++) nn[l] = a->nn[l];
+  return a->nlvls;
+}
+
+int nm_phmg_coarse_amg_map(const nm_phmg_t *hh, int lvl, int32_t *out) {
+  const Tamg *a = hh->h.amg;
+  if (!a || lvl < 1 || lvl > a->nlvls) {
+    set_error("nm_phmg_coarse_amg_map: no AMG level %d", lvl);
+    return NM_ERR_ARG;
+  }
+  std::copy(a->h_map[lvl].begin(), a->h_map[lvl].end(), out);
+  return NM_OK;
+}
 
 int nm_phmg_apply(nm_phmg_t *h, double *z, const double *r, hipStream_t s) {
   return phmg_cycle(&h->h, z, r, s);

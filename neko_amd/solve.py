@@ -617,6 +617,25 @@ class PHMG:
                                    _stream()),
               "nm_phmg_apply")
 
+    def set_coarse_amg(self, levels=3, iterations=1, cheby_degree=4):
+        """phmg's tree-AMG as the coarse solve (levels 0: back to the
+        Jacobi-CG)."""
+        check(lib().nm_phmg_set_coarse_amg(self.h, int(levels), int(iterations),
+                                           int(cheby_degree), _stream()),
+              "nm_phmg_set_coarse_amg")
+
+    def coarse_amg_levels(self):
+        nn = (C.c_int64 * 16)()
+        k = lib().nm_phmg_coarse_amg_levels(self.h, nn, 16)
+        return [nn[i] for i in range(k + 1)] if k else []
+
+    def coarse_amg_map(self, lvl):
+        n0 = self.coarse_amg_levels()[0]
+        out = np.zeros(n0, dtype=np.int32)
+        check(lib().nm_phmg_coarse_amg_map(self.h, int(lvl), out.ctypes.data),
+              "nm_phmg_coarse_amg_map")
+        return out
+
     def eigs(self):
         out = (C.c_double * 4)()
         check(lib().nm_phmg_eigs(self.h, out), "nm_phmg_eigs")
