@@ -634,7 +634,8 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
     Level operators with the fine h2 on every level (smoothers in FP32 and
     in FP64), and with Neko's coef default h2 = 1 on the coarser levels
     (phmg's own set-up, capped at 300 iterations: its coarse correction is
-    mis-scaled for this h2)."""
+    mis-scaled for this h2); the coarse solve is hsmg's Jacobi-CG or
+    phmg's own tree-AMG (3 levels, one V-cycle, Chebyshev(4))."""
     import numpy as np
     import torch
     t0 = time.time()
@@ -646,10 +647,15 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
     r0 = float(torch.sqrt(torch.sum(f * f * mult)) / np.sqrt(p.volume))
     out = {"workload": f"Helmholtz box {E}^3 lx={lx} periodic x, h1=1/1600, h2=1500",
            "local_dof": p.n, "setup_s": round(time.time() - t0, 2), "kernel_variants": tuned}
-    for lh2, smoother, cap in (("fine", "fp32", 2000), ("fine", "fp64", 2000),
-                               ("default", "fp64", default_cap)):
+    for lh2, smoother, cap, coarse in (("fine", "fp32", 2000, "cg"),
+                                       ("fine", "fp32", 2000, "amg"),
+                                       ("fine", "fp64", 2000, "cg"),
+                                       ("default", "fp64", default_cap, "cg"),
+                                       ("default", "fp64", default_cap, "amg")):
         t1 = time.time()
         ph = S.PHMG(p, facet_bc=2, smooth_its=4, level_h2=lh2, smoother=smoother)
+        if coarse == "amg":
+            ph.set_coarse_amg(3, 1, 4)  # phmg's own tree-AMG (defaults 3, 1, 4)
         torch.cuda.synchronize()
         t_pc = time.time() - t1
         g = S.GMRES(p, pc=ph, lgmres=30)
@@ -659,7 +665,7 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
         m = g.solve(x, f, rel_tol * r0, max_iter=cap)
         torch.cuda.synchronize()
         dt = time.time() - t2
-        key = f"gmres_phmg_level_h2_{lh2}_smoother_{smoother}"
+        key = f"gmres_phmg_level_h2_{lh2}_smoother_{smoother}_coarse_{coarse}"
         # roofline on a lower-bound byte count of the fine level per GMRES
         # iteration: the outer FP64 Ax+gs (in-kernel geometry), the 9 fine
         # Ax+gs of the V-cycle (4 pre- and 4 post-smoothing Chebyshev steps
@@ -683,8 +689,9 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
             "solves_per_s": round(1.0 / dt, 4) if m.converged else None,
             "ms_per_iteration": round(dt / max(m.iter, 1) * 1e3, 3),
             "res_final_rel": m.res_final / m.res_start, "pc_setup_s": round(t_pc, 2),
+            "coarse_levels": ph.coarse_amg_levels() if coarse == "amg" else None,
             "chebyshev_bounds": ph.eigs()}
-        log(f"[bench] helmholtz phmg({lh2}, {smoother}): {out[key]}")
+        log(f"[bench] helmholtz phmg({lh2}, {smoother}, coarse {coarse}): {out[key]}")
         g.free()
         ph.free()
     # Jacobi-PCG on the same system for reference
