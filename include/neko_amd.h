@@ -197,7 +197,8 @@ int nm_box_mask(int nelx, int nely, int nelz, int lx, int perx, int pery,
  * element vertices in Neko's symmetric order (the p1,p2,p4,p3,p5,p6,p8,p7
  * swap of nmsh_file.f90:159) and their point ids after the periodic zones'
  * id replacement (mesh_apply_periodic_facet, src/mesh/mesh.f90:1837-1866).
- * Curved meshes are rejected (NM_ERR_UNSUPPORTED). */
+ * The vertices only: the curve records of a curved mesh come from
+ * nm_nmsh_curves and go to nm_op_create_mesh_curved. */
 int nm_nmsh_info(const char *path, int64_t *nel, int64_t *nzones,
                  int64_t *nperiodic, int64_t *ncurves);
 int nm_nmsh_read(const char *path, int64_t nel, double *vxyz, int64_t *pid,
@@ -230,11 +231,34 @@ int nm_op_create_box(nm_op_t **op, int nelx, int nely, int nelz, int lx,
  * boundary), label (p_f of the record: the boundary label of a type-7
  * zone); nzones as nm_nmsh_info reports */
 int nm_nmsh_zones(const char *path, int64_t nzones, int32_t *out);
+/* curve records of a .nmsh file (nmsh_file_read_curves,
+ * src/io/nmsh_file.f90:575-639; record nmsh_curve_el_t,
+ * src/io/format/nmsh.f90:39-43): per record the element (0-based, file
+ * order), curve_data(5,12) as 60 doubles (5 per edge, Nek5000's cyclic edge
+ * order) and the 12 edge types (3 circular arc, 4 midpoint; others change
+ * nothing, as in dofmap_generate_xyz); ncurves as nm_nmsh_info reports */
+int nm_nmsh_curves(const char *path, int64_t ncurves, int32_t *el, double *data,
+                   int32_t *type);
+/* GLL coordinates (lx^3 each) of one element from its 8 vertices (Neko's
+ * symmetric order) and one curve record (dofmap_generate_xyz with
+ * dofmap_xyzquad, gh_face_extend_3d and arc_surface,
+ * src/sem/dofmap.f90:797-1285) */
+int nm_curve_xyz(int lx, const double *vxyz, const double *data, const int32_t *type,
+                 double *x, double *y, double *z);
 /* Whole problem on a general straight-edged hex mesh (e.g. from
  * nm_nmsh_read): the set-up of examples/poisson/driver.f90 (dofmap, gs,
  * coef, Dirichlet on every facet without a neighbour), constant h1/h2. */
 int nm_op_create_mesh(nm_op_t **op, int64_t nel, int lx, const double *vxyz,
                       const int64_t *pid, double h1, double h2, hipStream_t s);
+/* the same with curved elements (dofmap_generate_xyz,
+ * src/sem/dofmap.f90:797-864): ncurves records as nm_nmsh_curves returns
+ * them (at most one per element); the geometric factors then come from
+ * the curved coordinates (coef.f90:248-1156) and the in-kernel (trilinear)
+ * Ax variants are off.  Multigrid levels built on it are curved too. */
+int nm_op_create_mesh_curved(nm_op_t **op, int64_t nel, int lx, const double *vxyz,
+                             const int64_t *pid, int64_t ncurves, const int32_t *cel,
+                             const double *cdata, const int32_t *ctype, double h1,
+                             double h2, hipStream_t s);
 /* Problem from Neko's own host arrays (coef_t, gs_t, bc mask) -- what the
  * Fortran ksp_t/pc_t plug-ins pass at init (src/krylov/krylov.f90:73-201,
  * src/krylov/precon.f90:40-65). */

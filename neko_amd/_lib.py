@@ -83,6 +83,8 @@ SIGNATURES = {
     "nm_nmsh_info": (_i, [C.c_char_p, _pi64, _pi64, _pi64, _pi64]),
     "nm_nmsh_read": (_i, [C.c_char_p, _i64, _vp, _vp, _vp]),
     "nm_nmsh_zones": (_i, [C.c_char_p, _i64, _vp]),
+    "nm_nmsh_curves": (_i, [C.c_char_p, _i64, _vp, _vp, _vp]),
+    "nm_curve_xyz": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nm_mesh_facet_neigh": (_i, [_i64, _vp, _vp]),
     "nm_mesh_gs_sizes": (_i, [_i64, _i, _vp, _pi64, _pi64]),
     "nm_mesh_gs_map": (_i, [_i64, _i, _vp, _vp, _vp, _vp, _vp, _vp, _pi64,
@@ -91,6 +93,8 @@ SIGNATURES = {
     "nm_mesh_mask": (_i, [_i64, _i, _vp, _vp, _pi64]),
     "nm_op_create_mesh": (_i, [C.POINTER(_vp), _i64, _i, _vp, _vp, _d, _d,
                                _vp]),
+    "nm_op_create_mesh_curved": (_i, [C.POINTER(_vp), _i64, _i, _vp, _vp, _i64, _vp,
+                                      _vp, _vp, _d, _d, _vp]),
     "nm_op_set_pc": (_i, [_vp, _i, _vp]),
     "nm_op_set_coeffs": (_i, [_vp, _d, _d, _vp]),
     "nm_opgrad": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),

@@ -29,7 +29,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-pthread", "-ffp-contract=off",
 
 HIP_SOURCES = ["nm_kernels.hip", "nm_op.hip", "nm_cg.hip", "nm_abi.hip", "nm_hsmg.hip",
                "nm_gmres.hip"]
-CXX_SOURCES = ["nm_setup.cpp", "nm_fdm_setup.cpp", "nm_mesh.cpp"]
+CXX_SOURCES = ["nm_setup.cpp", "nm_fdm_setup.cpp", "nm_mesh.cpp", "nm_curve.cpp"]
 
 
 def _run(cmd):

@@ -664,8 +664,9 @@ static int level_op(const Op *fine, int lx, double h1, double h2, nm_op_t **out,
     return nm_op_create_box(out, b[0], b[1], b[2], lx, p[0], p[1], p[2], fine->amp,
                             fine->seed, h1, h2, s);
   }
-  return nm_op_create_mesh(out, fine->nel, lx, fine->h_vxyz.data(), fine->h_pid.data(), h1,
-                           h2, s);
+  return nm_op_create_mesh_curved(out, fine->nel, lx, fine->h_vxyz.data(), fine->h_pid.data(),
+                                  (int64_t)fine->h_cel.size(), fine->h_cel.data(),
+                                  fine->h_cdata.data(), fine->h_ctype.data(), h1, h2, s);
 }
 
 // 1 where element facet e*6+f has a neighbour (box lattice or mesh)

@@ -235,8 +235,9 @@ int nm_nmsh_info(const char *path, int64_t *nel, int64_t *nzones, int64_t *nperi
 // zone's glb_pt_ids (mesh_apply_periodic_facet, mesh.f90:1837-1866: a point
 // is one object, so the new id holds in every element that shares it; a
 // later zone overrides an earlier one).  el_idx (may be NULL): the file's
-// global element numbers.  Curved elements (ncurves > 0) are rejected: the
-// operator here takes straight-edged (trilinear) elements only.
+// global element numbers.  The vertices only: a file with curve records
+// (ncurves > 0) needs them too (nm_nmsh_curves), else its elements are
+// taken straight.
 // The zone records of a .nmsh file (nmsh_file.f90:326-380): per zone
 // {element (0-based, in file order), facet 1..6, type (5 periodic, 7
 // labelled), label (the zone's p_f: the boundary label of a type-7 zone)},
@@ -289,6 +290,65 @@ int nm_nmsh_zones(const char *path, int64_t nzones, int32_t *out) {
   return NM_OK;
 }
 
+// The curve records (nmsh_file_read_curves, nmsh_file.f90:575-639; the
+// packed layout of nmsh_curve_el_t, format/nmsh.f90:39-43, as the MPI
+// struct type of mpi_types.f90:225-249 writes it: int32 e, 60 f64
+// curve_data(5,12), 12 int32 types = 532 bytes): per record the element
+// (0-based, file order), its 60 curve values and 12 edge types.
+int nm_nmsh_curves(const char *path, int64_t ncurves, int32_t *el, double *data,
+                   int32_t *type) {
+  int64_t nel = 0, nz = 0, np = 0, nc = 0;
+  int st = nm_nmsh_info(path, &nel, &nz, &np, &nc);
+  if (st) return st;
+  if (nc != ncurves) {
+    set_error("nmsh %s: holds %lld curve records, caller expects %lld", path, (long long)nc,
+              (long long)ncurves);
+    return NM_ERR_ARG;
+  }
+  if (nc == 0) return NM_OK;
+  FILE *fp = std::fopen(path, "rb");
+  if (!fp) {
+    set_error("nmsh %s: cannot open", path);
+    return NM_ERR_ARG;
+  }
+  NmshHead h;
+  if ((st = nmsh_head(fp, path, h))) {
+    std::fclose(fp);
+    return st;
+  }
+  std::vector<unsigned char> hex((size_t)nel * 228), rec((size_t)nc * 532);
+  const bool ok = std::fseek(fp, 8, SEEK_SET) == 0 &&
+                  std::fread(hex.data(), 228, (size_t)nel, fp) == (size_t)nel &&
+                  std::fseek(fp, h.zone_off + 4 + 36L * h.nzones + 4, SEEK_SET) == 0 &&
+                  std::fread(rec.data(), 532, (size_t)nc, fp) == (size_t)nc;
+  std::fclose(fp);
+  if (!ok) {
+    set_error("nmsh %s: truncated curve records", path);
+    return NM_ERR_ARG;
+  }
+  std::unordered_map<int32_t, int64_t> glb2loc;
+  glb2loc.reserve((size_t)nel * 2);
+  for (int64_t e = 0; e < nel; e++) {
+    int32_t idx;
+    std::memcpy(&idx, hex.data() + e * 228, 4);
+    glb2loc[idx] = e;
+  }
+  for (int64_t c = 0; c < nc; c++) {
+    const unsigned char *r = rec.data() + c * 532;
+    int32_t e;
+    std::memcpy(&e, r, 4);
+    auto it = glb2loc.find(e);
+    if (it == glb2loc.end()) {
+      set_error("nmsh %s: curve record %lld names element %d", path, (long long)c, e);
+      return NM_ERR_ARG;
+    }
+    el[c] = (int32_t)it->second;
+    std::memcpy(data + c * 60, r + 4, 480);
+    std::memcpy(type + c * 12, r + 484, 48);
+  }
+  return NM_OK;
+}
+
 int nm_nmsh_read(const char *path, int64_t nel, double *vxyz, int64_t *pid,
                  int32_t *el_idx) {
   FILE *fp = std::fopen(path, "rb");
@@ -306,12 +366,6 @@ int nm_nmsh_read(const char *path, int64_t nel, double *vxyz, int64_t *pid,
     std::fclose(fp);
     set_error("nmsh %s: holds %d elements, caller expects %lld", path, h.nel, (long long)nel);
     return NM_ERR_ARG;
-  }
-  if (h.ncurves > 0) {
-    std::fclose(fp);
-    set_error("nmsh %s: %d curved elements (only straight-edged elements are supported)",
-              path, h.ncurves);
-    return NM_ERR_UNSUPPORTED;
   }
   std::vector<unsigned char> rec((size_t)nel * 228);
   std::vector<int32_t> z((size_t)h.nzones * 9);

@@ -13,6 +13,10 @@
 
 namespace nm {
 
+// GLL coordinates of one curved element (nm_curve.cpp)
+int curve_element_xyz(int lx, const double *zg, const double *v, const double *cd,
+                      const int32_t *type, double *x, double *y, double *z);
+
 template <typename T>
 struct OpFields {
   T *dx = nullptr;        // D(i,l), lx*lx
@@ -88,6 +92,10 @@ struct Op {
   std::vector<double> h_vxyz;
   std::vector<int64_t> h_pid;
   std::vector<int32_t> h_fneigh;
+  // curved elements of a mesh operator (Neko's curve records: element,
+  // 5 x 12 curve data, 12 edge types), kept for the multigrid levels
+  std::vector<int32_t> h_cel, h_ctype;
+  std::vector<double> h_cdata;
   bool trilinear = false;  // straight-edged elements: geometry in-kernel
   // diagonal preconditioner of the CG/GMRES solvers: 0 Jacobi (f*.dinv
   // holds 1/diag), 1 identity (f*.dinv holds ones); alt* keep the other

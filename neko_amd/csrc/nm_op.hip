@@ -563,6 +563,19 @@ __global__ void xyz_kernel(int64_t nel, int lx, const double *__restrict__ zg,
   }
 }
 
+// Curved elements: their coordinates, computed on the host, replace the
+// trilinear ones (element cel[c] <- buf[(3c + comp) n3 ..]).
+__global__ void curve_scatter_kernel(int64_t ncur, int64_t n3, const int32_t *__restrict__ cel,
+                                     const double *__restrict__ buf, double *__restrict__ X,
+                                     double *__restrict__ Y, double *__restrict__ Z) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncur * n3) return;
+  const int64_t c = idx / n3, p = idx % n3, e = cel[c];
+  X[e * n3 + p] = buf[(3 * c) * n3 + p];
+  Y[e * n3 + p] = buf[(3 * c + 1) * n3 + p];
+  Z[e * n3 + p] = buf[(3 * c + 2) * n3 + p];
+}
+
 }  // namespace nm
 
 using namespace nm;
@@ -621,6 +634,39 @@ static int op_build_linear(nm_op_t *h, int64_t nel, int lx, const std::vector<do
   hipLaunchKernelGGL(xyz_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                      nel, lx, d_z, d_v, op->xyz[0], op->xyz[1], op->xyz[2]);
   if (hipGetLastError() != hipSuccess) return fail(NM_ERR_HIP);
+  // curved elements (dofmap_generate_xyz): host coordinates scattered over
+  // the trilinear ones; the in-kernel (trilinear) geometry is then off
+  bool curved = false;
+  if (!op->h_cel.empty()) {
+    const int64_t nc = (int64_t)op->h_cel.size(), n3 = (int64_t)lx * lx * lx;
+    std::vector<double> buf(nc * 3 * n3);
+    for (int64_t c = 0; c < nc; c++) {
+      const int32_t *ty = op->h_ctype.data() + c * 12;
+      for (int q = 0; q < 12; q++)
+        curved |= (q < 8 && ty[q] == 3) || (ty[q] == 4 && lx > 2);
+      double *b = buf.data() + c * 3 * n3;
+      if ((st = curve_element_xyz(lx, z.data(), vxyz.data() + (int64_t)op->h_cel[c] * 24,
+                                  op->h_cdata.data() + c * 60, ty, b, b + n3, b + 2 * n3)))
+        return fail(st);
+    }
+    int32_t *d_cel = op->alloc<int32_t>(nc);
+    double *d_buf = nullptr;
+    if (!d_cel || hipMalloc(&d_buf, sizeof(double) * buf.size()) != hipSuccess)
+      return fail(NM_ERR_NOMEM);
+    const bool ok =
+        hipMemcpyAsync(d_cel, op->h_cel.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice,
+                       s) == hipSuccess &&
+        hipMemcpyAsync(d_buf, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice,
+                       s) == hipSuccess;
+    if (ok)
+      hipLaunchKernelGGL(curve_scatter_kernel, dim3((unsigned)((nc * n3 + 255) / 256)),
+                         dim3(256), 0, s, nc, n3, d_cel, d_buf, op->xyz[0], op->xyz[1],
+                         op->xyz[2]);
+    const bool done = ok && hipGetLastError() == hipSuccess &&
+                      hipStreamSynchronize(s) == hipSuccess;
+    (void)hipFree(d_buf);
+    if (!done) return fail(NM_ERR_HIP);
+  }
   if ((st = launch_coef(nel, lx, op->xyz[0], op->xyz[1], op->xyz[2], op->f64.dx, d_w3,
                         op->f64.g, op->jac, op->B, s)))
     return fail(st);
@@ -656,7 +702,7 @@ static int op_build_linear(nm_op_t *h, int64_t nel, int lx, const std::vector<do
             hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return fail(NM_ERR_HIP);
-    op->trilinear = true;
+    op->trilinear = !curved;
     op->h2c = h2;
   }
   // gs maps (already in op->h_*), then the device layout
@@ -731,10 +777,36 @@ int nm_op_create_box(nm_op_t **out, int nelx, int nely, int nelz, int lx,
 // element.
 int nm_op_create_mesh(nm_op_t **out, int64_t nel, int lx, const double *vxyz,
                       const int64_t *pid, double h1, double h2, hipStream_t s) {
+  return nm_op_create_mesh_curved(out, nel, lx, vxyz, pid, 0, nullptr, nullptr, nullptr, h1,
+                                  h2, s);
+}
+
+// The same with curved elements: ncurves records of Neko's curve data
+// (element cel[c], 0-based; cdata: 5 x 12 doubles; ctype: 12 edge types),
+// as nm_nmsh_curves reads them; at most one record per element.
+int nm_op_create_mesh_curved(nm_op_t **out, int64_t nel, int lx, const double *vxyz,
+                             const int64_t *pid, int64_t ncurves, const int32_t *cel,
+                             const double *cdata, const int32_t *ctype, double h1, double h2,
+                             hipStream_t s) {
   *out = nullptr;
   if (nel <= 0 || lx < 2 || lx > 12) {
     set_error("nm_op_create_mesh: nel=%lld lx=%d (lx must be 2..12)", (long long)nel, lx);
     return NM_ERR_ARG;
+  }
+  if (ncurves < 0 || (ncurves > 0 && (!cel || !cdata || !ctype))) {
+    set_error("nm_op_create_mesh_curved: bad curve arguments");
+    return NM_ERR_ARG;
+  }
+  {
+    std::vector<char> seen(nel, 0);
+    for (int64_t c = 0; c < ncurves; c++) {
+      if (cel[c] < 0 || cel[c] >= nel || seen[cel[c]]) {
+        set_error("nm_op_create_mesh_curved: curve record %lld names element %d "
+                  "(out of range or repeated)", (long long)c, cel[c]);
+        return NM_ERR_ARG;
+      }
+      seen[cel[c]] = 1;
+    }
   }
   std::vector<int32_t> fneigh(nel * 6);
   int st;
@@ -760,6 +832,9 @@ int nm_op_create_mesh(nm_op_t **out, int64_t nel, int lx, const double *vxyz,
   op->h_vxyz = v;
   op->h_pid.assign(pid, pid + nel * 8);
   op->h_fneigh = fneigh;
+  op->h_cel.assign(cel, cel + ncurves);
+  op->h_cdata.assign(cdata, cdata + ncurves * 60);
+  op->h_ctype.assign(ctype, ctype + ncurves * 12);
   if ((st = op_build_linear(h, nel, lx, v, h1, h2, nlocal, foff, nb, msk, m, s))) return st;
   *out = h;
   return NM_OK;

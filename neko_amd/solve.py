@@ -124,6 +124,35 @@ def read_nmsh(path):
     return vxyz, pid, el
 
 
+def read_nmsh_curves(path):
+    """Curve records of a .nmsh file (nm_nmsh_curves): (el (nc,) 0-based
+    element, data (nc, 60) curve_data(5,12), type (nc, 12) edge types)."""
+    b = os.fsencode(str(path))
+    nel, nz, nper, ncur = (C.c_int64() for _ in range(4))
+    check(lib().nm_nmsh_info(b, C.byref(nel), C.byref(nz), C.byref(nper),
+                             C.byref(ncur)), "nm_nmsh_info")
+    nc = ncur.value
+    el = np.zeros(max(nc, 1), dtype=np.int32)
+    data = np.zeros((max(nc, 1), 60), dtype=np.float64)
+    ty = np.zeros((max(nc, 1), 12), dtype=np.int32)
+    check(lib().nm_nmsh_curves(b, nc, el.ctypes.data, data.ctypes.data, ty.ctypes.data),
+          "nm_nmsh_curves")
+    return el[:nc], data[:nc], ty[:nc]
+
+
+def curve_xyz(lx, vxyz_e, data, type_):
+    """GLL coordinates (3, lx^3) of one element from its 8 vertices and one
+    curve record (nm_curve_xyz)."""
+    v = np.ascontiguousarray(vxyz_e, dtype=np.float64).reshape(24)
+    d = np.ascontiguousarray(data, dtype=np.float64).reshape(60)
+    t = np.ascontiguousarray(type_, dtype=np.int32).reshape(12)
+    out = np.zeros((3, lx ** 3))
+    check(lib().nm_curve_xyz(int(lx), v.ctypes.data, d.ctypes.data, t.ctypes.data,
+                             out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data),
+          "nm_curve_xyz")
+    return out
+
+
 def read_nmsh_zones(path):
     """Zone records of a .nmsh file as an (nzones, 4) int32 array: element
     (0-based), facet 1..6, type (5 periodic, 7 labelled boundary), label
@@ -177,25 +206,38 @@ class Problem:
         return pr
 
     @classmethod
-    def mesh(cls, vxyz, pid, lx, h1=1.0, h2=0.0):
-        """General straight-edged hex mesh (vxyz: (nel, 8, 3) vertices in
-        Neko's symmetric order, pid: (nel, 8) point ids), Dirichlet on every
-        facet without a neighbour (nm_op_create_mesh)."""
+    def mesh(cls, vxyz, pid, lx, h1=1.0, h2=0.0, curves=None):
+        """General hex mesh (vxyz: (nel, 8, 3) vertices in Neko's symmetric
+        order, pid: (nel, 8) point ids), Dirichlet on every facet without a
+        neighbour (nm_op_create_mesh); curves = (el, data, type) as
+        read_nmsh_curves returns them (nm_op_create_mesh_curved)."""
         v = np.ascontiguousarray(vxyz, dtype=np.float64).reshape(-1, 24)
         p = np.ascontiguousarray(pid, dtype=np.int64).reshape(-1, 8)
         if len(v) != len(p):
             raise ValueError("vxyz and pid disagree on the element count")
         h = C.c_void_p()
-        check(lib().nm_op_create_mesh(C.byref(h), len(v), int(lx), v.ctypes.data,
-                                      p.ctypes.data, float(h1), float(h2),
-                                      _stream()), "nm_op_create_mesh")
+        if curves is not None and len(curves[0]):
+            el = np.ascontiguousarray(curves[0], dtype=np.int32)
+            cd = np.ascontiguousarray(curves[1], dtype=np.float64).reshape(-1, 60)
+            ty = np.ascontiguousarray(curves[2], dtype=np.int32).reshape(-1, 12)
+            if not (len(el) == len(cd) == len(ty)):
+                raise ValueError("curve arrays disagree on the record count")
+            check(lib().nm_op_create_mesh_curved(
+                C.byref(h), len(v), int(lx), v.ctypes.data, p.ctypes.data, len(el),
+                el.ctypes.data, cd.ctypes.data, ty.ctypes.data, float(h1), float(h2),
+                _stream()), "nm_op_create_mesh_curved")
+        else:
+            check(lib().nm_op_create_mesh(C.byref(h), len(v), int(lx), v.ctypes.data,
+                                          p.ctypes.data, float(h1), float(h2),
+                                          _stream()), "nm_op_create_mesh")
         return cls(h.value)
 
     @classmethod
     def from_nmsh(cls, path, lx, h1=1.0, h2=0.0):
-        """Problem on a Neko .nmsh mesh (read_nmsh + mesh)."""
+        """Problem on a Neko .nmsh mesh, curved elements included
+        (read_nmsh + read_nmsh_curves + mesh)."""
         vxyz, pid, _ = read_nmsh(path)
-        return cls.mesh(vxyz, pid, lx, h1=h1, h2=h2)
+        return cls.mesh(vxyz, pid, lx, h1=h1, h2=h2, curves=read_nmsh_curves(path))
 
     def set_coeffs(self, h1: float, h2: float):
         """New constant Helmholtz coefficients (nm_op_set_coeffs)."""

@@ -627,6 +627,183 @@ void or_xyz(int nel, int lx, const double *zg, const double *vxyz, double *x,
   free(jx);
 }
 
+/* Lagrange weights of the nodes zn[0..nn-1] at xi (fd_weights_full with
+ * derivative order 0, speclib: the interpolation row). */
+static void lagrange_row(double xi, const double *zn, int nn, double *row) {
+  for (int a = 0; a < nn; a++) {
+    double p = 1.0;
+    for (int b = 0; b < nn; b++)
+      if (b != a) p *= (xi - zn[b]) / (zn[a] - zn[b]);
+    row[a] = p;
+  }
+}
+
+/* gh_face_extend_3d (dofmap.f90:1023-1127), gh_type 2: vertices and edges
+ * of x(n,n,n) extended into the faces and the interior by Gordon-Hall
+ * blending; e and v are n^3 work arrays. */
+static void gh_extend_edges(double *x, const double *zg, int n, double *e, double *v) {
+#define X3(a, b, c) x[(a) + n * ((b) + n * (c))]
+#define V3(a, b, c) v[(a) + n * ((b) + n * (c))]
+#define E3(a, b, c) e[(a) + n * ((b) + n * (c))]
+  const int ntot = n * n * n;
+  const int ends[2] = {0, n - 1};
+  for (int q = 0; q < ntot; q++) v[q] = 0.0;
+  /* blend toward node end 0 / end n-1 at zg[i] */
+  #define BL(end, i) (0.5 * ((double)((n - 1 - (end)) * (1 - zg[i])) + \
+                             (double)(end) * (1 + zg[i])) / (double)(n - 1))
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++)
+        for (int c = 0; c < 2; c++)
+          for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) {
+              const int ii = ends[a], jj = ends[b], kk = ends[c];
+              V3(i, j, k) += BL(ii, i) * BL(jj, j) * BL(kk, k) * X3(ii, jj, kk);
+            }
+  for (int q = 0; q < ntot; q++) e[q] = 0.0;
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++)
+        for (int c = 0; c < 2; c++)
+          for (int b = 0; b < 2; b++) {
+            const int p = ends[b], q = ends[c];
+            /* x-edges (along r), y-edges (along s), z-edges (along t) */
+            E3(i, j, k) += BL(p, j) * BL(q, k) * (X3(i, p, q) - V3(i, p, q));
+            E3(i, j, k) += BL(p, i) * BL(q, k) * (X3(p, j, q) - V3(p, j, q));
+            E3(i, j, k) += BL(p, i) * BL(q, j) * (X3(p, q, k) - V3(p, q, k));
+          }
+  for (int q = 0; q < ntot; q++) x[q] = e[q] + v[q];
+  #undef BL
+#undef X3
+#undef V3
+#undef E3
+}
+
+/* compute_h + arc_surface (dofmap.f90:1197-1285): the circular-arc
+ * perturbation of cyclic edge isid (1..8) of an element, in the x-y
+ * plane, added to x, y (lx^3 each).  Returns -1 when the radius is too
+ * small for the edge (the reference stops with an error). */
+static int arc_surface(int isid, const double *cd, double *x, double *y, const double *zg,
+                       int lx, const double *v) {
+  static const int fcyc_to_sym[6] = {3, 2, 4, 1, 5, 6};
+  static const int ecyc_to_sym[12] = {1, 6, 2, 5, 3, 8, 4, 7, 9, 10, 12, 11};
+  static const int edge_nodes[12][2] = {{1, 2}, {3, 4}, {5, 6}, {7, 8}, {1, 3}, {2, 4},
+                                        {5, 7}, {6, 8}, {1, 5}, {2, 6}, {3, 7}, {4, 8}};
+  const int itmp = ecyc_to_sym[isid - 1];
+  int n1 = edge_nodes[itmp - 1][0], n2 = edge_nodes[itmp - 1][1];
+  if ((isid >= 3 && isid <= 4) || (isid >= 7 && isid <= 8)) {
+    const int t = n1;
+    n1 = n2;
+    n2 = t;
+  }
+  const double pt1x = v[(n1 - 1) * 3], pt1y = v[(n1 - 1) * 3 + 1];
+  const double pt2x = v[(n2 - 1) * 3], pt2y = v[(n2 - 1) * 3 + 1];
+  const double radius = cd[0];
+  const double xs = pt2y - pt1y, ys = pt1x - pt2x;
+  const double xys = sqrt(xs * xs + ys * ys);
+  if (fabs(2.0 * radius) <= xys * 1.00001) return -1;
+  double dtheta = fabs(asin(0.5 * xys / radius));
+  const double pt12x = (pt1x + pt2x) / 2.0, pt12y = (pt1y + pt2y) / 2.0;
+  const double xcenn = pt12x - xs / xys * radius * cos(dtheta);
+  const double ycenn = pt12y - ys / xys * radius * cos(dtheta);
+  const double theta0 = atan2(pt12y - ycenn, pt12x - xcenn);
+  int isid1 = (isid + 4 - 1) % 4 + 1;
+  double *h = (double *)malloc(sizeof(double) * lx * 2);  /* h(ix, 1|2) */
+  for (int ix = 0; ix < lx; ix++) {
+    h[ix] = (1.0 - zg[ix]) * 0.5;
+    h[lx + ix] = (1.0 + zg[ix]) * 0.5;
+  }
+  if (radius < 0.0) dtheta = -dtheta;
+  double *xc = (double *)malloc(sizeof(double) * lx * 2), *yc = xc + lx;
+  for (int ix = 0; ix < lx; ix++) {
+    const int ixt = isid1 > 2 ? lx - 1 - ix : ix;
+    const double r = zg[ix];
+    xc[ixt] = xcenn + fabs(radius) * cos(theta0 + r * dtheta) - (h[ix] * pt1x + h[lx + ix] * pt2x);
+    yc[ixt] = ycenn + fabs(radius) * sin(theta0 + r * dtheta) - (h[ix] * pt1y + h[lx + ix] * pt2y);
+  }
+  isid1 = fcyc_to_sym[isid1 - 1];
+  const int izt = (isid - 1) / 4 + 1;
+  const int iyt = isid1 - 2, ixt = isid1;
+  const double *hz = h + (izt - 1) * lx;
+  for (int k = 0; k < lx; k++)
+    for (int j = 0; j < lx; j++)
+      for (int i = 0; i < lx; i++) {
+        double wx, wy;
+        if (isid1 <= 2) {  /* addtnsr(x, h(:,1,ixt), xcrved, h(:,3,izt)) */
+          const double hh = h[(ixt - 1) * lx + i] * hz[k];
+          wx = hh * xc[j];
+          wy = hh * yc[j];
+        } else {           /* addtnsr(x, xcrved, h(:,2,iyt), h(:,3,izt)) */
+          const double hh = h[(iyt - 1) * lx + j] * hz[k];
+          wx = hh * xc[i];
+          wy = hh * yc[i];
+        }
+        x[i + lx * (j + lx * k)] += wx;
+        y[i + lx * (j + lx * k)] += wy;
+      }
+  free(xc);
+  free(h);
+  return 0;
+}
+
+int or_curve_element(int lx, const double *zg, const double *v, const double *cd,
+                     const int *type, double *x, double *y, double *z) {
+  /* dofmap_generate_xyz for one curved element (dofmap.f90:797-864):
+   * dofmap_xyzquad when an edge has type 4 (midpoint) and lx > 2, then
+   * arc_surface for every edge 1..8 of type 3; other types change
+   * nothing.  x, y, z hold the element's trilinear coordinates on entry. */
+  static const int eindx[12] = {2, 6, 8, 4, 20, 24, 26, 22, 10, 12, 18, 16};
+  int midpoint = 0;
+  for (int j = 0; j < 12; j++) midpoint |= type[j] == 4;
+  if (midpoint && lx > 2) {
+    /* dofmap_xyzquad (dofmap.f90:941-1014) */
+    const double zq[3] = {-1.0, 0.0, 1.0};
+    double x3[3][27], e[27], w[27];
+    or_xyz(1, 3, zq, v, x3[0], x3[1], x3[2]);
+    for (int k = 0; k < 12; k++)
+      if (type[k] == 4)
+        for (int c = 0; c < 3; c++) x3[c][eindx[k] - 1] = cd[k * 5 + c];
+    for (int c = 0; c < 3; c++) gh_extend_edges(x3[c], zq, 3, e, w);
+    double *jx = (double *)malloc(sizeof(double) * lx * 3);
+    for (int i = 0; i < lx; i++) lagrange_row(zg[i], zq, 3, jx + 3 * i);
+    for (int c = 0; c < 3; c++) {
+      double *out = c == 0 ? x : (c == 1 ? y : z);
+      for (int k = 0; k < lx; k++)
+        for (int j = 0; j < lx; j++)
+          for (int i = 0; i < lx; i++) {
+            double acc = 0.0;
+            for (int kk = 0; kk < 3; kk++)
+              for (int jj = 0; jj < 3; jj++)
+                for (int ii = 0; ii < 3; ii++)
+                  acc += jx[3 * i + ii] * jx[3 * j + jj] * jx[3 * k + kk] *
+                         x3[c][ii + 3 * (jj + 3 * kk)];
+            out[i + lx * (j + lx * k)] = acc;
+          }
+    }
+    free(jx);
+  }
+  for (int j = 0; j < 8; j++)
+    if (type[j] == 3 && arc_surface(j + 1, cd + j * 5, x, y, zg, lx, v)) return -1;
+  return 0;
+}
+
+void or_xyz_curved(int nel, int lx, const double *zg, const double *vxyz, int ncurves,
+                   const int *cel, const double *cdata, const int *ctype, double *x,
+                   double *y, double *z, int *status) {
+  /* dofmap_generate_xyz (dofmap.f90:797-864): trilinear everywhere, then
+   * the curved elements cel[c] (0-based) with their 5 x 12 curve data and
+   * 12 edge types. */
+  or_xyz(nel, lx, zg, vxyz, x, y, z);
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  *status = 0;
+  for (int c = 0; c < ncurves; c++) {
+    const int64_t e = cel[c];
+    if (or_curve_element(lx, zg, vxyz + e * 24, cdata + (int64_t)c * 60, ctype + c * 12,
+                         x + e * n3, y + e * n3, z + e * n3))
+      *status = -1;
+  }
+}
+
 /* ===================================================================== */
 /* a6: gs mapping (gather_scatter.f90:513-1375)                          */
 /* ===================================================================== */

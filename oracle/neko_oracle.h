@@ -37,6 +37,13 @@ void or_dfrmd_el(int nel, const double *vxyz, int *dfrmd);
 void or_dof_ids(int nel, int lx, const int64_t *pid, int64_t *dof);
 void or_xyz(int nel, int lx, const double *zg, const double *vxyz,
             double *x, double *y, double *z);
+/* curved elements (dofmap_generate_xyz, dofmap.f90:797-1285): type 4
+ * (midpoint, Gordon-Hall quadratic) and type 3 (circular arc) edges */
+int or_curve_element(int lx, const double *zg, const double *v, const double *cd,
+                     const int *type, double *x, double *y, double *z);
+void or_xyz_curved(int nel, int lx, const double *zg, const double *vxyz, int ncurves,
+                   const int *cel, const double *cdata, const int *ctype, double *x,
+                   double *y, double *z, int *status);
 
 /* ---- a6: gs mapping (gather_scatter.f90: gs_init_mapping) */
 int or_gs_register(int nel, int lx, const int64_t *dof, const int *fneigh,

@@ -97,6 +97,10 @@ def lib():
     L.or_dfrmd_el.argtypes = [C.c_int, _dp, _ip]
     L.or_dof_ids.argtypes = [C.c_int, C.c_int, _lp, _lp]
     L.or_xyz.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.or_curve_element.restype = C.c_int
+    L.or_curve_element.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _dp, _dp, _dp]
+    L.or_xyz_curved.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _ip,
+                                _dp, _dp, _dp, C.POINTER(C.c_int)]
     L.or_gs_register.restype = C.c_int
     L.or_gs_register.argtypes = [C.c_int, C.c_int, _lp, _ip, _ip, _ip,
                                  C.POINTER(C.c_int)]
@@ -187,6 +191,29 @@ def read_nmsh(path):
     return vxyz, pid, zones.copy(), ncur
 
 
+def read_nmsh_curves(path):
+    """The curve records of a .nmsh file (nmsh_file_read_curves,
+    nmsh_file.f90:575-639; nmsh_curve_el_t, format/nmsh.f90:39-43, written
+    through the MPI struct type of mpi_types.f90:225-249, i.e. packed: int32
+    e, 5 x 12 f64 curve_data (column-major: 5 values per edge), 12 int32
+    edge types; 532 bytes).  Returns (el (nc,) 0-based file-order element
+    of each record, data (nc, 60), type (nc, 12))."""
+    b = open(path, "rb").read()
+    nel = int(np.frombuffer(b, dtype="<i4", count=1)[0])
+    hx = np.dtype([("el", "<i4"), ("v", [("idx", "<i4"), ("xyz", "<f8", 3)], 8)])
+    els = np.frombuffer(b, dtype=hx, count=nel, offset=8)["el"]
+    off = 8 + nel * hx.itemsize
+    nz = int(np.frombuffer(b, dtype="<i4", count=1, offset=off)[0])
+    off += 4 + 36 * nz
+    nc = int(np.frombuffer(b, dtype="<i4", count=1, offset=off)[0])
+    rec = np.dtype([("e", "<i4"), ("cd", "<f8", 60), ("t", "<i4", 12)])
+    assert rec.itemsize == 532
+    r = np.frombuffer(b, dtype=rec, count=nc, offset=off + 4)
+    loc = {int(g): e for e, g in enumerate(els)}
+    el = np.array([loc[int(e)] for e in r["e"]], dtype=np.int32)
+    return el, np.ascontiguousarray(r["cd"]), np.ascontiguousarray(r["t"], dtype=np.int32)
+
+
 class OracleCase:
     """A whole Neko-style problem on a (possibly perturbed/periodic) box."""
 
@@ -203,13 +230,15 @@ class OracleCase:
         self._setup(vxyz, pid, lx, h1, h2)
 
     @classmethod
-    def from_mesh(cls, vxyz, pid, lx, h1=1.0, h2=0.0):
-        """The same problem on a general conforming straight-edged hex mesh
-        (vertices in Neko's symmetric order, point ids), Dirichlet on every
-        facet without a neighbour."""
+    def from_mesh(cls, vxyz, pid, lx, h1=1.0, h2=0.0, curves=None):
+        """The same problem on a general conforming hex mesh (vertices in
+        Neko's symmetric order, point ids), Dirichlet on every facet without
+        a neighbour; curves = (el, data, type) as read_nmsh_curves returns
+        them, or None for straight edges."""
         c = cls.__new__(cls)
         c.nelx = c.nely = c.nelz = None
         c.amp, c.seed, c.periodic = None, None, None
+        c.curves = curves
         c._setup(np.ascontiguousarray(vxyz, dtype=np.float64).reshape(-1),
                  np.ascontiguousarray(pid, dtype=np.int64).reshape(-1), lx,
                  h1, h2)
@@ -221,7 +250,8 @@ class OracleCase:
         if self.nelx is not None:
             return OracleCase(self.nelx, self.nely, self.nelz, lx, amp=self.amp,
                               seed=self.seed, periodic=self.periodic, h1=h1, h2=h2)
-        return OracleCase.from_mesh(self.vxyz, self.pid, lx, h1=h1, h2=h2)
+        return OracleCase.from_mesh(self.vxyz, self.pid, lx, h1=h1, h2=h2,
+                                    curves=getattr(self, "curves", None))
 
     def _setup(self, vxyz, pid, lx, h1, h2):
         L = lib()
@@ -244,7 +274,18 @@ class OracleCase:
         self.x = np.zeros(n)
         self.y = np.zeros(n)
         self.zc = np.zeros(n)
-        L.or_xyz(nel, lx, self.z, self.vxyz, self.x, self.y, self.zc)
+        curves = getattr(self, "curves", None)
+        if curves is not None and len(curves[0]):
+            cel = np.ascontiguousarray(curves[0], dtype=np.int32)
+            cdata = np.ascontiguousarray(curves[1], dtype=np.float64).reshape(-1)
+            ctype = np.ascontiguousarray(curves[2], dtype=np.int32).reshape(-1)
+            st = C.c_int()
+            L.or_xyz_curved(nel, lx, self.z, self.vxyz, len(cel), cel, cdata, ctype,
+                            self.x, self.y, self.zc, C.byref(st))
+            if st.value:
+                raise ValueError("curve radius too small for an arc edge")
+        else:
+            L.or_xyz(nel, lx, self.z, self.vxyz, self.x, self.y, self.zc)
         # gs maps
         cap = n
         self.dg = np.zeros(cap, dtype=np.int32)
