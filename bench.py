@@ -158,6 +158,41 @@ def hbm_stream(n=134217728):
             "array_bytes": 8 * n}
 
 
+def c1_solve_section(S, reps=20):
+    """The small Poisson case on the GPU: 8^3 undeformed box, lx = 8,
+    RHS B 3 pi^2 sin(pi x) sin(pi y) sin(pi z), Jacobi-PCG to 1e-10
+    relative, FP64 and mixed; mean over `reps` solves from x = 0 after one
+    untimed solve, with the max-norm error against the exact solution."""
+    import numpy as np
+    import torch
+    p = S.Problem.box(8, 8, 8, 8)
+    f = p.rhs_sine()
+    mult = p.field(S.FIELD_MULT)
+    res0 = float(torch.sqrt(torch.sum(f * f * mult)) / np.sqrt(p.volume))
+    exact = (torch.sin(np.pi * p.field(S.FIELD_X)) * torch.sin(np.pi * p.field(S.FIELD_Y))
+             * torch.sin(np.pi * p.field(S.FIELD_Z)))
+    out = {"workload": "8^3 undeformed box, lx=8, sine RHS, Jacobi-PCG 1e-10",
+           "local_dof": p.n}
+    for prec in ("fp64", "mixed"):
+        cg = S.CG(p, prec)
+        x = torch.zeros_like(f)
+        cg.solve(x, f, 1e-10 * res0)
+        torch.cuda.synchronize()
+        t1 = time.time()
+        for _ in range(reps):
+            x.zero_()
+            mon = cg.solve(x, f, 1e-10 * res0)
+        torch.cuda.synchronize()
+        dt = (time.time() - t1) / reps
+        out[prec] = {"iterations": mon.iter, "outer": mon.outer, "converged": mon.converged,
+                     "seconds": round(dt, 5), "solves_per_s": round(1.0 / dt, 2),
+                     "max_err_vs_exact": float(torch.max(torch.abs(x - exact)))}
+        cg.free()
+    p.free()
+    log(f"[bench] c1 solve {out}")
+    return out
+
+
 def neko_array_operator(S, p, E, lx):
     """The same 64^3 problem as an operator built from Neko's own arrays
     (nm_op_create: the geometric factors G11..G23, mass B, Jacobi
@@ -369,6 +404,7 @@ def main():
         solves["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(xm - xd) / torch.linalg.norm(xd))
         solves["speedup_mixed"] = round(solves["fp64"]["seconds"] / solves["mixed"]["seconds"], 3)
         result["solves"] = solves
+        result["c1_solve"] = c1_solve_section(S)
 
     # ---- Helmholtz with the Chebyshev/Schwarz p-multigrid ---------------------
     if not args.no_helmholtz:

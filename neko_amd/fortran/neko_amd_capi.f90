@@ -195,5 +195,43 @@ module neko_amd_capi
        integer(c_int), value :: max_iter
        type(nm_gmres_monitor_t) :: mon
      end function nm_gmres_solve
+
+     ! ---- velocity solve: a solver's own zero-Dirichlet list (a velocity
+     ! component's bc list blstx / blsty / blstz, 1-based), solve_coupled
+     ! (src/krylov/bcknd/cpu/cg.f90:282-305) and compute_vector
+     ! (src/math/ax.f90:98-115)
+     integer(c_int) function nm_cg_set_mask(cg, msk, nmsk, s) &
+          bind(c, name='nm_cg_set_mask')
+       import c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: cg, s
+       integer(c_int), intent(in) :: msk(*)
+       integer(c_int64_t), value :: nmsk
+     end function nm_cg_set_mask
+
+     integer(c_int) function nm_cg_solve_coupled(cg, x, f, abs_tol, max_iter, &
+          inner_rel, mon, s) bind(c, name='nm_cg_solve_coupled')
+       import c_int, c_ptr, c_double, nm_monitor_t
+       type(c_ptr), intent(in) :: cg(3), x(3), f(3)
+       real(c_double), value :: abs_tol, inner_rel
+       integer(c_int), value :: max_iter
+       type(nm_monitor_t) :: mon(3)
+       type(c_ptr), value :: s
+     end function nm_cg_solve_coupled
+
+     integer(c_int) function nm_op_ax_local_vector(op, au, av, aw, u, v, w, s) &
+          bind(c, name='nm_op_ax_local_vector')
+       import c_int, c_ptr
+       type(c_ptr), value :: op, au, av, aw, u, v, w, s
+     end function nm_op_ax_local_vector
+
+     ! ---- phmg (Chebyshev/Schwarz p-multigrid) and its tree-AMG coarse
+     ! solve (src/multigrid/phmg.f90:138-187; levels, V-cycles and the
+     ! Chebyshev degree as phmg's json keys; levels 0 = hsmg's coarse CG)
+     integer(c_int) function nm_phmg_set_coarse_amg(h, levels, iterations, &
+          cheby_degree, s) bind(c, name='nm_phmg_set_coarse_amg')
+       import c_int, c_ptr
+       type(c_ptr), value :: h, s
+       integer(c_int), value :: levels, iterations, cheby_degree
+     end function nm_phmg_set_coarse_amg
   end interface
 end module neko_amd_capi
