@@ -293,12 +293,12 @@ __global__ void __launch_bounds__(RED_THREADS)
           wi = wi + w[(int64_t)d.x * N3 + face_point((d.y >> 3) & 7, ub, vb, LX)];
         }
       }
-      const T ri = r[i] - alpha_t * wi;
-      r[i] = ri;
-      const T mi = CNT ? tab[mcnt[i]] : mult[i];
+      const T ri = __builtin_nontemporal_load(r + i) - alpha_t * wi;
+      __builtin_nontemporal_store(ri, r + i);
+      const T mi = CNT ? tab[__builtin_nontemporal_load(mcnt + i)] : mult[i];
       const double rm = (double)ri * (double)ri * (double)mi;
       a += rm;
-      b += rm * (double)dinv[i];
+      b += rm * (double)__builtin_nontemporal_load(dinv + i);
     }
   }
   bsum2(a, b, sh);
@@ -405,6 +405,11 @@ __global__ void __launch_bounds__(RED_THREADS)
   }
 }
 
+// four floats as a native vector, for the streaming (nontemporal) loads
+// and stores: r and dinv are touched once per iteration, so they bypass
+// the caches that the partner values of w are reused from
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+
 // lx divisible by 4, FP32: four neighbouring points per thread (16-byte
 // loads and stores); with fewer quads per element than threads, a
 // workgroup takes 256 / (N3 / 4) elements per step.
@@ -453,8 +458,10 @@ __global__ void __launch_bounds__(RED_THREADS)
         if (h >= NH) break;
         const int64_t i = e * N3 + 4 * h;
         const float4 wv = *reinterpret_cast<const float4 *>(w + i);
-        const float4 rv = *reinterpret_cast<const float4 *>(r + i);
-        const float4 dv = *reinterpret_cast<const float4 *>(dinv + i);
+        const nt_f4 rn = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(r + i));
+        const nt_f4 dn = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(dinv + i));
+        const float4 rv = make_float4(rn.x, rn.y, rn.z, rn.w);
+        const float4 dv = make_float4(dn.x, dn.y, dn.z, dn.w);
         float m[4];
         if (CNT) {
           const uchar4 cv = *reinterpret_cast<const uchar4 *>(mcnt + i);
@@ -487,7 +494,12 @@ __global__ void __launch_bounds__(RED_THREADS)
         ro.y = rv.y - alpha_t * wi[1];
         ro.z = rv.z - alpha_t * wi[2];
         ro.w = rv.w - alpha_t * wi[3];
-        *reinterpret_cast<float4 *>(r + i) = ro;
+        nt_f4 rs;
+        rs.x = ro.x;
+        rs.y = ro.y;
+        rs.z = ro.z;
+        rs.w = ro.w;
+        __builtin_nontemporal_store(rs, reinterpret_cast<nt_f4 *>(r + i));
         const float ro_[4] = {ro.x, ro.y, ro.z, ro.w};
         const float dv_[4] = {dv.x, dv.y, dv.z, dv.w};
 #pragma unroll

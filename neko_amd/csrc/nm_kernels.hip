@@ -91,10 +91,10 @@ __global__ void __launch_bounds__(LX *LX *EPB)
 #pragma unroll
     for (int k = 0; k < LX; k++) {
       const int64_t ix = base + k * P;
-      po[k] = active ? p[ix] : T(0);
-      pr[k] = active ? cg_r[ix] : T(0);
-      pd[k] = active ? cg_dinv[ix] : T(0);
-      px[k] = active ? cg_x[ix] : T(0);
+      po[k] = active ? __builtin_nontemporal_load(p + ix) : T(0);
+      pr[k] = active ? __builtin_nontemporal_load(cg_r + ix) : T(0);
+      pd[k] = active ? __builtin_nontemporal_load(cg_dinv + ix) : T(0);
+      px[k] = active ? __builtin_nontemporal_load(cg_x + ix) : T(0);
     }
 #pragma unroll
     for (int k = 0; k < LX; k++) {
@@ -105,14 +105,14 @@ __global__ void __launch_bounds__(LX *LX *EPB)
 #pragma unroll
       for (int k = 0; k < LX; k++) {
         const int64_t ix = base + k * P;
-        p[ix] = uc[k];
-        cg_x[ix] = px[k] + alpha * po[k];
+        __builtin_nontemporal_store(uc[k], p + ix);
+        __builtin_nontemporal_store(px[k] + alpha * po[k], cg_x + ix);
       }
     }
   } else {
 #pragma unroll
     for (int k = 0; k < LX; k++) {
-      uc[k] = active ? u[base + k * P] : T(0);
+      uc[k] = active ? __builtin_nontemporal_load(u + base + k * P) : T(0);
       su[q][k * P + t] = uc[k];
     }
   }
@@ -138,8 +138,9 @@ __global__ void __launch_bounds__(LX *LX *EPB)
   T nG[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   T nh = h1c;
   if (PF && active) {
-    nG[0] = g11[base]; nG[1] = g22[base]; nG[2] = g33[base];
-    nG[3] = g12[base]; nG[4] = g13[base]; nG[5] = g23[base];
+    nG[0] = __builtin_nontemporal_load(g11 + base); nG[1] = __builtin_nontemporal_load(g22 + base);
+    nG[2] = __builtin_nontemporal_load(g33 + base); nG[3] = __builtin_nontemporal_load(g12 + base);
+    nG[4] = __builtin_nontemporal_load(g13 + base); nG[5] = __builtin_nontemporal_load(g23 + base);
     if (h1 != nullptr) nh = h1[base];
   }
 #pragma unroll 1
@@ -155,13 +156,15 @@ __global__ void __launch_bounds__(LX *LX *EPB)
       hh = nh;
       if (active && k + 1 < LX) {
         const int64_t pn = p + P;
-        nG[0] = g11[pn]; nG[1] = g22[pn]; nG[2] = g33[pn];
-        nG[3] = g12[pn]; nG[4] = g13[pn]; nG[5] = g23[pn];
+        nG[0] = __builtin_nontemporal_load(g11 + pn); nG[1] = __builtin_nontemporal_load(g22 + pn);
+        nG[2] = __builtin_nontemporal_load(g33 + pn); nG[3] = __builtin_nontemporal_load(g12 + pn);
+        nG[4] = __builtin_nontemporal_load(g13 + pn); nG[5] = __builtin_nontemporal_load(g23 + pn);
         if (h1 != nullptr) nh = h1[pn];
       }
     } else if (active) {
-      G11 = g11[p]; G22 = g22[p]; G33 = g33[p];
-      G12 = g12[p]; G13 = g13[p]; G23 = g23[p];
+      G11 = __builtin_nontemporal_load(g11 + p); G22 = __builtin_nontemporal_load(g22 + p);
+      G33 = __builtin_nontemporal_load(g33 + p); G12 = __builtin_nontemporal_load(g12 + p);
+      G13 = __builtin_nontemporal_load(g13 + p); G23 = __builtin_nontemporal_load(g23 + p);
       if (h1 != nullptr) hh = h1[p];
     }
     // D(k,:) is wave-uniform (scalar loads)
@@ -198,7 +201,7 @@ __global__ void __launch_bounds__(LX *LX *EPB)
     for (int m = 0; m < LX; m++) {
       T r = wc[m] + su[q][m * P + t];
       if (h2b != nullptr) r += h2b[base + m * P] * uc[m];
-      w[base + m * P] = r;
+      __builtin_nontemporal_store(r, w + base + m * P);
       if (CG) dsum += (double)r * (double)uc[m];
     }
   }
@@ -289,10 +292,10 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
 #pragma unroll
     for (int k = 0; k < LX; k++) {
       const int64_t ix = base + k * P;
-      po[k] = active ? cp.p[ix] : T(0);
-      pr[k] = active ? cp.r[ix] : T(0);
-      pd[k] = active ? cp.dinv[ix] : T(0);
-      px[k] = active ? cp.x[ix] : T(0);
+      po[k] = active ? __builtin_nontemporal_load(cp.p + ix) : T(0);
+      pr[k] = active ? __builtin_nontemporal_load(cp.r + ix) : T(0);
+      pd[k] = active ? __builtin_nontemporal_load(cp.dinv + ix) : T(0);
+      px[k] = active ? __builtin_nontemporal_load(cp.x + ix) : T(0);
     }
 #pragma unroll
     for (int k = 0; k < LX; k++) {
@@ -303,14 +306,14 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
 #pragma unroll
       for (int k = 0; k < LX; k++) {
         const int64_t ix = base + k * P;
-        cp.p[ix] = uc[k];
-        cp.x[ix] = px[k] + cp.alpha * po[k];
+        __builtin_nontemporal_store(uc[k], cp.p + ix);
+        __builtin_nontemporal_store(px[k] + cp.alpha * po[k], cp.x + ix);
       }
     }
   } else {
 #pragma unroll
     for (int k = 0; k < LX; k++) {
-      uc[k] = active ? u[base + k * P] : T(0);
+      uc[k] = active ? __builtin_nontemporal_load(u + base + k * P) : T(0);
       su[q][k * P + t] = uc[k];
     }
   }
@@ -426,7 +429,7 @@ __device__ __forceinline__ void ax_tri_body(T *__restrict__ w, const T *__restri
 #pragma unroll
     for (int m = 0; m < LX; m++) {
       const T r = wc[m] + su[q][m * P + t];
-      w[base + m * P] = r;
+      __builtin_nontemporal_store(r, w + base + m * P);
       if (DOT) dsum += (double)r * (double)uc[m];
     }
   }
@@ -750,7 +753,7 @@ __global__ void __launch_bounds__(256)
     const T o = __shfl(v, lane0 + q, 64);
     if (q < len) acc += o;
   }
-  if (g < nblks && m < len) u[idx] = acc;
+  if (g < nblks && m < len) __builtin_nontemporal_store(acc, u + idx);
 }
 
 template <typename T>
@@ -801,8 +804,8 @@ __global__ void __launch_bounds__(256)
   const int64_t ia = ea * n3 + face_point(fa, uu, vv, lx);
   const int64_t ib = eb * n3 + face_point(fb, ub, vb, lx);
   const T acc = u[ia] + u[ib];
-  u[ia] = acc;
-  u[ib] = acc;
+  __builtin_nontemporal_store(acc, u + ia);
+  __builtin_nontemporal_store(acc, u + ib);
 }
 
 __global__ void mult_count_kernel(const double *__restrict__ mult, uint8_t *__restrict__ cnt,
