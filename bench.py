@@ -683,11 +683,14 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
     r0 = float(torch.sqrt(torch.sum(f * f * mult)) / np.sqrt(p.volume))
     out = {"workload": f"Helmholtz box {E}^3 lx={lx} periodic x, h1=1/1600, h2=1500",
            "local_dof": p.n, "setup_s": round(time.time() - t0, 2), "kernel_variants": tuned}
-    for lh2, smoother, cap, coarse in (("fine", "fp32", 2000, "cg"),
-                                       ("fine", "fp32", 2000, "amg"),
-                                       ("fine", "fp64", 2000, "cg"),
-                                       ("default", "fp64", default_cap, "cg"),
-                                       ("default", "fp64", default_cap, "amg")):
+    runs = [("fine", "fp32", 2000, "cg", 0), ("fine", "fp32", 2000, "amg", 0),
+            ("fine", "fp64", 2000, "cg", 0), ("default", "fp64", default_cap, "cg", 0),
+            ("default", "fp64", default_cap, "amg", 0),
+            # the Schwarz FDM solves on the vector ALUs instead of the
+            # matrix cores, for comparison
+            ("fine", "fp32", 2000, "cg", 1), ("fine", "fp64", 2000, "cg", 1)]
+    for lh2, smoother, cap, coarse, fdm_valu in runs:
+        S.fdm_set_variant(fdm_valu)
         t1 = time.time()
         ph = S.PHMG(p, facet_bc=2, smooth_its=4, level_h2=lh2, smoother=smoother)
         if coarse == "amg":
@@ -702,6 +705,8 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
         torch.cuda.synchronize()
         dt = time.time() - t2
         key = f"gmres_phmg_level_h2_{lh2}_smoother_{smoother}_coarse_{coarse}"
+        if fdm_valu:
+            key += "_fdm_vector_alu"
         # roofline on a lower-bound byte count of the fine level per GMRES
         # iteration: the outer FP64 Ax+gs (in-kernel geometry), the 9 fine
         # Ax+gs of the V-cycle (4 pre- and 4 post-smoothing Chebyshev steps
@@ -727,9 +732,10 @@ def helmholtz_pmg(S, E=48, lx=8, rel_tol=1e-10, default_cap=300):
             "res_final_rel": m.res_final / m.res_start, "pc_setup_s": round(t_pc, 2),
             "coarse_levels": ph.coarse_amg_levels() if coarse == "amg" else None,
             "chebyshev_bounds": ph.eigs()}
-        log(f"[bench] helmholtz phmg({lh2}, {smoother}, coarse {coarse}): {out[key]}")
+        log(f"[bench] helmholtz {key}: {out[key]}")
         g.free()
         ph.free()
+    S.fdm_set_variant(0)
     # Jacobi-PCG on the same system for reference
     cg = S.CG(p, "fp64")
     x = torch.zeros_like(f)

@@ -419,6 +419,11 @@ int nm_cg_solve_coupled(nm_cg_t *const *cg, double *const *x,
                         hipStream_t stream);
 
 /* --- hsmg preconditioner (pc_t 'hsmg', src/krylov/pc_hsmg.f90) ---------- */
+/* Kernel family of the Schwarz FDM solves of hsmg and phmg (no
+ * counterpart in Neko, whose fdm_do_fast has one implementation per
+ * backend, src/math/fdm.f90:635-662): 0 the matrix cores (default), 1 the
+ * vector ALUs; the same solve to rounding, process-wide, for timing. */
+int nm_fdm_set_variant(int variant);
 /* Three-level V-cycle lx -> lx_mid (4, or max(lx-1,3) below lx 5) -> 2 on
  * the mesh of a box or mesh operator: overlapping Schwarz with FDM local solves
  * (src/math/schwarz.f90:112-634, src/math/fdm.f90:110-665) on the fine and

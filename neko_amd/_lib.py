@@ -126,6 +126,7 @@ SIGNATURES = {
     "nm_op_mask": (_i, [_vp, _vp, _vp]),
     "nm_op_set_all_deformed": (_i, [_vp, _vp]),
     "nm_op_rhs_normal": (_i, [_vp, C.c_uint64, _vp, _vp]),
+    "nm_fdm_set_variant": (_i, [_i]),
     "nm_hsmg_create": (_i, [C.POINTER(_vp), _vp, _i, _i, _vp]),
     "nm_hsmg_free": (None, [_vp]),
     "nm_hsmg_apply": (_i, [_vp, _vp, _vp, _vp]),

@@ -565,11 +565,196 @@ static int tnsr3d(double *v, int nv, const double *u, int nu, const double *A,
   return NM_OK;
 }
 
+// The same FDM solve on the matrix cores: one wave per element, each 1-D
+// transform an (NL x NL) by (NL x NL^2) product in 16x16x4 tiles
+// (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64): the tile's rows are the
+// transform's output index (NL of 16 used), its four k the summed index
+// (zero-padded to a multiple of 4), its 16 columns 16 of the NL^2 lines.
+// A lane holds A(row = lane % 16, k = 4 kk + lane / 16) and B(k, col =
+// lane % 16); the 4 accumulator entries of lane l are rows 4 (l / 16) + v
+// (FP32) or l / 16 + 4 v (FP64) of column l % 16 (checked on the device).
+// The block moves between the LDS arrays a and b once per direction; the
+// forward x, y, z, the D scaling and the backward z, y, x run in that
+// order, as in fdm_line_kernel.
+template <typename T> struct MfmaTile;
+template <> struct MfmaTile<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t fma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct MfmaTile<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t fma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+// index of (transform index k, line col) in the NL^3 block for direction
+// DIR (0: x, lines (y,z); 1: y, lines (x,z); 2: z, lines (x,y))
+template <int NL, int DIR>
+__device__ __forceinline__ int fdm_at(int k, int col) {
+  if (DIR == 0) return k + NL * col;
+  if (DIR == 1) return col % NL + NL * k + NL * NL * (col / NL);
+  return col + NL * NL * k;
+}
+
+// out(row, col) = sum_k A(row, k) in(k, col), A(row, k) = M[row + NL k]
+// (TRANS false) or M[k + NL row], over this wave's tiles (tile tt = wave,
+// wave + WPE, ...); SCALE multiplies by the D values prefetched in dsc
+// (per tile of the wave, per accumulator entry).  All of a tile's B
+// operands are read before its matrix instructions, so the LDS reads of
+// one tile overlap the products of the previous one.
+template <int NL, int WPE>
+struct FdmTiles {
+  static constexpr int N2 = NL * NL, KS = (NL + 3) / 4, NT = (N2 + 15) / 16,
+                       TPW = (NT + WPE - 1) / WPE;
+};
+
+template <int NL, int WPE, int DIR, bool TRANS, bool SCALE, typename T>
+__device__ __forceinline__ void fdm_mfma_pass(T *__restrict__ out, const T *__restrict__ in,
+                                              const T *__restrict__ M,
+                                              const T (*dsc)[4], int lane, int wave) {
+  using TL = FdmTiles<NL, WPE>;
+  constexpr int N2 = TL::N2, KS = TL::KS, NT = TL::NT, TPW = TL::TPW;
+  using acc_t = typename MfmaTile<T>::acc_t;
+  const int lr = lane & 15, lk = lane >> 4;
+  T am[KS];
+#pragma unroll
+  for (int kk = 0; kk < KS; kk++) {
+    const int k = 4 * kk + lk;
+    am[kk] = (lr < NL && k < NL) ? (TRANS ? M[k + NL * lr] : M[lr + NL * k]) : T(0);
+  }
+#pragma unroll
+  for (int m = 0; m < TPW; m++) {
+    const int tt = wave + WPE * m;
+    if (tt >= NT) break;
+    const int col = 16 * tt + lr;
+    T bv[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; kk++) {
+      const int k = 4 * kk + lk;
+      bv[kk] = (k < NL && col < N2) ? in[fdm_at<NL, DIR>(k, col)] : T(0);
+    }
+    acc_t acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int kk = 0; kk < KS; kk++) acc = MfmaTile<T>::fma(am[kk], bv[kk], acc);
+    if (col < N2) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = MfmaTile<T>::row(lane, v);
+        if (row < NL) out[fdm_at<NL, DIR>(row, col)] = SCALE ? dsc[m][v] * acc[v] : acc[v];
+      }
+    }
+  }
+}
+
+// WPE waves per element (the tiles of every transform dealt out over them)
+template <int NL, int WPE, bool FUSED, typename T>
+__global__ void __launch_bounds__(64 * WPE)
+    fdm_mfma_kernel(T *__restrict__ e, T *r, const T *__restrict__ S, const T *__restrict__ D,
+                    int64_t nel) {
+  static_assert(NL <= 16, "one 16-row tile per transform");
+  using TL = FdmTiles<NL, WPE>;
+  constexpr int N3 = NL * NL * NL, N2 = NL * NL, W = NL - 2, NF = 6 * W * W;
+  constexpr int NTH = 64 * WPE, TPW = TL::TPW, NT = TL::NT;
+  __shared__ T a[N3], b[N3];
+  __shared__ T sm[6 * N2];
+  const int64_t el = blockIdx.x;
+  if (el >= nel) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T *re = r + el * N3;
+  const T *se = S + el * 6 * N2;
+  // this lane's D values of the z-forward pass (its output points there)
+  T dsc[TPW][4];
+  {
+    const T *de = D + el * N3;
+    const int lr = lane & 15;
+#pragma unroll
+    for (int m = 0; m < TPW; m++) {
+      const int tt = wave + WPE * m, col = 16 * tt + lr;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = MfmaTile<T>::row(lane, v);
+        dsc[m][v] = (tt < NT && col < N2 && row < NL) ? de[fdm_at<NL, 2>(row, col)] : T(0);
+      }
+    }
+  }
+  for (int q = tid; q < 6 * N2; q += NTH) sm[q] = se[q];
+  for (int q = tid; q < N3; q += NTH) a[q] = re[q];
+  __syncthreads();
+  if (FUSED) {
+    // the neighbour's overlap only: face layer 0 -= layer 2 (face interiors)
+    for (int q = tid; q < NF; q += NTH) {
+      const int f = q / (W * W), u = 1 + (q % (W * W)) % W, v = 1 + (q % (W * W)) / W;
+      const int p0 = face_point<NL>(f, u, v, 0), p2 = face_point<NL>(f, u, v, 2);
+      a[p0] = a[p0] - a[p2];
+    }
+    __syncthreads();
+  }
+  const T *S1 = sm, *S1t = sm + N2, *S2 = sm + 2 * N2, *S2t = sm + 3 * N2, *S3 = sm + 4 * N2,
+          *S3t = sm + 5 * N2;
+  fdm_mfma_pass<NL, WPE, 0, false, false>(b, a, S1t, dsc, lane, wave);
+  __syncthreads();
+  fdm_mfma_pass<NL, WPE, 1, true, false>(a, b, S2, dsc, lane, wave);
+  __syncthreads();
+  fdm_mfma_pass<NL, WPE, 2, true, true>(b, a, S3, dsc, lane, wave);
+  __syncthreads();
+  fdm_mfma_pass<NL, WPE, 2, true, false>(a, b, S3t, dsc, lane, wave);
+  __syncthreads();
+  fdm_mfma_pass<NL, WPE, 1, true, false>(b, a, S2t, dsc, lane, wave);
+  __syncthreads();
+  fdm_mfma_pass<NL, WPE, 0, false, false>(a, b, S1, dsc, lane, wave);
+  __syncthreads();
+  T *ee = e + el * N3;
+  for (int q = tid; q < N3; q += NTH) ee[q] = a[q];
+  if (FUSED) {
+    // this element's own face layer 0 goes back into r (subtracted after
+    // the gs)
+    for (int q = tid; q < NF; q += NTH) {
+      const int f = q / (W * W), u = 1 + (q % (W * W)) % W, v = 1 + (q % (W * W)) / W;
+      const int p0 = face_point<NL>(f, u, v, 0);
+      re[p0] = a[p0];
+    }
+  }
+}
+
+// FDM kernel family: 0 matrix cores where they are faster (fdm_mfma_kernel),
+// 1 vector ALUs everywhere (the line kernel in FP32, the plane kernel in
+// FP64); process-wide
+static int g_fdm_variant = 0;
+constexpr int FDM_WPE = 4;  // waves per element of the matrix-core kernel
+
 // fused: r is the extended work array after its gs (face layer 0 is
 // corrected on load and receives e's layer 0 afterwards)
 template <typename T>
 static int fdm(T *e, T *r, const T *S, const T *D, int nl, int64_t nel, hipStream_t s,
                bool fused = false) {
+  // matrix cores for FP64 at every size and for FP32 from NL = 10 (timed on
+  // 48^3: NL 10 FP64 1.49 -> 0.97 ms, FP32 0.82 -> 0.65 ms; NL 6 FP64 0.29
+  // -> 0.27 ms, but FP32 0.18 -> 0.23 ms, where the line kernel stays)
+  if (g_fdm_variant == 0 && (std::is_same<T, double>::value || nl >= 10)) {
+    switch (nl) {
+#define NM_FDMM(N)                                                                       \
+  case N:                                                                                \
+    if (fused)                                                                           \
+      hipLaunchKernelGGL((fdm_mfma_kernel<N, FDM_WPE, true, T>), dim3((unsigned)nel),    \
+                         dim3(64 * FDM_WPE), 0, s, e, r, S, D, nel);                     \
+    else                                                                                 \
+      hipLaunchKernelGGL((fdm_mfma_kernel<N, FDM_WPE, false, T>), dim3((unsigned)nel),   \
+                         dim3(64 * FDM_WPE), 0, s, e, r, S, D, nel);                     \
+    NM_HIP_TRY(hipGetLastError());                                                       \
+    return NM_OK;
+      NM_FDMM(5) NM_FDMM(6) NM_FDMM(7) NM_FDMM(8) NM_FDMM(9) NM_FDMM(10) NM_FDMM(11)
+      NM_FDMM(12)
+#undef NM_FDMM
+      default:
+        set_error("fdm: extended size %d unsupported (lx 3..10)", nl);
+        return NM_ERR_UNSUPPORTED;
+    }
+  }
   switch (nl) {
 #define NM_FDM(N)                                                                       \
   case N:                                                                               \
@@ -1074,6 +1259,17 @@ struct nm_gmres_s;
 
 extern "C" {
 
+// Kernel family of every Schwarz FDM solve that follows (hsmg, phmg):
+// 0 matrix cores (default), 1 vector ALUs.  Both give the same solve to
+// rounding; the choice is for timing.
+int nm_fdm_set_variant(int variant) {
+  if (variant != 0 && variant != 1) {
+    set_error("nm_fdm_set_variant: %d (0 matrix cores, 1 vector ALUs)", variant);
+    return NM_ERR_ARG;
+  }
+  g_fdm_variant = variant;
+  return NM_OK;
+}
 
 // hsmg on an operator built on a box (nm_op_create_box) or a general mesh
 // (nm_op_create_mesh).  facet_bc: Neko's facet_type of the boundary facets

@@ -108,6 +108,12 @@ def box_mask(nelx, nely, nelz, lx, periodic=(0, 0, 0)):
     return msk[: mm.value].copy()
 
 
+def fdm_set_variant(variant: int) -> None:
+    """Kernel family of the Schwarz FDM solves (nm_fdm_set_variant): 0 the
+    matrix cores (default), 1 the vector ALUs."""
+    check(lib().nm_fdm_set_variant(int(variant)), "nm_fdm_set_variant")
+
+
 def read_nmsh(path):
     """Neko .nmsh file -> (vxyz (nel, 8, 3), pid (nel, 8), el_idx (nel,)),
     vertices in Neko's symmetric order, periodic zones applied

@@ -299,3 +299,59 @@ def test_phmg_on_reference_mesh():
     z = torch.zeros(p.n, dtype=torch.float64, device="cuda")
     ph.apply(z, _t(r))
     assert _relmax(_np(z), z_or) < 1e-9
+
+
+@pytest.mark.parametrize("case", [HS_CASES[0], HS_CASES[2], HS_CASES[5]])
+def test_fdm_matrix_cores_match_vector_alus(case):
+    """The Schwarz FDM solve on the matrix cores (v_mfma_f64_16x16x4f64,
+    the default) and on the vector ALUs (nm_fdm_set_variant(1)): the same
+    fine-level Schwarz application to 1e-13, each within 1e-11 of the
+    oracle (lx 8, 4, 10: extended sizes 10, 6, 12)."""
+    from neko_amd.solve import HSMG, fdm_set_variant
+    dims, lx, amp, per, fb = case
+    print("fdm variants", case, flush=True)
+    c, p = _pair(dims, lx, amp, per)
+    H = O.OracleHsmg(c, facet_bc=fb)
+    hs = HSMG(p, facet_bc=fb)
+    r = c.rhs_normal(31)
+    e_or = H.schwarz_fine(r)
+    out = []
+    try:
+        for variant in (0, 1):
+            fdm_set_variant(variant)
+            e = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+            hs.schwarz_fine(e, _t(r))
+            out.append(_np(e))
+    finally:
+        fdm_set_variant(0)
+    assert _relmax(out[0], out[1]) < 1e-13
+    for o in out:
+        assert _relmax(o, e_or) < 1e-11
+
+
+def test_fdm_matrix_cores_fp32_smoothers():
+    """FP32 smoothing with the extended size 10 (lx 8) FDM on the matrix
+    cores (v_mfma_f32_16x16x4f32) and on the vector ALUs: one V-cycle each
+    within 1e-5 of the FP64 oracle's and within 1e-5 of each other."""
+    from neko_amd.solve import PHMG, fdm_set_variant
+    dims, lx, amp, per, h1, h2, fb, lh2 = PH_CASES[0]
+    assert lx == 8
+    print("fdm variants fp32", flush=True)
+    c, p = _pair(dims, lx, amp, per, h1=h1, h2=h2)
+    P = O.OraclePhmg(c, facet_bc=fb, level_h2=lh2)
+    r = c.rhs_normal(41)
+    z_or = P.apply(r)
+    out = []
+    try:
+        for variant in (0, 1):
+            fdm_set_variant(variant)
+            ph = PHMG(p, facet_bc=fb, smooth_its=4, level_h2=lh2, smoother="fp32")
+            z = torch.zeros(p.n, dtype=torch.float64, device="cuda")
+            ph.apply(z, _t(r))
+            out.append(_np(z))
+            ph.free()
+    finally:
+        fdm_set_variant(0)
+    assert _relmax(out[0], out[1]) < 1e-5
+    for o in out:
+        assert _relmax(o, z_or) < 1e-5
