@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--no-neko-arrays", action="store_true",
                     help="skip the operator built from Neko's arrays")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the small (8^3) Poisson solve timing")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -404,7 +406,8 @@ def main():
         solves["mixed_vs_fp64_rel_l2"] = float(torch.linalg.norm(xm - xd) / torch.linalg.norm(xd))
         solves["speedup_mixed"] = round(solves["fp64"]["seconds"] / solves["mixed"]["seconds"], 3)
         result["solves"] = solves
-        result["c1_solve"] = c1_solve_section(S)
+        if not args.no_c1:
+            result["c1_solve"] = c1_solve_section(S)
 
     # ---- Helmholtz with the Chebyshev/Schwarz p-multigrid ---------------------
     if not args.no_helmholtz:

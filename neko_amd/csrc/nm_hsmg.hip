@@ -427,21 +427,23 @@ __global__ void toreg_fused_kernel(T *__restrict__ e, const T *__restrict__ w2,
   e[idx] = v;
 }
 
-// schwarz_wt3d: weights on the two outer layers of every face
-template <typename T>
-__global__ void wt3d_kernel(T *__restrict__ e, const T *__restrict__ wt, int n, int64_t nel) {
+// schwarz_wt3d (schwarz.f90:597-634): weights on the two outer layers of
+// every face.  The weights a point takes (r-faces for all (j,k); s-faces
+// for i in 3..n-2 (1-based); t-faces for i,j in 3..n-2; a point on several
+// layers takes every weight, in the reference's order) are multiplied
+// into one per-point factor at set-up, so the application is a coalesced
+// e *= wtp.
+__global__ void wtp_setup_kernel(double *__restrict__ wtp, const double *__restrict__ wt,
+                                 int n, int64_t nel) {
   const int64_t n3 = (int64_t)n * n * n;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nel * n3) return;
   const int64_t el = idx / n3;
   const int p = (int)(idx % n3);
   const int i = p % n, j = (p / n) % n, k = p / (n * n);
-  const T *we = wt + el * (int64_t)n * n * 12;
+  const double *we = wt + el * (int64_t)n * n * 12;
   auto W = [&](int a, int b, int c, int d) { return we[a + n * (b + n * (c + 4 * d))]; };
-  T v = e[idx];
-  // r-faces for all (j,k); s-faces for i in 3..n-2 (1-based); t-faces for
-  // i,j in 3..n-2 (schwarz.f90:606-630).  A point on several layers (n = 3)
-  // takes every weight, in the reference's order.
+  double v = 1.0;
   auto apply = [&](int x, int a, int b, int dir) {
     if (x == 0) v *= W(a, b, 0, dir);
     if (x == 1) v *= W(a, b, 1, dir);
@@ -452,7 +454,14 @@ __global__ void wt3d_kernel(T *__restrict__ e, const T *__restrict__ wt, int n, 
   const bool mid_i = (i >= 2 && i <= n - 3), mid_j = (j >= 2 && j <= n - 3);
   if (mid_i) apply(j, i, k, 1);
   if (mid_i && mid_j) apply(k, i, j, 2);
-  e[idx] = v;
+  wtp[idx] = v;
+}
+
+template <typename T>
+__global__ void wtp_mul_kernel(T *__restrict__ e, const T *__restrict__ wtp, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    e[i] *= __builtin_nontemporal_load(wtp + i);
 }
 
 // wt from the summed overlap counts (schwarz_setup_schwarz_wt3d_2)
@@ -494,7 +503,7 @@ struct Schwarz {
   int64_t nel = 0;
   const Op *op = nullptr;
   GsDev ext;
-  double *S = nullptr, *D = nullptr, *wt = nullptr;
+  double *S = nullptr, *D = nullptr, *wt = nullptr;  // wt: per-point weight product
   double *work1 = nullptr, *work2 = nullptr;
   // FP32 copies for the FP32 smoothers (allocated on request)
   float *S32 = nullptr, *D32 = nullptr, *wt32 = nullptr;
@@ -810,7 +819,8 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
                      lx, nel);
   if ((st = op.gs<T>(e, nullptr, s))) return st;
   if ((st = launch_mask<T>(e, op.msk, op.nmsk, nullptr, s))) return st;
-  hipLaunchKernelGGL(wt3d_kernel<T>, dim3(blocks_for(op.n)), dim3(256), 0, s, e, wt, lx, nel);
+  hipLaunchKernelGGL(wtp_mul_kernel<T>, dim3((unsigned)std::min<int64_t>(blocks_for(op.n), 16384)),
+                     dim3(256), 0, s, e, wt, op.n);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -969,7 +979,7 @@ static int schwarz_setup(H *h, Schwarz *sw, const Op *op, int facet_bc, hipStrea
   if (st) return st;
   const int64_t ns = nel * nl * nl * nl;
   if (!(sw->S = h->template alloc<double>(nel * 6 * nl * nl)) || !(sw->D = h->template alloc<double>(ns)) ||
-      !(sw->wt = h->template alloc<double>(nel * lx * lx * 12)) || !(sw->work1 = h->template alloc<double>(ns)) ||
+      !(sw->wt = h->template alloc<double>(n)) || !(sw->work1 = h->template alloc<double>(ns)) ||
       !(sw->work2 = h->template alloc<double>(ns))) {
     set_error("hsmg: out of device memory");
     return NM_ERR_NOMEM;
@@ -1005,10 +1015,20 @@ static int schwarz_setup(H *h, Schwarz *sw, const Op *op, int facet_bc, hipStrea
   if ((st = extrude(sw->work2, 2, 1.0, sw->work2, 0, 1.0, nl, nel, s))) return st;
   hipLaunchKernelGGL(toreg_kernel, dim3(blocks_for(n)), dim3(256), 0, s, e1, sw->work2, lx, nel);
   if ((st = op->gs<double>(e1, nullptr, s))) return st;
+  // the reference's face-layer weights (12 lx^2 per element, set-up only),
+  // then their product per point
+  double *wface = nullptr;
+  NM_HIP_TRY(hipMalloc(&wface, sizeof(double) * nel * lx * lx * 12));
   hipLaunchKernelGGL(wt_setup_kernel, dim3(blocks_for(nel * lx * lx * 12)), dim3(256), 0, s,
-                     sw->wt, e1, lx, nel);
-  NM_HIP_TRY(hipGetLastError());
-  NM_HIP_TRY(hipStreamSynchronize(s));
+                     wface, e1, lx, nel);
+  hipLaunchKernelGGL(wtp_setup_kernel, dim3(blocks_for(n)), dim3(256), 0, s, sw->wt, wface, lx,
+                     nel);
+  const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+  (void)hipFree(wface);
+  if (!ok) {
+    set_error("hsmg: Schwarz weight set-up failed");
+    return NM_ERR_HIP;
+  }
   return NM_OK;
 }
 
@@ -1443,7 +1463,7 @@ int nm_phmg_create(nm_phmg_t **out, nm_op_t *fine_h, int facet_bc, int smooth_it
       if ((st = op->ensure32(s))) return fail(st);
       Schwarz &w = h->sw[l];
       const int64_t n = op->n, nl = w.nl, ns = w.nel * nl * nl * nl;
-      const int64_t nS = w.nel * 6 * nl * nl, nW = w.nel * w.lx * w.lx * 12;
+      const int64_t nS = w.nel * 6 * nl * nl, nW = n;
       Cheby &c = h->ch[l];
       if (!(w.S32 = h->alloc<float>(nS)) || !(w.D32 = h->alloc<float>(ns)) ||
           !(w.wt32 = h->alloc<float>(nW)) || !(w.work1_32 = h->alloc<float>(ns)) ||
