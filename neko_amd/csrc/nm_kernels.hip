@@ -1073,15 +1073,31 @@ __global__ void __launch_bounds__(RED_THREADS)
                               const double *const *__restrict__ v,
                               const double *__restrict__ mult, int j, int64_t n,
                               double *__restrict__ part) {
+  // G vectors per sweep, so w and mult are read once per G vectors instead
+  // of once per vector; every vector's per-thread sum runs over the same
+  // points in the same order as a one-vector sweep, so the partials are
+  // the same bits whatever G is
+  constexpr int G = 8;
   __shared__ double sh[RED_THREADS / 64];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int l = 0; l < j; l++) {
-    const double *vl = v[l];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-      acc += w[i] * vl[i] * mult[i];
-    const double r = block_sum(acc, sh);
-    if (threadIdx.x == 0) part[(int64_t)l * RED_BLOCKS + blockIdx.x] = r;
+  for (int l0 = 0; l0 < j; l0 += G) {
+    const int ng = j - l0 < G ? j - l0 : G;
+    const double *vp[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) vp[g] = v[l0 + (g < ng ? g : 0)];
+    double acc[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[g] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const double wi = w[i], mi = mult[i];
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (g < ng) acc[g] += wi * __builtin_nontemporal_load(vp[g] + i) * mi;
+    }
+    for (int g = 0; g < ng; g++) {
+      const double r = block_sum(acc[g], sh);
+      if (threadIdx.x == 0) part[(int64_t)(l0 + g) * RED_BLOCKS + blockIdx.x] = r;
+    }
   }
 }
 
