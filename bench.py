@@ -510,7 +510,10 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
     the HBM-roofline fraction of the bytes the solves moved (iterations x
     bytes per DOF-iteration of the schedule run, plus the FP64 residual
     replacements of mixed solves).  The error of served results against the
-    FP64 CPU oracle is measured on requests of the smallest pool mesh."""
+    FP64 CPU oracle is measured on requests of the smallest pool mesh.  At
+    rates up to 100 req/s the mixed served configuration also runs the
+    stream live: a producer thread submits each request at its arrival
+    time while the server runs (nm_server_serve)."""
     import numpy as np
     import torch
 
@@ -549,11 +552,35 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
 
     oracle_err = {}
 
-    def run(label, arrivals, mesh_of, seed0):
+    def run(label, arrivals, mesh_of, seed0, live=False):
         sv, ids, precision = servers[label]
-        rids = [sv.submit(ids[m], seed0 + q, rel_tol=rel_tol, t_arrive=float(t))
-                for q, (t, m) in enumerate(zip(arrivals, mesh_of))]
-        wall = sv.run()
+        if live:
+            # a producer thread submits each request at its arrival time
+            # while the server runs (nm_server_serve); the arrival stamp is
+            # the submission time on the server's clock
+            import threading
+            rids = []
+
+            def producer():
+                t_start = time.perf_counter()
+                try:
+                    for q, (t, m) in enumerate(zip(arrivals, mesh_of)):
+                        wait = t_start + float(t) - time.perf_counter()
+                        if wait > 0:
+                            time.sleep(wait)
+                        rids.append(sv.submit(ids[m], seed0 + q, rel_tol=rel_tol,
+                                              t_arrive=-1.0))
+                finally:
+                    sv.close()
+
+            th = threading.Thread(target=producer)
+            th.start()
+            wall = sv.serve()
+            th.join()
+        else:
+            rids = [sv.submit(ids[m], seed0 + q, rel_tol=rel_tol, t_arrive=float(t))
+                    for q, (t, m) in enumerate(zip(arrivals, mesh_of))]
+            wall = sv.run()
         res = [sv.result(r) for r in rids]
         lat = np.array([r.t_done - r.t_arrive for r in res])
         _, peak = sv.memory()
@@ -584,7 +611,7 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
                     "iterations": res[q].iter, "oracle_iterations": m_or.iter,
                     "rel_l2_vs_oracle": float(np.linalg.norm(xs - x_or) / np.linalg.norm(x_or))}
                 break
-        log(f"[bench] serve {label}: {d}")
+        log(f"[bench] serve {label}{' (live arrivals)' if live else ''}: {d}")
         return d
 
     for k, rate in enumerate(rates):
@@ -593,6 +620,11 @@ def serving(S, rates, nreq, pool=12, budget=200e9, rel_tol=1e-8, seed=31):
         mesh_of = rng.integers(0, pool, nreq)
         out[f"{int(rate)}_req_s"] = {
             label: run(label, arrivals, mesh_of, 1000 * (k + 1)) for label, _, _ in configs}
+        if rate <= 100:
+            # the same stream with real-time arrivals (submitted while the
+            # server runs) on the mixed served configuration
+            out[f"{int(rate)}_req_s"]["mixed_served_live"] = run(
+                "mixed_served", arrivals, mesh_of, 1000 * (k + 1), live=True)
     out["error_vs_oracle"] = oracle_err
     for sv, _, _ in servers.values():
         sv.free()

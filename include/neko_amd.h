@@ -520,7 +520,7 @@ typedef struct {
   int state;            /* 0 queued, 1 running, 2 done, 3 failed */
   int iter, outer, converged;
   double res_start, res_final;
-  double t_arrive, t_admit, t_done;  /* seconds from nm_server_run start */
+  double t_arrive, t_admit, t_done;  /* seconds from the stream's start */
   int64_t n;            /* local dofs of the request's mesh */
 } nm_serve_result_t;
 /* precision 0 FP64 / 1 mixed; chunk_iters: CG iterations issued per
@@ -542,11 +542,24 @@ int nm_server_add_mesh(nm_server_t *sv, int64_t nel, int lx, const double *vxyz,
                        int *mesh_id, hipStream_t s);
 int nm_server_mesh_op(nm_server_t *sv, int mesh_id, nm_op_t **op);
 /* a request: mesh, RHS seed (nm_op_rhs_normal), relative tolerance,
- * arrival time in seconds after the start of nm_server_run */
+ * arrival time in seconds after the start of nm_server_run (or of
+ * nm_server_serve; negative: now, on a live stream's clock).  Safe to call
+ * from another thread while nm_server_serve runs. */
 int nm_server_submit(nm_server_t *sv, int mesh_id, uint64_t rhs_seed,
                      double rel_tol, int max_iter, double t_arrive,
                      int *req_id);
+/* serve the requests submitted so far (pre-timed arrivals); returns when
+ * all have completed */
 int nm_server_run(nm_server_t *sv, double *wall_s);
+/* live stream: serve requests as other threads submit them, until
+ * nm_server_close has been called and every request submitted before it
+ * has completed; meshes are registered before it starts.  nm_server_result
+ * may be polled meanwhile: a request's fields are final once its state
+ * reads 2. */
+int nm_server_serve(nm_server_t *sv, double *wall_s);
+int nm_server_close(nm_server_t *sv);
+/* seconds on the running live stream's clock (0 when none runs) */
+double nm_server_clock(nm_server_t *sv);
 int nm_server_result(const nm_server_t *sv, int req_id,
                      nm_serve_result_t *r);
 int nm_server_solution(const nm_server_t *sv, int req_id, double *x,

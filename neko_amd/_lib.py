@@ -153,6 +153,9 @@ SIGNATURES = {
     "nm_server_mesh_op": (_i, [_vp, _i, C.POINTER(_vp)]),
     "nm_server_submit": (_i, [_vp, _i, C.c_uint64, _d, _i, _d, C.POINTER(_i)]),
     "nm_server_run": (_i, [_vp, C.POINTER(_d)]),
+    "nm_server_serve": (_i, [_vp, C.POINTER(_d)]),
+    "nm_server_close": (_i, [_vp]),
+    "nm_server_clock": (_d, [_vp]),
     "nm_server_result": (_i, [_vp, _i, C.POINTER(ServeResult)]),
     "nm_server_solution": (_i, [_vp, _i, _vp, _vp]),
     "nm_server_memory": (_i, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

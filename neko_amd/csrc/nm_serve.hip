@@ -17,9 +17,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -42,7 +45,9 @@ struct ServeReq {
   double rel_tol = 1e-8;
   int max_iter = 10000;
   double t_arrive = 0.0, t_admit = -1.0, t_done = -1.0;
-  int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  // 0 queued, 1 running, 2 done, 3 failed; stored last (release) by the
+  // serving thread, so a reader that sees 2 sees every result field final
+  std::atomic<int> state{0};
   CgSolver *cg = nullptr;
   double *x_keep = nullptr;
   int iter = 0, outer = 0, converged = 0;
@@ -53,15 +58,26 @@ struct Server {
   int64_t budget = 0, used = 0, peak = 0;
   int precision = 1, max_inflight = 8, chunk_max = 2, keep = 0;
   std::vector<ServeMesh> meshes;
-  std::vector<ServeReq> reqs;
+  // requests: records never move once submitted; the vector of them is
+  // grown by nm_server_submit, possibly from another thread while a live
+  // stream runs, so it is touched only under `mu`
+  std::vector<std::unique_ptr<ServeReq>> reqs;
+  std::mutex mu;
+  std::vector<int> pending;  // submitted, not yet in `order`
+  bool live = false, closed = false;
+  double t0 = 0.0;           // stream clock origin (steady clock, seconds)
   std::vector<int> order, inflight;
   size_t next = 0;
   int completed = 0, failed = 0;
   double t_end = 0.0;
+  ServeReq *req(int id) {
+    std::lock_guard<std::mutex> g(mu);
+    return reqs[id].get();
+  }
   ~Server() {
-    for (ServeReq &q : reqs) {
-      if (q.cg) cg_solver_destroy(q.cg);
-      if (q.x_keep) (void)hipFree(q.x_keep);
+    for (auto &qp : reqs) {
+      if (qp->cg) cg_solver_destroy(qp->cg);
+      if (qp->x_keep) (void)hipFree(qp->x_keep);
     }
     for (ServeMesh &m : meshes) {
       for (CgSolver *c : m.idle) cg_solver_destroy(c);
@@ -132,8 +148,8 @@ static int admit(Server *sv, ServeReq &q, double t, bool *deferred) {
     return st;
   }
   q.cg = cg;
-  q.state = 1;
   q.t_admit = t;
+  q.state.store(1, std::memory_order_release);
   return NM_OK;
 }
 
@@ -145,7 +161,6 @@ static int finish(Server *sv, ServeReq &q, double t) {
   q.res_start = cg->res_start;
   q.res_final = cg->rnorm;
   q.t_done = t;
-  q.state = 2;
   if (sv->keep) {
     const int64_t nb = sizeof(double) * cg->op->n;
     if (hipMalloc((void **)&q.x_keep, nb) != hipSuccess) {
@@ -160,6 +175,7 @@ static int finish(Server *sv, ServeReq &q, double t) {
   cg->phase = CgSolver::IDLE;
   sv->meshes[q.mesh].idle.push_back(cg);
   q.cg = nullptr;
+  q.state.store(2, std::memory_order_release);
   return NM_OK;
 }
 
@@ -199,6 +215,14 @@ void nm_server_free(nm_server_t *h) { delete h; }
 static int register_mesh(nm_server_t *h, nm_op_t *op, int tune_reps, int prewarm,
                          int *mesh_id, hipStream_t s) {
   Server &sv = h->sv;
+  {
+    std::lock_guard<std::mutex> g(sv.mu);
+    if (sv.live) {
+      nm_op_free(op);
+      set_error("server: meshes are registered before a live stream starts");
+      return NM_ERR_STATE;
+    }
+  }
   ServeMesh m;
   m.op = op;
   int st;
@@ -261,65 +285,99 @@ int nm_server_submit(nm_server_t *h, int mesh_id, uint64_t rhs_seed, double rel_
     set_error("nm_server_submit: bad arguments");
     return NM_ERR_ARG;
   }
-  ServeReq q;
-  q.mesh = mesh_id;
-  q.seed = rhs_seed;
-  q.rel_tol = rel_tol;
-  q.max_iter = max_iter;
-  q.t_arrive = t_arrive;
-  sv.reqs.push_back(q);
+  auto q = std::make_unique<ServeReq>();
+  q->mesh = mesh_id;
+  q->seed = rhs_seed;
+  q->rel_tol = rel_tol;
+  q->max_iter = max_iter;
+  std::lock_guard<std::mutex> g(sv.mu);
+  if (sv.live && sv.closed) {
+    set_error("nm_server_submit: the live stream is closed");
+    return NM_ERR_STATE;
+  }
+  // negative: arrives now (live streams), on the stream clock
+  q->t_arrive = t_arrive >= 0.0 ? t_arrive : (sv.live ? now_s() - sv.t0 : 0.0);
+  sv.reqs.push_back(std::move(q));
   *req_id = (int)sv.reqs.size() - 1;
+  sv.pending.push_back(*req_id);
   return NM_OK;
+}
+
+// One pass of the request loop: admission in arrival order, then polling
+// of every in-flight request.  Returns the number of requests finished in
+// the pass (>= 0) or an error status (< 0).
+static int serve_pass(Server &sv, double t0, bool *progressed) {
+  const double t = now_s() - t0;
+  *progressed = false;
+  int done = 0;
+  while (sv.next < sv.order.size() && (int)sv.inflight.size() < sv.max_inflight) {
+    ServeReq &q = *sv.req(sv.order[sv.next]);
+    if (q.t_arrive > t) break;
+    bool deferred = false;
+    int st = admit(&sv, q, t, &deferred);
+    if (st) return -st;
+    if (deferred) {
+      if (sv.inflight.empty()) {
+        set_error("nm_server: a request does not fit the HBM budget");
+        return -NM_ERR_NOMEM;
+      }
+      break;
+    }
+    sv.inflight.push_back(sv.order[sv.next]);
+    sv.next++;
+    *progressed = true;
+  }
+  for (size_t k = 0; k < sv.inflight.size();) {
+    ServeReq &q = *sv.req(sv.inflight[k]);
+    const int st = cg_poll(q.cg);
+    if (st < 0) return st;
+    if (st == 1) {
+      const int fs = finish(&sv, q, now_s() - t0);
+      if (fs) return -fs;
+      sv.inflight.erase(sv.inflight.begin() + (long)k);
+      done++;
+      *progressed = true;
+    } else {
+      k++;
+    }
+  }
+  return done;
+}
+
+// move newly submitted requests into the admission order (the not yet
+// admitted tail stays sorted by arrival time)
+static void take_pending(Server &sv) {
+  std::lock_guard<std::mutex> g(sv.mu);
+  if (sv.pending.empty()) return;
+  for (int id : sv.pending) sv.order.push_back(id);
+  sv.pending.clear();
+  std::stable_sort(sv.order.begin() + (long)sv.next, sv.order.end(), [&](int a, int b) {
+    return sv.reqs[a]->t_arrive < sv.reqs[b]->t_arrive;
+  });
 }
 
 // Run the stream: arrival times are seconds after this call starts.
 // Returns when every submitted request has completed (or failed).
 int nm_server_run(nm_server_t *h, double *wall_s) {
   Server &sv = h->sv;
+  {
+    std::lock_guard<std::mutex> g(sv.mu);
+    if (sv.live) {
+      set_error("nm_server_run: a live stream is running");
+      return NM_ERR_STATE;
+    }
+  }
   sv.order.clear();
-  for (int i = 0; i < (int)sv.reqs.size(); i++)
-    if (sv.reqs[i].state == 0) sv.order.push_back(i);
-  std::stable_sort(sv.order.begin(), sv.order.end(), [&](int a, int b) {
-    return sv.reqs[a].t_arrive < sv.reqs[b].t_arrive;
-  });
   sv.next = 0;
+  take_pending(sv);
   const int todo = (int)sv.order.size();
   int done = 0;
   const double t0 = now_s();
   while (done < todo) {
-    const double t = now_s() - t0;
     bool progressed = false;
-    while (sv.next < sv.order.size() && (int)sv.inflight.size() < sv.max_inflight) {
-      ServeReq &q = sv.reqs[sv.order[sv.next]];
-      if (q.t_arrive > t) break;
-      bool deferred = false;
-      int st = admit(&sv, q, t, &deferred);
-      if (st) return st;
-      if (deferred) {
-        if (sv.inflight.empty()) {
-          set_error("nm_server_run: a request does not fit the HBM budget");
-          return NM_ERR_NOMEM;
-        }
-        break;
-      }
-      sv.inflight.push_back(sv.order[sv.next]);
-      sv.next++;
-      progressed = true;
-    }
-    for (size_t k = 0; k < sv.inflight.size();) {
-      ServeReq &q = sv.reqs[sv.inflight[k]];
-      const int st = cg_poll(q.cg);
-      if (st < 0) return st;
-      if (st == 1) {
-        const int fs = finish(&sv, q, now_s() - t0);
-        if (fs) return fs;
-        sv.inflight.erase(sv.inflight.begin() + (long)k);
-        done++;
-        progressed = true;
-      } else {
-        k++;
-      }
-    }
+    const int st = serve_pass(sv, t0, &progressed);
+    if (st < 0) return -st;
+    done += st;
     if (!progressed) std::this_thread::yield();
   }
   sv.t_end = now_s() - t0;
@@ -327,14 +385,75 @@ int nm_server_run(nm_server_t *h, double *wall_s) {
   return NM_OK;
 }
 
+// Live stream: nm_server_submit may be called from other threads while
+// this runs (t_arrive < 0 stamps the submission time on the stream
+// clock, which starts here); returns once nm_server_close has been called
+// and every request submitted before it has completed.
+int nm_server_serve(nm_server_t *h, double *wall_s) {
+  Server &sv = h->sv;
+  {
+    std::lock_guard<std::mutex> g(sv.mu);
+    if (sv.live) {
+      set_error("nm_server_serve: a live stream is already running");
+      return NM_ERR_STATE;
+    }
+    sv.live = true;  // a close() made before this call ends this stream
+    sv.t0 = now_s();
+  }
+  sv.order.clear();
+  sv.next = 0;
+  int rc = NM_OK;
+  for (;;) {
+    bool closed;
+    {
+      std::lock_guard<std::mutex> g(sv.mu);
+      closed = sv.closed;
+    }
+    take_pending(sv);
+    bool progressed = false;
+    const int st = serve_pass(sv, sv.t0, &progressed);
+    if (st < 0) {
+      rc = -st;
+      break;
+    }
+    if (closed && sv.next == sv.order.size() && sv.inflight.empty()) {
+      std::lock_guard<std::mutex> g(sv.mu);
+      if (sv.pending.empty()) break;
+      continue;
+    }
+    if (!progressed) std::this_thread::yield();
+  }
+  sv.t_end = now_s() - sv.t0;
+  if (wall_s) *wall_s = sv.t_end;
+  std::lock_guard<std::mutex> g(sv.mu);
+  sv.live = false;
+  sv.closed = false;
+  return rc;
+}
+
+// End a live stream: no further submissions; nm_server_serve returns once
+// the submitted requests have completed.
+int nm_server_close(nm_server_t *h) {
+  std::lock_guard<std::mutex> g(h->sv.mu);
+  h->sv.closed = true;
+  return NM_OK;
+}
+
+// Seconds on the stream clock of the running live stream (0 when none).
+double nm_server_clock(nm_server_t *h) {
+  std::lock_guard<std::mutex> g(h->sv.mu);
+  return h->sv.live ? now_s() - h->sv.t0 : 0.0;
+}
+
 int nm_server_result(const nm_server_t *h, int req_id, nm_serve_result_t *r) {
-  const Server &sv = h->sv;
+  Server &sv = const_cast<nm_server_t *>(h)->sv;
+  std::lock_guard<std::mutex> g(sv.mu);
   if (req_id < 0 || req_id >= (int)sv.reqs.size()) {
     set_error("nm_server_result: no request %d", req_id);
     return NM_ERR_ARG;
   }
-  const ServeReq &q = sv.reqs[req_id];
-  r->state = q.state;
+  const ServeReq &q = *sv.reqs[req_id];
+  r->state = q.state.load(std::memory_order_acquire);
   r->iter = q.iter;
   r->outer = q.outer;
   r->converged = q.converged;
@@ -348,12 +467,17 @@ int nm_server_result(const nm_server_t *h, int req_id, nm_serve_result_t *r) {
 }
 
 int nm_server_solution(const nm_server_t *h, int req_id, double *x, hipStream_t s) {
-  const Server &sv = h->sv;
-  if (req_id < 0 || req_id >= (int)sv.reqs.size() || !sv.reqs[req_id].x_keep) {
+  Server &sv = const_cast<nm_server_t *>(h)->sv;
+  ServeReq *qp = nullptr;
+  {
+    std::lock_guard<std::mutex> g(sv.mu);
+    if (req_id >= 0 && req_id < (int)sv.reqs.size()) qp = sv.reqs[req_id].get();
+  }
+  if (!qp || !qp->x_keep) {
     set_error("nm_server_solution: no kept solution for request %d", req_id);
     return NM_ERR_ARG;
   }
-  const ServeReq &q = sv.reqs[req_id];
+  const ServeReq &q = *qp;
   NM_HIP_TRY(hipMemcpyAsync(x, q.x_keep, sizeof(double) * sv.meshes[q.mesh].op->op.n,
                             hipMemcpyDeviceToDevice, s));
   return NM_OK;

@@ -868,6 +868,22 @@ class Server:
         check(lib().nm_server_run(self.h, C.byref(wall)), "nm_server_run")
         return wall.value
 
+    def serve(self) -> float:
+        """Live stream (nm_server_serve): blocks, serving what other threads
+        submit (ctypes releases the GIL for the call), until close() has
+        been called and the submitted requests are done; returns the wall
+        time."""
+        torch.cuda.synchronize()
+        wall = C.c_double()
+        check(lib().nm_server_serve(self.h, C.byref(wall)), "nm_server_serve")
+        return wall.value
+
+    def close(self):
+        check(lib().nm_server_close(self.h), "nm_server_close")
+
+    def clock(self) -> float:
+        return lib().nm_server_clock(self.h)
+
     def result(self, rid: int):
         from ._lib import ServeResult
         r = ServeResult()

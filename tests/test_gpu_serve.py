@@ -113,3 +113,45 @@ def test_serve_reference_meshes():
         x_alone, mon = _alone(sv, mid, seed, 1e-8, "mixed")
         assert mon.iter == r.iter
         assert torch.equal(sv.solution(rid), x_alone)
+
+
+def test_live_stream_submissions_from_another_thread():
+    """nm_server_serve with requests submitted by another thread while it
+    runs (arrival stamped at submission): every request completes, its
+    times are ordered on the stream clock, and each served solve equals
+    the same solve run alone bit for bit."""
+    import threading
+    import time
+    from neko_amd.solve import Server
+    print("serve live", flush=True)
+    sv = Server(8e9, max_inflight=3, precision="mixed", chunk_iters=8, keep_solutions=True)
+    ids = [sv.add_box_mesh(*m, lx=8, amp=0.1, seed=30 + k, prewarm=1,
+                           tune_reps=2 if k == 0 else 0)
+           for k, m in enumerate(MESHES)]
+    submitted = []
+
+    def producer():
+        try:
+            for k in range(8):
+                mid = ids[k % len(ids)]
+                rid = sv.submit(mid, 300 + k, rel_tol=1e-8, t_arrive=-1.0)
+                submitted.append((rid, mid, 300 + k))
+                time.sleep(0.005)
+        finally:
+            sv.close()
+
+    th = threading.Thread(target=producer)
+    th.start()
+    wall = sv.serve()
+    th.join()
+    assert wall > 0 and len(submitted) == 8
+    last_arrival = -1.0
+    for rid, mid, seed in submitted:
+        r = sv.result(rid)
+        assert r.state == 2 and r.converged
+        assert 0.0 <= r.t_arrive <= r.t_admit <= r.t_done <= wall + 1e-6
+        assert r.t_arrive >= last_arrival  # submission order = arrival order
+        last_arrival = r.t_arrive
+        x_alone, mon = _alone(sv, mid, seed, 1e-8, "mixed")
+        assert r.iter == mon.iter
+        assert torch.equal(sv.solution(rid), x_alone)
