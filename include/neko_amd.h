@@ -336,6 +336,27 @@ int nm_surface_info(const nm_surface_t *sf, int64_t *npoints);
  * normals, npoints each */
 int nm_surface_normals(const nm_surface_t *sf, int32_t *pts, double *nx,
                        double *ny, double *nz);
+/* Cyclic (rotationally periodic about z) boundaries: coef_cyclic
+ * (src/sem/coef.f90:1380-1430) over the cyclic facets (element 0-based,
+ * facet 1..6, 2 x nfacets int32, msh%periodic's order): per facet point
+ * R11, R12 = the unit normal's x-y direction signed by y n_x - x n_y. */
+typedef struct nm_cyclic_s nm_cyclic_t;
+int nm_cyclic_create(nm_cyclic_t **cy, nm_op_t *op, const int32_t *facets,
+                     int64_t nfacets, hipStream_t s);
+void nm_cyclic_free(nm_cyclic_t *cy);
+/* entry count; with msk != NULL also Neko's arrays (cyc_msk(1..ncyc),
+ * 1-based points, R11, R12) in entry order */
+int nm_cyclic_info(const nm_cyclic_t *cy, int64_t *ncyc, int32_t *msk,
+                   double *r11, double *r12);
+/* rotate_cyc (src/math/operators.f90:1078-1140, opr_cpu_rotate_cyc_r1,
+ * src/math/bcknd/cpu/opr_cpu.f90:333-357): idir 1 (vx, vy) -> (normal,
+ * tangential) at the cyclic points, 0 back; vz untouched */
+int nm_rotate_cyc(const nm_cyclic_t *cy, double *vx, double *vy, double *vz,
+                  int idir, hipStream_t s);
+/* drop-in for Neko's HIP rotate_cyc (opr_device.F90:159-168,
+ * opr_rotate_cyc.hip): cyc_msk[0] = ncyc + 1, then 1-based points */
+void hip_rotate_cyc(void *vx, void *vy, void *vz, void *x, void *y, void *z,
+                    void *cyc_msk, void *R11, void *R12, int *ncyc, int *idir);
 /* the whole pressure residual pnpn_prs_res_cpu_compute (pnpn_res_cpu.f90:
  * 31-180): nm_prs_res, then p_res -= (bd/dt) (u . n A) on the points of
  * prs (bc_prs_surface: the velocity-prescribed facets, u the current

@@ -109,6 +109,11 @@ SIGNATURES = {
     "nm_op_create": (_i, [C.POINTER(_vp), _i, _i] + [_vp] * 9 +
                      [_d, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i,
                       _vp, _vp]),
+    "nm_cyclic_create": (_i, [C.POINTER(_vp), _vp, _vp, _i64, _vp]),
+    "nm_cyclic_free": (None, [_vp]),
+    "nm_cyclic_info": (_i, [_vp, _pi64, _vp, _vp, _vp]),
+    "nm_rotate_cyc": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
+    "hip_rotate_cyc": (None, [_vp] * 9 + [C.POINTER(_i), C.POINTER(_i)]),
     "nm_op_free": (None, [_vp]),
     "nm_op_info": (_i, [_vp, C.POINTER(OpInfo)]),
     "nm_op_field": (_vp, [_vp, _i]),

@@ -12,10 +12,11 @@
 // res_c = -res_c - opgrad_c(p) + f_c, where res_c holds A u_c on entry.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <unordered_map>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 
 #include "neko_amd.h"
@@ -551,21 +552,26 @@ int surf_sub(double *p_res, const nm_surface_t *sf, const double *a0, const doub
 }  // namespace
 }  // namespace nm
 
-extern "C" {
+namespace nm {
+namespace {
 
-// Area-weighted normals of the given facets' points as
+// One point of a facet: local index, unit normal, area weight and x, y.
+struct FacetPoint {
+  int64_t loc;
+  double nx, ny, nz, area, x, y;
+};
+
+// The points of the given facets with the normal and area of each as
 // coef_generate_area_and_normal (src/sem/coef.f90:1195-1370) forms them:
 // tangent derivatives (D along the face's two directions), their cross
 // product with sign -,+ / +,- / -,+ on facets 1,2 / 3,4 / 5,6, area =
-// |cross| w_a w_b, unit normal above machine epsilon; summed per unique
-// local point over the facets in list order (facet_normal_finalize).
-int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int64_t nfacets,
-                      hipStream_t s) {
-  using namespace nm;
-  *out = nullptr;
-  Op *op = &h->op;
+// |cross| w_a w_b, unit normal above machine epsilon.  Facets in list
+// order, a facet's points with the first index fastest (the order of
+// Neko's i, j, k loops over the facet).
+int facet_points(Op *op, const int32_t *facets, int64_t nfacets, const char *who,
+                 std::vector<FacetPoint> &out, hipStream_t s) {
   if (!op->xyz[0]) {
-    set_error("nm_surface_create: the operator carries no coordinates (box or mesh operators do)");
+    set_error("%s: the operator carries no coordinates (box or mesh operators do)", who);
     return NM_ERR_STATE;
   }
   const int lx = op->lx;
@@ -575,7 +581,7 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
                             hipMemcpyDeviceToHost, s));
   NM_HIP_TRY(hipStreamSynchronize(s));
   if (nm_gll(lx, z.data(), wq.data(), dd.data(), ddt.data())) {
-    set_error("nm_surface_create: no GLL rule for lx = %d", lx);
+    set_error("%s: no GLL rule for lx = %d", who, lx);
     return NM_ERR_ARG;
   }
   auto Dm = [&](int i, int m) { return D[i + lx * m]; };
@@ -587,7 +593,7 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
     const int64_t e = facets[2 * q];
     const int f = facets[2 * q + 1];
     if (e < 0 || e >= op->nel || f < 1 || f > 6) {
-      set_error("nm_surface_create: facet %lld is (%lld, %d); elements 0..%lld, facets 1..6",
+      set_error("%s: facet %lld is (%lld, %d); elements 0..%lld, facets 1..6", who,
                 (long long)q, (long long)e, f, (long long)op->nel - 1);
       return NM_ERR_ARG;
     }
@@ -601,7 +607,7 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
     if (hipMalloc(&d_el, sizeof(int32_t) * ne) != hipSuccess ||
         hipMalloc(&d_x, sizeof(double) * ne * 3 * n3) != hipSuccess) {
       if (d_el) (void)hipFree(d_el);
-      set_error("nm_surface_create: out of device memory");
+      set_error("%s: out of device memory", who);
       return NM_ERR_NOMEM;
     }
     hipError_t e1 = hipMemcpyAsync(d_el, elist.data(), sizeof(int32_t) * ne,
@@ -619,11 +625,9 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
     (void)hipFree(d_x);
     NM_HIP_TRY(e1);
   }
-  // owned until the end: every early return below frees it and its buffers
-  std::unique_ptr<nm_surface_t> sf(new nm_surface_t);
-  sf->op = op;
-  std::unordered_map<int64_t, int64_t> slot;
   const double eps = 2.220446049250313e-16;
+  out.clear();
+  out.reserve((size_t)nfacets * lx * lx);
   for (int64_t q = 0; q < nfacets; q++) {
     const int64_t e = facets[2 * q];
     const int f = facets[2 * q + 1];
@@ -652,29 +656,60 @@ int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int
         const double A = t1[1] * t2[2] - t1[2] * t2[1];
         const double B = t1[2] * t2[0] - t1[0] * t2[2];
         const double C = t1[0] * t2[1] - t1[1] * t2[0];
-        const double area = std::sqrt(A * A + B * B + C * C) * (wq[a] * wq[b]);
-        double nx = sgn * A, ny = sgn * B, nz = sgn * C;
-        const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+        FacetPoint fp;
+        fp.area = std::sqrt(A * A + B * B + C * C) * (wq[a] * wq[b]);
+        fp.nx = sgn * A;
+        fp.ny = sgn * B;
+        fp.nz = sgn * C;
+        const double len = std::sqrt(fp.nx * fp.nx + fp.ny * fp.ny + fp.nz * fp.nz);
         if (len > eps) {
-          nx = nx / len;
-          ny = ny / len;
-          nz = nz / len;
+          fp.nx = fp.nx / len;
+          fp.ny = fp.ny / len;
+          fp.nz = fp.nz / len;
         }
-        const int64_t loc = e * n3 + i + lx * (j + (int64_t)lx * k);
-        auto it = slot.find(loc);
-        int64_t sl;
-        if (it == slot.end()) {
-          sl = (int64_t)sf->h_pts.size();
-          slot.emplace(loc, sl);
-          sf->h_pts.push_back((int32_t)loc);
-          for (int c = 0; c < 3; c++) sf->h_nrm[c].push_back(0.0);
-        } else {
-          sl = it->second;
-        }
-        sf->h_nrm[0][sl] = sf->h_nrm[0][sl] + nx * area;
-        sf->h_nrm[1][sl] = sf->h_nrm[1][sl] + ny * area;
-        sf->h_nrm[2][sl] = sf->h_nrm[2][sl] + nz * area;
+        fp.loc = e * n3 + i + lx * (j + (int64_t)lx * k);
+        fp.x = at(0, i, j, k);
+        fp.y = at(1, i, j, k);
+        out.push_back(fp);
       }
+  }
+  return NM_OK;
+}
+
+}  // namespace
+}  // namespace nm
+
+extern "C" {
+
+// Area-weighted normals of the given facets' points (facet_points), summed
+// per unique local point over the facets in list order
+// (facet_normal_finalize).
+int nm_surface_create(nm_surface_t **out, nm_op_t *h, const int32_t *facets, int64_t nfacets,
+                      hipStream_t s) {
+  using namespace nm;
+  *out = nullptr;
+  Op *op = &h->op;
+  std::vector<FacetPoint> fps;
+  int st = facet_points(op, facets, nfacets, "nm_surface_create", fps, s);
+  if (st) return st;
+  // owned until the end: every early return below frees it and its buffers
+  std::unique_ptr<nm_surface_t> sf(new nm_surface_t);
+  sf->op = op;
+  std::unordered_map<int64_t, int64_t> slot;
+  for (const FacetPoint &fp : fps) {
+    auto it = slot.find(fp.loc);
+    int64_t sl;
+    if (it == slot.end()) {
+      sl = (int64_t)sf->h_pts.size();
+      slot.emplace(fp.loc, sl);
+      sf->h_pts.push_back((int32_t)fp.loc);
+      for (int c = 0; c < 3; c++) sf->h_nrm[c].push_back(0.0);
+    } else {
+      sl = it->second;
+    }
+    sf->h_nrm[0][sl] = sf->h_nrm[0][sl] + fp.nx * fp.area;
+    sf->h_nrm[1][sl] = sf->h_nrm[1][sl] + fp.ny * fp.area;
+    sf->h_nrm[2][sl] = sf->h_nrm[2][sl] + fp.nz * fp.area;
   }
   sf->np = (int64_t)sf->h_pts.size();
   const size_t np1 = (size_t)std::max<int64_t>(sf->np, 1);
@@ -764,3 +799,199 @@ static int prs_res_impl(nm_op_t *h, double mu, double rho, const double *p, cons
   if (sym && (st = surf_sub(p_res, sym, ta[0], ta[1], ta[2], 1.0, false, s))) return st;
   return NM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Cyclic (rotationally periodic about z) velocity components: rotate_cyc
+// (src/math/operators.f90:1078-1140, opr_cpu_rotate_cyc_r1,
+// src/math/bcknd/cpu/opr_cpu.f90:333-357) with the point list and rotations
+// of coef_cyclic (src/sem/coef.f90:1380-1430).
+// ---------------------------------------------------------------------------
+namespace nm {
+namespace {
+
+// idir 1: (vx, vy) -> (normal, tangential) components; 0: back.  Each
+// thread takes one local point and applies its entries in list order (a
+// point on two cyclic facets of one element is listed twice and rotated
+// twice, as the reference's sequential loop does).
+__global__ void rotate_cyc_chain_kernel(double *__restrict__ vx, double *__restrict__ vy,
+                                        const int32_t *__restrict__ upts,
+                                        const int32_t *__restrict__ uptr,
+                                        const double *__restrict__ r11,
+                                        const double *__restrict__ r12, int64_t nu, int idir) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nu) return;
+  const int64_t j = upts[u];
+  double x = vx[j], y = vy[j];
+  for (int32_t q = uptr[u]; q < uptr[u + 1]; q++) {
+    const double a = r11[q], b = r12[q];
+    double vn, vt;
+    if (idir == 1) {
+      vn = x * a + y * b;
+      vt = -x * b + y * a;
+    } else {
+      vn = x * a - y * b;
+      vt = x * b + y * a;
+    }
+    x = vn;
+    y = vt;
+  }
+  vx[j] = x;
+  vy[j] = y;
+}
+
+// Neko's device layout (hip_rotate_cyc): cyc_msk[0] = ncyc + 1, then the
+// 1-based points; one thread per entry
+__global__ void rotate_cyc_entry_kernel(double *__restrict__ vx, double *__restrict__ vy,
+                                        const int *__restrict__ cyc_msk,
+                                        const double *__restrict__ r11,
+                                        const double *__restrict__ r12, int ncyc, int idir) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncyc; i += stride) {
+    const int64_t j = cyc_msk[i + 1] - 1;
+    const double x = vx[j], y = vy[j], a = r11[i], b = r12[i];
+    if (idir == 1) {
+      vx[j] = x * a + y * b;
+      vy[j] = -x * b + y * a;
+    } else {
+      vx[j] = x * a - y * b;
+      vy[j] = x * b + y * a;
+    }
+  }
+}
+
+}  // namespace
+}  // namespace nm
+
+struct nm_cyclic_s {
+  int64_t ncyc = 0, nu = 0;
+  std::vector<int32_t> h_msk;           // 0-based points, entry order
+  std::vector<double> h_r11, h_r12;
+  int32_t *upts = nullptr, *uptr = nullptr;
+  double *r11 = nullptr, *r12 = nullptr;  // entries grouped by point
+  ~nm_cyclic_s() {
+    for (void *q : {(void *)upts, (void *)uptr, (void *)r11, (void *)r12})
+      if (q) (void)hipFree(q);
+  }
+};
+
+extern "C" {
+
+// coef_cyclic: for each cyclic facet (element 0-based, facet 1..6, in list
+// order, as msh%periodic lists them) and each of its points, the unit
+// normal's x-y direction n / |n| signed by d = y n_x - x n_y gives R11,
+// R12; a normal without an x-y part is an error (rotations about z only).
+int nm_cyclic_create(nm_cyclic_t **out, nm_op_t *h, const int32_t *facets, int64_t nfacets,
+                     hipStream_t s) {
+  using namespace nm;
+  *out = nullptr;
+  std::vector<FacetPoint> fps;
+  int st = facet_points(&h->op, facets, nfacets, "nm_cyclic_create", fps, s);
+  if (st) return st;
+  std::unique_ptr<nm_cyclic_t> cy(new nm_cyclic_t);
+  const double eps = 2.220446049250313e-16;
+  for (const FacetPoint &fp : fps) {
+    const double len = std::sqrt(fp.nx * fp.nx + fp.ny * fp.ny);
+    if (!(len > eps)) {
+      set_error("nm_cyclic_create: a cyclic facet point's normal has no x-y part "
+                "(cyclic rotations are about z)");
+      return NM_ERR_ARG;
+    }
+    const double d = fp.y * fp.nx - fp.x * fp.ny;
+    const double sg = std::copysign(1.0, d);
+    cy->h_msk.push_back((int32_t)fp.loc);
+    cy->h_r11.push_back(fp.nx / len * sg);
+    cy->h_r12.push_back(fp.ny / len * sg);
+  }
+  cy->ncyc = (int64_t)cy->h_msk.size();
+  // group the entries by point, list order kept within a point
+  std::vector<int32_t> order(cy->ncyc);
+  for (int64_t q = 0; q < cy->ncyc; q++) order[q] = (int32_t)q;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return cy->h_msk[a] < cy->h_msk[b]; });
+  std::vector<int32_t> upts, uptr;
+  std::vector<double> r11(cy->ncyc), r12(cy->ncyc);
+  for (int64_t q = 0; q < cy->ncyc; q++) {
+    const int32_t e = order[q];
+    if (q == 0 || cy->h_msk[e] != cy->h_msk[order[q - 1]]) {
+      upts.push_back(cy->h_msk[e]);
+      uptr.push_back((int32_t)q);
+    }
+    r11[q] = cy->h_r11[e];
+    r12[q] = cy->h_r12[e];
+  }
+  uptr.push_back((int32_t)cy->ncyc);
+  cy->nu = (int64_t)upts.size();
+  const size_t n1 = (size_t)std::max<int64_t>(cy->ncyc, 1), u1 = (size_t)std::max<int64_t>(cy->nu, 1);
+  if (hipMalloc(&cy->upts, sizeof(int32_t) * u1) != hipSuccess ||
+      hipMalloc(&cy->uptr, sizeof(int32_t) * (u1 + 1)) != hipSuccess ||
+      hipMalloc(&cy->r11, sizeof(double) * n1) != hipSuccess ||
+      hipMalloc(&cy->r12, sizeof(double) * n1) != hipSuccess) {
+    set_error("nm_cyclic_create: out of device memory");
+    return NM_ERR_NOMEM;
+  }
+  if (cy->ncyc > 0) {
+    NM_HIP_TRY(hipMemcpyAsync(cy->upts, upts.data(), sizeof(int32_t) * cy->nu,
+                              hipMemcpyHostToDevice, s));
+    NM_HIP_TRY(hipMemcpyAsync(cy->uptr, uptr.data(), sizeof(int32_t) * (cy->nu + 1),
+                              hipMemcpyHostToDevice, s));
+    NM_HIP_TRY(hipMemcpyAsync(cy->r11, r11.data(), sizeof(double) * cy->ncyc,
+                              hipMemcpyHostToDevice, s));
+    NM_HIP_TRY(hipMemcpyAsync(cy->r12, r12.data(), sizeof(double) * cy->ncyc,
+                              hipMemcpyHostToDevice, s));
+  }
+  NM_HIP_TRY(hipStreamSynchronize(s));
+  *out = cy.release();
+  return NM_OK;
+}
+
+void nm_cyclic_free(nm_cyclic_t *cy) { delete cy; }
+
+// host copies: entry count, and (when msk is not NULL) Neko's arrays --
+// cyc_msk(1..ncyc) 1-based, R11, R12 -- in entry order
+int nm_cyclic_info(const nm_cyclic_t *cy, int64_t *ncyc, int32_t *msk, double *r11,
+                   double *r12) {
+  *ncyc = cy->ncyc;
+  if (msk)
+    for (int64_t q = 0; q < cy->ncyc; q++) {
+      msk[q] = cy->h_msk[q] + 1;
+      r11[q] = cy->h_r11[q];
+      r12[q] = cy->h_r12[q];
+    }
+  return NM_OK;
+}
+
+int nm_rotate_cyc(const nm_cyclic_t *cy, double *vx, double *vy, double *vz, int idir,
+                  hipStream_t s) {
+  using namespace nm;
+  (void)vz;
+  if (idir != 0 && idir != 1) {
+    set_error("nm_rotate_cyc: idir %d (1 to normal/tangential, 0 back)", idir);
+    return NM_ERR_ARG;
+  }
+  if (cy->nu == 0) return NM_OK;
+  hipLaunchKernelGGL(rotate_cyc_chain_kernel, dim3((unsigned)((cy->nu + 255) / 256)),
+                     dim3(256), 0, s, vx, vy, cy->upts, cy->uptr, cy->r11, cy->r12, cy->nu,
+                     idir);
+  NM_HIP_TRY(hipGetLastError());
+  return NM_OK;
+}
+
+// Drop-in for Neko's HIP backend (opr_device.F90:159-168): device
+// pointers by value, integers by reference, on glb_cmd_queue.  One thread
+// per entry, as the reference kernel: a point listed twice must not occur.
+void hip_rotate_cyc(void *vx, void *vy, void *vz, void *x, void *y, void *z, void *cyc_msk,
+                    void *R11, void *R12, int *ncyc, int *idir) {
+  using namespace nm;
+  (void)vz;
+  (void)x;
+  (void)y;
+  (void)z;
+  if (*ncyc <= 0) return;
+  const int nb = std::min(65535, (*ncyc + 255) / 256);
+  hipLaunchKernelGGL(rotate_cyc_entry_kernel, dim3((unsigned)nb), dim3(256), 0,
+                     (hipStream_t)glb_cmd_queue, (double *)vx, (double *)vy,
+                     (const int *)cyc_msk, (const double *)R11, (const double *)R12, *ncyc,
+                     *idir);
+}
+
+}  // extern "C"

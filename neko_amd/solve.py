@@ -108,6 +108,45 @@ def box_mask(nelx, nely, nelz, lx, periodic=(0, 0, 0)):
     return msk[: mm.value].copy()
 
 
+class Cyclic:
+    """Cyclic (rotationally periodic about z) facets of a problem
+    (nm_cyclic_create): rotate(vx, vy, vz, idir) is Neko's rotate_cyc."""
+
+    def __init__(self, problem, facets):
+        fc = np.ascontiguousarray(np.asarray(facets, dtype=np.int32).reshape(-1, 2))
+        h = C.c_void_p()
+        check(lib().nm_cyclic_create(C.byref(h), problem.h, fc.ctypes.data, len(fc),
+                                     _stream()), "nm_cyclic_create")
+        self.h = h
+        self.n = problem.n
+
+    def info(self):
+        """(cyc_msk 1-based, R11, R12) in entry order."""
+        nc = C.c_int64()
+        check(lib().nm_cyclic_info(self.h, C.byref(nc), None, None, None), "nm_cyclic_info")
+        m = np.zeros(max(nc.value, 1), dtype=np.int32)
+        a = np.zeros(max(nc.value, 1))
+        b = np.zeros(max(nc.value, 1))
+        check(lib().nm_cyclic_info(self.h, C.byref(nc), m.ctypes.data, a.ctypes.data,
+                                   b.ctypes.data), "nm_cyclic_info")
+        return m[: nc.value], a[: nc.value], b[: nc.value]
+
+    def rotate(self, vx, vy, vz, idir):
+        check(lib().nm_rotate_cyc(self.h, _ptr(vx, self.n), _ptr(vy, self.n),
+                                  _ptr(vz, self.n), int(idir), _stream()), "nm_rotate_cyc")
+
+    def free(self):
+        if self.h:
+            lib().nm_cyclic_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def fdm_set_variant(variant: int) -> None:
     """Kernel family of the Schwarz FDM solves (nm_fdm_set_variant): 0 the
     matrix cores (default), 1 the vector ALUs."""

@@ -482,7 +482,44 @@ class OracleCase:
         pr = self.ax_local_coeffs(p, 1.0 / rho, 0.0)
         return (-pr) + wa[0] + wa[1] + wa[2]
 
-    def facet_normals(self, facets):
+    def cyclic_setup(self, facets):
+        """coef_cyclic (src/sem/coef.f90:1380-1430): for each cyclic facet in
+        list order and its points in Neko's i, j, k loop order, the unit
+        normal n (coef_get_normal) gives R11 = n_x/|n_xy| sign(d), R12 =
+        n_y/|n_xy| sign(d), d = y n_x - x n_y (Fortran SIGN(1, d): the sign
+        bit of d).  Returns (cyc_msk 1-based, R11, R12)."""
+        pts = []
+        self.facet_normals(facets, points=pts)
+        msk, r11, r12 = [], [], []
+        eps = np.finfo(np.float64).eps
+        for loc, nx, ny, _ in pts:
+            ln = np.sqrt(nx * nx + ny * ny)
+            if not ln > eps:
+                raise ValueError("normal without an x-y part")
+            d = self.y[loc] * nx - self.x[loc] * ny
+            sg = np.copysign(1.0, d)
+            msk.append(loc + 1)
+            r11.append(nx / ln * sg)
+            r12.append(ny / ln * sg)
+        return np.array(msk, dtype=np.int64), np.array(r11), np.array(r12)
+
+    @staticmethod
+    def rotate_cyc(vx, vy, idir, setup):
+        """opr_cpu_rotate_cyc_r1 (src/math/bcknd/cpu/opr_cpu.f90:333-357),
+        in place, entries in order."""
+        msk, r11, r12 = setup
+        for i in range(len(msk)):
+            j = msk[i] - 1
+            if idir == 1:
+                vnor = vx[j] * r11[i] + vy[j] * r12[i]
+                vtan = -vx[j] * r12[i] + vy[j] * r11[i]
+            else:
+                vnor = vx[j] * r11[i] - vy[j] * r12[i]
+                vtan = vx[j] * r12[i] + vy[j] * r11[i]
+            vx[j] = vnor
+            vy[j] = vtan
+
+    def facet_normals(self, facets, points=None):
         """facet_normal_t over a list of (element, facet 1..6) pairs
         (src/bc/facet_normal.f90:233-326): the unique local points of the
         facets in first-seen order and their area-weighted normals summed over
@@ -531,6 +568,11 @@ class OracleCase:
             nz = np.where(big, nz / np.where(big, ln, 1.0), nz)
             kk, jj, ii = np.meshgrid(np.arange(lx), np.arange(lx), np.arange(lx), indexing="ij")
             loc = (e * lx ** 3 + ii + lx * (jj + lx * kk))[sel]
+            if points is not None:
+                for q in range(lx * lx):
+                    s_, f_ = divmod(q, lx)
+                    points.append((int(loc[s_, f_]), nx[s_, f_], ny[s_, f_], nz[s_, f_]))
+                continue
             for q in range(lx * lx):
                 s_, f_ = divmod(q, lx)
                 k = int(loc[s_, f_])
