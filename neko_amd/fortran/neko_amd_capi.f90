@@ -91,6 +91,13 @@ module neko_amd_capi
        integer(c_int) :: nel, lx
      end subroutine hip_jacobi_update
 
+     subroutine hip_rotate_cyc(vx, vy, vz, x, y, z, cyc_msk, r11, r12, &
+          ncyc, idir) bind(c, name='hip_rotate_cyc')
+       import c_ptr, c_int
+       type(c_ptr), value :: vx, vy, vz, x, y, z, cyc_msk, r11, r12
+       integer(c_int) :: ncyc, idir
+     end subroutine hip_rotate_cyc
+
      ! ---- handle API
      function nm_last_error() bind(c, name='nm_last_error')
        import c_ptr
