@@ -89,10 +89,11 @@ __global__ void __launch_bounds__(CHUNK)
     if (S.done) continue;  // uniform over the block
     const float alpha = (float)S.alpha;
     const int64_t i = c * CHUNK + threadIdx.x;
-    const float ri = r[i] - alpha * w[i];
-    r[i] = ri;
-    double a = (double)ri * (double)ri * (double)mult[i];
-    double b = a * (double)dinv[i];
+    // every array is touched once per step: streamed past the caches
+    const float ri = __builtin_nontemporal_load(r + i) - alpha * __builtin_nontemporal_load(w + i);
+    __builtin_nontemporal_store(ri, r + i);
+    double a = (double)ri * (double)ri * (double)__builtin_nontemporal_load(mult + i);
+    double b = a * (double)__builtin_nontemporal_load(dinv + i);
     bsum_pair(a, b);
     if (threadIdx.x == 0) {
       part[c] = a;

@@ -775,7 +775,7 @@ __global__ void __launch_bounds__(256)
     if (act) v = u[idx];
   }
   const T o = __shfl_xor(v, 1, 64);
-  if (act) u[idx] = v + o;  // a + b == b + a exactly
+  if (act) __builtin_nontemporal_store(v + o, u + idx);  // a + b == b + a exactly
 }
 
 // Facet pairs of one element face shared by two elements, from a
