@@ -376,10 +376,9 @@ __global__ void __launch_bounds__(NL *NL)
 
 // ext (NL^3, zero) <- interior r (lx^3) plus face layer 0 = interior layer
 // 2 (schwarz_toext3d + extrude(.., 0, 0, .., 2, 1)); one thread per point
-template <typename T>
-__global__ void toext_fused_kernel(T *__restrict__ a, const T *__restrict__ r, int n,
-                                   int64_t nel) {
-  const int m = n + 2;
+template <typename T, int n>
+__global__ void toext_fused_kernel(T *__restrict__ a, const T *__restrict__ r, int64_t nel) {
+  constexpr int m = n + 2;
   const int64_t m3 = (int64_t)m * m * m;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nel * m3) return;
@@ -406,10 +405,10 @@ __global__ void toext_fused_kernel(T *__restrict__ a, const T *__restrict__ r, i
 // every face += (work2 - work1) at layer 0 (extrude(w2, 0, 1, w1, 0, -1)
 // then extrude(w2, 2, 1, w2, 0, 1), r faces then s then t, low before
 // high), then schwarz_toreg3d
-template <typename T>
+template <typename T, int n>
 __global__ void toreg_fused_kernel(T *__restrict__ e, const T *__restrict__ w2,
-                                   const T *__restrict__ w1, int n, int64_t nel) {
-  const int m = n + 2;
+                                   const T *__restrict__ w1, int64_t nel) {
+  constexpr int m = n + 2;
   const int64_t n3 = (int64_t)n * n * n, m3 = (int64_t)m * m * m;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nel * n3) return;
@@ -810,13 +809,37 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
   // the reference's sequence mask, toext3d, extrude x4, fdm, toreg3d runs
   // here as three fused passes around the two extended-layout gs
   if ((st = launch_mask<T>(r, op.msk, op.nmsk, nullptr, s))) return st;
-  hipLaunchKernelGGL(toext_fused_kernel<T>, dim3(blocks_for(ns)), dim3(256), 0, s, w1, r, lx,
-                     nel);
+  // the extents are template constants, so the point index splits by
+  // multiplications instead of 64-bit divisions
+  switch (lx) {
+#define NM_TOEXT(N)                                                                      \
+  case N:                                                                                \
+    hipLaunchKernelGGL((toext_fused_kernel<T, N>), dim3(blocks_for(ns)), dim3(256), 0, s, \
+                       w1, r, nel);                                                      \
+    break;
+    NM_TOEXT(3) NM_TOEXT(4) NM_TOEXT(5) NM_TOEXT(6) NM_TOEXT(7) NM_TOEXT(8) NM_TOEXT(9)
+    NM_TOEXT(10)
+#undef NM_TOEXT
+    default:
+      set_error("schwarz: lx %d unsupported (3..10)", lx);
+      return NM_ERR_UNSUPPORTED;
+  }
   if ((st = gs_dev<T>(sw.ext, w1, s))) return st;
   if ((st = fdm<T>(w2, w1, S, D, nx, nel, s, true))) return st;
   if ((st = gs_dev<T>(sw.ext, w2, s))) return st;
-  hipLaunchKernelGGL(toreg_fused_kernel<T>, dim3(blocks_for(op.n)), dim3(256), 0, s, e, w2, w1,
-                     lx, nel);
+  switch (lx) {
+#define NM_TOREG(N)                                                                      \
+  case N:                                                                                \
+    hipLaunchKernelGGL((toreg_fused_kernel<T, N>), dim3(blocks_for(op.n)), dim3(256), 0, s, \
+                       e, w2, w1, nel);                                                  \
+    break;
+    NM_TOREG(3) NM_TOREG(4) NM_TOREG(5) NM_TOREG(6) NM_TOREG(7) NM_TOREG(8) NM_TOREG(9)
+    NM_TOREG(10)
+#undef NM_TOREG
+    default:
+      set_error("schwarz: lx %d unsupported (3..10)", lx);
+      return NM_ERR_UNSUPPORTED;
+  }
   if ((st = op.gs<T>(e, nullptr, s))) return st;
   if ((st = launch_mask<T>(e, op.msk, op.nmsk, nullptr, s))) return st;
   hipLaunchKernelGGL(wtp_mul_kernel<T>, dim3((unsigned)std::min<int64_t>(blocks_for(op.n), 16384)),
