@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <vector>
 
 #include "neko_amd.h"
@@ -331,11 +332,10 @@ int tamg_create(Tamg **out, const Op *op, const std::vector<int32_t> &fneigh, in
               (long long)nel);
     return NM_ERR_ARG;
   }
-  Tamg *h = new Tamg;
-  auto fail = [&](int c) {
-    delete h;
-    return c;
-  };
+  // owned until the end: every early return (NM_HIP_TRY included) frees it
+  std::unique_ptr<Tamg> hold(new Tamg);
+  Tamg *h = hold.get();
+  auto fail = [](int c) { return c; };
   h->op = op;
   h->n0 = n0;
   h->max_iter = max_iter;
@@ -414,7 +414,7 @@ int tamg_create(Tamg **out, const Op *op, const std::vector<int32_t> &fneigh, in
   }
   for (int l = 0; l < levels; l++)
     if ((st = power(h, l, power_its, s))) return fail(st);
-  *out = h;
+  *out = hold.release();
   return NM_OK;
 }
 
