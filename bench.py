@@ -647,16 +647,29 @@ def measured_traffic(kernel):
     return None
 
 
+def ax_kernel_name(code, lx):
+    """Name of the FP64 Ax kernel a variant code launches (no fused dot or
+    CG prologue)."""
+    if code >= 300 and lx in (4, 8):
+        return f"ax_tri_mfma_kernel<{lx}>"
+    if code >= 200:
+        return f"ax_tri_kernel<double,{lx},{code % 100 or 1}>"
+    return f"ax_kstep_kernel<double,{lx},{code % 100 or 1}>"
+
+
 def roofline_entry(code, lx, gbs, tfs, nbytes, ms, n):
     """Roofline object of the dominant kernel (the FP64 Ax launch).  The
     stored-geometry variants stream 64 B/DOF and are HBM-bound.  The
-    in-kernel-geometry variant reads 16.6 B/DOF and executes ~180 FP64
-    flop/DOF (a count modelled from the kernel source, ax_flops_per_dof) on
-    the vector ALUs -- no matrix instructions -- so it is graded against the
-    FP64 vector peak and labelled "fp64-valu"; both fractions are kept."""
+    in-kernel-geometry variants read 16.6 B/DOF and execute ~180 FP64
+    flop/DOF (a count modelled from the kernel source, ax_flops_per_dof), so
+    they are graded against the FP64 peak, 78.6 TFLOP/s for the vector ALUs
+    and for the matrix cores alike on MI355X: "fp64-valu" for the vector
+    kernel, "fp64" for the kernel that runs the contractions (12 lx
+    flop/DOF) on v_mfma_f64_4x4x4f64 and the geometry on the vector ALUs;
+    both fractions are kept."""
     tri = code >= 200
-    name = (f"ax_tri_kernel<double,{lx},{code % 100 or 1}>" if tri else
-            f"ax_kstep_kernel<double,{lx},{code % 100 or 1}>")
+    mfma = code >= 300 and lx in (4, 8)
+    name = ax_kernel_name(code, lx)
     hbm_frac = gbs / HBM_PEAK_GBS
     fp_frac = tfs / FP64_PEAK_TFS
     common = {"kernel": name, "traffic": measured_traffic(name),
@@ -665,9 +678,11 @@ def roofline_entry(code, lx, gbs, tfs, nbytes, ms, n):
               "ms_per_launch": round(ms, 4),
               "hbm_GBs": round(gbs, 1), "hbm_frac": round(hbm_frac, 4),
               "fp64_TFs": round(tfs, 2), "fp64_frac": round(fp_frac, 4)}
+    if mfma:
+        common["matrix_flops_per_dof"] = 12 * lx
     if tri:
-        return {"bound": "fp64-valu", "achieved": round(tfs, 2), "peak": FP64_PEAK_TFS,
-                "unit": "TFLOP/s", "frac": round(fp_frac, 4), **common}
+        return {"bound": "fp64" if mfma else "fp64-valu", "achieved": round(tfs, 2),
+                "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": round(fp_frac, 4), **common}
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(hbm_frac, 4), **common}
 

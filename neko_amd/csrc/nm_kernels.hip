@@ -456,6 +456,253 @@ __global__ void __launch_bounds__(LX *LX *EPB)
                                     DOT ? dot_part + blockIdx.x : nullptr, cp);
 }
 
+// ---------------------------------------------------------------------------
+// Ax with in-kernel trilinear geometry, contractions on the matrix cores
+// (FP64, lx a multiple of 4).
+//
+// v_mfma_f64_4x4x4f64 computes, in each of four 4x4 blocks, C += A B with
+// K = 4, where lane l holds C(row l/16, column l%4) of block (l%16)/4,
+// B(k = l/16, column l%4) and A(row l%4, k = l/16).  A line of lx points
+// (the values a derivative contracts over) therefore fits in the four lanes
+// n, n+16, n+32, n+48 (n = l % 16), lane n + 16h holding the line's entries
+// h, h+4, h+8, ..., and the product D x line lands in the same lanes with
+// the same entries: D (or D^T) is the A operand, shared by the blocks, and
+// sixteen lines go through one instruction.  Lines are indexed c = 16g + n,
+// g < lx^2/16.  The t-lines (c = i + lx j) are how the element is loaded,
+// kept and stored; the r-lines (c = j + lx k) and s-lines (c = i + lx k) go
+// through LDS both ways.  Per element a derivative takes lx^4/256
+// instructions (16 at lx = 8) and no vector FMA; the geometric factors of
+// ax_tri_body (cofactors of the trilinear map, jac, h1 w3 / jac) are formed
+// per point on the vector ALUs in the t-line layout.  One wave per element.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double mfma_f64_4x4x4(double a, double b, double c) {
+  return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+// Exchange point between the lanes of a one-wave workgroup: a wave's LDS
+// instructions execute in issue order, so ordering the accesses in the
+// compiler (wavefront-scope fences) is enough; no wait for the LDS queue
+// to drain, no s_barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int LX>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
+    ax_tri_mfma_kernel(double *__restrict__ w, const double *__restrict__ u,
+                       const double *__restrict__ dx, const double *__restrict__ zg,
+                       const double *__restrict__ wg, const double *__restrict__ edges,
+                       double h1c, double h2c, int64_t nel, const int *__restrict__ skip) {
+  static_assert(LX % 4 == 0 && LX <= 8, "lines of 4-entry chunks, registers for lx <= 8");
+  constexpr int P = LX * LX, N3 = LX * P;
+  constexpr int NL = LX / 4;   // entries of a line per lane
+  constexpr int NG = P / 16;   // line groups (16 lines each)
+  // element arrays in LDS with point (i,j,k) at (P + LX) k + LX j + (i ^ j):
+  // the padded plane stride moves odd planes to the other half of the
+  // banks and the xor spreads a row, so the 16 lanes that one LDS cycle
+  // serves touch 16 distinct bank pairs along t-lines, r-lines and s-lines
+  constexpr int NS = (P + LX) * LX;
+  __shared__ double sa[NS];    // the element's u, then s-parts
+  __shared__ double sb[NS];    // r-parts
+  __shared__ double tr[LX][LX][3], ts[LX][LX][3];
+  __shared__ double sedge[36], swg[LX];
+  if (skip != nullptr && *skip) return;
+  const int64_t e = blockIdx.x;
+  if (e >= nel) return;  // whole wave: no exchange is skipped by part of it
+  const int64_t base = e * (int64_t)N3;
+  const int l = threadIdx.x;
+  const int n = l & 15, h = l >> 4;
+  // LDS positions of the lane's entries: t-lines (c = i + LX j, entry k),
+  // r-lines (c = j + LX k, entry i), s-lines (c = i + LX k, entry j); each
+  // position is owned by one lane in each layout
+  auto pos = [&](int i, int j, int k) { return (P + LX) * k + LX * j + (i ^ j); };
+  auto pt = [&](int g, int s) {
+    const int c = 16 * g + n;
+    return pos(c % LX, c / LX, h + 4 * s);
+  };
+  auto pr = [&](int g, int s) {
+    const int c = 16 * g + n;
+    return pos(h + 4 * s, c % LX, c / LX);
+  };
+  auto ps = [&](int g, int s) {
+    const int c = 16 * g + n;
+    return pos(c % LX, h + 4 * s, c / LX);
+  };
+  // offset in the element (global memory) of the lane's t-line entries
+  auto gt = [&](int g, int s) { return 16 * g + n + P * (h + 4 * s); };
+
+  if (l < LX) swg[l] = wg[l];
+  // A operands: D(4 mb + m, 4 ks + kk) and D^T, m = l % 4, kk = l / 16
+  double Ad[NL][NL], At[NL][NL];
+#pragma unroll
+  for (int mb = 0; mb < NL; mb++)
+#pragma unroll
+    for (int ks = 0; ks < NL; ks++) {
+      Ad[mb][ks] = dx[(4 * mb + (l & 3)) + LX * (4 * ks + h)];
+      At[mb][ks] = dx[(4 * ks + h) + LX * (4 * mb + (l & 3))];
+    }
+  double uc[NG][NL];
+#pragma unroll
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+      uc[g][s] = __builtin_nontemporal_load(u + base + gt(g, s));
+      sa[pt(g, s)] = uc[g][s];
+    }
+  if (l < 36) sedge[l] = edges[e * 36 + l];
+  wave_lds_sync();
+
+  // geometry tables: tr[k][j] = dX/dr(s_j, t_k), ts[k][i] = dX/ds(r_i, t_k)
+  // (bilinear interpolants of the half edge vectors: r-edges [0,12),
+  // s-edges [12,24), t-edges [24,36), component c at c*4 + (low/high of the
+  // first other coordinate) + 2*(of the second)); dX/dt(r_i, s_j) per lane
+  for (int q = l; q < P; q += 64) {
+    const int a = q % LX, b = q / LX;
+    const double pa1 = (1.0 + zg[a]) * 0.5, pa0 = 1.0 - pa1;
+    const double pb1 = (1.0 + zg[b]) * 0.5, pb0 = 1.0 - pb1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      tr[b][a][c] = (sedge[c * 4 + 0] * pa0 + sedge[c * 4 + 1] * pa1) * pb0 +
+                    (sedge[c * 4 + 2] * pa0 + sedge[c * 4 + 3] * pa1) * pb1;
+      ts[b][a][c] = (sedge[12 + c * 4 + 0] * pa0 + sedge[12 + c * 4 + 1] * pa1) * pb0 +
+                    (sedge[12 + c * 4 + 2] * pa0 + sedge[12 + c * 4 + 3] * pa1) * pb1;
+    }
+  }
+  // the lane's t-line columns share i = n % LX (16 g is a multiple of LX):
+  // the t-edges interpolated in r once, then in s per column
+  double tlo[3], thi[3];
+  {
+    const int i = n % LX;
+    const double r1 = (1.0 + zg[i]) * 0.5, r0 = 1.0 - r1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      tlo[c] = sedge[24 + c * 4 + 0] * r0 + sedge[24 + c * 4 + 1] * r1;
+      thi[c] = sedge[24 + c * 4 + 2] * r0 + sedge[24 + c * 4 + 3] * r1;
+    }
+  }
+
+  // ur on the r-lines straight to sb; us on the s-lines, stored over u
+  // once every lane has read its lines
+  double usl[NG][NL];
+#pragma unroll
+  for (int g = 0; g < NG; g++) {
+    double lr[NL], ls[NL];
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+      lr[s] = sa[pr(g, s)];
+      ls[s] = sa[ps(g, s)];
+    }
+#pragma unroll
+    for (int mb = 0; mb < NL; mb++) {
+      double cr = 0.0, cs = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < NL; ks++) {
+        cr = mfma_f64_4x4x4(Ad[mb][ks], lr[ks], cr);
+        cs = mfma_f64_4x4x4(Ad[mb][ks], ls[ks], cs);
+      }
+      sb[pr(g, mb)] = cr;
+      usl[g][mb] = cs;
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int s = 0; s < NL; s++) sa[ps(g, s)] = usl[g][s];
+  wave_lds_sync();
+
+  // per t-line group: ut on the matrix cores, the geometry per point, the
+  // r- and s-parts over ur/us (the lane's own positions), the t-part
+  // contracted at once (its accumulator starts at the mass term)
+  double wt[NG][NL];
+  const bool mass = h2c != 0.0;
+#pragma unroll
+  for (int g = 0; g < NG; g++) {
+    double vt[NL], rt[NL], m0[NL], cv[3];
+    {
+      const int j = (16 * g + n) / LX;
+      const double s1 = (1.0 + zg[j]) * 0.5, s0 = 1.0 - s1;
+#pragma unroll
+      for (int c = 0; c < 3; c++) cv[c] = tlo[c] * s0 + thi[c] * s1;
+    }
+#pragma unroll
+    for (int mb = 0; mb < NL; mb++) {
+      double ct = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < NL; ks++) ct = mfma_f64_4x4x4(Ad[mb][ks], uc[g][ks], ct);
+      vt[mb] = ct;
+    }
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+      const int c = 16 * g + n, i = c % LX, j = c / LX, k = h + 4 * s;
+      const int p = pt(g, s);
+      const double vr = sb[p], vs = sa[p];
+      const double a0 = tr[k][j][0], a1 = tr[k][j][1], a2 = tr[k][j][2];
+      const double b0 = ts[k][i][0], b1 = ts[k][i][1], b2 = ts[k][i][2];
+      const double c0 = cv[0], c1 = cv[1], c2 = cv[2];
+      const double rx = b1 * c2 - c1 * b2;
+      const double ry = c0 * b2 - b0 * c2;
+      const double rz = b0 * c1 - c0 * b1;
+      const double sx = c1 * a2 - a1 * c2;
+      const double sy = a0 * c2 - c0 * a2;
+      const double sz = c0 * a1 - a0 * c1;
+      const double tx = a1 * b2 - b1 * a2;
+      const double ty = b0 * a2 - a0 * b2;
+      const double tz = a0 * b1 - b0 * a1;
+      const double jac = c0 * tx + c1 * ty + c2 * tz;
+      const double w3 = swg[i] * swg[j] * swg[k];
+      const double sc = h1c * w3 * recip(jac);
+      const double gx = sc * (vr * rx + vs * sx + vt[s] * tx);
+      const double gy = sc * (vr * ry + vs * sy + vt[s] * ty);
+      const double gz = sc * (vr * rz + vs * sz + vt[s] * tz);
+      sb[p] = rx * gx + ry * gy + rz * gz;
+      sa[p] = sx * gx + sy * gy + sz * gz;
+      rt[s] = tx * gx + ty * gy + tz * gz;
+      m0[s] = mass ? h2c * w3 * jac * uc[g][s] : 0.0;
+    }
+#pragma unroll
+    for (int mb = 0; mb < NL; mb++) {
+      double acc = m0[mb];
+#pragma unroll
+      for (int ks = 0; ks < NL; ks++) acc = mfma_f64_4x4x4(At[mb][ks], rt[ks], acc);
+      wt[g][mb] = acc;
+    }
+  }
+  wave_lds_sync();
+  // D^T contractions of the r-parts (r-lines) and s-parts (s-lines), the
+  // results over their inputs (the lane's own positions)
+#pragma unroll
+  for (int g = 0; g < NG; g++) {
+    double lr[NL], ls[NL];
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+      lr[s] = sb[pr(g, s)];
+      ls[s] = sa[ps(g, s)];
+    }
+#pragma unroll
+    for (int mb = 0; mb < NL; mb++) {
+      double cr = 0.0, cs = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < NL; ks++) {
+        cr = mfma_f64_4x4x4(At[mb][ks], lr[ks], cr);
+        cs = mfma_f64_4x4x4(At[mb][ks], ls[ks], cs);
+      }
+      sb[pr(g, mb)] = cr;
+      sa[ps(g, mb)] = cs;
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int s = 0; s < NL; s++) {
+      const int p = pt(g, s);
+      __builtin_nontemporal_store(wt[g][s] + (sb[p] + sa[p]), w + base + gt(g, s));
+    }
+}
+
 // Batched: workgroup b of a launch is global workgroup blk_base + b, which
 // belongs to instance blk_inst[.] (instances never share a workgroup); the
 // instance's flag `done` (CgScal-compatible layout: int at done_off bytes)
@@ -646,8 +893,34 @@ static int ax_epb(const AxArgs<T> &a, int code, hipStream_t s) {
   return NM_ERR_UNSUPPORTED;
 }
 
+// Variant 300 + epb: the matrix-core kernel where it applies (FP64, lx 4 or
+// 8, no fused dot or CG prologue), else the vector kernel 200 + epb.
+static bool ax_mfma_applies(int lx, bool fp64, bool fused) {
+  return fp64 && (lx == 4 || lx == 8) && !fused;
+}
+
 template <typename T>
 int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s) {
+  if (epb >= 300) {
+    const bool fused = a.dot_part != nullptr || a.cg_r != nullptr;
+    if (!ax_mfma_applies(a.lx, std::is_same<T, double>::value, fused)) {
+      epb -= 100;
+    } else if constexpr (std::is_same<T, double>::value) {
+      if (a.nel <= 0) return NM_OK;
+      if (!a.edges || !a.zg || !a.wg) {
+        set_error("ax: the matrix-core variant needs trilinear geometry");
+        return NM_ERR_ARG;
+      }
+      if (a.lx == 8)
+        hipLaunchKernelGGL(ax_tri_mfma_kernel<8>, dim3((unsigned)a.nel), dim3(64), 0, s, a.w,
+                           a.u, a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
+      else
+        hipLaunchKernelGGL(ax_tri_mfma_kernel<4>, dim3((unsigned)a.nel), dim3(64), 0, s, a.w,
+                           a.u, a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
+      NM_HIP_TRY(hipGetLastError());
+      return NM_OK;
+    }
+  }
   switch (a.lx) {
     case 2: return ax_epb<T, 2>(a, epb, s);
     case 3: return ax_epb<T, 3>(a, epb, s);

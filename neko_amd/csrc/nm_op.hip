@@ -1089,7 +1089,10 @@ int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
   NM_HIP_TRY(hipEventCreate(&e1));
   launch_vec<double>(V_CFILL, u, nullptr, nullptr, nullptr, 1.0, n, s);
   std::vector<int> codes = {0, 2, 4, 100, 102, 104};
-  if (op.trilinear && !op.f64.h1) codes.insert(codes.end(), {201, 202, 204});
+  if (op.trilinear && !op.f64.h1) {
+    codes.insert(codes.end(), {201, 202, 204});
+    if (op.lx == 4 || op.lx == 8) codes.push_back(300);  // FP64 on the matrix cores
+  }
   auto timeit = [&](auto fn) -> float {
     for (int r = 0; r < 2; r++) fn();
     hipEventRecord(e0, s);

@@ -135,10 +135,11 @@ def test_64cube_gs_bit_exact(c2):
     assert np.array_equal(_np(ud).view(np.int64), ref.view(np.int64))
 
 
-@pytest.mark.parametrize("code", [1, 201])
+@pytest.mark.parametrize("code", [1, 201, 300])
 def test_64cube_ax_every_element(c2, code):
     """code 1: geometric factors streamed from HBM; 201: geometry evaluated
-    in the kernel from the element edges."""
+    in the kernel from the element edges; 300: the same with the
+    contractions on the matrix cores."""
     p, c, _ = c2
     p.set_epb(code, code)
     u = np.random.default_rng(42).standard_normal(c.n)

@@ -261,6 +261,7 @@ TRI_CASES = [
     (3, 3, 2, 8, 0.1, (0, 0, 0), 0.0),
     (3, 2, 3, 8, 0.1, (1, 0, 0), 1500.0),
     (3, 3, 3, 2, 0.1, (0, 0, 0), 0.0),
+    (3, 2, 3, 4, 0.1, (0, 0, 1), 5.0),
     (2, 3, 2, 3, 0.1, (0, 0, 0), 10.0),
     (2, 2, 3, 5, 0.08, (0, 0, 0), 0.0),
     (2, 3, 2, 6, 0.1, (0, 1, 0), 0.0),
@@ -272,11 +273,13 @@ TRI_CASES = [
 
 
 @pytest.mark.parametrize("case", TRI_CASES)
-@pytest.mark.parametrize("code", [201, 202, 204])
+@pytest.mark.parametrize("code", [201, 202, 204, 300])
 def test_ax_trilinear_matches_oracle(case, code):
     """G and B formed in the kernel from the element's edge vectors agree
     with the oracle's coef-based operator (spectral derivatives of the
-    trilinear GLL coordinates) within 1e-12; FP32 within 2e-5."""
+    trilinear GLL coordinates) within 1e-12; FP32 within 2e-5.  Code 300
+    runs the FP64 contractions on the matrix cores at lx 4 and 8 (the
+    vector kernel elsewhere and in FP32)."""
     from neko_amd import solve as S
     nx, ny, nz, lx, amp, per, h2 = case
     if lx * lx * (code % 100) > 256:
