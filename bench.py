@@ -651,7 +651,7 @@ def ax_kernel_name(code, lx):
     """Name of the FP64 Ax kernel a variant code launches (no fused dot or
     CG prologue)."""
     if code >= 300 and lx in (4, 8):
-        return f"ax_tri_mfma_kernel<{lx}>"
+        return f"ax_tri_mfma_kernel<{lx}, false>"  # the Poisson operator: no mass term
     if code >= 200:
         return f"ax_tri_kernel<double,{lx},{code % 100 or 1}>"
     return f"ax_kstep_kernel<double,{lx},{code % 100 or 1}>"

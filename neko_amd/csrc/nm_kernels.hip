@@ -489,7 +489,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int LX>
+template <int LX, bool MASS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
     ax_tri_mfma_kernel(double *__restrict__ w, const double *__restrict__ u,
                        const double *__restrict__ dx, const double *__restrict__ zg,
@@ -617,15 +617,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
   // r- and s-parts over ur/us (the lane's own positions), the t-part
   // contracted at once (its accumulator starts at the mass term)
   double wt[NG][NL];
-  const bool mass = h2c != 0.0;
 #pragma unroll
   for (int g = 0; g < NG; g++) {
-    double vt[NL], rt[NL], m0[NL], cv[3];
+    double vt[NL], rt[NL], m0[NL], cv[3], h1ij, h2ij;
     {
-      const int j = (16 * g + n) / LX;
+      const int c = 16 * g + n, i = c % LX, j = c / LX;
       const double s1 = (1.0 + zg[j]) * 0.5, s0 = 1.0 - s1;
 #pragma unroll
-      for (int c = 0; c < 3; c++) cv[c] = tlo[c] * s0 + thi[c] * s1;
+      for (int q = 0; q < 3; q++) cv[q] = tlo[q] * s0 + thi[q] * s1;
+      const double wij = swg[i] * swg[j];
+      h1ij = h1c * wij;
+      h2ij = h2c * wij;
     }
 #pragma unroll
     for (int mb = 0; mb < NL; mb++) {
@@ -652,15 +654,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3)))
       const double ty = b0 * a2 - a0 * b2;
       const double tz = a0 * b1 - b0 * a1;
       const double jac = c0 * tx + c1 * ty + c2 * tz;
-      const double w3 = swg[i] * swg[j] * swg[k];
-      const double sc = h1c * w3 * recip(jac);
+      const double wk = swg[k];
+      const double sc = h1ij * wk * recip(jac);
       const double gx = sc * (vr * rx + vs * sx + vt[s] * tx);
       const double gy = sc * (vr * ry + vs * sy + vt[s] * ty);
       const double gz = sc * (vr * rz + vs * sz + vt[s] * tz);
       sb[p] = rx * gx + ry * gy + rz * gz;
       sa[p] = sx * gx + sy * gy + sz * gz;
       rt[s] = tx * gx + ty * gy + tz * gz;
-      m0[s] = mass ? h2c * w3 * jac * uc[g][s] : 0.0;
+      m0[s] = MASS ? h2ij * wk * jac * uc[g][s] : 0.0;
     }
 #pragma unroll
     for (int mb = 0; mb < NL; mb++) {
@@ -911,12 +913,20 @@ int launch_ax(const AxArgs<T> &a, int epb, hipStream_t s) {
         set_error("ax: the matrix-core variant needs trilinear geometry");
         return NM_ERR_ARG;
       }
-      if (a.lx == 8)
-        hipLaunchKernelGGL(ax_tri_mfma_kernel<8>, dim3((unsigned)a.nel), dim3(64), 0, s, a.w,
-                           a.u, a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
+      const dim3 grid((unsigned)a.nel), block(64);
+      const bool mass = a.h2c != 0.0;
+      if (a.lx == 8 && mass)
+        hipLaunchKernelGGL((ax_tri_mfma_kernel<8, true>), grid, block, 0, s, a.w, a.u, a.dx,
+                           a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
+      else if (a.lx == 8)
+        hipLaunchKernelGGL((ax_tri_mfma_kernel<8, false>), grid, block, 0, s, a.w, a.u, a.dx,
+                           a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
+      else if (mass)
+        hipLaunchKernelGGL((ax_tri_mfma_kernel<4, true>), grid, block, 0, s, a.w, a.u, a.dx,
+                           a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
       else
-        hipLaunchKernelGGL(ax_tri_mfma_kernel<4>, dim3((unsigned)a.nel), dim3(64), 0, s, a.w,
-                           a.u, a.dx, a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
+        hipLaunchKernelGGL((ax_tri_mfma_kernel<4, false>), grid, block, 0, s, a.w, a.u, a.dx,
+                           a.zg, a.wg, a.edges, a.h1c, a.h2c, a.nel, a.skip);
       NM_HIP_TRY(hipGetLastError());
       return NM_OK;
     }
