@@ -639,7 +639,7 @@ def measured_traffic(kernel):
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
-                val, name = line.split(":", 1)[1].split()
+                val, name = line.split(":", 1)[1].strip().split(None, 1)
                 if name == f"kernel={kernel}":
                     return int(val)
     except OSError:

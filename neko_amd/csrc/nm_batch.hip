@@ -4,7 +4,8 @@
 // scheme of nm_cg.hip).
 //
 // Layout: every problem's vectors are one slice of shared batch vectors,
-// padded to whole 256-dof chunks (padding stays zero: dinv = mult = 0), so
+// padded to whole 256-dof chunks (padding stays zero: dinv = mult = 0 and
+// multiplicity count 0), so
 // a chunk belongs to exactly one problem.  Per iteration:
 //   p = dinv r + beta_i p            one launch over all chunks
 //   w = A_i p (FP32, trilinear)      one launch per lx, workgroups tagged
@@ -13,8 +14,9 @@
 //                                    gs + one mask launch for all problems
 //   alpha_i, x32 += a p, r -= a w, rtr_i, rtz_i, beta_i, stop test
 // Problems that have stopped skip their work chunk by chunk.  Reductions
-// are per problem in a fixed order (chunk partials, then one workgroup per
-// problem), so results do not depend on the other problems of the batch.
+// are per problem in a fixed order (partials of runs of up to 16 chunks
+// starting at the problem's first chunk, then one workgroup per problem),
+// so results do not depend on the other problems of the batch.
 // Neko solves one system at a time (krylov.f90 ksp_t%solve); this has no
 // counterpart there.
 #include <hip/hip_runtime.h>
@@ -77,42 +79,86 @@ __global__ void __launch_bounds__(CHUNK)
   }
 }
 
-// fused schedule (p and x updates in the Ax prologue): r -= alpha w;
-// chunk partials of rtr and rtz
+// fused schedule (p and x updates in the Ax prologue): r -= alpha w and
+// the rtr/rtz partials.  Workgroup u takes the run of up to RUN chunks of
+// one problem that starts at chunk ub_c0[u] (a problem's runs start at its
+// first chunk, so its partials do not depend on the rest of the batch);
+// four points per thread (16-byte accesses), one reduction per run.  With
+// CNT the multiplicity weight is looked up from the one-byte count (mult
+// is exactly 1/count: the same bits as mult itself).
+constexpr int RUN = 16;
+typedef float bnt_f4 __attribute__((ext_vector_type(4)));
+template <bool CNT>
 __global__ void __launch_bounds__(CHUNK)
-    bupd2_kernel(float *__restrict__ r, const float *__restrict__ w,
-                 const float *__restrict__ mult, const float *__restrict__ dinv,
-                 const int32_t *__restrict__ chunk_inst, int64_t nchunk,
-                 const CgScal *__restrict__ sc, double *__restrict__ part) {
-  for (int64_t c = blockIdx.x; c < nchunk; c += gridDim.x) {
-    const CgScal &S = sc[chunk_inst[c]];
-    if (S.done) continue;  // uniform over the block
-    const float alpha = (float)S.alpha;
-    const int64_t i = c * CHUNK + threadIdx.x;
-    // every array is touched once per step: streamed past the caches
-    const float ri = __builtin_nontemporal_load(r + i) - alpha * __builtin_nontemporal_load(w + i);
-    __builtin_nontemporal_store(ri, r + i);
-    double a = (double)ri * (double)ri * (double)__builtin_nontemporal_load(mult + i);
-    double b = a * (double)__builtin_nontemporal_load(dinv + i);
-    bsum_pair(a, b);
-    if (threadIdx.x == 0) {
-      part[c] = a;
-      part[nchunk + c] = b;
+    bupd3_kernel(float *__restrict__ r, const float *__restrict__ w,
+                 const float *__restrict__ mult, const uint8_t *__restrict__ mcnt,
+                 const float *__restrict__ dinv, const int32_t *__restrict__ ub_c0,
+                 const int32_t *__restrict__ ub_inst, int64_t nub,
+                 const BatchInst *__restrict__ inst, const CgScal *__restrict__ sc,
+                 double *__restrict__ part) {
+  const int64_t u = blockIdx.x;
+  const int ii = ub_inst[u];
+  const CgScal &S = sc[ii];
+  if (S.done) return;  // uniform over the workgroup
+  __shared__ float tab[CNT ? 256 : 1];
+  if (CNT) {
+    tab[threadIdx.x] = threadIdx.x ? (float)(1.0 / (double)threadIdx.x) : 0.0f;
+    __syncthreads();
+  }
+  const float alpha = (float)S.alpha;
+  const int64_t cb = ub_c0[u];
+  const int64_t ce = cb + RUN < inst[ii].c1 ? cb + RUN : inst[ii].c1;
+  double a = 0.0, b = 0.0;
+  // every array is touched once per step: streamed past the caches
+  for (int64_t i = cb * CHUNK + 4 * threadIdx.x; i < ce * CHUNK; i += 4 * CHUNK) {
+    const float4 wv = *reinterpret_cast<const float4 *>(w + i);
+    const bnt_f4 rv = __builtin_nontemporal_load(reinterpret_cast<const bnt_f4 *>(r + i));
+    const bnt_f4 dv = __builtin_nontemporal_load(reinterpret_cast<const bnt_f4 *>(dinv + i));
+    float m[4];
+    if (CNT) {
+      const uchar4 cv = *reinterpret_cast<const uchar4 *>(mcnt + i);
+      m[0] = tab[cv.x];
+      m[1] = tab[cv.y];
+      m[2] = tab[cv.z];
+      m[3] = tab[cv.w];
+    } else {
+      const float4 mv = *reinterpret_cast<const float4 *>(mult + i);
+      m[0] = mv.x;
+      m[1] = mv.y;
+      m[2] = mv.z;
+      m[3] = mv.w;
     }
+    const float wq[4] = {wv.x, wv.y, wv.z, wv.w};
+    bnt_f4 ro;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const float ri = rv[c] - alpha * wq[c];
+      ro[c] = ri;
+      const double rm = (double)ri * (double)ri * (double)m[c];
+      a += rm;
+      b += rm * (double)dv[c];
+    }
+    __builtin_nontemporal_store(ro, reinterpret_cast<bnt_f4 *>(r + i));
+  }
+  bsum_pair(a, b);
+  if (threadIdx.x == 0) {
+    part[u] = a;
+    part[nub + u] = b;
   }
 }
 
-// per problem: rtr, rtz, beta, norm, stop test (cg_fin semantics)
+// per problem: rtr, rtz, beta, norm, stop test (cg_fin semantics) over the
+// problem's run partials
 __global__ void __launch_bounds__(CHUNK)
-    bfin_kernel(const double *__restrict__ part, int64_t nchunk,
+    bfin_kernel(const double *__restrict__ part, int64_t nub,
                 const BatchInst *__restrict__ inst, CgScal *__restrict__ sc) {
   CgScal &S = sc[blockIdx.x];
   if (S.done) return;
   const BatchInst I = inst[blockIdx.x];
   double a = 0.0, b = 0.0;
-  for (int64_t c = I.c0 + threadIdx.x; c < I.c1; c += CHUNK) {
+  for (int64_t c = I.u0 + threadIdx.x; c < I.u1; c += CHUNK) {
     a += part[c];
-    b += part[nchunk + c];
+    b += part[nub + c];
   }
   bsum_pair(a, b);
   if (threadIdx.x == 0) {
@@ -150,7 +196,12 @@ struct Batch {
   hipStream_t s = nullptr;
   // device
   BatchInst *inst = nullptr;
-  int32_t *blk_inst = nullptr, *chunk_inst = nullptr;
+  int32_t *blk_inst = nullptr;
+  // update runs: first chunk and problem of every update workgroup
+  int32_t *ub_c0 = nullptr, *ub_inst = nullptr;
+  int64_t nub = 0;
+  uint8_t *mcnt = nullptr;  // multiplicity counts (every problem's mcnt_ok)
+  bool cnt = false;
   int32_t *gs_ptr = nullptr, *gs_idx = nullptr, *msk = nullptr;
   int64_t gs_nblks = 0, gs_pair0 = 0, gs_npairs = 0, nmsk = 0;
   int gs_maxblk = 0;
@@ -197,7 +248,6 @@ struct Batch {
 // batched Ax kernel (x is one step behind; a replacement flushes it), the
 // update kernel does r -= alpha w and the rtr/rtz partials.
 static int batch_iteration(Batch *B, hipStream_t s) {
-  const unsigned vb = (unsigned)std::min<int64_t>(B->nchunk, 65536);
   for (const LxGroup &g : B->groups) {
     AxBatchArgs a;
     a.w = B->w32;
@@ -243,10 +293,15 @@ static int batch_iteration(Batch *B, hipStream_t s) {
   if (st) return st;
   const unsigned ni = (unsigned)B->ops.size();
   hipLaunchKernelGGL(balpha_kernel, dim3(ni), dim3(CHUNK), 0, s, B->dot_part, B->inst, B->sc);
-  hipLaunchKernelGGL(bupd2_kernel, dim3(vb), dim3(CHUNK), 0, s, B->r32, B->w32, B->mult,
-                     B->dinv, B->chunk_inst, B->nchunk, B->sc, B->part);
-  hipLaunchKernelGGL(bfin_kernel, dim3(ni), dim3(CHUNK), 0, s, B->part, B->nchunk, B->inst,
-                     B->sc);
+  if (B->cnt)
+    hipLaunchKernelGGL(bupd3_kernel<true>, dim3((unsigned)B->nub), dim3(CHUNK), 0, s, B->r32,
+                       B->w32, B->mult, B->mcnt, B->dinv, B->ub_c0, B->ub_inst, B->nub, B->inst,
+                       B->sc, B->part);
+  else
+    hipLaunchKernelGGL(bupd3_kernel<false>, dim3((unsigned)B->nub), dim3(CHUNK), 0, s, B->r32,
+                       B->w32, B->mult, B->mcnt, B->dinv, B->ub_c0, B->ub_inst, B->nub, B->inst,
+                       B->sc, B->part);
+  hipLaunchKernelGGL(bfin_kernel, dim3(ni), dim3(CHUNK), 0, s, B->part, B->nub, B->inst, B->sc);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -383,7 +438,8 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       !(B->dot_part = B->alloc<double>(nblk_max)) ||
       !(B->rpart = B->alloc<double>(2 * RED_BLOCKS)) || !(B->sc = B->alloc<CgScal>(ninst)) ||
       !(B->inst = B->alloc<BatchInst>(ninst)) || !(B->blk_inst = B->alloc<int32_t>(nblk_max)) ||
-      !(B->chunk_inst = B->alloc<int32_t>(B->nchunk)) ||
+      !(B->ub_c0 = B->alloc<int32_t>(B->nchunk)) || !(B->ub_inst = B->alloc<int32_t>(B->nchunk)) ||
+      !(B->mcnt = B->alloc<uint8_t>(n)) ||
       !(B->gs_ptr = B->alloc<int32_t>(gs_blocks + 32 * (int64_t)ninst + 1)) ||
       !(B->gs_idx = B->alloc<int32_t>(gs_entries + 256 * (int64_t)ninst)) ||
       !(B->msk = B->alloc<int32_t>(ms + 256 * (int64_t)ninst))) {
@@ -402,7 +458,10 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       return fail(NM_ERR_HIP);
   // instance table, chunk table, merged gs and mask (offsets added)
   B->h_inst.resize(ninst);
-  std::vector<int32_t> cinst(B->nchunk), gptr, gidx, gpairs, msk, wgb, wgp, wgm;
+  std::vector<int32_t> ubc0, ubinst, gptr, gidx, gpairs, msk, wgb, wgp, wgm;
+  B->cnt = true;
+  for (int i = 0; i < ninst; i++) B->cnt = B->cnt && B->ops[i]->mcnt_ok;
+  if (B->cnt && hipMemsetAsync(B->mcnt, 0, (size_t)n, B->s) != hipSuccess) return fail(NM_ERR_HIP);
   gptr.reserve(gs_blocks + 1);
   gidx.reserve(gs_entries);
   gpairs.reserve(2 * gs_pairs);
@@ -419,7 +478,15 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
     I.h2c = (float)(op->ifh2 ? op->h2c : 0.0);
     I.c0 = B->voff[i] / CHUNK;
     I.c1 = (B->voff[i] + B->vlen[i]) / CHUNK;
-    for (int64_t c = I.c0; c < I.c1; c++) cinst[c] = i;
+    I.u0 = (int64_t)ubc0.size();
+    for (int64_t c = I.c0; c < I.c1; c += RUN) {
+      ubc0.push_back((int32_t)c);
+      ubinst.push_back(i);
+    }
+    I.u1 = (int64_t)ubc0.size();
+    if (B->cnt && hipMemcpyAsync(B->mcnt + I.voff, op->mcnt, (size_t)op->n,
+                                 hipMemcpyDeviceToDevice, B->s) != hipSuccess)
+      return fail(NM_ERR_HIP);
     std::vector<int32_t> ptr(op->nblks + 1), gi(op->nlocal), mk(op->nmsk);
     if (hipMemcpy(ptr.data(), op->blk_ptr, sizeof(int32_t) * (op->nblks + 1),
                   hipMemcpyDeviceToHost) != hipSuccess ||
@@ -481,7 +548,10 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
                        hipMemcpyHostToDevice, B->s) != hipSuccess)
       return fail(NM_ERR_HIP);
   }
-  if (hipMemcpyAsync(B->chunk_inst, cinst.data(), sizeof(int32_t) * cinst.size(),
+  B->nub = (int64_t)ubc0.size();
+  if (hipMemcpyAsync(B->ub_c0, ubc0.data(), sizeof(int32_t) * ubc0.size(),
+                     hipMemcpyHostToDevice, B->s) != hipSuccess ||
+      hipMemcpyAsync(B->ub_inst, ubinst.data(), sizeof(int32_t) * ubinst.size(),
                      hipMemcpyHostToDevice, B->s) != hipSuccess ||
       hipMemcpyAsync(B->gs_ptr, gptr.data(), sizeof(int32_t) * gptr.size(),
                      hipMemcpyHostToDevice, B->s) != hipSuccess ||
