@@ -147,6 +147,126 @@ __global__ void __launch_bounds__(CHUNK)
   }
 }
 
+// The same update with the face sums of gs (the batch's gs then runs the
+// blocks only): a face-interior point of an element face that a descriptor
+// covers adds its partner's value, w_a + w_b, the bits the facet-pair pass
+// would have stored (nm_cg.hip, cg_update2f_kernel).  Launched per lx
+// group; workgroup u takes a run of up to RK elements of one problem
+// (runs start at the problem's first element); fnb holds every problem's
+// {neighbour element, code} per element face from the problem's e0 on.
+template <int LX>
+__global__ void __launch_bounds__(CHUNK)
+    bupd_face_kernel(float *__restrict__ r, const float *__restrict__ w,
+                     const uint8_t *__restrict__ mcnt, const float *__restrict__ dinv,
+                     const int2 *__restrict__ fnb, const int32_t *__restrict__ run_inst,
+                     const int32_t *__restrict__ run_k0, int64_t r_base, int64_t nrun,
+                     const BatchInst *__restrict__ inst, const CgScal *__restrict__ sc,
+                     double *__restrict__ part) {
+  constexpr int N3 = LX * LX * LX;
+  // four neighbouring points per thread (16-byte accesses) where lx^3 is a
+  // multiple of 4 (element starts stay 4-aligned), else one
+  constexpr int V = N3 % 4 == 0 ? 4 : 1;
+  constexpr int NH = N3 / V;                                     // vectors per element
+  constexpr int EPI = NH >= CHUNK ? 1 : CHUNK / NH;              // elements per step
+  constexpr int NQ = NH >= CHUNK ? (NH + CHUNK - 1) / CHUNK : 1;  // vectors per thread
+  constexpr int RK = N3 >= 4096 ? 1 : 4096 / N3;
+  const int64_t u = r_base + blockIdx.x;
+  const int ii = run_inst[u];
+  const CgScal &S = sc[ii];
+  if (S.done) return;  // uniform over the workgroup
+  __shared__ float tab[256];
+  tab[threadIdx.x] = threadIdx.x ? (float)(1.0 / (double)threadIdx.x) : 0.0f;
+  __syncthreads();
+  const BatchInst I = inst[ii];
+  const float alpha = (float)S.alpha;
+  const int eo = EPI > 1 ? (int)threadIdx.x / NH : 0;
+  const int h0 = EPI > 1 ? (int)threadIdx.x % NH : (int)threadIdx.x;
+  int fq[NQ][V], uq[NQ][V], vq[NQ][V];
+#pragma unroll
+  for (int t = 0; t < NQ; t++)
+#pragma unroll
+    for (int c = 0; c < V; c++) {
+      const int q = V * (h0 + t * CHUNK) + c;
+      const int i2 = q % LX, jj = (q / LX) % LX, kk = q / (LX * LX);
+      const bool bi = i2 == 0 || i2 == LX - 1, bj = jj == 0 || jj == LX - 1,
+                 bk = kk == 0 || kk == LX - 1;
+      fq[t][c] = (q < N3 && (int)bi + (int)bj + (int)bk == 1)
+                     ? (bi ? (i2 != 0) : (bj ? 2 + (jj != 0) : 4 + (kk != 0)))
+                     : -1;
+      uq[t][c] = bi ? jj : i2;
+      vq[t][c] = bk ? jj : kk;
+    }
+  const int64_t k0 = run_k0[u];
+  const int64_t k1 = k0 + RK < I.nel ? k0 + RK : I.nel;
+  double a = 0.0, b = 0.0;
+  if (eo < EPI) {
+    for (int64_t k = k0 + eo; k < k1; k += EPI) {
+#pragma unroll
+      for (int t = 0; t < NQ; t++) {
+        const int h = h0 + t * CHUNK;
+        if (h >= NH) break;
+        const int64_t i = I.voff + k * N3 + (int64_t)V * h;
+        float wv[V], rv[V], dv[V], mv[V];
+        if constexpr (V == 4) {
+          const float4 w4 = *reinterpret_cast<const float4 *>(w + i);
+          const bnt_f4 r4 = __builtin_nontemporal_load(reinterpret_cast<const bnt_f4 *>(r + i));
+          const bnt_f4 d4 = __builtin_nontemporal_load(reinterpret_cast<const bnt_f4 *>(dinv + i));
+          const uchar4 c4 = *reinterpret_cast<const uchar4 *>(mcnt + i);
+          wv[0] = w4.x; wv[1] = w4.y; wv[2] = w4.z; wv[3] = w4.w;
+          mv[0] = tab[c4.x]; mv[1] = tab[c4.y]; mv[2] = tab[c4.z]; mv[3] = tab[c4.w];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            rv[c] = r4[c];
+            dv[c] = d4[c];
+          }
+        } else {
+          wv[0] = w[i];
+          rv[0] = __builtin_nontemporal_load(r + i);
+          dv[0] = __builtin_nontemporal_load(dinv + i);
+          mv[0] = tab[__builtin_nontemporal_load(mcnt + i)];
+        }
+        float ro[V];
+#pragma unroll
+        for (int c = 0; c < V; c++) {
+          float wi = wv[c];
+          if (fq[t][c] >= 0) {
+            const int2 d = fnb[(I.e0 + k) * 6 + fq[t][c]];
+            if (d.x >= 0) {
+              int ub = (d.y & 64) ? vq[t][c] : uq[t][c], vb = (d.y & 64) ? uq[t][c] : vq[t][c];
+              if (d.y & 128) ub = LX - 1 - ub;
+              if (d.y & 256) vb = LX - 1 - vb;
+              wi = wi + w[I.voff + (int64_t)d.x * N3 + face_point((d.y >> 3) & 7, ub, vb, LX)];
+            }
+          }
+          ro[c] = rv[c] - alpha * wi;
+          const double rm = (double)ro[c] * (double)ro[c] * (double)mv[c];
+          a += rm;
+          b += rm * (double)dv[c];
+        }
+        if constexpr (V == 4) {
+          bnt_f4 o;
+#pragma unroll
+          for (int c = 0; c < 4; c++) o[c] = ro[c];
+          __builtin_nontemporal_store(o, reinterpret_cast<bnt_f4 *>(r + i));
+        } else {
+          __builtin_nontemporal_store(ro[0], r + i);
+        }
+      }
+    }
+  }
+  bsum_pair(a, b);
+  if (threadIdx.x == 0) {
+    part[u] = a;
+    part[nrun + u] = b;
+  }
+}
+
+// elements per update run of an lx (bupd_face_kernel's RK)
+static inline int64_t face_run(int lx) {
+  const int64_t n3 = (int64_t)lx * lx * lx;
+  return n3 >= 4096 ? 1 : 4096 / n3;
+}
+
 // per problem: rtr, rtz, beta, norm, stop test (cg_fin semantics) over the
 // problem's run partials
 __global__ void __launch_bounds__(CHUNK)
@@ -183,6 +303,7 @@ __global__ void d2f_slice_kernel(float *__restrict__ dst, const double *__restri
 struct LxGroup {
   int lx = 0, epb = 1;
   int64_t blk_base = 0, nblocks = 0;
+  int64_t r0 = 0, r1 = 0;  // its update runs (face-sum schedule)
   std::vector<int> members;
   const float *dx = nullptr, *zg = nullptr, *wg = nullptr;
 };
@@ -202,6 +323,13 @@ struct Batch {
   int64_t nub = 0;
   uint8_t *mcnt = nullptr;  // multiplicity counts (every problem's mcnt_ok)
   bool cnt = false;
+  // face-sum schedule (every problem can defer its face sums past its mask
+  // and has no facet pair outside a whole-face descriptor): per-element face
+  // table and update runs of elements
+  bool face = false;
+  int2 *fnb = nullptr;
+  int32_t *run_inst = nullptr, *run_k0 = nullptr;
+  int64_t nrun = 0;
   int32_t *gs_ptr = nullptr, *gs_idx = nullptr, *msk = nullptr;
   int64_t gs_nblks = 0, gs_pair0 = 0, gs_npairs = 0, nmsk = 0;
   int gs_maxblk = 0;
@@ -273,8 +401,8 @@ static int batch_iteration(Batch *B, hipStream_t s) {
     int st = launch_ax_batch(a, s);
     if (st) return st;
   }
-  GsArgs<float> ga{B->w32, B->gs_ptr, B->gs_idx, B->gs_nblks, B->gs_pair0, B->gs_npairs,
-                   nullptr};
+  GsArgs<float> ga{B->w32, B->gs_ptr, B->gs_idx, B->gs_nblks, B->gs_pair0,
+                   B->face ? 0 : B->gs_npairs, nullptr};
   ga.variant = B->gs_maxblk <= 8 ? 1 : 0;
   if (ga.variant == 1) {
     ga.wg_blk = B->wg_blk;
@@ -293,6 +421,30 @@ static int batch_iteration(Batch *B, hipStream_t s) {
   if (st) return st;
   const unsigned ni = (unsigned)B->ops.size();
   hipLaunchKernelGGL(balpha_kernel, dim3(ni), dim3(CHUNK), 0, s, B->dot_part, B->inst, B->sc);
+  if (B->face) {
+    for (const LxGroup &g : B->groups) {
+      if (g.r1 <= g.r0) continue;
+      const dim3 grid((unsigned)(g.r1 - g.r0)), block(CHUNK);
+#define NM_BF_CASE(L)                                                                      \
+  case L:                                                                                  \
+    hipLaunchKernelGGL(bupd_face_kernel<L>, grid, block, 0, s, B->r32, B->w32, B->mcnt,      \
+                       B->dinv, B->fnb, B->run_inst, B->run_k0, g.r0, B->nrun, B->inst,      \
+                       B->sc, B->part);                                                    \
+    break;
+      switch (g.lx) {
+        NM_BF_CASE(3) NM_BF_CASE(4) NM_BF_CASE(5) NM_BF_CASE(6) NM_BF_CASE(7) NM_BF_CASE(8)
+        NM_BF_CASE(9) NM_BF_CASE(10) NM_BF_CASE(11) NM_BF_CASE(12)
+        default:
+          set_error("batch: face-sum update unsupported for lx=%d", g.lx);
+          return NM_ERR_UNSUPPORTED;
+      }
+#undef NM_BF_CASE
+    }
+    hipLaunchKernelGGL(bfin_kernel, dim3(ni), dim3(CHUNK), 0, s, B->part, B->nrun, B->inst,
+                       B->sc);
+    NM_HIP_TRY(hipGetLastError());
+    return NM_OK;
+  }
   if (B->cnt)
     hipLaunchKernelGGL(bupd3_kernel<true>, dim3((unsigned)B->nub), dim3(CHUNK), 0, s, B->r32,
                        B->w32, B->mult, B->mcnt, B->dinv, B->ub_c0, B->ub_inst, B->nub, B->inst,
@@ -461,6 +613,12 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
   std::vector<int32_t> ubc0, ubinst, gptr, gidx, gpairs, msk, wgb, wgp, wgm;
   B->cnt = true;
   for (int i = 0; i < ninst; i++) B->cnt = B->cnt && B->ops[i]->mcnt_ok;
+  B->face = B->cnt;
+  for (int i = 0; i < ninst; i++) {
+    const Op *op = B->ops[i];
+    B->face = B->face && op->face_fuse && op->face_fuse_on && op->nrest == 0 &&
+              op->lx >= 3 && op->lx <= 12 && op->h_face_nbr.size() == (size_t)op->nel * 12;
+  }
   if (B->cnt && hipMemsetAsync(B->mcnt, 0, (size_t)n, B->s) != hipSuccess) return fail(NM_ERR_HIP);
   gptr.reserve(gs_blocks + 1);
   gidx.reserve(gs_entries);
@@ -549,6 +707,49 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       return fail(NM_ERR_HIP);
   }
   B->nub = (int64_t)ubc0.size();
+  if (B->face) {
+    // face tables from element e0 of each problem, update runs per lx group
+    // (a problem's runs contiguous: [u0, u1) into the run partials)
+    std::vector<int32_t> fn, rinst, rk0;
+    int64_t e0 = 0;
+    for (int i = 0; i < ninst; i++) {
+      B->h_inst[i].e0 = e0;
+      fn.insert(fn.end(), B->ops[i]->h_face_nbr.begin(), B->ops[i]->h_face_nbr.end());
+      e0 += B->ops[i]->nel;
+    }
+    for (LxGroup &g : B->groups) {
+      g.r0 = (int64_t)rinst.size();
+      const int64_t rk = face_run(g.lx);
+      for (int m : g.members) {
+        BatchInst &I = B->h_inst[m];
+        I.u0 = (int64_t)rinst.size();
+        for (int64_t k = 0; k < I.nel; k += rk) {
+          rinst.push_back(m);
+          rk0.push_back((int32_t)k);
+        }
+        I.u1 = (int64_t)rinst.size();
+      }
+      g.r1 = (int64_t)rinst.size();
+    }
+    B->nrun = (int64_t)rinst.size();
+    if (B->nrun > B->nchunk) {  // a run holds at least one chunk's worth of points
+      set_error("nm_batch_create: %lld update runs exceed the partial buffer",
+                (long long)B->nrun);
+      return fail(NM_ERR_STATE);
+    }
+    if (!(B->fnb = B->alloc<int2>((int64_t)fn.size() / 2)) ||
+        !(B->run_inst = B->alloc<int32_t>(B->nrun)) || !(B->run_k0 = B->alloc<int32_t>(B->nrun))) {
+      set_error("nm_batch_create: out of device memory");
+      return fail(NM_ERR_NOMEM);
+    }
+    if (hipMemcpyAsync(B->fnb, fn.data(), sizeof(int32_t) * fn.size(), hipMemcpyHostToDevice,
+                       B->s) != hipSuccess ||
+        hipMemcpyAsync(B->run_inst, rinst.data(), sizeof(int32_t) * rinst.size(),
+                       hipMemcpyHostToDevice, B->s) != hipSuccess ||
+        hipMemcpyAsync(B->run_k0, rk0.data(), sizeof(int32_t) * rk0.size(),
+                       hipMemcpyHostToDevice, B->s) != hipSuccess)
+      return fail(NM_ERR_HIP);
+  }
   if (hipMemcpyAsync(B->ub_c0, ubc0.data(), sizeof(int32_t) * ubc0.size(),
                      hipMemcpyHostToDevice, B->s) != hipSuccess ||
       hipMemcpyAsync(B->ub_inst, ubinst.data(), sizeof(int32_t) * ubinst.size(),

@@ -79,7 +79,8 @@ struct BatchInst {
   float h1c, h2c;
   int64_t blk0, nblk;    // its Ax workgroups (global numbering)
   int64_t c0, c1;        // its 256-dof chunks [c0, c1)
-  int64_t u0, u1;        // its update workgroups [u0, u1) (runs of chunks)
+  int64_t u0, u1;        // its update workgroups [u0, u1) (runs of chunks or elements)
+  int64_t e0;            // its first element in the batch's face table
 };
 struct AxBatchArgs {
   float *w;

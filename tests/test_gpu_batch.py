@@ -42,10 +42,16 @@ def _build():
     return probs, cases, fs
 
 
-def test_batch_matches_oracle():
+@pytest.mark.parametrize("face_sums", [True, False])
+def test_batch_matches_oracle(face_sums):
+    """face_sums: the update adds the face partners' values (the batch's gs
+    runs the blocks only); switched off on the problems, the batch runs the
+    facet-pair gs and the chunk-run update instead."""
     from neko_amd.solve import Batch
-    print("batch vs oracle", flush=True)
+    print("batch vs oracle", face_sums, flush=True)
     probs, cases, fs = _build()
+    for p in probs:
+        p.set_cg_face_sums(face_sums)
     b = Batch(probs)
     tuned = b.autotune(reps=2)
     assert set(tuned) == {4, 5, 6, 8, 10}
