@@ -50,6 +50,29 @@ __device__ inline void bsum2(double &a, double &b, double *sh) {
   __syncthreads();
 }
 
+// alpha = rtz / p.Ap from the p.Ap partials exactly as cg_alpha_kernel forms
+// it (same partials, same per-thread order, same tree), taken by every
+// workgroup of an update kernel itself instead of a separate launch;
+// workgroup 0 also stores pap and alpha for the kernels that follow (the
+// next Ax's x update, a flush).  Blocks of RED_THREADS threads.
+__device__ inline double upd_alpha(const double *__restrict__ apart, int64_t napart,
+                                   CgScal *__restrict__ sc, double *sh) {
+  __shared__ double s_alpha;
+  double a = 0.0, b = 0.0;
+  for (int64_t q = threadIdx.x; q < napart; q += blockDim.x) a += apart[q];
+  bsum2(a, b, sh);
+  if (threadIdx.x == 0) {
+    const double al = sc->rtz / a;
+    s_alpha = al;
+    if (blockIdx.x == 0) {
+      sc->pap = a;
+      sc->alpha = al;
+    }
+  }
+  __syncthreads();
+  return s_alpha;
+}
+
 // init: x = 0, p = 0, r = scale_in * f; partial rtr = sum r r mult,
 // rtz = sum r r mult dinv.  scale_in comes from the device (refinement) or 1.
 template <typename T, typename TX, typename TF>
@@ -206,11 +229,12 @@ __global__ void cg_fin_kernel(const double *__restrict__ part, int npart,
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS)
     cg_update2_kernel(T *__restrict__ r, const T *__restrict__ w, const T *__restrict__ mult,
-                      const T *__restrict__ dinv, int64_t n, const CgScal *__restrict__ sc,
-                      double *__restrict__ part) {
+                      const T *__restrict__ dinv, int64_t n, CgScal *__restrict__ sc,
+                      double *__restrict__ part, const double *__restrict__ apart,
+                      int64_t napart) {
   if (sc->done) return;
   __shared__ double sh[16];
-  const T alpha_t = (T)sc->alpha;
+  const T alpha_t = (T)upd_alpha(apart, napart, sc, sh);
   double a = 0.0, b = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -243,14 +267,20 @@ __global__ void __launch_bounds__(RED_THREADS)
 // face partners run on the same XCD at the same time, timed slower: the
 // partner lines cost less than the lost spread of the element walk.)
 constexpr int UPF_BLOCKS = 2048;
+// The grid is one round of the chip at 8 waves per SIMD (2048 workgroups of
+// four waves on 256 CUs): the kernels below are held to 64 VGPRs up to lx =
+// 10, since a second, partial round of workgroups (each with an equal share
+// of the elements) would leave most of the chip idle at the end; beyond,
+// the points per thread need more registers than that without spilling.
 // With CNT the multiplicity weight comes from the one-byte count (mult is
 // exactly 1/count): tab[count] = (T)(1.0 / count), the bits of mult itself.
 template <typename T, int LX, bool CNT>
-__global__ void __launch_bounds__(RED_THREADS)
+__global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(LX <= 10 ? 8 : 1)))
     cg_update2f_kernel(T *__restrict__ r, const T *__restrict__ w, const T *__restrict__ mult,
                        const uint8_t *__restrict__ mcnt, const T *__restrict__ dinv,
-                       const int2 *__restrict__ fnb, int64_t nel, const CgScal *__restrict__ sc,
-                       double *__restrict__ part) {
+                       const int2 *__restrict__ fnb, int64_t nel, CgScal *__restrict__ sc,
+                       double *__restrict__ part, const double *__restrict__ apart,
+                       int64_t napart) {
   if (sc->done) return;
   constexpr int N3 = LX * LX * LX;
   constexpr int NQ = (N3 + RED_THREADS - 1) / RED_THREADS;  // points per thread and element
@@ -260,7 +290,7 @@ __global__ void __launch_bounds__(RED_THREADS)
     for (int c = threadIdx.x; c < 256; c += RED_THREADS) tab[c] = c ? (T)(1.0 / (double)c) : T(0);
     __syncthreads();
   }
-  const T alpha_t = (T)sc->alpha;
+  const T alpha_t = (T)upd_alpha(apart, napart, sc, sh);
   // where this thread's points sit in an element (the same for every
   // element): face f of a face-interior point and its (u, v), else f = -1
   int fq[NQ], uq[NQ], vq[NQ];
@@ -319,12 +349,13 @@ template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
 
 template <typename T, int LX, bool CNT>
-__global__ void __launch_bounds__(RED_THREADS)
+__global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(LX <= 10 ? 8 : 1)))
     cg_update2f_pair_kernel(T *__restrict__ r, const T *__restrict__ w,
                             const T *__restrict__ mult, const uint8_t *__restrict__ mcnt,
                             const T *__restrict__ dinv, const int2 *__restrict__ fnb,
-                            int64_t nel, const CgScal *__restrict__ sc,
-                            double *__restrict__ part) {
+                            int64_t nel, CgScal *__restrict__ sc,
+                            double *__restrict__ part, const double *__restrict__ apart,
+                            int64_t napart) {
   static_assert(LX % 2 == 0, "point pairs need an even lx");
   using T2 = typename Vec2<T>::type;
   if (sc->done) return;
@@ -336,7 +367,7 @@ __global__ void __launch_bounds__(RED_THREADS)
     for (int c = threadIdx.x; c < 256; c += RED_THREADS) tab[c] = c ? (T)(1.0 / (double)c) : T(0);
     __syncthreads();
   }
-  const T alpha_t = (T)sc->alpha;
+  const T alpha_t = (T)upd_alpha(apart, napart, sc, sh);
   // per pair slot and point: face of a face-interior point (else -1), (u, v)
   int fq[NQ][2], uq[NQ][2], vq[NQ][2];
 #pragma unroll
@@ -414,12 +445,13 @@ typedef float nt_f4 __attribute__((ext_vector_type(4)));
 // loads and stores); with fewer quads per element than threads, a
 // workgroup takes 256 / (N3 / 4) elements per step.
 template <int LX, bool CNT>
-__global__ void __launch_bounds__(RED_THREADS)
+__global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(LX <= 10 ? 8 : 1)))
     cg_update2f_quad_kernel(float *__restrict__ r, const float *__restrict__ w,
                             const float *__restrict__ mult, const uint8_t *__restrict__ mcnt,
                             const float *__restrict__ dinv, const int2 *__restrict__ fnb,
-                            int64_t nel, const CgScal *__restrict__ sc,
-                            double *__restrict__ part) {
+                            int64_t nel, CgScal *__restrict__ sc,
+                            double *__restrict__ part, const double *__restrict__ apart,
+                            int64_t napart) {
   static_assert(LX % 4 == 0, "point quads need lx divisible by 4");
   constexpr int N3 = LX * LX * LX, NH = N3 / 4;
   constexpr int EPI = NH >= RED_THREADS ? 1 : RED_THREADS / NH;  // elements per step
@@ -431,7 +463,7 @@ __global__ void __launch_bounds__(RED_THREADS)
     for (int c = threadIdx.x; c < 256; c += RED_THREADS) tab[c] = c ? (float)(1.0 / (double)c) : 0.0f;
     __syncthreads();
   }
-  const float alpha_t = (float)sc->alpha;
+  const float alpha_t = (float)upd_alpha(apart, napart, sc, sh);
   const int eo = EPI > 1 ? (int)threadIdx.x / NH : 0;  // element offset in the step
   const int h0 = EPI > 1 ? (int)threadIdx.x % NH : (int)threadIdx.x;
   int fq[NQ][4], uq[NQ][4], vq[NQ][4];
@@ -518,33 +550,38 @@ __global__ void __launch_bounds__(RED_THREADS)
   }
 }
 
+// apart/napart: the p.Ap partials alpha is formed from (never `part`, the
+// update's own partials, which other workgroups write meanwhile)
 template <typename T, int LX, bool CNT>
-static void launch_update2f_lx(const Op &op, T *r, const T *w, const int2 *fnb,
-                               const CgScal *sc, double *part, hipStream_t s) {
+static void launch_update2f_lx(const Op &op, T *r, const T *w, const int2 *fnb, CgScal *sc,
+                               double *part, const double *apart, int64_t napart,
+                               hipStream_t s) {
   const OpFields<T> &F = op.fields<T>();
   if constexpr (LX % 4 == 0 && std::is_same<T, float>::value)
     hipLaunchKernelGGL((cg_update2f_quad_kernel<LX, CNT>), dim3(UPF_BLOCKS), dim3(RED_THREADS),
-                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part);
+                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart,
+                       napart);
   else if constexpr (LX % 2 == 0 && std::is_same<T, float>::value)
     hipLaunchKernelGGL((cg_update2f_pair_kernel<T, LX, CNT>), dim3(UPF_BLOCKS),
                        dim3(RED_THREADS), 0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc,
-                       part);
+                       part, apart, napart);
   else
     hipLaunchKernelGGL((cg_update2f_kernel<T, LX, CNT>), dim3(UPF_BLOCKS), dim3(RED_THREADS),
-                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part);
+                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart,
+                       napart);
 }
 
 template <typename T>
-static int launch_update2f(const Op &op, T *r, const T *w, const CgScal *sc, double *part,
-                           hipStream_t s) {
+static int launch_update2f(const Op &op, T *r, const T *w, CgScal *sc, double *part,
+                           const double *apart, int64_t napart, hipStream_t s) {
   const OpFields<T> &F = op.fields<T>();
   const int2 *fnb = (const int2 *)op.face_nbr;
-#define NM_U2F(L)                                                     \
-  case L:                                                             \
-    if (op.mcnt_ok)                                                   \
-      launch_update2f_lx<T, L, true>(op, r, w, fnb, sc, part, s);     \
-    else                                                              \
-      launch_update2f_lx<T, L, false>(op, r, w, fnb, sc, part, s);    \
+#define NM_U2F(L)                                                                   \
+  case L:                                                                           \
+    if (op.mcnt_ok)                                                                 \
+      launch_update2f_lx<T, L, true>(op, r, w, fnb, sc, part, apart, napart, s);    \
+    else                                                                            \
+      launch_update2f_lx<T, L, false>(op, r, w, fnb, sc, part, apart, napart, s);   \
     break;
   switch (op.lx) {
     NM_U2F(3) NM_U2F(4) NM_U2F(5) NM_U2F(6) NM_U2F(7) NM_U2F(8) NM_U2F(9) NM_U2F(10)
@@ -745,20 +782,28 @@ static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
       const bool ff = m.face_fuse && op.face_fuse_on;
       int st = op.apply_cgp<T>(w, p, r, x, &sc->beta, &sc->alpha, &sc->done, s, dpart, ff, &m);
       if (st) return st;
-      hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, dpart,
-                         npart, part, sc);
-      hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+      // alpha is formed inside the update kernel from apart.  Up to
+      // RED_BLOCKS Ax partials it reads them directly (folding them would
+      // copy them unchanged) and the update's partials go to `part`;
+      // beyond, they are folded into `part` first and the update's
+      // partials go to dpart.
+      const bool small = npart <= RED_BLOCKS;
+      const double *apart = small ? dpart : part;
+      const int64_t napart = small ? npart : RED_BLOCKS;
+      double *upart = small ? part : dpart;
+      if (!small)
+        hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s,
+                           dpart, npart, part, sc);
       if (ff) {
-        // partials into dpart (its p.Ap partials are folded by now)
-        if ((st = launch_update2f<T>(op, r, w, sc, dpart, s))) return st;
-        hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, dpart, UPF_BLOCKS,
+        if ((st = launch_update2f<T>(op, r, w, sc, upart, apart, napart, s))) return st;
+        hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, upart, UPF_BLOCKS,
                            sc);
         NM_HIP_TRY(hipGetLastError());
         return NM_OK;
       }
       hipLaunchKernelGGL(cg_update2_kernel<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, r, w,
-                         F.mult, F.dinv, n, sc, part);
-      hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, RED_BLOCKS, sc);
+                         F.mult, F.dinv, n, sc, upart, apart, napart);
+      hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, upart, RED_BLOCKS, sc);
       NM_HIP_TRY(hipGetLastError());
       return NM_OK;
     }
@@ -933,7 +978,8 @@ int cg_solver_create(CgSolver **out, Op *op, int precision) {
   }
   if (!(cg->x = cg->alloc<double>(n)) || !(cg->r = cg->alloc<double>(n)) ||
       !(cg->w = cg->alloc<double>(n)) || !(cg->f = cg->alloc<double>(n)) ||
-      !(cg->part = cg->alloc<double>(2 * RED_BLOCKS)) || !(cg->sc = cg->alloc<CgScal>(2))) {
+      !(cg->part = cg->alloc<double>(2 * std::max(RED_BLOCKS, UPF_BLOCKS))) ||
+      !(cg->sc = cg->alloc<CgScal>(2))) {
     set_error("cg: out of device memory");
     return fail(NM_ERR_NOMEM);
   }
