@@ -495,6 +495,10 @@ struct GsDev {
   int32_t *faces = nullptr;
   int64_t nfaces = 0;
   int lx = 0;
+  // every block entry lies on two or more element boundaries (an edge or
+  // vertex of the element): where a field is zero on those points, as the
+  // Schwarz extension leaves it, the block sums are zeros and can be skipped
+  bool blocks_on_edges = false;
 };
 
 struct Schwarz {
@@ -544,8 +548,8 @@ struct Hsmg {
 };
 
 template <typename T>
-static int gs_dev(const GsDev &g, T *u, hipStream_t s) {
-  GsArgs<T> a{u, g.blk_ptr, g.gidx, g.nblks, g.pair0, g.npairs, nullptr};
+static int gs_dev(const GsDev &g, T *u, hipStream_t s, bool blocks = true) {
+  GsArgs<T> a{u, g.blk_ptr, g.gidx, blocks ? g.nblks : 0, g.pair0, g.npairs, nullptr};
   a.variant = g.max_blk <= 8 ? (g.faces ? 2 : 1) : 0;
   a.faces = g.faces;
   a.nfaces = g.nfaces;
@@ -824,7 +828,9 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
       set_error("schwarz: lx %d unsupported (3..10)", lx);
       return NM_ERR_UNSUPPORTED;
   }
-  if ((st = gs_dev<T>(sw.ext, w1, s))) return st;
+  // the extension leaves the edges and vertices of every extended element
+  // zero, so the first exchange's block sums are zeros: faces only
+  if ((st = gs_dev<T>(sw.ext, w1, s, !sw.ext.blocks_on_edges))) return st;
   if ((st = fdm<T>(w2, w1, S, D, nx, nel, s, true))) return st;
   if ((st = gs_dev<T>(sw.ext, w2, s))) return st;
   switch (lx) {
@@ -963,6 +969,17 @@ static int gsdev_level(H *own, GsDev *g, const Op *op, int lx, hipStream_t s) {
   {
     const int st2 = gs_regroup(op->nel, (int64_t)lx * lx * lx, nb, g->pair0, g->npairs, ptr, gi);
     if (st2) return st2;
+  }
+  g->blocks_on_edges = true;
+  {
+    const int64_t n3 = (int64_t)lx * lx * lx;
+    for (int64_t q = 0; q < ptr[nb] && g->blocks_on_edges; q++) {
+      const int64_t l = gi[q] % n3;
+      const int c[3] = {(int)(l % lx), (int)((l / lx) % lx), (int)(l / ((int64_t)lx * lx))};
+      int on = 0;
+      for (int d = 0; d < 3; d++) on += (c[d] == 0 || c[d] == lx - 1);
+      g->blocks_on_edges = on >= 2;
+    }
   }
   if (!(g->blk_ptr = own->template alloc<int32_t>(nb + 1)) ||
       !(g->gidx = own->template alloc<int32_t>(nl))) {
