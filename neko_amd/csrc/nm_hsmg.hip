@@ -832,7 +832,9 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
   // zero, so the first exchange's block sums are zeros: faces only
   if ((st = gs_dev<T>(sw.ext, w1, s, !sw.ext.blocks_on_edges))) return st;
   if ((st = fdm<T>(w2, w1, S, D, nx, nel, s, true))) return st;
-  if ((st = gs_dev<T>(sw.ext, w2, s))) return st;
+  // the restriction below reads the extended face interiors and the element
+  // interior only, never an extended edge or vertex: faces only again
+  if ((st = gs_dev<T>(sw.ext, w2, s, !sw.ext.blocks_on_edges))) return st;
   switch (lx) {
 #define NM_TOREG(N)                                                                      \
   case N:                                                                                \
