@@ -375,55 +375,69 @@ __global__ void __launch_bounds__(NL *NL)
 }
 
 // ext (NL^3, zero) <- interior r (lx^3) plus face layer 0 = interior layer
-// 2 (schwarz_toext3d + extrude(.., 0, 0, .., 2, 1)); one thread per point
+// 2 (schwarz_toext3d + extrude(.., 0, 0, .., 2, 1)).  One workgroup per
+// element: r staged in LDS by coalesced loads, the extended element
+// written coalesced, one thread per extended point.
 template <typename T, int n>
-__global__ void toext_fused_kernel(T *__restrict__ a, const T *__restrict__ r, int64_t nel) {
-  constexpr int m = n + 2;
-  const int64_t m3 = (int64_t)m * m * m;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nel * m3) return;
-  const int64_t e = idx / m3;
-  const int p = (int)(idx % m3);
-  int c[3] = {p % m, (p / m) % m, p / (m * m)};
-  int nb = 0, dir = -1;
-  for (int d = 0; d < 3; d++)
-    if (c[d] == 0 || c[d] == m - 1) {
-      nb++;
-      dir = d;
+__global__ void __launch_bounds__(256)
+    toext_el_kernel(T *__restrict__ a, const T *__restrict__ r, int64_t nel) {
+  constexpr int m = n + 2, n3 = n * n * n, m3 = m * m * m;
+  __shared__ T sr[n3];
+  const int64_t e = blockIdx.x;
+  if (e >= nel) return;
+  for (int q = threadIdx.x; q < n3; q += 256) sr[q] = r[e * n3 + q];
+  __syncthreads();
+  for (int p = threadIdx.x; p < m3; p += 256) {
+    int c[3] = {p % m, (p / m) % m, p / (m * m)};
+    int nb = 0, dir = -1;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      if (c[d] == 0 || c[d] == m - 1) {
+        nb++;
+        dir = d;
+      }
+    T v = T(0);
+    if (nb <= 1) {
+      if (nb == 1) c[dir] = c[dir] == 0 ? 2 : m - 3;
+      v = sr[(c[0] - 1) + n * ((c[1] - 1) + n * (c[2] - 1))];
     }
-  T v = T(0);
-  if (nb == 0) {
-    v = r[e * (int64_t)n * n * n + i3(c[0] - 1, c[1] - 1, c[2] - 1, n)];
-  } else if (nb == 1) {
-    c[dir] = c[dir] == 0 ? 2 : m - 3;
-    v = r[e * (int64_t)n * n * n + i3(c[0] - 1, c[1] - 1, c[2] - 1, n)];
+    a[e * m3 + p] = v;
   }
-  a[idx] = v;
 }
 
 // e (lx^3) <- interior of work2 after the overlap exchange: layer 2 of
 // every face += (work2 - work1) at layer 0 (extrude(w2, 0, 1, w1, 0, -1)
 // then extrude(w2, 2, 1, w2, 0, 1), r faces then s then t, low before
-// high), then schwarz_toreg3d
+// high), then schwarz_toreg3d.  One workgroup per element: the
+// extended element of work2 and the face interiors of work1 staged in LDS
+// by coalesced loads, the element written coalesced.
 template <typename T, int n>
-__global__ void toreg_fused_kernel(T *__restrict__ e, const T *__restrict__ w2,
-                                   const T *__restrict__ w1, int64_t nel) {
-  constexpr int m = n + 2;
-  const int64_t n3 = (int64_t)n * n * n, m3 = (int64_t)m * m * m;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nel * n3) return;
-  const int64_t el = idx / n3;
-  const int p = (int)(idx % n3);
-  const int I = 1 + p % n, J = 1 + (p / n) % n, K = 1 + p / (n * n);
+__global__ void __launch_bounds__(256)
+    toreg_el_kernel(T *__restrict__ e, const T *__restrict__ w2, const T *__restrict__ w1,
+                    int64_t nel) {
+  constexpr int m = n + 2, n3 = n * n * n, m3 = m * m * m;
+  __shared__ T sa[m3], sb[m3];
+  const int64_t el = blockIdx.x;
+  if (el >= nel) return;
   const T *A = w2 + el * m3, *B = w1 + el * m3;
-  T v = A[i3(I, J, K, m)];
-  if (I == 2) v += A[i3(0, J, K, m)] - B[i3(0, J, K, m)];
-  if (I == m - 3) v += A[i3(m - 1, J, K, m)] - B[i3(m - 1, J, K, m)];
-  if (J == 2) v += A[i3(I, 0, K, m)] - B[i3(I, 0, K, m)];
-  if (J == m - 3) v += A[i3(I, m - 1, K, m)] - B[i3(I, m - 1, K, m)];
-  if (K == 2) v += A[i3(I, J, 0, m)] - B[i3(I, J, 0, m)];
-  if (K == m - 3) v += A[i3(I, J, m - 1, m)] - B[i3(I, J, m - 1, m)];
-  e[idx] = v;
+  for (int p = threadIdx.x; p < m3; p += 256) {
+    sa[p] = A[p];
+    const int i = p % m, j = (p / m) % m, k = p / (m * m);
+    const int nb = (i == 0 || i == m - 1) + (j == 0 || j == m - 1) + (k == 0 || k == m - 1);
+    if (nb == 1) sb[p] = B[p];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n3; q += 256) {
+    const int I = 1 + q % n, J = 1 + (q / n) % n, K = 1 + q / (n * n);
+    T v = sa[I + m * (J + m * K)];
+    if (I == 2) v += sa[0 + m * (J + m * K)] - sb[0 + m * (J + m * K)];
+    if (I == m - 3) v += sa[(m - 1) + m * (J + m * K)] - sb[(m - 1) + m * (J + m * K)];
+    if (J == 2) v += sa[I + m * (0 + m * K)] - sb[I + m * (0 + m * K)];
+    if (J == m - 3) v += sa[I + m * ((m - 1) + m * K)] - sb[I + m * ((m - 1) + m * K)];
+    if (K == 2) v += sa[I + m * (J + m * 0)] - sb[I + m * (J + m * 0)];
+    if (K == m - 3) v += sa[I + m * (J + m * (m - 1))] - sb[I + m * (J + m * (m - 1))];
+    e[el * n3 + q] = v;
+  }
 }
 
 // schwarz_wt3d (schwarz.f90:597-634): weights on the two outer layers of
@@ -818,7 +832,7 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
   switch (lx) {
 #define NM_TOEXT(N)                                                                      \
   case N:                                                                                \
-    hipLaunchKernelGGL((toext_fused_kernel<T, N>), dim3(blocks_for(ns)), dim3(256), 0, s, \
+    hipLaunchKernelGGL((toext_el_kernel<T, N>), dim3((unsigned)nel), dim3(256), 0, s,   \
                        w1, r, nel);                                                      \
     break;
     NM_TOEXT(3) NM_TOEXT(4) NM_TOEXT(5) NM_TOEXT(6) NM_TOEXT(7) NM_TOEXT(8) NM_TOEXT(9)
@@ -838,7 +852,7 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
   switch (lx) {
 #define NM_TOREG(N)                                                                      \
   case N:                                                                                \
-    hipLaunchKernelGGL((toreg_fused_kernel<T, N>), dim3(blocks_for(op.n)), dim3(256), 0, s, \
+    hipLaunchKernelGGL((toreg_el_kernel<T, N>), dim3((unsigned)nel), dim3(256), 0, s,   \
                        e, w2, w1, nel);                                                  \
     break;
     NM_TOREG(3) NM_TOREG(4) NM_TOREG(5) NM_TOREG(6) NM_TOREG(7) NM_TOREG(8) NM_TOREG(9)
