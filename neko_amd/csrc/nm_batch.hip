@@ -63,6 +63,17 @@ __device__ inline void bsum_pair(double &a, double &b) {
   __syncthreads();
 }
 
+// Start of a step: a problem whose residual replacement has finished
+// beside the graph (rearm, stored with release order after its scalars)
+// runs again from this step on; within a step done never goes 1 -> 0.
+__global__ void brearm_kernel(CgScal *__restrict__ sc, int ni) {
+  for (int i = threadIdx.x; i < ni; i += blockDim.x)
+    if (__hip_atomic_load(&sc[i].rearm, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+      sc[i].rearm = 0;
+      sc[i].done = 0;
+    }
+}
+
 // alpha_i from the fused p.Ap partials of the problem's workgroups
 __global__ void __launch_bounds__(CHUNK)
     balpha_kernel(const double *__restrict__ dot_part, const BatchInst *__restrict__ inst,
@@ -159,9 +170,9 @@ __global__ void __launch_bounds__(CHUNK)
     bupd_face_kernel(float *__restrict__ r, const float *__restrict__ w,
                      const uint8_t *__restrict__ mcnt, const float *__restrict__ dinv,
                      const int2 *__restrict__ fnb, const int32_t *__restrict__ run_inst,
-                     const int32_t *__restrict__ run_k0, int64_t r_base, int64_t nrun,
-                     const BatchInst *__restrict__ inst, const CgScal *__restrict__ sc,
-                     double *__restrict__ part) {
+                     const int32_t *__restrict__ run_k0, const int32_t *__restrict__ run_map,
+                     int64_t r_base, int64_t nrun, const BatchInst *__restrict__ inst,
+                     const CgScal *__restrict__ sc, double *__restrict__ part) {
   constexpr int N3 = LX * LX * LX;
   // four neighbouring points per thread (16-byte accesses) where lx^3 is a
   // multiple of 4 (element starts stay 4-aligned), else one
@@ -170,7 +181,7 @@ __global__ void __launch_bounds__(CHUNK)
   constexpr int EPI = NH >= CHUNK ? 1 : CHUNK / NH;              // elements per step
   constexpr int NQ = NH >= CHUNK ? (NH + CHUNK - 1) / CHUNK : 1;  // vectors per thread
   constexpr int RK = N3 >= 4096 ? 1 : 4096 / N3;
-  const int64_t u = r_base + blockIdx.x;
+  const int64_t u = run_map[r_base + blockIdx.x];
   const int ii = run_inst[u];
   const CgScal &S = sc[ii];
   if (S.done) return;  // uniform over the workgroup
@@ -304,6 +315,7 @@ struct LxGroup {
   int lx = 0, epb = 1;
   int64_t blk_base = 0, nblocks = 0;
   int64_t r0 = 0, r1 = 0;  // its update runs (face-sum schedule)
+  int64_t m0 = 0, m1 = 0;  // its running problems' runs: run_map[m0, m1)
   std::vector<int> members;
   const float *dx = nullptr, *zg = nullptr, *wg = nullptr;
 };
@@ -338,13 +350,26 @@ struct Batch {
   // problem of every logical workgroup, so a converged or replacing
   // problem's gs workgroups exit at once
   int32_t *wg_blk = nullptr, *wg_pair = nullptr, *wg_msk = nullptr;
+  // the workgroups of the problems still running (rebuilt with the Ax
+  // tables): gs block and mask workgroups, update runs; host copies of the
+  // full problem-per-workgroup tables they are drawn from
+  int32_t *map_blk = nullptr, *map_msk = nullptr, *map_run = nullptr;
+  int64_t nmap_blk = 0, nmap_msk = 0;
+  std::vector<int32_t> h_wg_blk, h_wg_msk, h_run_inst;
   double *x = nullptr, *f = nullptr, *w64 = nullptr, *part = nullptr, *dot_part = nullptr;
   double *rpart = nullptr;  // scratch of the per-problem replacement
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
   float *dinv = nullptr, *mult = nullptr;
-  CgScal *sc = nullptr, *h_sc = nullptr;
+  CgScal *sc = nullptr, *h_sc = nullptr;  // h_sc: two read-back slots of ninst
+  // residual replacements run on s2 beside the iteration graph on s: each
+  // problem's decided scalars land in rep_out (device) and h_rep (pinned),
+  // its completion in rep_ev; gev marks each read-back slot
+  hipStream_t s2 = nullptr;
+  CgScal *rep_out = nullptr, *h_rep = nullptr;
+  std::vector<hipEvent_t> rep_ev;
+  hipEvent_t gev[2] = {nullptr, nullptr}, s2_ev = nullptr;
   hipGraphExec_t graph = nullptr;
-  int graph_iters = 8;
+  int graph_iters = 4;
   bool compacted = false;   // tables currently cover a subset
   int64_t covered_dof = 0;  // local dofs of the problems the tables cover
   // statistics of the last solve: graph steps, and dof-iterations the Ax
@@ -354,10 +379,17 @@ struct Batch {
   int64_t bytes = 0;
   std::vector<void *> owned;
   ~Batch() {
-    if (graph) (void)hipGraphExecDestroy(graph);
     if (s) (void)hipStreamSynchronize(s);
+    if (s2) (void)hipStreamSynchronize(s2);
+    if (graph) (void)hipGraphExecDestroy(graph);
     for (void *q : owned) (void)hipFree(q);
     if (h_sc) (void)hipHostFree(h_sc);
+    if (h_rep) (void)hipHostFree(h_rep);
+    for (hipEvent_t e : rep_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {gev[0], gev[1], s2_ev})
+      if (e) (void)hipEventDestroy(e);
+    if (s2) (void)hipStreamDestroy(s2);
     if (s) (void)hipStreamDestroy(s);
   }
   template <typename T>
@@ -376,6 +408,7 @@ struct Batch {
 // batched Ax kernel (x is one step behind; a replacement flushes it), the
 // update kernel does r -= alpha w and the rtr/rtz partials.
 static int batch_iteration(Batch *B, hipStream_t s) {
+  hipLaunchKernelGGL(brearm_kernel, dim3(1), dim3(256), 0, s, B->sc, (int)B->ops.size());
   for (const LxGroup &g : B->groups) {
     AxBatchArgs a;
     a.w = B->w32;
@@ -410,12 +443,14 @@ static int batch_iteration(Batch *B, hipStream_t s) {
     ga.scal = (const char *)B->sc;
     ga.scal_size = (int)sizeof(CgScal);
     ga.done_off = (int)offsetof(CgScal, done);
+    ga.wg_map = B->map_blk;
+    ga.nwg_map = B->nmap_blk;
   }
   int st = launch_gs<float>(ga, s);
   if (st) return st;
   if (B->wg_msk)
     st = launch_mask_wg(B->w32, B->msk, B->nmsk, B->wg_msk, B->sc, (int)sizeof(CgScal),
-                        (int)offsetof(CgScal, done), s);
+                        (int)offsetof(CgScal, done), s, B->map_msk, B->nmap_msk);
   else
     st = launch_mask<float>(B->w32, B->msk, B->nmsk, nullptr, s);
   if (st) return st;
@@ -423,13 +458,13 @@ static int batch_iteration(Batch *B, hipStream_t s) {
   hipLaunchKernelGGL(balpha_kernel, dim3(ni), dim3(CHUNK), 0, s, B->dot_part, B->inst, B->sc);
   if (B->face) {
     for (const LxGroup &g : B->groups) {
-      if (g.r1 <= g.r0) continue;
-      const dim3 grid((unsigned)(g.r1 - g.r0)), block(CHUNK);
+      if (g.m1 <= g.m0) continue;
+      const dim3 grid((unsigned)(g.m1 - g.m0)), block(CHUNK);
 #define NM_BF_CASE(L)                                                                      \
   case L:                                                                                  \
     hipLaunchKernelGGL(bupd_face_kernel<L>, grid, block, 0, s, B->r32, B->w32, B->mcnt,      \
-                       B->dinv, B->fnb, B->run_inst, B->run_k0, g.r0, B->nrun, B->inst,      \
-                       B->sc, B->part);                                                    \
+                       B->dinv, B->fnb, B->run_inst, B->run_k0, B->map_run, g.m0, B->nrun,   \
+                       B->inst, B->sc, B->part);                                           \
     break;
       switch (g.lx) {
         NM_BF_CASE(3) NM_BF_CASE(4) NM_BF_CASE(5) NM_BF_CASE(6) NM_BF_CASE(7) NM_BF_CASE(8)
@@ -502,6 +537,30 @@ static int batch_tables(Batch *B, const std::vector<char> *finished = nullptr) {
     set_error("batch: workgroup table grew beyond its allocation");
     return NM_ERR_STATE;
   }
+  // the gs block, mask and update-run workgroups of the running problems
+  auto running = [&](int32_t m) { return !finished || !(*finished)[m]; };
+  std::vector<int32_t> mb, mm, mr;
+  for (size_t b = 0; b < B->h_wg_blk.size(); b++)
+    if (running(B->h_wg_blk[b])) mb.push_back((int32_t)b);
+  for (size_t b = 0; b < B->h_wg_msk.size(); b++)
+    if (running(B->h_wg_msk[b])) mm.push_back((int32_t)b);
+  for (LxGroup &g : B->groups) {
+    g.m0 = (int64_t)mr.size();
+    for (int64_t u = g.r0; u < g.r1; u++)
+      if (running(B->h_run_inst[u])) mr.push_back((int32_t)u);
+    g.m1 = (int64_t)mr.size();
+  }
+  B->nmap_blk = (int64_t)mb.size();
+  B->nmap_msk = (int64_t)mm.size();
+  if (B->map_blk && !mb.empty())
+    NM_HIP_TRY(hipMemcpyAsync(B->map_blk, mb.data(), sizeof(int32_t) * mb.size(),
+                              hipMemcpyHostToDevice, B->s));
+  if (B->map_msk && !mm.empty())
+    NM_HIP_TRY(hipMemcpyAsync(B->map_msk, mm.data(), sizeof(int32_t) * mm.size(),
+                              hipMemcpyHostToDevice, B->s));
+  if (B->map_run && !mr.empty())
+    NM_HIP_TRY(hipMemcpyAsync(B->map_run, mr.data(), sizeof(int32_t) * mr.size(),
+                              hipMemcpyHostToDevice, B->s));
   NM_HIP_TRY(hipMemcpyAsync(B->blk_inst, bi.data(), sizeof(int32_t) * bi.size(),
                             hipMemcpyHostToDevice, B->s));
   NM_HIP_TRY(hipMemcpyAsync(B->inst, B->h_inst.data(), sizeof(BatchInst) * ni,
@@ -598,8 +657,20 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
     set_error("nm_batch_create: out of device memory");
     return fail(NM_ERR_NOMEM);
   }
-  if (hipHostMalloc((void **)&B->h_sc, sizeof(CgScal) * ninst, hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc((void **)&B->h_sc, 2 * sizeof(CgScal) * ninst, hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&B->h_rep, sizeof(CgScal) * ninst, hipHostMallocDefault) !=
+          hipSuccess ||
+      !(B->rep_out = B->alloc<CgScal>(ninst)))
     return fail(NM_ERR_NOMEM);
+  if (hipStreamCreateWithFlags(&B->s2, hipStreamNonBlocking) != hipSuccess)
+    return fail(NM_ERR_HIP);
+  B->rep_ev.assign(ninst, nullptr);
+  for (hipEvent_t *e : {&B->gev[0], &B->gev[1], &B->s2_ev})
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return fail(NM_ERR_HIP);
+  for (int i = 0; i < ninst; i++)
+    if (hipEventCreateWithFlags(&B->rep_ev[i], hipEventDisableTiming) != hipSuccess)
+      return fail(NM_ERR_HIP);
   // zero every vector (padding must stay zero), then the FP32 fields
   for (void *v : {(void *)B->x, (void *)B->f, (void *)B->w64})
     if (launch_vec<double>(V_RZERO, (double *)v, nullptr, nullptr, nullptr, 0.0, n, B->s))
@@ -684,6 +755,8 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
   gidx.insert(gidx.end(), gpairs.begin(), gpairs.end());
   B->nmsk = (int64_t)msk.size();
   if (!wgm.empty()) {
+    B->h_wg_msk = wgm;
+    if (!(B->map_msk = B->alloc<int32_t>((int64_t)wgm.size()))) return fail(NM_ERR_NOMEM);
     if (!(B->wg_msk = B->alloc<int32_t>((int64_t)wgm.size()))) return fail(NM_ERR_NOMEM);
     if (hipMemcpyAsync(B->wg_msk, wgm.data(), sizeof(int32_t) * wgm.size(),
                        hipMemcpyHostToDevice, B->s) != hipSuccess)
@@ -697,7 +770,9 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
     const int64_t np = NM_NXCD * ((B->gs_npairs / 128 + NM_NXCD - 1) / NM_NXCD);
     wgb.resize(std::max<int64_t>(nb, 1), 0);
     wgp.resize(std::max<int64_t>(np, 1), 0);
-    if (!(B->wg_blk = B->alloc<int32_t>((int64_t)wgb.size())) ||
+    B->h_wg_blk = wgb;
+    if (!(B->map_blk = B->alloc<int32_t>((int64_t)wgb.size())) ||
+        !(B->wg_blk = B->alloc<int32_t>((int64_t)wgb.size())) ||
         !(B->wg_pair = B->alloc<int32_t>((int64_t)wgp.size())))
       return fail(NM_ERR_NOMEM);
     if (hipMemcpyAsync(B->wg_blk, wgb.data(), sizeof(int32_t) * wgb.size(),
@@ -737,8 +812,10 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
                 (long long)B->nrun);
       return fail(NM_ERR_STATE);
     }
+    B->h_run_inst = rinst;
     if (!(B->fnb = B->alloc<int2>((int64_t)fn.size() / 2)) ||
-        !(B->run_inst = B->alloc<int32_t>(B->nrun)) || !(B->run_k0 = B->alloc<int32_t>(B->nrun))) {
+        !(B->run_inst = B->alloc<int32_t>(B->nrun)) || !(B->run_k0 = B->alloc<int32_t>(B->nrun)) ||
+        !(B->map_run = B->alloc<int32_t>(B->nrun))) {
       set_error("nm_batch_create: out of device memory");
       return fail(NM_ERR_NOMEM);
     }
@@ -868,30 +945,67 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
     if ((st = batch_tables(B)) || (st = batch_capture(B))) return st;
     B->compacted = false;
   }
-  std::vector<char> state(ni, 0);  // 0 running, 1 finished
+  // Pipelined: graph k+1 is launched before the host reads graph k's
+  // scalars, and a problem whose FP32 slice has stopped gets its residual
+  // replacement on s2 while the graphs go on (its done flag keeps every
+  // graph kernel off its slices until the replacement rearms it).  A done,
+  // unfinished problem read at the same iteration count as its last
+  // replacement is one whose rearm is still pending, not a new stop.
+  std::vector<char> state(ni, 0);  // 0 running, 1 finished, 2 replacing on s2
+  std::vector<int> rep_iter(ni, -1);
+  std::vector<CgScal> last(ni);
   int left = ni, launches = 0, covered = ni;
   const int limit = max_iter / B->graph_iters + 64;
   B->stat_steps = 0;
   B->stat_launched = 0.0;
-  while (left > 0) {
+  auto launch = [&]() -> int {
     NM_HIP_TRY(hipGraphLaunch(B->graph, B->s));
+    NM_HIP_TRY(hipMemcpyAsync(B->h_sc + (size_t)(launches % 2) * ni, B->sc,
+                              sizeof(CgScal) * ni, hipMemcpyDeviceToHost, B->s));
+    NM_HIP_TRY(hipEventRecord(B->gev[launches % 2], B->s));
     launches++;
     B->stat_steps += B->graph_iters;
     B->stat_launched += (double)B->graph_iters * (double)B->covered_dof;
-    NM_HIP_TRY(hipMemcpyAsync(B->h_sc, B->sc, sizeof(CgScal) * ni, hipMemcpyDeviceToHost, B->s));
-    NM_HIP_TRY(hipStreamSynchronize(B->s));
+    return NM_OK;
+  };
+  if ((st = launch())) return st;
+  while (left > 0) {
+    const int cur = launches - 1;
+    if ((st = launch())) return st;
+    NM_HIP_TRY(hipEventSynchronize(B->gev[cur % 2]));
+    const CgScal *H = B->h_sc + (size_t)(cur % 2) * ni;
     for (int i = 0; i < ni; i++) {
-      if (state[i]) continue;
-      const CgScal &S = B->h_sc[i];
+      if (state[i] == 1) continue;
+      if (state[i] == 2) {
+        const hipError_t q = hipEventQuery(B->rep_ev[i]);
+        if (q == hipErrorNotReady) continue;
+        NM_HIP_TRY(q);
+        if (B->h_rep[i].fin) {
+          state[i] = 1;
+          last[i] = B->h_rep[i];
+          left--;
+        } else {
+          state[i] = 0;
+        }
+        continue;  // H may predate the replacement's end
+      }
+      const CgScal &S = H[i];
       if (S.fin) {
         state[i] = 1;
+        last[i] = S;
         left--;
-      } else if (S.done) {
+      } else if (S.done && S.iter != rep_iter[i]) {
         const int64_t o = B->voff[i];
-        if ((st = mx_slice_flush(B->x32 + o, B->p32 + o, B->ops[i]->n, B->sc + i, B->s)) ||
+        if ((st = mx_slice_flush(B->x32 + o, B->p32 + o, B->ops[i]->n, B->sc + i, B->s2)) ||
             (st = mx_slice_replace(*B->ops[i], B->x + o, B->x32 + o, B->r32 + o, B->f + o,
-                                   B->w64 + o, B->sc + i, B->rpart, B->s)))
+                                   B->w64 + o, B->sc + i, B->rpart, B->s2, nullptr,
+                                   B->rep_out + i)))
           return st;
+        NM_HIP_TRY(hipMemcpyAsync(B->h_rep + i, B->rep_out + i, sizeof(CgScal),
+                                  hipMemcpyDeviceToHost, B->s2));
+        NM_HIP_TRY(hipEventRecord(B->rep_ev[i], B->s2));
+        rep_iter[i] = S.iter;
+        state[i] = 2;
       }
     }
     if (launches > limit) {
@@ -900,15 +1014,21 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
     }
     // drop finished problems from the Ax launches once a quarter of the
     // covered ones are done (a re-capture costs far less than the Ax work
-    // of the finished problems it removes)
+    // of the finished problems it removes); batch_tables waits for the
+    // graph in flight before the graph is replaced
     if (left > 0 && 4 * left <= 3 * covered) {
-      if ((st = batch_tables(B, &state)) || (st = batch_capture(B))) return st;
+      std::vector<char> fin(ni);
+      for (int i = 0; i < ni; i++) fin[i] = state[i] == 1;
+      if ((st = batch_tables(B, &fin)) || (st = batch_capture(B))) return st;
       B->compacted = true;
       covered = left;
     }
   }
+  // the last replacements (all observed finished) precede the copies out
+  NM_HIP_TRY(hipEventRecord(B->s2_ev, B->s2));
+  NM_HIP_TRY(hipStreamWaitEvent(B->s, B->s2_ev, 0));
   for (int i = 0; i < ni; i++) {
-    const CgScal &S = B->h_sc[i];
+    const CgScal &S = last[i];
     mon[i].iter = S.iter;
     mon[i].outer = S.updates;
     mon[i].res_start = S.res_start;

@@ -22,6 +22,10 @@ struct CgScal {
   double true_rnorm;  // mixed: norm of the last true residual
   int iter, done, max_iter, converged, updates;
   int fin;            // mixed: solve finished (set with the true residual)
+  // batched mixed solves: a replacement that ran beside the iteration
+  // graph leaves done set and raises rearm; the graph's next step clears
+  // both, so a step never sees a problem resume half-way
+  int rearm;
 };
 
 struct CgSolver {
@@ -84,8 +88,12 @@ int mx_slice_begin(const Op &op, double *x, float *x32, float *r32, float *p32,
                    double delta, int max_iter, hipStream_t s);
 // x32 += alpha p32, alpha = 0 (the fused schedule's owed x update)
 int mx_slice_flush(float *x32, const float *p32, int64_t n, CgScal *sc, hipStream_t s);
+// With rep_out the replacement may run beside the iteration graph: a
+// continuing problem keeps done set and raises rearm (with release order
+// after its other writes), and the scalars it decided on are copied to
+// *rep_out, which nothing else writes.
 int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
                      double *w, CgScal *sc, double *part, hipStream_t s,
-                     const MaskRef *mk = nullptr);
+                     const MaskRef *mk = nullptr, CgScal *rep_out = nullptr);
 
 }  // namespace nm

@@ -705,8 +705,13 @@ __global__ void __launch_bounds__(RED_THREADS)
 }
 
 // replacement scalars: true norm, new rtz and beta (the next direction is
-// p = z + beta p with z from the replaced residual), next threshold
-__global__ void mx_resid_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ sc) {
+// p = z + beta p with z from the replaced residual), next threshold.
+// With out (a replacement beside the iteration graph): a continuing
+// problem keeps done = 1, its scalars go to *out, then rearm = 1 is stored
+// with release order at device scope (the graph's step that clears it sees
+// every write before it); a finished one copies its scalars to *out.
+__global__ void mx_resid_fin_kernel(const double *__restrict__ part, CgScal *__restrict__ sc,
+                                    CgScal *__restrict__ out) {
   __shared__ double sh[16];
   double a = 0.0, b = 0.0;
   for (int q = threadIdx.x; q < RED_BLOCKS; q += blockDim.x) {
@@ -733,8 +738,12 @@ __global__ void mx_resid_fin_kernel(const double *__restrict__ part, CgScal *__r
       sc->converged = 0;
       sc->fin = 1;
     } else {
-      sc->done = 0;
+      sc->done = out ? 1 : 0;
       sc->converged = 0;
+    }
+    if (out) {
+      *out = *sc;
+      if (!sc->fin) __hip_atomic_store(&sc->rearm, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1048,14 +1057,15 @@ int mx_slice_flush(float *x32, const float *p32, int64_t n, CgScal *sc, hipStrea
 }
 
 int mx_slice_replace(const Op &op, double *x, float *x32, float *r32, const double *f,
-                     double *w, CgScal *sc, double *part, hipStream_t s, const MaskRef *mk) {
+                     double *w, CgScal *sc, double *part, hipStream_t s, const MaskRef *mk,
+                     CgScal *rep_out) {
   const int64_t n = op.n;
   hipLaunchKernelGGL(mx_absorb_kernel, dim3(vec_blocks(n)), dim3(256), 0, s, x, x32, n);
   int st = op.apply<double>(w, x, nullptr, s, true, true, nullptr, mk);
   if (st) return st;
   hipLaunchKernelGGL(mx_resid_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, r32, f, w,
                      op.f64.mult, op.f32.dinv, n, part);
-  hipLaunchKernelGGL(mx_resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc);
+  hipLaunchKernelGGL(mx_resid_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, part, sc, rep_out);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }

@@ -135,6 +135,10 @@ struct GsArgs {
   const int32_t *wg_blk = nullptr, *wg_pair = nullptr;
   const char *scal = nullptr;
   int scal_size = 0, done_off = 0;
+  // optional (variant 1 blocks): only the nwg_map logical workgroups listed
+  // in wg_map are launched (those of the problems still running)
+  const int32_t *wg_map = nullptr;
+  int64_t nwg_map = 0;
 };
 template <typename T>
 int launch_gs(const GsArgs<T> &a, hipStream_t s);
@@ -152,9 +156,11 @@ template <typename T>
 int launch_mask(T *x, const int32_t *msk0, int64_t m, const int *skip,
                 hipStream_t s);
 // batched FP32 mask with per-workgroup problem skip (entries padded to 256
-// per problem with -1; wg[b] = problem of workgroup b)
+// per problem with -1; wg[b] = problem of workgroup b); with map only the
+// nmap workgroups listed there are launched
 int launch_mask_wg(float *x, const int32_t *msk, int64_t m, const int32_t *wg,
-                   const void *scal, int scal_size, int done_off, hipStream_t s);
+                   const void *scal, int scal_size, int done_off, hipStream_t s,
+                   const int32_t *map = nullptr, int64_t nmap = 0);
 
 // Deterministic reductions: partial sums into part[RED_BLOCKS * nv] then a
 // one-block combine into out[nv] (double).

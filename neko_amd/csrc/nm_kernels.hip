@@ -1012,9 +1012,14 @@ __global__ void __launch_bounds__(256)
     gs_blk8_kernel(T *__restrict__ u, const int32_t *__restrict__ blk_ptr,
                    const int32_t *__restrict__ gidx, int64_t nblks,
                    const int *__restrict__ skip, int64_t per, const int32_t *__restrict__ wg,
-                   const char *__restrict__ scal, int scal_size, int done_off) {
+                   const char *__restrict__ scal, int scal_size, int done_off,
+                   const int32_t *__restrict__ wg_map, int64_t nmap) {
   if (skip != nullptr && *skip) return;
-  const int64_t lb = xcd_block(per);
+  int64_t lb = xcd_block(per);
+  if (wg_map != nullptr) {
+    if (lb >= nmap) return;
+    lb = wg_map[lb];
+  }
   if (wg_done(wg, scal, scal_size, done_off, lb)) return;
   const int64_t t = lb * blockDim.x + threadIdx.x;
   const int64_t g = t >> 3;
@@ -1120,7 +1125,8 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
     if (a.nblks > 0) {
       const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
-                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, nullptr, nullptr, 0, 0);
+                         a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, nullptr, nullptr, 0, 0,
+                         nullptr, (int64_t)0);
     }
     if (a.nfaces > 0) {
       const int64_t LL = (int64_t)(a.lx - 2) * (a.lx - 2);
@@ -1134,11 +1140,11 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
                          a.u, a.rest, (int64_t)0, a.nrest, a.skip, per, nullptr, nullptr, 0, 0);
     }
   } else if (a.variant == 1) {
-    if (a.nblks > 0) {
-      const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
+    if (a.nblks > 0 && (a.wg_map == nullptr || a.nwg_map > 0)) {
+      const int64_t per = a.wg_map ? xcd_per(a.nwg_map) : xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
                          a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, a.wg_blk, a.scal,
-                         a.scal_size, a.done_off);
+                         a.scal_size, a.done_off, a.wg_map, a.nwg_map);
     }
     if (a.npairs > 0) {
       const int64_t per = xcd_per((2 * a.npairs + bs - 1) / bs);
@@ -1275,9 +1281,11 @@ __global__ void mask_kernel(T *__restrict__ x, const int32_t *__restrict__ msk,
 // scal + problem * scal_size + done_off is set.
 __global__ void mask_wg_kernel(float *__restrict__ x, const int32_t *__restrict__ msk,
                                int64_t m, const int32_t *__restrict__ wg,
-                               const char *__restrict__ scal, int scal_size, int done_off) {
-  if (*(const int *)(scal + (int64_t)wg[blockIdx.x] * scal_size + done_off)) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                               const char *__restrict__ scal, int scal_size, int done_off,
+                               const int32_t *__restrict__ map) {
+  const int64_t b = map ? (int64_t)map[blockIdx.x] : (int64_t)blockIdx.x;
+  if (*(const int *)(scal + (int64_t)wg[b] * scal_size + done_off)) return;
+  const int64_t t = b * blockDim.x + threadIdx.x;
   if (t < m) {
     const int32_t k = msk[t];
     if (k >= 0) x[k] = 0.0f;
@@ -1285,10 +1293,12 @@ __global__ void mask_wg_kernel(float *__restrict__ x, const int32_t *__restrict_
 }
 
 int launch_mask_wg(float *x, const int32_t *msk, int64_t m, const int32_t *wg,
-                   const void *scal, int scal_size, int done_off, hipStream_t s) {
-  if (m <= 0) return NM_OK;
-  hipLaunchKernelGGL(mask_wg_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, x, msk,
-                     m, wg, (const char *)scal, scal_size, done_off);
+                   const void *scal, int scal_size, int done_off, hipStream_t s,
+                   const int32_t *map, int64_t nmap) {
+  const int64_t nwg = map ? nmap : (m + 255) / 256;
+  if (m <= 0 || nwg <= 0) return NM_OK;
+  hipLaunchKernelGGL(mask_wg_kernel, dim3((unsigned)nwg), dim3(256), 0, s, x, msk, m, wg,
+                     (const char *)scal, scal_size, done_off, map);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
