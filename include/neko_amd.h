@@ -280,8 +280,9 @@ int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
 /* Ax variant code: elements per workgroup (1,2,4,8,16; 0 = default) plus
  * 100 for the G-prefetching variant, 200 for the geometry evaluated in the
  * kernel (straight-edged operators); gs variant 0 (thread per block), 1
- * (lane-parallel, blocks <= 8 entries) or 2 (as 1, facet pairs as whole
- * shared faces with computed indices). */
+ * (lane-parallel, blocks <= 8 entries), 2 (as 1, facet pairs as whole
+ * shared faces with computed indices) or 3 (as 2, the blocks as packed
+ * segments with one lane per entry). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
 int nm_op_set_gs_variant(nm_op_t *op, int variant);
 /* Fused CG schedule: let (1, the default) or forbid (0) its update kernel

@@ -135,6 +135,12 @@ struct GsArgs {
   const int32_t *wg_blk = nullptr, *wg_pair = nullptr;
   const char *scal = nullptr;
   int scal_size = 0, done_off = 0;
+  // variant 3 (as 2 for the faces): the blocks as packed segments, one lane
+  // per entry: seg_idx[t] the local index (-1: padding), seg_pl[t] its
+  // place in its block | block length << 4, no block across a wavefront
+  const int32_t *seg_idx = nullptr;
+  const uint8_t *seg_pl = nullptr;
+  int64_t nseg = 0;
   // optional (variant 1 blocks): only the nwg_map logical workgroups listed
   // in wg_map are launched (those of the problems still running)
   const int32_t *wg_map = nullptr;

@@ -1044,6 +1044,37 @@ __global__ void __launch_bounds__(256)
   if (g < nblks && m < len) __builtin_nontemporal_store(acc, u + idx);
 }
 
+// Blocks of <= 8 entries as packed segments: one lane per entry (the
+// lane-per-slot form above leaves about half its lanes idle on edge
+// blocks of 4), a block's entries on consecutive lanes of one wavefront;
+// each lane gathers its block's values in block order, so the sum is the
+// same bits as the other block kernels'.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    gs_seg_kernel(T *__restrict__ u, const int32_t *__restrict__ seg_idx,
+                  const uint8_t *__restrict__ seg_pl, int64_t nseg,
+                  const int *__restrict__ skip, int64_t per) {
+  if (skip != nullptr && *skip) return;
+  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
+  int32_t idx = -1;
+  int off = 0, len = 0;
+  if (t < nseg) {
+    idx = seg_idx[t];
+    const int pl = seg_pl[t];
+    off = pl & 15;
+    len = pl >> 4;
+  }
+  const T v = idx >= 0 ? u[idx] : T(0);
+  const int lane0 = (int)(threadIdx.x & 63) - off;
+  T acc = __shfl(v, lane0, 64);
+#pragma unroll
+  for (int q = 1; q < 8; q++) {
+    const T o = __shfl(v, lane0 + q, 64);
+    if (q < len) acc += o;
+  }
+  if (idx >= 0) __builtin_nontemporal_store(acc, u + idx);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     gs_pair_kernel(T *__restrict__ u, const int32_t *__restrict__ gidx,
@@ -1121,8 +1152,12 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
   const int64_t items = a.nblks + a.npairs;
   if (items <= 0) return NM_OK;
   const int bs = 256;
-  if (a.variant == 2) {
-    if (a.nblks > 0) {
+  if (a.variant == 2 || a.variant == 3) {
+    if (a.variant == 3 && a.nseg > 0) {
+      const int64_t per = xcd_per((a.nseg + bs - 1) / bs);
+      hipLaunchKernelGGL(gs_seg_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                         a.u, a.seg_idx, a.seg_pl, a.nseg, a.skip, per);
+    } else if (a.nblks > 0) {
       const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
                          a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, nullptr, nullptr, 0, 0,

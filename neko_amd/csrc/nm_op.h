@@ -47,6 +47,10 @@ struct Op {
   // gs (device layout)
   int64_t nlocal = 0, nblks = 0, npairs = 0, pair0 = 0;
   int32_t *blk_ptr = nullptr, *gidx = nullptr;
+  // the blocks as packed segments (gs variant 3; blocks of <= 8 entries)
+  int32_t *seg_idx = nullptr;
+  uint8_t *seg_pl = nullptr;
+  int64_t nseg = 0;
   // facet pairs as whole-face descriptors (3 int32 each) + the rest
   int32_t *gs_faces = nullptr, *gs_rest = nullptr;
   int64_t nfaces = 0, nrest = 0;

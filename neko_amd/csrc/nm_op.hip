@@ -77,7 +77,12 @@ template <typename T>
 int Op::gs(T *u, const int *skip, hipStream_t s) const {
   GsArgs<T> a{u, blk_ptr, gidx, nblks, pair0, npairs, skip};
   a.variant = (gs_variant >= 1 && max_blk <= 8) ? gs_variant : 0;
-  if (a.variant == 2) {
+  if (a.variant == 3) {
+    a.seg_idx = seg_idx;
+    a.seg_pl = seg_pl;
+    a.nseg = nseg;
+  }
+  if (a.variant == 2 || a.variant == 3) {
     a.faces = gs_faces;
     a.nfaces = nfaces;
     a.lx = lx;
@@ -177,6 +182,14 @@ int Op::apply_cgp(T *w, T *p, const T *r, T *x, const double *beta, const double
     // the blocks, then the facet pairs no descriptor covers
     GsArgs<T> b{w, blk_ptr, gidx, nblks, pair0, 0, skip};
     b.variant = max_blk <= 8 ? 1 : 0;
+    if (gs_variant == 3 && max_blk <= 8) {
+      // the packed-segment blocks (no faces: nfaces stays 0)
+      b.variant = 3;
+      b.seg_idx = seg_idx;
+      b.seg_pl = seg_pl;
+      b.nseg = nseg;
+      b.lx = lx;
+    }
     if (nblks > 0 && (st = launch_gs<T>(b, s))) return st;
     GsArgs<T> c{w, nullptr, gs_rest, 0, 0, nrest, skip};
     c.variant = 1;
@@ -430,6 +443,33 @@ int op_set_gs(Op *op, int64_t nlocal, int64_t facet_offset, int64_t nblks,
     if (!rest.empty())
       NM_HIP_TRY(hipMemcpyAsync(op->gs_rest, rest.data(), sizeof(int32_t) * rest.size(),
                                 hipMemcpyHostToDevice, s));
+    NM_HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (op->max_blk <= 8 && nblks > 0) {
+    // packed segments: a block's entries on consecutive lanes, never across
+    // a 64-lane wavefront (padding entries -1)
+    std::vector<int32_t> si;
+    std::vector<uint8_t> sp;
+    si.reserve((size_t)ptr[nblks] + (size_t)ptr[nblks] / 16 + 64);
+    sp.reserve(si.capacity());
+    for (int64_t b = 0; b < nblks; b++) {
+      const int len = ptr[b + 1] - ptr[b];
+      if ((int64_t)(si.size() % 64) + len > 64)
+        while (si.size() % 64 != 0) {
+          si.push_back(-1);
+          sp.push_back(0);
+        }
+      for (int q = 0; q < len; q++) {
+        si.push_back(gi[ptr[b] + q]);
+        sp.push_back((uint8_t)(q | (len << 4)));
+      }
+    }
+    op->nseg = (int64_t)si.size();
+    NM_ALLOC(op->seg_idx, int32_t, op->nseg);
+    NM_ALLOC(op->seg_pl, uint8_t, op->nseg);
+    NM_HIP_TRY(hipMemcpyAsync(op->seg_idx, si.data(), sizeof(int32_t) * si.size(),
+                              hipMemcpyHostToDevice, s));
+    NM_HIP_TRY(hipMemcpyAsync(op->seg_pl, sp.data(), si.size(), hipMemcpyHostToDevice, s));
     NM_HIP_TRY(hipStreamSynchronize(s));
   }
   NM_ALLOC(op->blk_ptr, int32_t, nblks + 1);
@@ -1120,7 +1160,7 @@ int nm_op_autotune(nm_op_t *h, int reps, int *best64, int *best32,
   op.epb32 = c32;
   float bg = 1e30f;
   int cg = 0;
-  for (int v = 0; v < 3; v++) {
+  for (int v = 0; v < 4; v++) {
     if (v >= 1 && op.max_blk > 8) continue;
     op.gs_variant = v;
     const float tg = timeit([&] { op.gs<double>(w, nullptr, s); });

@@ -80,7 +80,7 @@ def test_ax_h1_field_and_h2(lx):
     assert _relmax(_np(w), c.ax_local(u)) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("case", CASES[:7])
 def test_gs_bit_exact(case, variant):
     nx, ny, nz, lx, amp, per = case
@@ -193,12 +193,14 @@ def test_box_setup_matches_oracle(case):
 
 
 def _solve_pair(nx, ny, nz, lx, amp, per, rhs, precision="fp64", h1=1.0, h2=0.0,
-                code=None, face_sums=None):
+                code=None, face_sums=None, gs_variant=None):
     from neko_amd import solve as S
     c = O.OracleCase(nx, ny, nz, lx, amp=amp, seed=21, periodic=per, h1=h1, h2=h2)
     p = S.Problem.box(nx, ny, nz, lx, periodic=per, amp=amp, seed=21, h1=h1, h2=h2)
     if code is not None:
         p.set_epb(code, code)
+    if gs_variant is not None:
+        p.set_gs_variant(gs_variant)
     if face_sums is not None:
         p.set_cg_face_sums(face_sums)
         assert p.info.cg_face_sums == int(face_sums)
@@ -339,6 +341,24 @@ def test_cg_update_face_sums(precision, lx, per, face_sums):
     print("cg face sums", precision, lx, per, face_sums, flush=True)
     mon, mon_or, err = _solve_pair(3, 4, 3, lx, 0.1, per, "normal",
                                    precision=precision, code=201, face_sums=face_sums)
+    assert mon.converged
+    if precision == "fp64":
+        assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
+    else:
+        assert mon.res_final < 1e-10 * mon.res_start
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("lx,per,code", [(8, (0, 1, 0), 201), (5, (1, 0, 1), 201),
+                                         (6, (0, 0, 1), 102)])
+def test_cg_packed_segment_blocks(precision, lx, per, code):
+    """The fused CG with the gs blocks as packed segments (gs variant 3),
+    in-kernel and stored geometry: oracle iterations +-1 (FP64), 1e-10
+    reached (mixed), solution within 1e-9."""
+    print("cg packed segments", precision, lx, per, code, flush=True)
+    mon, mon_or, err = _solve_pair(3, 4, 3, lx, 0.1, per, "normal", precision=precision,
+                                   code=code, face_sums=True, gs_variant=3)
     assert mon.converged
     if precision == "fp64":
         assert abs(mon.iter - mon_or.iter) <= 1, (mon.iter, mon_or.iter)
