@@ -924,6 +924,9 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
   hipEvent_t ev = guard.ev;
   NM_HIP_TRY(hipEventRecord(ev, stream));
   NM_HIP_TRY(hipStreamWaitEvent(B->s, ev, 0));
+  // replacements a failed earlier solve may have left on s2 finish first
+  NM_HIP_TRY(hipEventRecord(B->s2_ev, B->s2));
+  NM_HIP_TRY(hipStreamWaitEvent(B->s, B->s2_ev, 0));
   for (int i = 0; i < ni; i++) {
     CgScal init{};
     init.norm_fac = 1.0 / std::sqrt(B->ops[i]->volume);
