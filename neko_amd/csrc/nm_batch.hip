@@ -356,6 +356,13 @@ struct Batch {
   int32_t *map_blk = nullptr, *map_msk = nullptr, *map_run = nullptr;
   int64_t nmap_blk = 0, nmap_msk = 0;
   std::vector<int32_t> h_wg_blk, h_wg_msk, h_run_inst;
+  // the gs blocks as packed segments (one lane per entry, a block on
+  // consecutive lanes of one wavefront), each problem's padded to whole
+  // workgroups of 256 lanes: problem of each workgroup, running ones' map
+  int32_t *seg_idx = nullptr, *wg_seg = nullptr, *map_seg = nullptr;
+  uint8_t *seg_pl = nullptr;
+  int64_t nseg = 0, nmap_seg = 0;
+  std::vector<int32_t> h_wg_seg;
   double *x = nullptr, *f = nullptr, *w64 = nullptr, *part = nullptr, *dot_part = nullptr;
   double *rpart = nullptr;  // scratch of the per-problem replacement
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *w32 = nullptr;
@@ -445,6 +452,14 @@ static int batch_iteration(Batch *B, hipStream_t s) {
     ga.done_off = (int)offsetof(CgScal, done);
     ga.wg_map = B->map_blk;
     ga.nwg_map = B->nmap_blk;
+    if (B->seg_idx) {
+      ga.seg_idx = B->seg_idx;
+      ga.seg_pl = B->seg_pl;
+      ga.nseg = B->nseg;
+      ga.wg_blk = B->wg_seg;
+      ga.wg_map = B->map_seg;
+      ga.nwg_map = B->nmap_seg;
+    }
   }
   int st = launch_gs<float>(ga, s);
   if (st) return st;
@@ -544,6 +559,13 @@ static int batch_tables(Batch *B, const std::vector<char> *finished = nullptr) {
     if (running(B->h_wg_blk[b])) mb.push_back((int32_t)b);
   for (size_t b = 0; b < B->h_wg_msk.size(); b++)
     if (running(B->h_wg_msk[b])) mm.push_back((int32_t)b);
+  std::vector<int32_t> ms2;
+  for (size_t b = 0; b < B->h_wg_seg.size(); b++)
+    if (running(B->h_wg_seg[b])) ms2.push_back((int32_t)b);
+  B->nmap_seg = (int64_t)ms2.size();
+  if (B->map_seg && !ms2.empty())
+    NM_HIP_TRY(hipMemcpyAsync(B->map_seg, ms2.data(), sizeof(int32_t) * ms2.size(),
+                              hipMemcpyHostToDevice, B->s));
   for (LxGroup &g : B->groups) {
     g.m0 = (int64_t)mr.size();
     for (int64_t u = g.r0; u < g.r1; u++)
@@ -681,7 +703,8 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
       return fail(NM_ERR_HIP);
   // instance table, chunk table, merged gs and mask (offsets added)
   B->h_inst.resize(ninst);
-  std::vector<int32_t> ubc0, ubinst, gptr, gidx, gpairs, msk, wgb, wgp, wgm;
+  std::vector<int32_t> ubc0, ubinst, gptr, gidx, gpairs, msk, wgb, wgp, wgm, sidx, wgs;
+  std::vector<uint8_t> spl;
   B->cnt = true;
   for (int i = 0; i < ninst; i++) B->cnt = B->cnt && B->ops[i]->mcnt_ok;
   B->face = B->cnt;
@@ -724,6 +747,27 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
         (op->nmsk > 0 && hipMemcpy(mk.data(), op->msk, sizeof(int32_t) * op->nmsk,
                                    hipMemcpyDeviceToHost) != hipSuccess))
       return fail(NM_ERR_HIP);
+    if (pad) {
+      // packed segments of this problem's blocks, then padding to a whole
+      // workgroup (a block never crosses a 64-lane wavefront)
+      for (int64_t b = 0; b < op->nblks; b++) {
+        const int len = ptr[b + 1] - ptr[b];
+        if ((int64_t)(sidx.size() % 64) + len > 64)
+          while (sidx.size() % 64 != 0) {
+            sidx.push_back(-1);
+            spl.push_back(0);
+          }
+        for (int q = 0; q < len; q++) {
+          sidx.push_back(gi[ptr[b] + q] + (int32_t)I.voff);
+          spl.push_back((uint8_t)(q | (len << 4)));
+        }
+      }
+      while (sidx.size() % 256 != 0) {
+        sidx.push_back(-1);
+        spl.push_back(0);
+      }
+      while ((int64_t)wgs.size() * 256 < (int64_t)sidx.size()) wgs.push_back(i);
+    }
     const int32_t base = (int32_t)gidx.size();
     for (int64_t b = 0; b < op->nblks; b++) gptr.push_back(base + ptr[b]);
     for (int64_t q = 0; q < ptr[op->nblks]; q++) gidx.push_back(gi[q] + (int32_t)I.voff);
@@ -780,6 +824,26 @@ int nm_batch_create(nm_batch_t **out, int ninst, nm_op_t *const *ops, hipStream_
         hipMemcpyAsync(B->wg_pair, wgp.data(), sizeof(int32_t) * wgp.size(),
                        hipMemcpyHostToDevice, B->s) != hipSuccess)
       return fail(NM_ERR_HIP);
+    if (!sidx.empty()) {
+      // the segment tables, padded up to a multiple of 8 workgroups (the
+      // XCD order's rounding; those workgroups hold only padding)
+      const int64_t nwg = NM_NXCD * (((int64_t)wgs.size() + NM_NXCD - 1) / NM_NXCD);
+      sidx.resize((size_t)nwg * 256, -1);
+      spl.resize((size_t)nwg * 256, 0);
+      wgs.resize((size_t)nwg, 0);
+      B->nseg = (int64_t)sidx.size();
+      B->h_wg_seg = wgs;
+      if (!(B->seg_idx = B->alloc<int32_t>(B->nseg)) || !(B->seg_pl = B->alloc<uint8_t>(B->nseg)) ||
+          !(B->wg_seg = B->alloc<int32_t>(nwg)) || !(B->map_seg = B->alloc<int32_t>(nwg)))
+        return fail(NM_ERR_NOMEM);
+      if (hipMemcpyAsync(B->seg_idx, sidx.data(), sizeof(int32_t) * sidx.size(),
+                         hipMemcpyHostToDevice, B->s) != hipSuccess ||
+          hipMemcpyAsync(B->seg_pl, spl.data(), spl.size(), hipMemcpyHostToDevice, B->s) !=
+              hipSuccess ||
+          hipMemcpyAsync(B->wg_seg, wgs.data(), sizeof(int32_t) * wgs.size(),
+                         hipMemcpyHostToDevice, B->s) != hipSuccess)
+        return fail(NM_ERR_HIP);
+    }
   }
   B->nub = (int64_t)ubc0.size();
   if (B->face) {

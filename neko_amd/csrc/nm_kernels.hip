@@ -1053,9 +1053,17 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     gs_seg_kernel(T *__restrict__ u, const int32_t *__restrict__ seg_idx,
                   const uint8_t *__restrict__ seg_pl, int64_t nseg,
-                  const int *__restrict__ skip, int64_t per) {
+                  const int *__restrict__ skip, int64_t per, const int32_t *__restrict__ wg,
+                  const char *__restrict__ scal, int scal_size, int done_off,
+                  const int32_t *__restrict__ wg_map, int64_t nmap) {
   if (skip != nullptr && *skip) return;
-  const int64_t t = xcd_block(per) * blockDim.x + threadIdx.x;
+  int64_t lb = xcd_block(per);
+  if (wg_map != nullptr) {
+    if (lb >= nmap) return;
+    lb = wg_map[lb];
+  }
+  if (wg_done(wg, scal, scal_size, done_off, lb)) return;
+  const int64_t t = lb * blockDim.x + threadIdx.x;
   int32_t idx = -1;
   int off = 0, len = 0;
   if (t < nseg) {
@@ -1156,7 +1164,8 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
     if (a.variant == 3 && a.nseg > 0) {
       const int64_t per = xcd_per((a.nseg + bs - 1) / bs);
       hipLaunchKernelGGL(gs_seg_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
-                         a.u, a.seg_idx, a.seg_pl, a.nseg, a.skip, per);
+                         a.u, a.seg_idx, a.seg_pl, a.nseg, a.skip, per, nullptr, nullptr, 0, 0,
+                         nullptr, (int64_t)0);
     } else if (a.nblks > 0) {
       const int64_t per = xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
@@ -1175,7 +1184,17 @@ int launch_gs(const GsArgs<T> &a, hipStream_t s) {
                          a.u, a.rest, (int64_t)0, a.nrest, a.skip, per, nullptr, nullptr, 0, 0);
     }
   } else if (a.variant == 1) {
-    if (a.nblks > 0 && (a.wg_map == nullptr || a.nwg_map > 0)) {
+    if (a.seg_idx != nullptr) {
+      // batched: the blocks as packed segments, whole workgroups of 256
+      // lanes per problem (wg_blk: problem of each), optionally only the
+      // workgroups in wg_map
+      if (a.nseg > 0 && (a.wg_map == nullptr || a.nwg_map > 0)) {
+        const int64_t per = a.wg_map ? xcd_per(a.nwg_map) : xcd_per((a.nseg + bs - 1) / bs);
+        hipLaunchKernelGGL(gs_seg_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
+                           a.u, a.seg_idx, a.seg_pl, a.nseg, a.skip, per, a.wg_blk, a.scal,
+                           a.scal_size, a.done_off, a.wg_map, a.nwg_map);
+      }
+    } else if (a.nblks > 0 && (a.wg_map == nullptr || a.nwg_map > 0)) {
       const int64_t per = a.wg_map ? xcd_per(a.nwg_map) : xcd_per((8 * a.nblks + bs - 1) / bs);
       hipLaunchKernelGGL(gs_blk8_kernel<T>, dim3((unsigned)(NM_NXCD * per)), dim3(bs), 0, s,
                          a.u, a.blk_ptr, a.gidx, a.nblks, a.skip, per, a.wg_blk, a.scal,
