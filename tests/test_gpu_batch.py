@@ -91,6 +91,36 @@ def test_batch_member_independent_of_batch():
     assert all(torch.equal(a, c) for a, c in zip(xs, xs2))
 
 
+def test_batch_zero_rhs_member_and_reuse():
+    """A member with a zero right-hand side is finished at the start (0
+    iterations, x = 0) while the others iterate, replace their residuals
+    beside the iteration graphs and drop out of the launches as they
+    finish; each of the others is bit-identical to its batch of one, and a
+    second solve of the same batch (tables compacted by the first) gives
+    the same bits."""
+    from neko_amd.solve import Batch
+    print("batch zero rhs", flush=True)
+    probs, cases, fs = _build()
+    fs = list(fs)
+    fs[2] = torch.zeros_like(fs[2])
+    b = Batch(probs)
+    xs = [torch.full_like(f, 3.0) for f in fs]
+    mons = b.solve(xs, fs, rel_tol=1e-10)
+    assert mons[2].iter == 0 and mons[2].converged
+    assert torch.count_nonzero(xs[2]).item() == 0
+    for k in (0, 4, 6):
+        assert mons[k].converged and mons[k].iter > 0
+        one = Batch([probs[k]])
+        x1 = torch.zeros_like(fs[k])
+        m1 = one.solve([x1], [fs[k]], rel_tol=1e-10)[0]
+        assert m1.iter == mons[k].iter and torch.equal(x1, xs[k])
+        one.free()
+    xs2 = [torch.zeros_like(f) for f in fs]
+    mons2 = b.solve(xs2, fs, rel_tol=1e-10)
+    assert [m.iter for m in mons2] == [m.iter for m in mons]
+    assert all(torch.equal(a, c) for a, c in zip(xs, xs2))
+
+
 def test_batch_rejects_operator_without_geometry():
     from neko_amd._lib import NekoAmdError
     from neko_amd.solve import Batch, Problem
