@@ -842,6 +842,7 @@ def batched(S, ninst, seed=47, rel_tol=1e-10):
     seq_iters, max_dev = [], 0.0
     for p, f, xb in zip(probs, fs, xs):
         p.set_epb(201, 201)  # in-kernel geometry and the fused CG, as the batch
+        p.set_gs_variant(3)  # block sums as packed segments, as the batch
         cg = S.CG(p, "mixed")
         x = torch.zeros_like(f)
         mult = p.field(S.FIELD_MULT)
