@@ -282,7 +282,8 @@ int nm_op_gs_maps(const nm_op_t *op, int32_t *dg, int32_t *gd,
  * kernel (straight-edged operators); gs variant 0 (thread per block), 1
  * (lane-parallel, blocks <= 8 entries), 2 (as 1, facet pairs as whole
  * shared faces with computed indices) or 3 (as 2, the blocks as packed
- * segments with one lane per entry). */
+ * segments with one lane per entry; the default, 0 where a block has more
+ * than 8 entries). */
 int nm_op_set_epb(nm_op_t *op, int epb64, int epb32);
 int nm_op_set_gs_variant(nm_op_t *op, int variant);
 /* Fused CG schedule: let (1, the default) or forbid (0) its update kernel

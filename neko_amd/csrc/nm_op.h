@@ -84,7 +84,9 @@ struct Op {
   OpFields<float> f32;
   bool has32 = false;
   int epb64 = 0, epb32 = 0;  // Ax variant codes (0 = default)
-  int gs_variant = 0;        // gather-scatter kernel variant
+  // gather-scatter kernel variant (3 by default; variant 0 wherever a
+  // block has more than 8 entries)
+  int gs_variant = 3;
   int max_blk = 0;           // longest gs block
   double *red_part = nullptr;   // RED_BLOCKS * 4 doubles
   double *red_out = nullptr;    // 4 doubles

@@ -131,7 +131,7 @@ def test_mesh_gs_variants_bit_exact(name):
     assert p.info.nrest == 0 and p.info.nfaces * 9 == p.info.npairs
     u = np.random.default_rng(4).standard_normal(c.n)
     ref = c.gs(u.copy())
-    for v in (0, 1, 2):
+    for v in (0, 1, 2, 3):
         p.set_gs_variant(v)
         ud = torch.from_numpy(u.copy()).cuda()
         p.gs_op(ud)
