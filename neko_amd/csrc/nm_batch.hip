@@ -67,8 +67,11 @@ __device__ inline void bsum_pair(double &a, double &b) {
 // beside the graph (rearm, stored with release order after its scalars)
 // runs again from this step on; within a step done never goes 1 -> 0.
 __global__ void brearm_kernel(CgScal *__restrict__ sc, int ni) {
+  // a relaxed look first: the acquire (a cache invalidation at device
+  // scope) only where a flag is up
   for (int i = threadIdx.x; i < ni; i += blockDim.x)
-    if (__hip_atomic_load(&sc[i].rearm, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) {
+    if (__hip_atomic_load(&sc[i].rearm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       sc[i].rearm = 0;
       sc[i].done = 0;
     }
