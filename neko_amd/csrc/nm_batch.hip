@@ -1038,12 +1038,32 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
     B->stat_launched += (double)B->graph_iters * (double)B->covered_dof;
     return NM_OK;
   };
-  if ((st = launch())) return st;
+  // a graph is launched only while some problem runs; with every
+  // unfinished problem replacing, the host waits for the second stream
+  auto any_running = [&]() {
+    for (int i = 0; i < ni; i++)
+      if (state[i] == 0) return true;
+    return false;
+  };
+  int inflight = -1;  // the launch whose scalars are still to be read
   while (left > 0) {
-    const int cur = launches - 1;
-    if ((st = launch())) return st;
-    NM_HIP_TRY(hipEventSynchronize(B->gev[cur % 2]));
-    const CgScal *H = B->h_sc + (size_t)(cur % 2) * ni;
+    if (inflight < 0 && any_running()) {
+      if ((st = launch())) return st;
+      inflight = launches - 1;
+    }
+    const int cur = inflight;
+    inflight = -1;
+    if (cur >= 0 && any_running()) {
+      if ((st = launch())) return st;
+      inflight = launches - 1;
+    }
+    const CgScal *H = nullptr;
+    if (cur >= 0) {
+      NM_HIP_TRY(hipEventSynchronize(B->gev[cur % 2]));
+      H = B->h_sc + (size_t)(cur % 2) * ni;
+    } else {
+      NM_HIP_TRY(hipStreamSynchronize(B->s2));
+    }
     for (int i = 0; i < ni; i++) {
       if (state[i] == 1) continue;
       if (state[i] == 2) {
@@ -1059,6 +1079,7 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
         }
         continue;  // H may predate the replacement's end
       }
+      if (!H) continue;
       const CgScal &S = H[i];
       if (S.fin) {
         state[i] = 1;
@@ -1092,6 +1113,8 @@ int nm_batch_solve(nm_batch_t *h, double *const *x, const double *const *f, doub
       if ((st = batch_tables(B, &fin)) || (st = batch_capture(B))) return st;
       B->compacted = true;
       covered = left;
+      // batch_tables waited for the graph in flight; its scalars are read
+      // by the next pass as usual (the slot and event stay valid)
     }
   }
   // the last replacements (all observed finished) precede the copies out
