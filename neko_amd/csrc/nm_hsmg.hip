@@ -810,9 +810,11 @@ static int fdm(T *e, T *r, const T *S, const T *D, int nl, int64_t nel, hipStrea
 }
 
 // schwarz_compute (CPU branch semantics, schwarz.f90:565-595); r masked in
-// place as in the reference.
+// place as in the reference.  weight = false leaves out the final multiply
+// by the Schwarz weights, for a caller that applies them in its own next
+// pass (the Chebyshev step).
 template <typename T>
-static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
+static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s, bool weight = true) {
   const Op &op = *sw.op;
   const int nx = sw.nl, lx = sw.lx;
   const int64_t nel = sw.nel, ns = nel * nx * nx * nx;
@@ -864,8 +866,10 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s) {
   }
   if ((st = op.gs<T>(e, nullptr, s))) return st;
   if ((st = launch_mask<T>(e, op.msk, op.nmsk, nullptr, s))) return st;
-  hipLaunchKernelGGL(wtp_mul_kernel<T>, dim3((unsigned)std::min<int64_t>(blocks_for(op.n), 16384)),
-                     dim3(256), 0, s, e, wt, op.n);
+  if (weight)
+    hipLaunchKernelGGL(wtp_mul_kernel<T>,
+                       dim3((unsigned)std::min<int64_t>(blocks_for(op.n), 16384)), dim3(256), 0,
+                       s, e, wt, op.n);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -1105,13 +1109,15 @@ __global__ void sub3_kernel(T *__restrict__ r, const T *__restrict__ f, const T 
     r[i] = f[i] - w[i];
 }
 // d = a d + b w; x += d.  The first step (first != 0) sets d = b w without
-// reading d (which holds no value yet).
+// reading d (which holds no value yet).  With wt, w is the Schwarz result
+// before its weights: w wt, rounded as the weight pass would store it.
 template <typename T>
 __global__ void cheby_step_kernel(T *__restrict__ d, T *__restrict__ x, const T *__restrict__ w,
-                                  T a, T b, int64_t n, int first) {
+                                  T a, T b, int64_t n, int first, const T *__restrict__ wt) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const T di = first ? b * w[i] : a * d[i] + b * w[i];
+    const T wi = wt ? w[i] * wt[i] : w[i];
+    const T di = first ? b * wi : a * d[i] + b * wi;
     d[i] = di;
     x[i] = x[i] + di;
   }
@@ -1207,9 +1213,13 @@ static int cheby_solve(Cheby &c, T *x, const T *f, int niter, bool zero_init, hi
   } else {
     NM_HIP_TRY(hipMemcpyAsync(r, f, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
   }
-  if ((st = schwarz_apply<T>(*c.sw, w, r, s))) return st;
+  // the Schwarz weights ride in the Chebyshev step (one pass over w less)
+  const T *S_, *D_, *wt;
+  T *w1_, *w2_;
+  c.sw->arrays<T>(S_, D_, wt, w1_, w2_);
+  if ((st = schwarz_apply<T>(*c.sw, w, r, s, false))) return st;
   hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(g), dim3(256), 0, s, d, x, w, T(0),
-                     T(1.0 / c.tha), n, 1);
+                     T(1.0 / c.tha), n, 1, wt);
   const double sig1 = c.tha / c.dlt;
   double rhok = 1.0 / sig1;
   for (int iter = 2; iter <= niter; iter++) {
@@ -1219,9 +1229,9 @@ static int cheby_solve(Cheby &c, T *x, const T *f, int niter, bool zero_init, hi
     rhok = rhokp1;
     if ((st = op.apply<T>(w, x, nullptr, s, true, true))) return st;
     hipLaunchKernelGGL(sub3_kernel<T>, dim3(g), dim3(256), 0, s, r, f, w, n);
-    if ((st = schwarz_apply<T>(*c.sw, w, r, s))) return st;
+    if ((st = schwarz_apply<T>(*c.sw, w, r, s, false))) return st;
     hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(g), dim3(256), 0, s, d, x, w, (T)tmp1,
-                       (T)tmp2, n, 0);
+                       (T)tmp2, n, 0, wt);
   }
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
