@@ -564,7 +564,9 @@ struct Hsmg {
 template <typename T>
 static int gs_dev(const GsDev &g, T *u, hipStream_t s, bool blocks = true) {
   GsArgs<T> a{u, g.blk_ptr, g.gidx, blocks ? g.nblks : 0, g.pair0, g.npairs, nullptr};
-  a.variant = g.max_blk <= 8 ? (g.faces ? 2 : 1) : 0;
+  // the block length limit of the lane-parallel kernels matters only when
+  // the blocks run
+  a.variant = (!blocks || g.max_blk <= 8) ? (g.faces ? 2 : 1) : 0;
   a.faces = g.faces;
   a.nfaces = g.nfaces;
   a.lx = g.lx;
@@ -1006,7 +1008,9 @@ static int gsdev_level(H *own, GsDev *g, const Op *op, int lx, hipStream_t s) {
     set_error("hsmg: out of device memory");
     return NM_ERR_NOMEM;
   }
-  if (lx > 2 && g->max_blk <= 8) {
+  if (lx > 2) {
+    // (used also where blocks exceed 8 entries, by exchanges that skip the
+    // block sums)
     std::vector<int32_t> faces, rest;
     gs_face_pairs(op->nel, lx, gi.data() + g->pair0, g->npairs, faces, rest);
     if (rest.empty() && !faces.empty()) {
