@@ -261,8 +261,9 @@ __global__ void __launch_bounds__(RED_THREADS)
 // (i, j, k) come from the thread index.  UPF_BLOCKS workgroups (more than
 // the plain update's RED_BLOCKS: the partner load waits on the face-table
 // load, and the extra waves hide that chain; 2048 timed best against 1024,
-// 4096 and 8192, and against keeping 2 or 4 elements in flight per thread),
-// each writing its partial rtr to part[b] and rtz to part[UPF_BLOCKS + b].
+// 4096 and 8192, and against keeping 2 or 4 elements in flight per thread;
+// fewer on problems with fewer element steps), workgroup b of G writing its
+// partial rtr to part[b] and rtz to part[G + b].
 // (Walking chunks of consecutive elements, dealt out so that a point's
 // face partners run on the same XCD at the same time, timed slower: the
 // partner lines cost less than the lost spread of the element walk.)
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_e
     vq[t] = bk ? jj : kk;
   }
   double a = 0.0, b = 0.0;
-  for (int64_t e = blockIdx.x; e < nel; e += UPF_BLOCKS) {
+  for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
 #pragma unroll
     for (int t = 0; t < NQ; t++) {
       const int q = threadIdx.x + t * RED_THREADS;
@@ -334,7 +335,7 @@ __global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_e
   bsum2(a, b, sh);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
-    part[UPF_BLOCKS + blockIdx.x] = b;
+    part[gridDim.x + blockIdx.x] = b;
   }
 }
 
@@ -385,7 +386,7 @@ __global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_e
       vq[t][c] = bk ? jj : kk;
     }
   double a = 0.0, b = 0.0;
-  for (int64_t e = blockIdx.x; e < nel; e += UPF_BLOCKS) {
+  for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
 #pragma unroll
     for (int t = 0; t < NQ; t++) {
       const int h = threadIdx.x + t * RED_THREADS;
@@ -432,7 +433,7 @@ __global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_e
   bsum2(a, b, sh);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
-    part[UPF_BLOCKS + blockIdx.x] = b;
+    part[gridDim.x + blockIdx.x] = b;
   }
 }
 
@@ -483,7 +484,7 @@ __global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_e
     }
   double a = 0.0, b = 0.0;
   if (eo < EPI) {
-    for (int64_t e = (int64_t)blockIdx.x * EPI + eo; e < nel; e += (int64_t)UPF_BLOCKS * EPI) {
+    for (int64_t e = (int64_t)blockIdx.x * EPI + eo; e < nel; e += (int64_t)gridDim.x * EPI) {
 #pragma unroll
       for (int t = 0; t < NQ; t++) {
         const int h = h0 + t * RED_THREADS;
@@ -546,42 +547,54 @@ __global__ void __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_e
   bsum2(a, b, sh);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = a;
-    part[UPF_BLOCKS + blockIdx.x] = b;
+    part[gridDim.x + blockIdx.x] = b;
   }
 }
 
 // apart/napart: the p.Ap partials alpha is formed from (never `part`, the
 // update's own partials, which other workgroups write meanwhile)
+// Workgroups of the face-summing update: UPF_BLOCKS, or fewer when the
+// operator has fewer element steps than that (small problems: no idle
+// workgroups to launch and no zero partials to sum)
+template <typename T, int LX>
+static int upf_grid(int64_t nel) {
+  int64_t epi = 1;
+  if constexpr (LX % 4 == 0 && std::is_same<T, float>::value) {
+    constexpr int NH = LX * LX * LX / 4;
+    epi = NH >= RED_THREADS ? 1 : RED_THREADS / NH;
+  }
+  return (int)std::max<int64_t>(1, std::min<int64_t>(UPF_BLOCKS, (nel + epi - 1) / epi));
+}
+
 template <typename T, int LX, bool CNT>
-static void launch_update2f_lx(const Op &op, T *r, const T *w, const int2 *fnb, CgScal *sc,
-                               double *part, const double *apart, int64_t napart,
-                               hipStream_t s) {
+static int launch_update2f_lx(const Op &op, T *r, const T *w, const int2 *fnb, CgScal *sc,
+                              double *part, const double *apart, int64_t napart,
+                              hipStream_t s) {
   const OpFields<T> &F = op.fields<T>();
+  const int grid = upf_grid<T, LX>(op.nel);
   if constexpr (LX % 4 == 0 && std::is_same<T, float>::value)
-    hipLaunchKernelGGL((cg_update2f_quad_kernel<LX, CNT>), dim3(UPF_BLOCKS), dim3(RED_THREADS),
-                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart,
-                       napart);
+    hipLaunchKernelGGL((cg_update2f_quad_kernel<LX, CNT>), dim3(grid), dim3(RED_THREADS), 0, s,
+                       r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart, napart);
   else if constexpr (LX % 2 == 0 && std::is_same<T, float>::value)
-    hipLaunchKernelGGL((cg_update2f_pair_kernel<T, LX, CNT>), dim3(UPF_BLOCKS),
-                       dim3(RED_THREADS), 0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc,
-                       part, apart, napart);
+    hipLaunchKernelGGL((cg_update2f_pair_kernel<T, LX, CNT>), dim3(grid), dim3(RED_THREADS), 0,
+                       s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart, napart);
   else
-    hipLaunchKernelGGL((cg_update2f_kernel<T, LX, CNT>), dim3(UPF_BLOCKS), dim3(RED_THREADS),
-                       0, s, r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart,
-                       napart);
+    hipLaunchKernelGGL((cg_update2f_kernel<T, LX, CNT>), dim3(grid), dim3(RED_THREADS), 0, s,
+                       r, w, F.mult, op.mcnt, F.dinv, fnb, op.nel, sc, part, apart, napart);
+  return grid;
 }
 
 template <typename T>
+// *grid: the workgroups launched (= partial pairs written to part)
 static int launch_update2f(const Op &op, T *r, const T *w, CgScal *sc, double *part,
-                           const double *apart, int64_t napart, hipStream_t s) {
-  const OpFields<T> &F = op.fields<T>();
+                           const double *apart, int64_t napart, hipStream_t s, int *grid) {
   const int2 *fnb = (const int2 *)op.face_nbr;
-#define NM_U2F(L)                                                                   \
-  case L:                                                                           \
-    if (op.mcnt_ok)                                                                 \
-      launch_update2f_lx<T, L, true>(op, r, w, fnb, sc, part, apart, napart, s);    \
-    else                                                                            \
-      launch_update2f_lx<T, L, false>(op, r, w, fnb, sc, part, apart, napart, s);   \
+#define NM_U2F(L)                                                                           \
+  case L:                                                                                   \
+    if (op.mcnt_ok)                                                                         \
+      *grid = launch_update2f_lx<T, L, true>(op, r, w, fnb, sc, part, apart, napart, s);    \
+    else                                                                                    \
+      *grid = launch_update2f_lx<T, L, false>(op, r, w, fnb, sc, part, apart, napart, s);   \
     break;
   switch (op.lx) {
     NM_U2F(3) NM_U2F(4) NM_U2F(5) NM_U2F(6) NM_U2F(7) NM_U2F(8) NM_U2F(9) NM_U2F(10)
@@ -804,9 +817,9 @@ static int cg_iteration(const Op &op, TX *x, T *r, T *p, T *w, CgScal *sc,
         hipLaunchKernelGGL(fold_partials_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s,
                            dpart, npart, part, sc);
       if (ff) {
-        if ((st = launch_update2f<T>(op, r, w, sc, upart, apart, napart, s))) return st;
-        hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, upart, UPF_BLOCKS,
-                           sc);
+        int ug = 0;
+        if ((st = launch_update2f<T>(op, r, w, sc, upart, apart, napart, s, &ug))) return st;
+        hipLaunchKernelGGL(cg_fin_kernel, dim3(1), dim3(RED_THREADS), 0, s, upart, ug, sc);
         NM_HIP_TRY(hipGetLastError());
         return NM_OK;
       }
