@@ -119,13 +119,27 @@ __global__ void __launch_bounds__(256)
   if (e >= nel) return;
   const double *ue = u + e * (int64_t)nu * nu * nu;
   double *ve = v + e * (int64_t)nv * nv * nv;
-  const int nn = nv * nu;
+  const int nn = nv * nu, nu3 = nu * nu * nu;
   for (int q = threadIdx.x; q < nn; q += blockDim.x) {
     sA[q] = A[e * mstride + q];
     sB[q] = Bt[e * mstride + q];
     sC[q] = Ct[e * mstride + q];
   }
-  for (int q = threadIdx.x; q < nu * nu * nu; q += blockDim.x) su[q] = ue[q];
+  {
+    // every load of u issued before the first LDS store (blockDim.x 256)
+    constexpr int NR = (MAXN * MAXN * MAXN + 255) / 256;
+    double lu[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int q = threadIdx.x + 256 * k;
+      lu[k] = q < nu3 ? ue[q] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int q = threadIdx.x + 256 * k;
+      if (q < nu3) su[q] = lu[k];
+    }
+  }
   __syncthreads();
   // pass 1: w1(i,b,c) = sum_a A(i,a) u(a,b,c)        (nv x nu x nu)
   for (int q = threadIdx.x; q < nv * nu * nu; q += blockDim.x) {
@@ -382,10 +396,22 @@ template <typename T, int n>
 __global__ void __launch_bounds__(256)
     toext_el_kernel(T *__restrict__ a, const T *__restrict__ r, int64_t nel) {
   constexpr int m = n + 2, n3 = n * n * n, m3 = m * m * m;
+  constexpr int NR = (n3 + 255) / 256;
   __shared__ T sr[n3];
   const int64_t e = blockIdx.x;
   if (e >= nel) return;
-  for (int q = threadIdx.x; q < n3; q += 256) sr[q] = r[e * n3 + q];
+  // every load of the element issued before the first LDS store
+  T ld[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const int q = threadIdx.x + 256 * k;
+    ld[k] = q < n3 ? r[e * n3 + q] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const int q = threadIdx.x + 256 * k;
+    if (q < n3) sr[q] = ld[k];
+  }
   __syncthreads();
   for (int p = threadIdx.x; p < m3; p += 256) {
     int c[3] = {p % m, (p / m) % m, p / (m * m)};
@@ -420,11 +446,24 @@ __global__ void __launch_bounds__(256)
   const int64_t el = blockIdx.x;
   if (el >= nel) return;
   const T *A = w2 + el * m3, *B = w1 + el * m3;
-  for (int p = threadIdx.x; p < m3; p += 256) {
-    sa[p] = A[p];
+  // every load of the element issued before the first LDS store
+  constexpr int NR = (m3 + 255) / 256;
+  T la[NR], lb[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) {
+    const int p = threadIdx.x + 256 * q;
     const int i = p % m, j = (p / m) % m, k = p / (m * m);
     const int nb = (i == 0 || i == m - 1) + (j == 0 || j == m - 1) + (k == 0 || k == m - 1);
-    if (nb == 1) sb[p] = B[p];
+    la[q] = p < m3 ? A[p] : T(0);
+    lb[q] = (p < m3 && nb == 1) ? B[p] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < NR; q++) {
+    const int p = threadIdx.x + 256 * q;
+    if (p < m3) {
+      sa[p] = la[q];
+      sb[p] = lb[q];
+    }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < n3; q += 256) {
@@ -583,12 +622,25 @@ static int extrude(double *a1, int l1, double f1, const double *a2, int l2,
 
 static int tnsr3d(double *v, int nv, const double *u, int nu, const double *A,
                   const double *Bt, const double *Ct, int64_t nel, hipStream_t s) {
-  if (std::max(nv, nu) > 14) {
-    set_error("tnsr3d: size %d unsupported", std::max(nv, nu));
-    return NM_ERR_UNSUPPORTED;
+  // LDS sized for the larger of the two extents: at 14^3 a workgroup holds
+  // 49 KB (three per CU), at 8^3 10 KB
+  switch (std::max(nv, nu)) {
+#define NM_TNSR(N)                                                                     \
+  case N:                                                                              \
+    hipLaunchKernelGGL((tnsr3d_kernel<N>), dim3((unsigned)nel), dim3(256), 0, s, v, nv, \
+                       u, nu, A, Bt, Ct, (int64_t)0, nel);                             \
+    break;
+    NM_TNSR(2) NM_TNSR(3) NM_TNSR(4) NM_TNSR(5) NM_TNSR(6) NM_TNSR(7) NM_TNSR(8)
+    NM_TNSR(9) NM_TNSR(10)
+#undef NM_TNSR
+    case 11: case 12: case 13: case 14:
+      hipLaunchKernelGGL((tnsr3d_kernel<14>), dim3((unsigned)nel), dim3(256), 0, s, v, nv,
+                         u, nu, A, Bt, Ct, (int64_t)0, nel);
+      break;
+    default:
+      set_error("tnsr3d: size %d unsupported", std::max(nv, nu));
+      return NM_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL((tnsr3d_kernel<14>), dim3((unsigned)nel), dim3(256), 0, s, v, nv,
-                     u, nu, A, Bt, Ct, (int64_t)0, nel);
   NM_HIP_TRY(hipGetLastError());
   return NM_OK;
 }
@@ -710,8 +762,21 @@ __global__ void __launch_bounds__(64 * WPE)
       }
     }
   }
-  for (int q = tid; q < 6 * N2; q += NTH) sm[q] = se[q];
-  for (int q = tid; q < N3; q += NTH) a[q] = re[q];
+  {
+    // every load of the element issued before the first LDS store
+    constexpr int NS = (6 * N2 + NTH - 1) / NTH, NA = (N3 + NTH - 1) / NTH;
+    T ls[NS], la[NA];
+#pragma unroll
+    for (int k = 0; k < NS; k++) ls[k] = tid + NTH * k < 6 * N2 ? se[tid + NTH * k] : T(0);
+#pragma unroll
+    for (int k = 0; k < NA; k++) la[k] = tid + NTH * k < N3 ? re[tid + NTH * k] : T(0);
+#pragma unroll
+    for (int k = 0; k < NS; k++)
+      if (tid + NTH * k < 6 * N2) sm[tid + NTH * k] = ls[k];
+#pragma unroll
+    for (int k = 0; k < NA; k++)
+      if (tid + NTH * k < N3) a[tid + NTH * k] = la[k];
+  }
   __syncthreads();
   if (FUSED) {
     // the neighbour's overlap only: face layer 0 -= layer 2 (face interiors)
