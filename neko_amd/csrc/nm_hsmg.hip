@@ -392,9 +392,12 @@ __global__ void __launch_bounds__(NL *NL)
 // 2 (schwarz_toext3d + extrude(.., 0, 0, .., 2, 1)).  One workgroup per
 // element: r staged in LDS by coalesced loads, the extended element
 // written coalesced, one thread per extended point.
-template <typename T, int n>
+// With SUB the interior is f - r instead of r (the Chebyshev residual
+// formed on load).
+template <typename T, int n, bool SUB>
 __global__ void __launch_bounds__(256)
-    toext_el_kernel(T *__restrict__ a, const T *__restrict__ r, int64_t nel) {
+    toext_el_kernel(T *__restrict__ a, const T *__restrict__ r, const T *__restrict__ f,
+                    int64_t nel) {
   constexpr int m = n + 2, n3 = n * n * n, m3 = m * m * m;
   constexpr int NR = (n3 + 255) / 256;
   __shared__ T sr[n3];
@@ -405,7 +408,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int k = 0; k < NR; k++) {
     const int q = threadIdx.x + 256 * k;
-    ld[k] = q < n3 ? r[e * n3 + q] : T(0);
+    ld[k] = q < n3 ? (SUB ? f[e * n3 + q] - r[e * n3 + q] : r[e * n3 + q]) : T(0);
   }
 #pragma unroll
   for (int k = 0; k < NR; k++) {
@@ -876,12 +879,23 @@ static int fdm(T *e, T *r, const T *S, const T *D, int nl, int64_t nel, hipStrea
   return NM_OK;
 }
 
+// w[msk] = f[msk]: f - w is then +0 at every masked point, as masking
+// f - w after the subtraction leaves it
+template <typename T>
+__global__ void mask_copy_kernel(T *__restrict__ w, const T *__restrict__ f,
+                                 const int32_t *__restrict__ msk, int64_t m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) w[msk[i]] = f[msk[i]];
+}
+
 // schwarz_compute (CPU branch semantics, schwarz.f90:565-595); r masked in
 // place as in the reference.  weight = false leaves out the final multiply
 // by the Schwarz weights, for a caller that applies them in its own next
-// pass (the Chebyshev step).
+// pass (the Chebyshev step).  With f the input residual is mask(f - r),
+// formed inside the extension (r is overwritten at the masked points).
 template <typename T>
-static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s, bool weight = true) {
+static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s, bool weight = true,
+                         const T *f = nullptr) {
   const Op &op = *sw.op;
   const int nx = sw.nl, lx = sw.lx;
   const int64_t nel = sw.nel, ns = nel * nx * nx * nx;
@@ -895,14 +909,24 @@ static int schwarz_apply(const Schwarz &sw, T *e, T *r, hipStream_t s, bool weig
   int st;
   // the reference's sequence mask, toext3d, extrude x4, fdm, toreg3d runs
   // here as three fused passes around the two extended-layout gs
-  if ((st = launch_mask<T>(r, op.msk, op.nmsk, nullptr, s))) return st;
+  if (f) {
+    if (op.nmsk > 0)
+      hipLaunchKernelGGL(mask_copy_kernel<T>, dim3((unsigned)((op.nmsk + 255) / 256)),
+                         dim3(256), 0, s, r, f, op.msk, op.nmsk);
+  } else if ((st = launch_mask<T>(r, op.msk, op.nmsk, nullptr, s))) {
+    return st;
+  }
   // the extents are template constants, so the point index splits by
   // multiplications instead of 64-bit divisions
   switch (lx) {
 #define NM_TOEXT(N)                                                                      \
   case N:                                                                                \
-    hipLaunchKernelGGL((toext_el_kernel<T, N>), dim3((unsigned)nel), dim3(256), 0, s,   \
-                       w1, r, nel);                                                      \
+    if (f)                                                                               \
+      hipLaunchKernelGGL((toext_el_kernel<T, N, true>), dim3((unsigned)nel), dim3(256),  \
+                         0, s, w1, r, f, nel);                                           \
+    else                                                                                 \
+      hipLaunchKernelGGL((toext_el_kernel<T, N, false>), dim3((unsigned)nel), dim3(256), \
+                         0, s, w1, r, nullptr, nel);                                     \
     break;
     NM_TOEXT(3) NM_TOEXT(4) NM_TOEXT(5) NM_TOEXT(6) NM_TOEXT(7) NM_TOEXT(8) NM_TOEXT(9)
     NM_TOEXT(10)
@@ -1169,14 +1193,6 @@ __global__ void cheby_start_kernel(double *__restrict__ d, int64_t n, uint64_t s
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     d[i] = drng_uniform(seed, (uint64_t)i) + 10.0;
 }
-// r = f - w
-template <typename T>
-__global__ void sub3_kernel(T *__restrict__ r, const T *__restrict__ f, const T *__restrict__ w,
-                            int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    r[i] = f[i] - w[i];
-}
 // d = a d + b w; x += d.  The first step (first != 0) sets d = b w without
 // reading d (which holds no value yet).  With wt, w is the Schwarz result
 // before its weights: w wt, rounded as the weight pass would store it.
@@ -1265,7 +1281,6 @@ static int cheby_power(Cheby &c, uint64_t seed, int its, double *part, double *o
   return NM_OK;
 }
 
-// cheby_impl (cheby.f90:317-397): niter steps on A x = f, all on stream s
 // cheby_impl (cheby.f90:317-397): niter steps on A x = f, all on stream s;
 // T = float runs the whole smoother (operator, gs, Schwarz) in FP32
 template <typename T>
@@ -1276,17 +1291,19 @@ static int cheby_solve(Cheby &c, T *x, const T *f, int niter, bool zero_init, hi
   T *d, *w, *r;
   c.work<T>(d, w, r);
   int st;
-  if (!zero_init) {
-    if ((st = op.apply<T>(w, x, nullptr, s, true, true))) return st;
-    hipLaunchKernelGGL(sub3_kernel<T>, dim3(g), dim3(256), 0, s, r, f, w, n);
-  } else {
-    NM_HIP_TRY(hipMemcpyAsync(r, f, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-  }
   // the Schwarz weights ride in the Chebyshev step (one pass over w less)
   const T *S_, *D_, *wt;
   T *w1_, *w2_;
   c.sw->arrays<T>(S_, D_, wt, w1_, w2_);
-  if ((st = schwarz_apply<T>(*c.sw, w, r, s, false))) return st;
+  // the residual f - A x is formed inside the Schwarz extension (w holds
+  // A x until the Schwarz result replaces it)
+  if (!zero_init) {
+    if ((st = op.apply<T>(w, x, nullptr, s, true, true))) return st;
+    if ((st = schwarz_apply<T>(*c.sw, w, w, s, false, f))) return st;
+  } else {
+    NM_HIP_TRY(hipMemcpyAsync(r, f, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    if ((st = schwarz_apply<T>(*c.sw, w, r, s, false))) return st;
+  }
   hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(g), dim3(256), 0, s, d, x, w, T(0),
                      T(1.0 / c.tha), n, 1, wt);
   const double sig1 = c.tha / c.dlt;
@@ -1297,8 +1314,7 @@ static int cheby_solve(Cheby &c, T *x, const T *f, int niter, bool zero_init, hi
     const double tmp2 = 2.0 * rhokp1 / c.dlt;
     rhok = rhokp1;
     if ((st = op.apply<T>(w, x, nullptr, s, true, true))) return st;
-    hipLaunchKernelGGL(sub3_kernel<T>, dim3(g), dim3(256), 0, s, r, f, w, n);
-    if ((st = schwarz_apply<T>(*c.sw, w, r, s, false))) return st;
+    if ((st = schwarz_apply<T>(*c.sw, w, w, s, false, f))) return st;
     hipLaunchKernelGGL(cheby_step_kernel<T>, dim3(g), dim3(256), 0, s, d, x, w, (T)tmp1,
                        (T)tmp2, n, 0, wt);
   }
