@@ -29,7 +29,35 @@ __global__ void __launch_bounds__(RED_THREADS)
   __syncthreads();
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  // the basis vectors eight at a time (their loads in flight together) and
+  // two points per trip; every sum runs in the order of the plain loop
+  // over l and over the thread's points
+  constexpr int G = 8;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + stride < n; i += 2 * stride) {
+    const int64_t i2 = i + stride;
+    double t = 0.0, t2 = 0.0;
+    for (int l0 = 0; l0 < j; l0 += G) {
+      double a[G], b[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        a[g] = l0 + g < j ? v[l0 + g][i] : 0.0;
+        b[g] = l0 + g < j ? v[l0 + g][i2] : 0.0;
+      }
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (l0 + g < j) {
+          t -= hs[l0 + g] * a[g];
+          t2 -= hs[l0 + g] * b[g];
+        }
+    }
+    const double wi = w[i] + t, wi2 = w[i2] + t2;
+    w[i] = wi;
+    w[i2] = wi2;
+    acc += wi * wi * mult[i];
+    acc += wi2 * wi2 * mult[i2];
+  }
+  if (i < n) {
     double t = 0.0;
     for (int l = 0; l < j; l++) t -= hs[l] * v[l][i];
     const double wi = w[i] + t;

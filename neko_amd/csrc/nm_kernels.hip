@@ -1435,7 +1435,26 @@ __global__ void __launch_bounds__(RED_THREADS)
     double acc[G];
 #pragma unroll
     for (int g = 0; g < G; g++) acc[g] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    // two points per trip (both points' loads in flight together), summed
+    // in the same order as one point per trip
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + stride < n; i += 2 * stride) {
+      const int64_t i2 = i + stride;
+      const double wi = w[i], mi = mult[i], wi2 = w[i2], mi2 = mult[i2];
+      double a[G], b[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        a[g] = g < ng ? __builtin_nontemporal_load(vp[g] + i) : 0.0;
+        b[g] = g < ng ? __builtin_nontemporal_load(vp[g] + i2) : 0.0;
+      }
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (g < ng) {
+          acc[g] += wi * a[g] * mi;
+          acc[g] += wi2 * b[g] * mi2;
+        }
+    }
+    if (i < n) {
       const double wi = w[i], mi = mult[i];
 #pragma unroll
       for (int g = 0; g < G; g++)
