@@ -832,15 +832,18 @@ static int fdm(T *e, T *r, const T *S, const T *D, int nl, int64_t nel, hipStrea
   // 48^3: NL 10 FP64 1.49 -> 0.97 ms, FP32 0.82 -> 0.65 ms; NL 6 FP64 0.29
   // -> 0.27 ms, but FP32 0.18 -> 0.23 ms, where the line kernel stays)
   if (g_fdm_variant == 0 && (std::is_same<T, double>::value || nl >= 10)) {
+    // FP32: two waves per element (a workgroup's LDS is half the FP64 one,
+    // so twice as many elements fit on a CU)
+    constexpr int WP = std::is_same<T, float>::value ? 2 : FDM_WPE;
     switch (nl) {
 #define NM_FDMM(N)                                                                       \
   case N:                                                                                \
     if (fused)                                                                           \
-      hipLaunchKernelGGL((fdm_mfma_kernel<N, FDM_WPE, true, T>), dim3((unsigned)nel),    \
-                         dim3(64 * FDM_WPE), 0, s, e, r, S, D, nel);                     \
+      hipLaunchKernelGGL((fdm_mfma_kernel<N, WP, true, T>), dim3((unsigned)nel),         \
+                         dim3(64 * WP), 0, s, e, r, S, D, nel);                          \
     else                                                                                 \
-      hipLaunchKernelGGL((fdm_mfma_kernel<N, FDM_WPE, false, T>), dim3((unsigned)nel),   \
-                         dim3(64 * FDM_WPE), 0, s, e, r, S, D, nel);                     \
+      hipLaunchKernelGGL((fdm_mfma_kernel<N, WP, false, T>), dim3((unsigned)nel),        \
+                         dim3(64 * WP), 0, s, e, r, S, D, nel);                          \
     NM_HIP_TRY(hipGetLastError());                                                       \
     return NM_OK;
       NM_FDMM(5) NM_FDMM(6) NM_FDMM(7) NM_FDMM(8) NM_FDMM(9) NM_FDMM(10) NM_FDMM(11)
